@@ -1,0 +1,199 @@
+"""Benchmark of the MI355X radiosity-bake path.
+
+Headline (value): visibility throughput in Mrays/s on the "plain GPU path at
+scale" workload: a procedural 8x8-room level (~175k triangles, chop 32,
+16 point lights + sun) and batches of 4M patch-pair TestLine segments
+(MakeTransfers candidates that pass the facing cull), resident in HBM.
+One step = one 4M-segment batch through the visibility kernel.
+
+Also reported (same run): BounceLight gather bandwidth on the precision-form
+transfer shape (250k rows, lognormal lengths, mean 2000, FP32 and FP16),
+and the CPU restatement timed on a bounded sample of the same segments.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
+
+
+def pick_segments(s, n_seg, seed, eps=0.1):
+    """Random patch pairs (i < j) passing the facing cull of MakeTransfers,
+    turned into its TestLine segments (o + eps n)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    o, nrm = s.p_origin, s.p_normal
+    n = len(o)
+    out_i, out_j, have = [], [], 0
+    while have < n_seg:
+        i = rng.integers(0, n, 2 * n_seg)
+        j = rng.integers(0, n, 2 * n_seg)
+        lo, hi = np.minimum(i, j), np.maximum(i, j)
+        keep = lo != hi
+        lo, hi = lo[keep], hi[keep]
+        dv = o[hi] - o[lo]
+        di = np.einsum("ij,ij->i", nrm[lo], dv)
+        dj = -np.einsum("ij,ij->i", nrm[hi], dv)
+        keep = (di > 0) & (dj > 0)
+        out_i.append(lo[keep])
+        out_j.append(hi[keep])
+        have += int(keep.sum())
+    i = np.concatenate(out_i)[:n_seg]
+    j = np.concatenate(out_j)[:n_seg]
+    e = np.float32(eps)
+    st = (o[i] + e * nrm[i]).astype(np.float32)
+    en = (o[j] + e * nrm[j]).astype(np.float32)
+    return st, en
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--segments", type=int, default=4 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--no-bounce", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from vrad_amd import _lib, scenes
+    from vrad_amd.raytracer import Environment
+    from vrad_amd.patches import Transfers
+    from vrad_amd import lightmap
+
+    lib, check = _lib.lib, _lib.check
+    check(lib.vr_init(0))
+    dev = torch.device("cuda:0")
+
+    # ---- workload: configuration 2 level ----
+    t0 = time.time()
+    s = scenes.level(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
+    env = Environment()
+    env.AddTriangles(s.verts9, s.ids)
+    tb = time.time()
+    env.SetupAccelerationStructure()
+    build_s = time.time() - tb
+    info = env.info()
+    st, en = pick_segments(s, args.segments, seed=20)
+    n = len(st)
+    cols = [torch.from_numpy(np.ascontiguousarray(a[:, k])).to(dev) for a in (st, en) for k in range(3)]
+    vis = torch.empty(n, dtype=torch.uint8, device=dev)
+    sky = torch.empty(n, dtype=torch.uint8, device=dev)
+    segs = _lib.Segs(*[_lib.C.cast(c.data_ptr(), _lib.FP) for c in cols])
+    stream = torch.cuda.current_stream(dev)
+    sptr = _lib.C.c_void_p(stream.cuda_stream)
+
+    def step():
+        check(lib.vr_test_line_dev(env.handle, _lib.C.byref(segs), n, -1,
+                                   _lib.C.c_void_p(vis.data_ptr()),
+                                   _lib.C.c_void_p(sky.data_ptr()), sptr))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    check(lib.vr_profile_reset())
+    check(lib.vr_profile_enable(1))
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    check(lib.vr_profile_enable(0))
+    ms, launches = _lib.C.c_double(), _lib.C.c_int64()
+    check(lib.vr_profile_get(b"test_line", _lib.C.byref(ms), _lib.C.byref(launches)))
+    kern_ms = ms.value / max(launches.value, 1)
+    vis_frac = float(vis.float().mean().item())
+    mrays = n * args.steps / elapsed / 1e6
+    achieved = SEG_BYTES * n / (kern_ms * 1e-3) / 1e9
+
+    # ---- CPU restatement on a bounded sample of the same segments ----
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        m = min(args.cpu_sample, n)
+        tri = oracle.precompute(s.verts9, s.ids)
+        kd = oracle.KdTree(s.verts9)
+        c0 = time.perf_counter()
+        cv, _ = oracle.testline(kd, tri, st[:m], en[:m], threads=threads)
+        cpu_s = time.perf_counter() - c0
+        agree = bool(np.array_equal(cv, vis[:m].cpu().numpy()))
+        cpu = {"value": round(m / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {m} of the {n} segments, kd-tree TestLine, {cpu_s:.2f} s",
+               "bitexact_vs_gpu": agree}
+
+    # ---- BounceLight bandwidth: precision-form transfer shape ----
+    bounce = None
+    if not args.no_bounce:
+        bounce = {}
+        N = 250_000
+        rng = np.random.default_rng(8)
+        e0 = rng.random((N, 3)).astype(np.float32)
+        refl = rng.uniform(0.2, 0.8, (N, 3)).astype(np.float32)
+        area = rng.uniform(16, 1024, N).astype(np.float32)
+        xf = Transfers.synthetic(seed=3, n_rows=N, mean_len=2000.0, sigma=1.0, max_len=40000)
+        T = xf.info().nnz
+        for name, prec, cb, sb in (("fp32", _lib.VR_COEF_F32, 4, 0), ("fp16", _lib.VR_COEF_F16, 2, 4)):
+            xf.set_precision(prec)
+            check(lib.vr_profile_reset())
+            check(lib.vr_profile_enable(1))
+            tot, energy, k = lightmap.BounceLight(xf, e0, refl, area, max_bounces=8, stop_frac=0.0)
+            check(lib.vr_profile_enable(0))
+            check(lib.vr_profile_get(b"bounce_gather", _lib.C.byref(ms), _lib.C.byref(launches)))
+            per = ms.value / max(launches.value, 1)
+            B = T * (4 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
+            gbps = B / (per * 1e-3) / 1e9
+            bounce[name] = {"ms_per_bounce": round(per, 4), "GBps": round(gbps, 1),
+                            "frac": round(gbps / HBM_PEAK_GBPS, 4), "bytes_per_bounce": B,
+                            "transfers": T, "rows": N}
+        xf.close()
+
+    out = {
+        "metric": "Mrays/s visibility; bounce HBM GB/s; re-bake p50/p99 ms vs plain GPU",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "plain GPU path at scale: 8x8-room procedural level, "
+                               "4M MakeTransfers TestLine segments per batch",
+                   "triangles": int(info.n_tri), "patches": int(len(s.p_area)),
+                   "kd_nodes": int(info.n_nodes), "kd_leaf_refs": int(info.n_leaf_refs),
+                   "kd_build_s": round(build_s, 2), "segments_per_step": n,
+                   "visible_fraction": round(vis_frac, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
+                     "traffic": None, "kernel": "test_line", "kernel_ms": round(kern_ms, 4),
+                     "algorithmic_bytes_per_launch": SEG_BYTES * n},
+        "cpu_baseline": cpu,
+        "bounce": bounce,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,267 @@
+/*
+ * vrad_hip.h -- C-ABI of the MI355X radiosity-bake path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (VR_OK = 0, negative VR_E_* on failure) and never aborts; vr_last_error()
+ * returns the calling thread's last message.  Host pointers passed in are
+ * read or written only for the duration of the call (cgo pointer rules).
+ *
+ * Each declaration names the reference (Galaco/VRAD, Go) interface it
+ * replaces as file:line; where the reference has a stub or nothing, the
+ * comment says so.  The Go-side binding is shown in INTEGRATION.md.
+ *
+ * Threading: calls are serialised by one library lock; the host-pointer
+ * forms are synchronous, the *_dev forms are asynchronous on the stream
+ * given (NULL = the library stream).
+ */
+#ifndef VRAD_HIP_H
+#define VRAD_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define VR_OK 0
+#define VR_E_INVALID -1     /* bad argument */
+#define VR_E_NOMEM -2       /* host or device allocation failed */
+#define VR_E_HIP -3         /* HIP runtime error */
+#define VR_E_UNSUPPORTED -4 /* e.g. a transparent-triangle callback */
+#define VR_E_STATE -5       /* wrong object state (scene not built, ...) */
+#define VR_E_NODEVICE -6    /* no usable GPU */
+#define VR_E_CAPACITY -7    /* request exceeds a configured capacity */
+
+/* raytracer/constants.go:9-11 */
+#define VR_TRACE_ID_SKY 0x01000000
+#define VR_TRACE_ID_OPAQUE 0x02000000
+#define VR_TRACE_ID_STATICPROP 0x04000000
+/* raytracer/constants.go:5-7 */
+#define VR_RTE_FLAGS_FAST_TREE_GENERATION 1
+#define VR_RTE_FLAGS_DONT_STORE_TRIANGLE_COLORS 2
+#define VR_RTE_FLAGS_DONT_STORE_TRIANGLE_MATERIALS 4
+/* raytracer/cache/optimisedkdnode.go:3-6 */
+#define VR_KDNODE_STATE_XSPLIT 0
+#define VR_KDNODE_STATE_YSPLIT 1
+#define VR_KDNODE_STATE_ZSPLIT 2
+#define VR_KDNODE_STATE_LEAF 3
+/* emittype_t as used by rad/lightmap/lights.go:71,216,235,388,392 */
+#define VR_EMIT_SURFACE 0
+#define VR_EMIT_POINT 1
+#define VR_EMIT_SPOTLIGHT 2
+#define VR_EMIT_SKYLIGHT 3
+#define VR_EMIT_SKYAMBIENT 5
+
+/* kernel paths: the optimised path (LDS traversal stacks, wave-per-row
+ * gathers, device-side stop test) and the plain path it is compared with
+ * (scratch-memory stacks, one thread per row, host energy read per bounce) */
+#define VR_PATH_OPT 0
+#define VR_PATH_PLAIN 1
+
+/* transfer-coefficient storage */
+#define VR_COEF_F32 0
+#define VR_COEF_F16 1
+#define VR_COEF_BF16 2
+
+/* ---- library ----------------------------------------------------------- */
+/* Thread-local text of the last failure of this thread ("" if none). */
+const char *vr_last_error(void);
+/* Select the HIP device and create the library stream.  Called implicitly
+ * by the first entry point that needs the GPU.  No reference counterpart
+ * (the reference is single-process CPU code). */
+int vr_init(int device);
+/* Select VR_PATH_OPT / VR_PATH_PLAIN for the calls that follow. */
+int vr_set_path(int path);
+/* Kernel timing with HIP events on the launching stream.  name = kernel
+ * family ("trace_closest", "trace_occluded", "bounce_gather", "energy",
+ * "make_transfers", "gather_light"). */
+int vr_profile_enable(int on);
+int vr_profile_reset(void);
+int vr_profile_get(const char *name, double *total_ms, int64_t *launches);
+
+/* ---- scene: raytracer.Environment ------------------------------------ */
+typedef struct vr_scene vr_scene;
+
+/* Bulk form of Environment.AddTriangleWithMaterial / AddTriangle
+ * (raytracer/environment.go:41-69): verts9 holds n_tri x VertexCoordData[9]
+ * (TriGeometryData, raytracer/cache/triangle/trigeometrydata.go:3-12),
+ * ids the NTriangleID values, flags the NFlags (may be NULL). */
+int vr_scene_create(vr_scene **out, const float *verts9, const int32_t *ids,
+                    const uint8_t *flags, int64_t n_tri);
+
+/* Environment.SetupAccelerationStructure (raytracer/environment.go:119-138):
+ * SAH kd-tree after RefineNode/CalculateCostsOfSplit (:181-387) with the
+ * costs of raytracer/kdtree/constants.go:25-28, then
+ * ChangeIntoIntersectionFormat per triangle
+ * (raytracer/cache/optimisedtriangle.go:30-78), upload to HBM.
+ * n_threads <= 0 uses every host core. */
+int vr_scene_build(vr_scene *s, int32_t n_threads);
+/* The host half of vr_scene_build (kd-tree + intersection format, no GPU):
+ * lets vr_scene_get_kdtree / vr_scene_get_triangles run on a host without a
+ * device.  vr_scene_build calls it when it has not run. */
+int vr_scene_build_host(vr_scene *s, int32_t n_threads);
+
+typedef struct {
+  int64_t n_tri, n_nodes, n_leaf_refs;
+  int32_t max_depth, has_sky;
+  float bmin[3], bmax[3]; /* Environment.MinBound / MaxBound */
+  int64_t device_bytes;
+} vr_scene_info;
+int vr_scene_get_info(const vr_scene *s, vr_scene_info *out);
+
+/* Environment.OptimizedKDTree + TriangleIndexList
+ * (raytracer/environment.go:33-35; node encoding
+ * raytracer/cache/optimisedkdnode.go:15-54). */
+int vr_scene_get_kdtree(const vr_scene *s, uint32_t *children, float *split,
+                        int32_t *triangle_index_list);
+/* Environment.OptimizedTriangleList in intersection format: n_tri records of
+ * 48 bytes laid out as TriIntersectData
+ * (raytracer/cache/triangle/triintersectdata.go:3-22); GetTriangle (:422). */
+int vr_scene_get_triangles(const vr_scene *s, void *tri_intersect_data);
+int vr_scene_destroy(vr_scene *s);
+
+/* ---- visibility ------------------------------------------------------ */
+typedef struct {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+} vr_rays;
+typedef struct {
+  const float *sx, *sy, *sz, *ex, *ey, *ez;
+} vr_segs;
+
+/* Environment.Trace4Rays (raytracer/environment.go:140-145, a stub in the
+ * reference), batched: for each ray the closest triangle with
+ * tmin <= t < tmax (ties to the lower index), skipping triangles whose
+ * NTriangleID == skip_id.  hit_idx = -1 for no hit, else the index into
+ * OptimizedTriangleList (raytracer/types/result.go:8-12);
+ * hit_normal (3n, nullable) = the hit triangle's plane normal.
+ * A non-NULL transparency callback has no C form (VR_E_UNSUPPORTED is
+ * returned by the Go wrapper before calling here). */
+int vr_trace_closest(vr_scene *s, const vr_rays *rays, int64_t n,
+                     int32_t skip_id, int32_t *hit_idx, float *hit_t,
+                     float *hit_normal);
+
+/* trace.TestLineDoesHitSky (raytracer/trace/testline.go:18-94; panics at
+ * :21 in the reference), batched: dir = (end-start)*(1/len), trace
+ * [0, len); occluded iff the closest hit is not a TRACE_ID_SKY triangle.
+ * visible[i] = 1 - occluded (fractionVisible, :91-93); hit_sky nullable. */
+int vr_test_line(vr_scene *s, const vr_segs *segs, int64_t n, int32_t skip_id,
+                 uint8_t *visible, uint8_t *hit_sky);
+
+/* Device-pointer forms of the two calls above: every pointer (including the
+ * members of *rays / *segs, read on the host) is device memory; the call
+ * returns after enqueueing on `stream` (hipStream_t, NULL = library stream).
+ * No reference counterpart: they let callers keep ray batches resident. */
+int vr_trace_closest_dev(vr_scene *s, const vr_rays *rays, int64_t n,
+                         int32_t skip_id, int32_t *hit_idx, float *hit_t,
+                         float *hit_normal, void *stream);
+int vr_test_line_dev(vr_scene *s, const vr_segs *segs, int64_t n,
+                     int32_t skip_id, uint8_t *visible, uint8_t *hit_sky,
+                     void *stream);
+
+/* ---- transfers: rad/patches MakeTransfers ----------------------------- */
+typedef struct {
+  const float *ox, *oy, *oz, *nx, *ny, *nz, *area;
+} vr_patches;
+typedef struct {
+  float normal_offset;  /* segment end offset along each normal (0.1) */
+  int32_t skip_id;
+  int32_t normalize;    /* scale rows with FP64 sum > 1 by 1/sum */
+  int64_t batch_pairs;  /* candidate pairs per visibility batch (<=0: 8M) */
+} vr_xfer_params;
+typedef struct vr_xfers vr_xfers;
+typedef struct {
+  int64_t n_rows, nnz, stored_nnz, stored_bytes;
+  int32_t precision;
+  int64_t pairs_considered, segments_traced;
+} vr_xfers_info;
+
+/* MakeTransfers (absent in the reference; result records
+ * common/types/transfer.go:3-6 and Patch.NumTransfers/Transfers,
+ * common/types/patch.go:60-61): leaf patches i<j passing the facing cull
+ * are tested with TestLine between the normal-offset origins, visible pairs
+ * get F_ij = cos_i cos_j A_j / (pi r^2) and the CSR stays resident. */
+int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t n_patches,
+                      const vr_xfer_params *params, vr_xfers **out);
+/* Upload a host CSR (row_ptr n_rows+1, col/coef nnz) as transfers. */
+int vr_xfers_from_csr(vr_xfers **out, int64_t n_rows, const int64_t *row_ptr,
+                      const int32_t *col, const float *coef, int32_t precision);
+/* Synthetic transfers on the device: row lengths lognormal
+ * (mu = ln(mean) - sigma^2/2), clipped to [1, max_len] and rescaled to the
+ * mean; columns one per equal stratum of [0,n)\{i} (sorted, unique);
+ * coef = u/sum(u) * s_i with s_i ~ U[0.3,0.9]. */
+int vr_xfers_synthetic(vr_xfers **out, uint64_t seed, int64_t n_rows,
+                       double mean_len, double sigma, int64_t max_len,
+                       int32_t precision);
+/* Re-encode the coefficients: FP16/BF16 store coef/row_max with the row
+ * max as an FP32 per-row scale. */
+int vr_xfers_set_precision(vr_xfers *x, int32_t precision);
+int vr_xfers_get_info(const vr_xfers *x, vr_xfers_info *out);
+/* Copy out the logical CSR (coef decoded to FP32 without the row scale for
+ * FP16/BF16; raw_coef16 nullable gets the stored 16-bit words; row_scale
+ * nullable). */
+int vr_xfers_download(const vr_xfers *x, int64_t *row_ptr, int32_t *col,
+                      float *coef, uint16_t *raw_coef16, float *row_scale);
+int vr_xfers_destroy(vr_xfers *x);
+
+/* ---- direct light: rad/lightmap GatherSampleLight --------------------- */
+/* worldlight.WorldLight fields used by rad/lightmap/lights.go:168-341 plus
+ * DirectLight's fade/cap (common/types/light.go:31-36). */
+typedef struct {
+  int32_t type;
+  float origin[3];
+  float intensity[3];
+  float normal[3];
+  float constant_attn, linear_attn, quadratic_attn;
+  float stopdot, stopdot2, exponent;
+  float start_fade, end_fade, cap_distance;
+} vr_light;
+typedef struct {
+  const float *px, *py, *pz, *nx, *ny, *nz;
+} vr_samples;
+
+/* GatherSampleLight (absent in the reference): for every sample, the sum
+ * over lights (list order) of intensity x falloff x TestLine visibility. */
+int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n,
+                           const vr_light *lights, int32_t n_lights,
+                           float normal_offset, int32_t skip_id, float *rgb);
+
+/* ---- bounce: rad/lightmap BounceLight --------------------------------- */
+typedef struct {
+  int32_t max_bounces;  /* -bounce (cmd/args.go:66); numBounce default 2 */
+  double stop_frac;     /* stop when E_k <= stop_frac * sum_m E_m */
+} vr_bounce_params;
+
+/* BounceLight (absent in the reference; state Patch.TotalLight,
+ * Reflectivity, Area, common/types/patch.go:33-42): Jacobi gather
+ * add = F.emit, total += add, emit = add*refl, FP64 energy per bounce.
+ * emit0/refl/total are 3n RGB, area n.  energy gets one FP64 per bounce. */
+int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl,
+                    const float *area, float *total,
+                    const vr_bounce_params *params, double *energy,
+                    int32_t *n_bounces);
+
+/* ---- procedural scenes (inputs for tests and benchmarks) --------------- */
+typedef struct vr_gen vr_gen;
+/* Closed room 1024x1024x256 with n_pillars seeded axis-aligned pillars,
+ * shell tessellated at `tess` units, patches chopped at `chop`, one point
+ * light (intensity 10000, quadratic attenuation 1). */
+int vr_gen_room(vr_gen **out, uint64_t seed, int32_t n_pillars, float chop,
+                float tess);
+/* Grid of rooms_x x rooms_y connected rooms (sizes 256-2048, doorways
+ * 128x192, pillars, ramps, skylights), n_point_lights point lights and one
+ * sun when sun != 0. */
+int vr_gen_level(vr_gen **out, uint64_t seed, int32_t rooms_x, int32_t rooms_y,
+                 float chop, float tess, int32_t n_point_lights, int32_t sun);
+int vr_gen_counts(const vr_gen *g, int64_t *n_tri, int64_t *n_patches,
+                  int32_t *n_lights);
+/* verts9 9n, ids n, flags n, patch arrays (o/n/refl 3 per patch, area 1),
+ * lights n_lights; any pointer may be NULL. */
+int vr_gen_copy(const vr_gen *g, float *verts9, int32_t *ids, uint8_t *flags,
+                float *patch_origin, float *patch_normal, float *patch_area,
+                float *patch_refl, vr_light *lights);
+int vr_gen_destroy(vr_gen *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,271 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker of the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+TRI_DTYPE = np.dtype([("nx", "<f4"), ("ny", "<f4"), ("nz", "<f4"), ("d", "<f4"), ("id", "<i4"),
+                      ("e", "<f4", 6), ("c0", "u1"), ("c1", "u1"), ("flags", "u1"),
+                      ("unused", "u1")])
+assert TRI_DTYPE.itemsize == 48
+LIGHT_DTYPE = np.dtype([("type", "<i4"), ("origin", "<f4", 3), ("intensity", "<f4", 3),
+                        ("normal", "<f4", 3), ("constant_attn", "<f4"), ("linear_attn", "<f4"),
+                        ("quadratic_attn", "<f4"), ("stopdot", "<f4"), ("stopdot2", "<f4"),
+                        ("exponent", "<f4"), ("start_fade", "<f4"), ("end_fade", "<f4"),
+                        ("cap_distance", "<f4")])
+
+FP = C.POINTER(C.c_float)
+P = C.c_void_p
+
+
+class _Rays(C.Structure):
+    _fields_ = [(n, FP) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+
+
+class _Segs(C.Structure):
+    _fields_ = [(n, FP) for n in ("sx", "sy", "sz", "ex", "ey", "ez")]
+
+
+class _Patches(C.Structure):
+    _fields_ = [(n, FP) for n in ("ox", "oy", "oz", "nx", "ny", "nz", "area")]
+
+
+class _Samples(C.Structure):
+    _fields_ = [(n, FP) for n in ("px", "py", "pz", "nx", "ny", "nz")]
+
+
+class _Kd(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("children", C.POINTER(C.c_uint32)), ("split", FP),
+                ("n_idx", C.c_int64), ("idx", C.POINTER(C.c_int32)), ("bmin", C.c_float * 3),
+                ("bmax", C.c_float * 3), ("max_depth", C.c_int)]
+
+
+class _Csr(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("nnz", C.c_int64), ("row_ptr", C.POINTER(C.c_int64)),
+                ("col", C.POINTER(C.c_int32)), ("coef", FP), ("row_sum", C.POINTER(C.c_double))]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built (make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "or_precompute": (None, [P, P, P, C.c_int64, P]),
+        "or_trace_brute": (None, [P, C.c_int64, C.POINTER(_Rays), C.c_int64, C.c_int32, P, P]),
+        "or_trace_brute_f64": (None, [P, P, C.c_int64, C.POINTER(_Rays), C.c_int64, C.c_int32,
+                                      P, P]),
+        "or_kd_build": (C.c_int, [P, C.c_int64, C.POINTER(_Kd)]),
+        "or_kd_free": (None, [C.POINTER(_Kd)]),
+        "or_trace_kd": (None, [C.POINTER(_Kd), P, C.POINTER(_Rays), C.c_int64, C.c_int32, P, P,
+                               C.c_int]),
+        "or_testline": (None, [C.POINTER(_Kd), P, C.c_int64, C.POINTER(_Segs), C.c_int64,
+                               C.c_int32, P, P, C.c_int]),
+        "or_make_transfers": (C.c_int, [C.POINTER(_Kd), P, C.c_int64, C.POINTER(_Patches),
+                                        C.c_int64, C.c_float, C.c_int32, C.c_int, C.c_int,
+                                        C.POINTER(_Csr)]),
+        "or_csr_free": (None, [C.POINTER(_Csr)]),
+        "or_row_sum_f64": (C.c_double, [P, C.c_int64]),
+        "or_gather_light": (None, [C.POINTER(_Kd), P, C.c_int64, C.POINTER(_Samples), C.c_int64,
+                                   P, C.c_int32, C.c_float, C.c_int32, P, C.c_int]),
+        "or_sky_dirs": (None, [P]),
+        "or_bounce_f32": (C.c_int, [C.c_int64, P, P, P, C.c_int, P, P, P, P, P, C.c_int,
+                                    C.c_double, P, C.c_int]),
+        "or_bounce_f64": (C.c_int, [C.c_int64, P, P, P, P, P, P, P, C.c_int, C.c_double, P]),
+        "or_energy_reduce": (C.c_double, [P, C.c_int64]),
+        "or_f32_to_f16": (C.c_uint16, [C.c_float]),
+        "or_f16_to_f32": (C.c_float, [C.c_uint16]),
+        "or_f32_to_bf16": (C.c_uint16, [C.c_float]),
+        "or_bf16_to_f32": (C.c_float, [C.c_uint16]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    return lib
+
+
+lib = _load()
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f(a):
+    return a.ctypes.data_as(FP)
+
+
+def _cols(arr, k):
+    a = np.ascontiguousarray(arr, np.float32).reshape(-1, k)
+    return [np.ascontiguousarray(a[:, i]) for i in range(k)]
+
+
+def precompute(v9, ids, flags=None):
+    v9 = np.ascontiguousarray(v9, np.float32).reshape(-1, 9)
+    ids = np.ascontiguousarray(ids, np.int32)
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    out = np.zeros(len(ids), TRI_DTYPE)
+    lib.or_precompute(_p(v9), _p(ids), _p(fl), len(ids), _p(out))
+    return out
+
+
+class KdTree:
+    def __init__(self, v9):
+        self.v9 = np.ascontiguousarray(v9, np.float32).reshape(-1, 9)
+        self.kd = _Kd()
+        rc = lib.or_kd_build(_p(self.v9), len(self.v9), C.byref(self.kd))
+        if rc:
+            raise MemoryError("oracle kd build failed")
+        n, m = self.kd.n_nodes, self.kd.n_idx
+        self.children = np.ctypeslib.as_array(self.kd.children, (n,)).copy()
+        self.split = np.ctypeslib.as_array(self.kd.split, (n,)).copy()
+        self.idx = np.ctypeslib.as_array(self.kd.idx, (max(m, 1),))[:m].copy()
+        self.max_depth = self.kd.max_depth
+        self.bmin = np.array(self.kd.bmin[:], np.float32)
+        self.bmax = np.array(self.kd.bmax[:], np.float32)
+
+    def __del__(self):
+        try:
+            lib.or_kd_free(C.byref(self.kd))
+        except Exception:
+            pass
+
+
+def _rays(o, d, tmin, tmax):
+    n = len(np.asarray(o).reshape(-1, 3))
+    cols = _cols(o, 3) + _cols(d, 3) + [np.ascontiguousarray(np.broadcast_to(tmin, (n,)), np.float32),
+                                        np.ascontiguousarray(np.broadcast_to(tmax, (n,)), np.float32)]
+    return _Rays(*[_f(c) for c in cols]), cols, n
+
+
+def trace_brute(tris, o, d, tmin, tmax, skip=-1):
+    r, keep, n = _rays(o, d, tmin, tmax)
+    hit = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    lib.or_trace_brute(_p(tris), len(tris), C.byref(r), n, skip, _p(hit), _p(t))
+    return hit, t
+
+
+def trace_brute_f64(v9, ids, o, d, tmin, tmax, skip=-1):
+    v9 = np.ascontiguousarray(v9, np.float32)
+    ids = np.ascontiguousarray(ids, np.int32)
+    r, keep, n = _rays(o, d, tmin, tmax)
+    hit = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float64)
+    lib.or_trace_brute_f64(_p(v9), _p(ids), len(ids), C.byref(r), n, skip, _p(hit), _p(t))
+    return hit, t
+
+
+def trace_kd(kd, tris, o, d, tmin, tmax, skip=-1, threads=1):
+    r, keep, n = _rays(o, d, tmin, tmax)
+    hit = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    lib.or_trace_kd(C.byref(kd.kd), _p(tris), C.byref(r), n, skip, _p(hit), _p(t), threads)
+    return hit, t
+
+
+def testline(kd, tris, s, e, skip=-1, threads=1):
+    cols = _cols(s, 3) + _cols(e, 3)
+    g = _Segs(*[_f(c) for c in cols])
+    n = len(cols[0])
+    vis = np.zeros(n, np.uint8)
+    sky = np.zeros(n, np.uint8)
+    lib.or_testline(C.byref(kd.kd) if kd is not None else None, _p(tris), len(tris), C.byref(g),
+                    n, skip, _p(vis), _p(sky), threads)
+    return vis, sky
+
+
+def make_transfers(kd, tris, origin, normal, area, eps=0.1, skip=-1, normalize=True, threads=1):
+    cols = _cols(origin, 3) + _cols(normal, 3) + [np.ascontiguousarray(area, np.float32)]
+    pt = _Patches(*[_f(c) for c in cols])
+    n = len(cols[0])
+    csr = _Csr()
+    rc = lib.or_make_transfers(C.byref(kd.kd) if kd is not None else None, _p(tris), len(tris),
+                               C.byref(pt), n, eps, skip, 1 if normalize else 0, threads,
+                               C.byref(csr))
+    if rc:
+        raise MemoryError("oracle transfers failed")
+    nnz = csr.nnz
+    rp = np.ctypeslib.as_array(csr.row_ptr, (n + 1,)).copy()
+    col = np.ctypeslib.as_array(csr.col, (max(nnz, 1),))[:nnz].copy()
+    coef = np.ctypeslib.as_array(csr.coef, (max(nnz, 1),))[:nnz].copy()
+    rs = np.ctypeslib.as_array(csr.row_sum, (max(n, 1),))[:n].copy()
+    lib.or_csr_free(C.byref(csr))
+    return rp, col, coef, rs
+
+
+def gather_light(kd, tris, pos, nrm, lights, eps=0.1, skip=-1, threads=1):
+    cols = _cols(pos, 3) + _cols(nrm, 3)
+    s = _Samples(*[_f(c) for c in cols])
+    n = len(cols[0])
+    L = np.ascontiguousarray(lights).view(LIGHT_DTYPE) if lights.dtype != LIGHT_DTYPE else \
+        np.ascontiguousarray(lights)
+    rgb = np.zeros((n, 3), np.float32)
+    lib.or_gather_light(C.byref(kd.kd) if kd is not None else None, _p(tris), len(tris),
+                        C.byref(s), n, _p(L), len(L), eps, skip, _p(rgb), threads)
+    return rgb
+
+
+def sky_dirs():
+    d = np.zeros((162, 3), np.float32)
+    lib.or_sky_dirs(_p(d))
+    return d
+
+
+def bounce_f32(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, stop_frac=1e-3,
+               kind=0, row_scale=None, threads=1):
+    n = len(rp) - 1
+    rp = np.ascontiguousarray(rp, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    coef = np.ascontiguousarray(coef, np.float32 if kind == 0 else np.uint16)
+    sc = None if row_scale is None else np.ascontiguousarray(row_scale, np.float32)
+    e0 = np.ascontiguousarray(emit0, np.float32).reshape(n, 3)
+    rf = np.ascontiguousarray(refl, np.float32).reshape(n, 3)
+    ar = np.ascontiguousarray(area, np.float32).reshape(n)
+    tot = np.zeros((n, 3), np.float32) if total is None else \
+        np.ascontiguousarray(total, np.float32).reshape(n, 3).copy()
+    en = np.zeros(max(max_bounces, 1), np.float64)
+    k = lib.or_bounce_f32(n, _p(rp), _p(col), _p(coef), kind, _p(sc), _p(e0), _p(rf), _p(ar),
+                          _p(tot), max_bounces, stop_frac, _p(en), threads)
+    return tot, en[:k], k
+
+
+def bounce_f64(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, stop_frac=1e-3):
+    n = len(rp) - 1
+    rp = np.ascontiguousarray(rp, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    coef = np.ascontiguousarray(coef, np.float32)
+    e0 = np.ascontiguousarray(emit0, np.float32).reshape(n, 3)
+    rf = np.ascontiguousarray(refl, np.float32).reshape(n, 3)
+    ar = np.ascontiguousarray(area, np.float32).reshape(n)
+    tot = np.zeros((n, 3), np.float64) if total is None else \
+        np.ascontiguousarray(total, np.float64).reshape(n, 3).copy()
+    en = np.zeros(max(max_bounces, 1), np.float64)
+    k = lib.or_bounce_f64(n, _p(rp), _p(col), _p(coef), _p(e0), _p(rf), _p(ar), _p(tot),
+                          max_bounces, stop_frac, _p(en))
+    return tot, en[:k], k
+
+
+def energy_reduce(e):
+    e = np.ascontiguousarray(e, np.float64)
+    return lib.or_energy_reduce(_p(e), len(e))
+
+
+def row_sum_f64(coef):
+    c = np.ascontiguousarray(coef, np.float32)
+    return lib.or_row_sum_f64(_p(c), len(c))
+
+
+def f32_to_f16(x):
+    return lib.or_f32_to_f16(float(x))
+
+
+def f32_to_bf16(x):
+    return lib.or_f32_to_bf16(float(x))

@@ -1,0 +1,1131 @@
+/*
+ * oracle.c -- CPU restatement of the VRAD radiosity-bake hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (checker for tests/, smoke() and bench.py's
+ * cpu_baseline leg).  Nothing in the product links or calls it.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fno-fast-math -msse2 -mfpmath=sse
+ *        (see oracle/Makefile).  All FP32 expressions below are written with
+ *        explicit left-to-right association; with contraction off every
+ *        operation rounds once, exactly as on the GPU build.
+ *
+ * Pinning (what each function is checked against, tests/test_oracle.py):
+ *   or_precompute      closed-form triangles (tests/golden/tri_precompute.json),
+ *                      edge-equation invariants E.(p,1) in {0,1}
+ *   or_trace_brute     analytic axis-aligned rays (exact t), FP64 brute force
+ *   or_kd_build        node encode/decode, every triangle reachable, the
+ *                      RefineNode cost function on hand-worked 3-4 triangle sets
+ *   or_trace_kd        equality with or_trace_brute (bit-exact) on random rays
+ *   or_testline        closed-form occluder scenes (box between two points)
+ *   or_make_transfers  parallel / perpendicular unit-square form factors
+ *                      (Howell catalogue), reciprocity, enclosure row sums
+ *   or_gather_light    unoccluded closed form I*cos/d^2, occluded -> 0
+ *   or_bounce_*        furnace closed form E*sum(rho^k) (dyadic, exact),
+ *                      scipy sparse solve of (I - rho F) B = E on tiny N
+ *   or_energy_reduce   exact integer-valued sums, long-double pairwise sum
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* small helpers                                                             */
+/* ------------------------------------------------------------------------ */
+static inline float f_min(float a, float b) { return a < b ? a : b; }
+static inline float f_max(float a, float b) { return a > b ? a : b; }
+
+typedef struct {
+  void (*fn)(void *ctx, int64_t begin, int64_t end);
+  void *ctx;
+  int64_t begin, end;
+} or_job_t;
+
+static void *or_job_run(void *arg) {
+  or_job_t *j = (or_job_t *)arg;
+  j->fn(j->ctx, j->begin, j->end);
+  return NULL;
+}
+
+/* run fn over [0,n) split into nthreads contiguous ranges */
+static void or_parallel(int64_t n, int nthreads,
+                        void (*fn)(void *, int64_t, int64_t), void *ctx) {
+  if (nthreads <= 1 || n < 64) {
+    fn(ctx, 0, n);
+    return;
+  }
+  if (nthreads > 256) nthreads = 256;
+  pthread_t th[256];
+  or_job_t jobs[256];
+  int64_t chunk = (n + nthreads - 1) / nthreads;
+  int started = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    int64_t b = (int64_t)t * chunk, e = b + chunk;
+    if (b >= n) break;
+    if (e > n) e = n;
+    jobs[t].fn = fn;
+    jobs[t].ctx = ctx;
+    jobs[t].begin = b;
+    jobs[t].end = e;
+    if (pthread_create(&th[t], NULL, or_job_run, &jobs[t]) != 0) {
+      fn(ctx, b, e); /* run inline if a thread cannot start */
+      jobs[t].fn = NULL;
+    }
+    started = t + 1;
+  }
+  for (int t = 0; t < started; ++t)
+    if (jobs[t].fn) pthread_join(th[t], NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ChangeIntoIntersectionFormat: raytracer/cache/optimisedtriangle.go:30-78, */
+/* GetEdgeEquation: vmath/polygon/edge.go:5-25.  mgl32 Sub/Cross/Normalize/  */
+/* Dot op order (go-gl/mathgl mgl32, pinned 5ab0e04e, Gopkg.lock:65-67).     */
+/* ------------------------------------------------------------------------ */
+static void or_edge(const float p1[3], const float p2[3], int c0, int c1,
+                    const float in[3], float out[3]) {
+  float nx = p1[c1] - p2[c1];                   /* edge.go:6 */
+  float ny = p2[c0] - p1[c0];                   /* edge.go:7 */
+  float d = -(nx * p1[c0] + ny * p1[c1]);       /* edge.go:8 */
+  float tr = (in[c0] * nx + in[c1] * ny) + d;   /* edge.go:13 */
+  if (tr < 0) {                                 /* edge.go:14-19 */
+    nx = -nx;
+    ny = -ny;
+    d = -d;
+    tr = -tr;
+  }
+  out[0] = nx / tr;                             /* edge.go:20-22 */
+  out[1] = ny / tr;
+  out[2] = d / tr;
+}
+
+void or_precompute(const float *v9, const int32_t *ids, const uint8_t *flags,
+                   int64_t n, or_tri_t *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const float *v = v9 + 9 * i;
+    float p1[3] = {v[0], v[1], v[2]}, p2[3] = {v[3], v[4], v[5]},
+          p3[3] = {v[6], v[7], v[8]};
+    float e1[3], e2[3], N[3];
+    for (int c = 0; c < 3; ++c) {
+      e1[c] = p2[c] - p1[c];                    /* optimisedtriangle.go:43 */
+      e2[c] = p3[c] - p1[c];                    /* :44 */
+    }
+    N[0] = e1[1] * e2[2] - e1[2] * e2[1];       /* mgl32 Cross, :46 */
+    N[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    N[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    /* mgl32 Normalize: l = 1/float32(sqrt(float64(x*x+y*y+z*z))), :47 */
+    float len2 = (N[0] * N[0] + N[1] * N[1]) + N[2] * N[2];
+    float len = (float)sqrt((double)len2);
+    float l = 1.0f / len;
+    N[0] = N[0] * l;
+    N[1] = N[1] * l;
+    N[2] = N[2] * l;
+    int drop = 0;                               /* :50-55, strict > */
+    for (int c = 1; c < 3; ++c)
+      if (fabs((double)N[c]) > fabs((double)N[drop])) drop = c;
+    or_tri_t *t = out + i;
+    memset(t, 0, sizeof *t);
+    t->d = (N[0] * p1[0] + N[1] * p1[1]) + N[2] * p1[2]; /* mgl32 Dot, :57 */
+    t->nx = N[0];
+    t->ny = N[1];
+    t->nz = N[2];
+    int c0 = (drop + 1) % 3, c1 = (drop + 2) % 3;        /* :63-64 */
+    t->c0 = (uint8_t)c0;
+    t->c1 = (uint8_t)c1;
+    or_edge(p1, p2, c0, c1, p3, t->e + 0);               /* :69-72 */
+    or_edge(p2, p3, c0, c1, p1, t->e + 3);               /* :74-77 */
+    t->id = ids ? ids[i] : 0;
+    t->flags = flags ? flags[i] : 0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-triangle hit test (frozen; the reference's Trace4Rays is a stub,     */
+/* raytracer/environment.go:140-145).  Projected-barycentric test that the   */
+/* TriIntersectData layout encodes.  Accepts t in [tmin, tbest) and resolves */
+/* equal t to the smaller triangle index, so the result is the lexicographic */
+/* minimum of (t, index) and does not depend on visiting order.              */
+/* ------------------------------------------------------------------------ */
+static inline void or_hit(const or_tri_t *T, int32_t idx, const float o[3],
+                          const float d[3], float tmin, float *tbest,
+                          int32_t *best) {
+  float den = (T->nx * d[0] + T->ny * d[1]) + T->nz * d[2];
+  if (!(den != 0.0f)) return;
+  float num = T->d - ((T->nx * o[0] + T->ny * o[1]) + T->nz * o[2]);
+  float t = num / den;
+  if (!(t >= tmin && (t < *tbest || (t == *tbest && idx < *best)))) return;
+  float h0 = o[T->c0] + t * d[T->c0];
+  float h1 = o[T->c1] + t * d[T->c1];
+  float b0 = (T->e[0] * h0 + T->e[1] * h1) + T->e[2];
+  float b1 = (T->e[3] * h0 + T->e[4] * h1) + T->e[5];
+  if (b0 >= 0.0f && b1 >= 0.0f && (b0 + b1) <= 1.0f) {
+    *tbest = t;
+    *best = idx;
+  }
+}
+
+void or_trace_brute(const or_tri_t *tris, int64_t ntri, const or_rays_t *r,
+                    int64_t n, int32_t skip_id, int32_t *hit, float *hit_t) {
+  for (int64_t i = 0; i < n; ++i) {
+    float o[3] = {r->ox[i], r->oy[i], r->oz[i]};
+    float d[3] = {r->dx[i], r->dy[i], r->dz[i]};
+    float tmin = r->tmin[i], tbest = r->tmax[i];
+    int32_t best = -1;
+    for (int64_t k = 0; k < ntri; ++k) {
+      if (tris[k].id == skip_id) continue;
+      or_hit(tris + k, (int32_t)k, o, d, tmin, &tbest, &best);
+    }
+    hit[i] = best;
+    hit_t[i] = best >= 0 ? tbest : 0.0f;
+  }
+}
+
+/* FP64 version: same construction and test, all in double. */
+void or_trace_brute_f64(const float *v9, const int32_t *ids, int64_t ntri,
+                        const or_rays_t *r, int64_t n, int32_t skip_id,
+                        int32_t *hit, double *hit_t) {
+  typedef struct { double n[3], d, e[6]; int c0, c1; } T64;
+  T64 *T = (T64 *)malloc(sizeof(T64) * (size_t)(ntri > 0 ? ntri : 1));
+  for (int64_t i = 0; i < ntri; ++i) {
+    const float *v = v9 + 9 * i;
+    double p1[3] = {v[0], v[1], v[2]}, p2[3] = {v[3], v[4], v[5]},
+           p3[3] = {v[6], v[7], v[8]}, e1[3], e2[3], N[3];
+    for (int c = 0; c < 3; ++c) { e1[c] = p2[c] - p1[c]; e2[c] = p3[c] - p1[c]; }
+    N[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    N[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    N[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    double l = 1.0 / sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    for (int c = 0; c < 3; ++c) N[c] *= l;
+    int drop = 0;
+    for (int c = 1; c < 3; ++c) if (fabs(N[c]) > fabs(N[drop])) drop = c;
+    int c0 = (drop + 1) % 3, c1 = (drop + 2) % 3;
+    T[i].c0 = c0; T[i].c1 = c1;
+    memcpy(T[i].n, N, sizeof N);
+    T[i].d = N[0] * p1[0] + N[1] * p1[1] + N[2] * p1[2];
+    const double *P[3] = {p1, p2, p3};
+    for (int ed = 0; ed < 2; ++ed) {
+      const double *a = P[ed], *b = P[ed + 1], *in = P[(ed + 2) % 3];
+      double nx = a[c1] - b[c1], ny = b[c0] - a[c0], d = -(nx * a[c0] + ny * a[c1]);
+      double tr = in[c0] * nx + in[c1] * ny + d;
+      if (tr < 0) { nx = -nx; ny = -ny; d = -d; tr = -tr; }
+      T[i].e[3 * ed + 0] = nx / tr;
+      T[i].e[3 * ed + 1] = ny / tr;
+      T[i].e[3 * ed + 2] = d / tr;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    double o[3] = {r->ox[i], r->oy[i], r->oz[i]}, d[3] = {r->dx[i], r->dy[i], r->dz[i]};
+    double tmin = r->tmin[i], tbest = r->tmax[i];
+    int32_t best = -1;
+    for (int64_t k = 0; k < ntri; ++k) {
+      if (ids && ids[k] == skip_id) continue;
+      const T64 *t = T + k;
+      double den = t->n[0] * d[0] + t->n[1] * d[1] + t->n[2] * d[2];
+      if (!(den != 0.0)) continue;
+      double tt = (t->d - (t->n[0] * o[0] + t->n[1] * o[1] + t->n[2] * o[2])) / den;
+      if (!(tt >= tmin && (tt < tbest || (tt == tbest && k < best)))) continue;
+      double h0 = o[t->c0] + tt * d[t->c0], h1 = o[t->c1] + tt * d[t->c1];
+      double b0 = t->e[0] * h0 + t->e[1] * h1 + t->e[2];
+      double b1 = t->e[3] * h0 + t->e[4] * h1 + t->e[5];
+      if (b0 >= 0 && b1 >= 0 && b0 + b1 <= 1) { tbest = tt; best = (int32_t)k; }
+    }
+    hit[i] = best;
+    hit_t[i] = best >= 0 ? tbest : 0.0;
+  }
+  free(T);
+}
+
+/* ------------------------------------------------------------------------ */
+/* SAH kd-tree, following the intent of RefineNode / CalculateCostsOfSplit   */
+/* (raytracer/environment.go:181-387) with the documented defects repaired:  */
+/* real triangle bounds (trianglelist.go:9-21 never writes its outputs),     */
+/* real vertices (optimisedtriangle.go:22-28 returns one coordinate thrice), */
+/* midpoint 0.5*(min+max) (environment.go:274), split value and counts      */
+/* returned from the cost function (environment.go:181-236 takes them by    */
+/* value), and the partition writes before incrementing (:347-349).         */
+/* ------------------------------------------------------------------------ */
+enum { PC_POS = 1, PC_NEG = -1, PC_STRAD = 0 };
+
+typedef struct {
+  const float *v9;
+  float *tmin3, *tmax3; /* per-triangle per-axis vertex extents */
+  uint32_t *children;
+  float *split;
+  int64_t n_nodes, cap_nodes;
+  int32_t *idx;
+  int64_t n_idx, cap_idx;
+  int max_depth;
+  signed char *lab;     /* per-triangle scratch label */
+} or_kdb_t;
+
+static int kdb_push_nodes(or_kdb_t *b, int k) {
+  if (b->n_nodes + k > b->cap_nodes) {
+    int64_t nc = b->cap_nodes ? b->cap_nodes * 2 : 1024;
+    while (nc < b->n_nodes + k) nc *= 2;
+    uint32_t *c = (uint32_t *)realloc(b->children, sizeof(uint32_t) * nc);
+    if (!c) return -1;
+    b->children = c;
+    float *s = (float *)realloc(b->split, sizeof(float) * nc);
+    if (!s) return -1;
+    b->split = s;
+    b->cap_nodes = nc;
+  }
+  for (int i = 0; i < k; ++i) {
+    b->children[b->n_nodes + i] = 0;
+    b->split[b->n_nodes + i] = 0.0f;
+  }
+  b->n_nodes += k;
+  return 0;
+}
+
+/* ClassifyAgainstAxisSplit, optimisedtriangle.go:83-103 (real vertices) */
+static inline int kdb_classify(const or_kdb_t *b, int32_t t, int axis, float sv) {
+  float mn = b->tmin3[3 * t + axis], mx = b->tmax3[3 * t + axis];
+  if (mn >= sv) return PC_POS;
+  if (mx <= sv) return PC_NEG;
+  if (mn == mx) return PC_POS;
+  return PC_STRAD;
+}
+
+/* BoxSurfaceArea, vmath/polygon/surface.go:5-8 */
+static inline float box_sa(const float mn[3], const float mx[3]) {
+  float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+  return 2.0f * ((d0 * d2 + d0 * d1) + d1 * d2);
+}
+
+/* CalculateCostsOfSplit, environment.go:181-236 */
+static float kdb_cost(const or_kdb_t *b, int axis, const int32_t *list, int64_t n,
+                      const float mn[3], const float mx[3], float *split_value,
+                      int64_t *nl, int64_t *nr, int64_t *nb) {
+  int64_t l = 0, r = 0, both = 0;
+  double minc = 1.0e23, maxc = -1.0e23;
+  float sv = *split_value;
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t t = list[i];
+    double a = b->tmin3[3 * t + axis], c = b->tmax3[3 * t + axis];
+    if (a < minc) minc = a;
+    if (c > maxc) maxc = c;
+    switch (kdb_classify(b, t, axis, sv)) {
+      case PC_NEG: ++l; break;
+      case PC_POS: ++r; break;
+      default: ++both; break;
+    }
+  }
+  if (l != 0 && both == 0 && r == 0) sv = (float)maxc; /* :216-218 */
+  if (r != 0 && both == 0 && l == 0) sv = (float)minc; /* :219-221 */
+  float lmx[3] = {mx[0], mx[1], mx[2]}, rmn[3] = {mn[0], mn[1], mn[2]};
+  lmx[axis] = sv;
+  rmn[axis] = sv;
+  float sal = box_sa(mn, lmx), sar = box_sa(rmn, mx);
+  float isa = 1.0f / box_sa(mn, mx);
+  float cost = (float)OR_COST_OF_TRAVERSAL +
+               (float)OR_COST_OF_INTERSECTION *
+                   (((float)both + (sal * isa) * (float)l) + (sar * isa) * (float)r);
+  *split_value = sv;
+  *nl = l;
+  *nr = r;
+  *nb = both;
+  return cost;
+}
+
+static int kdb_leaf(or_kdb_t *b, int64_t node, const int32_t *list, int64_t n,
+                    int depth) {
+  b->children[node] = 3u + ((uint32_t)b->n_idx << 2);  /* :243 / :313 */
+  b->split[node] = (float)n;                           /* :244 / :314 */
+  if (b->n_idx + n > b->cap_idx) {
+    int64_t nc = b->cap_idx ? b->cap_idx * 2 : 4096;
+    while (nc < b->n_idx + n) nc *= 2;
+    int32_t *p = (int32_t *)realloc(b->idx, sizeof(int32_t) * nc);
+    if (!p) return -1;
+    b->idx = p;
+    b->cap_idx = nc;
+  }
+  memcpy(b->idx + b->n_idx, list, sizeof(int32_t) * (size_t)n);
+  b->n_idx += n;
+  if (depth > b->max_depth) b->max_depth = depth;
+  return 0;
+}
+
+/* RefineNode, environment.go:238-387.  `level` is the true tree depth;
+ * `depth` is the reference's counter that jumps by 100. */
+static int kdb_refine(or_kdb_t *b, int64_t node, const int32_t *list, int64_t n,
+                      const float mn[3], const float mx[3], int depth, int level) {
+  if (n < 3 || depth > OR_MAX_TREE_DEPTH)  /* :241, :311 (search result unused) */
+    return kdb_leaf(b, node, list, n, level);
+  float best_cost = 1.0e23f, best_moved = 0.0f, best_trial = 0.0f;
+  int64_t bnl = 0, bnr = 0, bnb = 0;
+  int best_axis = 0;
+  int64_t skip = 1 + n / 10;                                     /* :265 */
+  for (int axis = 0; axis < 3; ++axis) {
+    for (int64_t ts = -1; ts < n; ts += skip) {
+      for (int tv = 0; tv < 3; ++tv) {
+        float trial;
+        if (ts == -1) {
+          trial = 0.5f * (mn[axis] + mx[axis]);
+        } else {
+          trial = b->v9[9 * (int64_t)list[ts] + 3 * tv + axis];
+          if (trial > mx[axis] || trial < mn[axis]) continue;    /* :279-281 */
+        }
+        float sv = trial;
+        int64_t nl, nr, nb;
+        float cost = kdb_cost(b, axis, list, n, mn, mx, &sv, &nl, &nr, &nb);
+        if (cost < best_cost) {                                  /* :288-300 */
+          best_axis = axis;
+          best_cost = cost;
+          bnl = nl; bnr = nr; bnb = nb;
+          best_moved = sv;
+          best_trial = trial;
+        }
+        if (ts == -1) break;
+      }
+    }
+  }
+  float no_split = (float)((int64_t)OR_COST_OF_INTERSECTION * n); /* :310 */
+  if (no_split <= best_cost) return kdb_leaf(b, node, list, n, level);
+  /* labels are those of the winning trial value (saved at :296-299) */
+  int32_t *nlist = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  if (!nlist) return -1;
+  int64_t lo = 0, ro = 0, bo = 0;
+  for (int64_t i = 0; i < n; ++i) {                              /* :343-359 */
+    int32_t t = list[i];
+    switch (kdb_classify(b, t, best_axis, best_trial)) {
+      case PC_NEG: nlist[lo++] = t; break;
+      case PC_POS: ++ro; nlist[n - ro] = t; break;
+      default: nlist[bnl + bo] = t; ++bo; break;
+    }
+  }
+  int64_t left = b->n_nodes;                                     /* :361-376 */
+  if (kdb_push_nodes(b, 2)) { free(nlist); return -1; }
+  b->children[node] = (uint32_t)best_axis + ((uint32_t)left << 2);
+  b->split[node] = best_moved;
+  if (n < 20 && (bnl == 0 || bnr == 0)) depth += 100;            /* :378-380 */
+  float lmx[3] = {mx[0], mx[1], mx[2]}, rmn[3] = {mn[0], mn[1], mn[2]};
+  lmx[best_axis] = best_moved;
+  rmn[best_axis] = best_moved;
+  int rc = kdb_refine(b, left, nlist, bnl + bnb, mn, lmx, depth + 1, level + 1);
+  if (!rc)
+    rc = kdb_refine(b, left + 1, nlist + bnl, bnr + bnb, rmn, mx, depth + 1,
+                    level + 1);
+  free(nlist);
+  return rc;
+}
+
+int or_kd_build(const float *v9, int64_t ntri, or_kdtree_t *out) {
+  memset(out, 0, sizeof *out);
+  or_kdb_t b;
+  memset(&b, 0, sizeof b);
+  b.v9 = v9;
+  int64_t m = ntri > 0 ? ntri : 1;
+  b.tmin3 = (float *)malloc(sizeof(float) * 3 * m);
+  b.tmax3 = (float *)malloc(sizeof(float) * 3 * m);
+  int32_t *root = (int32_t *)malloc(sizeof(int32_t) * m);
+  if (!b.tmin3 || !b.tmax3 || !root) return -1;
+  float mn[3] = {1.0e23f, 1.0e23f, 1.0e23f}, mx[3] = {-1.0e23f, -1.0e23f, -1.0e23f};
+  for (int64_t t = 0; t < ntri; ++t) {
+    root[t] = (int32_t)t;
+    for (int c = 0; c < 3; ++c) {
+      float a = v9[9 * t + c], bb = v9[9 * t + 3 + c], cc = v9[9 * t + 6 + c];
+      float lo = f_min(a, f_min(bb, cc)), hi = f_max(a, f_max(bb, cc));
+      b.tmin3[3 * t + c] = lo;
+      b.tmax3[3 * t + c] = hi;
+      if (lo < mn[c]) mn[c] = lo;
+      if (hi > mx[c]) mx[c] = hi;
+    }
+  }
+  if (ntri == 0)
+    for (int c = 0; c < 3; ++c) mn[c] = mx[c] = 0.0f;
+  int rc = kdb_push_nodes(&b, 1);
+  if (!rc) rc = kdb_refine(&b, 0, root, ntri, mn, mx, 0, 0);
+  free(root);
+  free(b.tmin3);
+  free(b.tmax3);
+  if (rc) {
+    free(b.children); free(b.split); free(b.idx);
+    return rc;
+  }
+  out->n_nodes = b.n_nodes;
+  out->children = b.children;
+  out->split = b.split;
+  out->n_idx = b.n_idx;
+  out->idx = b.idx ? b.idx : (int32_t *)malloc(sizeof(int32_t));
+  memcpy(out->bmin, mn, sizeof mn);
+  memcpy(out->bmax, mx, sizeof mx);
+  out->max_depth = b.max_depth;
+  return 0;
+}
+
+void or_kd_free(or_kdtree_t *t) {
+  free(t->children);
+  free(t->split);
+  free(t->idx);
+  memset(t, 0, sizeof *t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* kd traversal with a conservative slab margin.  Every decision only culls: */
+/* a cell is visited over the ray interval where the ray lies within dl of   */
+/* it, dl = 2^-15 * (scene max |coord| + max |origin|), which exceeds the    */
+/* rounding of the hit test and of the interval arithmetic by > 100x.  Leaf  */
+/* tests use the global [tmin, tbest) window, so results equal brute force.  */
+/* ------------------------------------------------------------------------ */
+static float kd_scene_maxabs(const or_kdtree_t *kd) {
+  float m = 1.0f;
+  for (int c = 0; c < 3; ++c) {
+    m = f_max(m, fabsf(kd->bmin[c]));
+    m = f_max(m, fabsf(kd->bmax[c]));
+  }
+  return m;
+}
+
+#define OR_KD_STACK 64
+static void kd_trace_one(const or_kdtree_t *kd, float M, const or_tri_t *tris,
+                         const float o[3], const float d[3], float tmin, float tmax,
+                         int32_t skip, int anyhit, int32_t *best_out, float *t_out) {
+  float tbest = tmax;
+  int32_t best = -1;
+  float om = f_max(fabsf(o[0]), f_max(fabsf(o[1]), fabsf(o[2])));
+  float dl = (M + om) * 0x1p-15f;
+  float inv[3];
+  for (int c = 0; c < 3; ++c) {
+    float dd = fabsf(d[c]) < 1e-20f ? copysignf(1e-20f, d[c]) : d[c];
+    inv[c] = 1.0f / dd;
+  }
+  float ta = tmin, tb = tmax;
+  for (int c = 0; c < 3; ++c) {
+    float t1 = ((kd->bmin[c] - dl) - o[c]) * inv[c];
+    float t2 = ((kd->bmax[c] + dl) - o[c]) * inv[c];
+    float lo = t1 < t2 ? t1 : t2, hi = t1 < t2 ? t2 : t1;
+    ta = ta > lo ? ta : lo;
+    tb = tb < hi ? tb : hi;
+  }
+  if (!(ta <= tb) || kd->n_nodes == 0) {
+    *best_out = -1;
+    *t_out = 0.0f;
+    return;
+  }
+  uint32_t st_node[OR_KD_STACK];
+  float st_a[OR_KD_STACK], st_b[OR_KD_STACK];
+  int sp = 0;
+  uint32_t node = 0;
+  for (;;) {
+    uint32_t ch = kd->children[node];
+    uint32_t type = ch & 3u;
+    if (type != 3u) {
+      float sv = kd->split[node];
+      float lo = ((sv - dl) - o[type]) * inv[type];
+      float hi = ((sv + dl) - o[type]) * inv[type];
+      uint32_t left = ch >> 2, near_n, far_n;
+      float near_end, far_start;
+      if (inv[type] >= 0.0f) { near_n = left; far_n = left + 1; near_end = hi; far_start = lo; }
+      else { near_n = left + 1; far_n = left; near_end = lo; far_start = hi; }
+      float tbe = tb < tbest ? tb : tbest;
+      float ne = near_end < tbe ? near_end : tbe;
+      float fs = far_start > ta ? far_start : ta;
+      int near_ok = ta <= ne, far_ok = fs <= tbe;
+      if (near_ok && far_ok && sp < OR_KD_STACK) {
+        st_node[sp] = far_n; st_a[sp] = fs; st_b[sp] = tb; ++sp;
+        node = near_n; tb = near_end < tb ? near_end : tb;
+        continue;
+      }
+      if (near_ok && far_ok) { /* stack full (cannot happen for depth<=64) */
+        node = near_n; tb = near_end < tb ? near_end : tb;
+        continue;
+      }
+      if (near_ok) { node = near_n; tb = near_end < tb ? near_end : tb; continue; }
+      if (far_ok) { node = far_n; ta = fs; continue; }
+    } else {
+      uint32_t start = ch >> 2, cnt = (uint32_t)kd->split[node];
+      for (uint32_t k = start; k < start + cnt; ++k) {
+        int32_t t = kd->idx[k];
+        if (tris[t].id == skip) continue;
+        or_hit(tris + t, t, o, d, tmin, &tbest, &best);
+      }
+      if (anyhit && best >= 0) break;
+    }
+    /* pop */
+    int found = 0;
+    while (sp > 0) {
+      --sp;
+      if (st_a[sp] > tbest) continue;
+      node = st_node[sp]; ta = st_a[sp]; tb = st_b[sp];
+      found = 1;
+      break;
+    }
+    if (!found) break;
+  }
+  *best_out = best;
+  *t_out = best >= 0 ? tbest : 0.0f;
+}
+
+typedef struct {
+  const or_kdtree_t *kd;
+  const or_tri_t *tris;
+  const or_rays_t *r;
+  int32_t skip;
+  int32_t *hit;
+  float *hit_t;
+  float M;
+} kd_ctx_t;
+
+static void kd_trace_range(void *vc, int64_t b, int64_t e) {
+  kd_ctx_t *c = (kd_ctx_t *)vc;
+  for (int64_t i = b; i < e; ++i) {
+    float o[3] = {c->r->ox[i], c->r->oy[i], c->r->oz[i]};
+    float d[3] = {c->r->dx[i], c->r->dy[i], c->r->dz[i]};
+    kd_trace_one(c->kd, c->M, c->tris, o, d, c->r->tmin[i], c->r->tmax[i], c->skip,
+                 0, c->hit + i, c->hit_t + i);
+  }
+}
+
+void or_trace_kd(const or_kdtree_t *kd, const or_tri_t *tris, const or_rays_t *r,
+                 int64_t n, int32_t skip_id, int32_t *hit, float *hit_t,
+                 int nthreads) {
+  kd_ctx_t c = {kd, tris, r, skip_id, hit, hit_t, kd_scene_maxabs(kd)};
+  or_parallel(n, nthreads, kd_trace_range, &c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* TestLineDoesHitSky intent, raytracer/trace/testline.go:22-51,91-93:       */
+/*   dir = (stop-start) * (1/|stop-start|), trace [0, len), occluded iff the */
+/*   closest hit is not a TRACE_ID_SKY triangle.  The 3D-skybox recursion    */
+/*   (:60-89) and texture coverage (:53-55) are off by default and absent.  */
+/* ------------------------------------------------------------------------ */
+static void testline_one(const or_kdtree_t *kd, float M, const or_tri_t *tris,
+                         int64_t ntri, const float s[3], const float e[3],
+                         int32_t skip, uint8_t *vis, uint8_t *sky) {
+  float dv[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+  float len2 = (dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2];
+  float len = sqrtf(len2);
+  float inv = 1.0f / len;
+  float d[3] = {dv[0] * inv, dv[1] * inv, dv[2] * inv};
+  int32_t best = -1;
+  float tb = 0.0f;
+  if (kd) {
+    kd_trace_one(kd, M, tris, s, d, 0.0f, len, skip, 0, &best, &tb);
+  } else {
+    float tbest = len;
+    for (int64_t k = 0; k < ntri; ++k) {
+      if (tris[k].id == skip) continue;
+      or_hit(tris + k, (int32_t)k, s, d, 0.0f, &tbest, &best);
+    }
+    tb = tbest;
+  }
+  int hit = best >= 0 && tb < len;
+  int is_sky = hit && (tris[best].id & OR_TRACE_ID_SKY) != 0;
+  *vis = (uint8_t)!(hit && !is_sky);
+  if (sky) *sky = (uint8_t)is_sky;
+}
+
+typedef struct {
+  const or_kdtree_t *kd;
+  const or_tri_t *tris;
+  int64_t ntri;
+  const or_segs_t *s;
+  int32_t skip;
+  uint8_t *vis, *sky;
+  float M;
+} tl_ctx_t;
+
+static void tl_range(void *vc, int64_t b, int64_t e) {
+  tl_ctx_t *c = (tl_ctx_t *)vc;
+  for (int64_t i = b; i < e; ++i) {
+    float s[3] = {c->s->sx[i], c->s->sy[i], c->s->sz[i]};
+    float en[3] = {c->s->ex[i], c->s->ey[i], c->s->ez[i]};
+    testline_one(c->kd, c->M, c->tris, c->ntri, s, en, c->skip, c->vis + i,
+                 c->sky ? c->sky + i : NULL);
+  }
+}
+
+void or_testline(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                 const or_segs_t *s, int64_t n, int32_t skip_id, uint8_t *visible,
+                 uint8_t *hit_sky, int nthreads) {
+  tl_ctx_t c = {kd, tris, ntri, s, skip_id, visible, hit_sky,
+                kd ? kd_scene_maxabs(kd) : 1.0f};
+  or_parallel(n, nthreads, tl_range, &c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* MakeTransfers (absent in the reference; output record Transfer,          */
+/* common/types/transfer.go:3-6, Patch.NumTransfers/Transfers,              */
+/* common/types/patch.go:60-61).  Frozen semantics:                         */
+/*   unordered pair i<j, facing cull n_i.(o_j-o_i) > 0 && -(n_j.(o_j-o_i))>0 */
+/*   visibility TestLine(o_i + eps n_i -> o_j + eps n_j) from the lower index */
+/*   F_ij = ((di*dj)*A_j) / ((pi_f*r2)*r2)                                   */
+/*   rows ascending in column; FP64 row sum in kernel wave order; rows with   */
+/*   sum > 1 scaled by 1/sum in FP64 when normalize != 0.                    */
+/* ------------------------------------------------------------------------ */
+#define OR_PI_F 3.14159265358979323846f
+
+double or_row_sum_f64(const float *coef, int64_t len) {
+  /* lane l accumulates positions p with (p/4) % 64 == l, then butterfly */
+  double acc[64];
+  for (int l = 0; l < 64; ++l) acc[l] = 0.0;
+  for (int64_t p = 0; p < len; ++p) {
+    int l = (int)((p / 4) % 64);
+    acc[l] = acc[l] + (double)coef[p];
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    double nw[64];
+    for (int l = 0; l < 64; ++l) nw[l] = acc[l] + acc[l ^ off];
+    memcpy(acc, nw, sizeof acc);
+  }
+  return acc[0];
+}
+
+typedef struct {
+  const or_kdtree_t *kd;
+  const or_tri_t *tris;
+  int64_t ntri;
+  const or_patches_t *p;
+  int64_t np;
+  float eps;
+  int32_t skip;
+  uint32_t *bits; /* np x W words, symmetric */
+  int64_t W;
+  float M;
+} mt_ctx_t;
+
+static void mt_range(void *vc, int64_t b, int64_t e) {
+  mt_ctx_t *c = (mt_ctx_t *)vc;
+  const or_patches_t *p = c->p;
+  for (int64_t i = b; i < e; ++i) {
+    for (int64_t j = i + 1; j < c->np; ++j) {
+      float dx = p->ox[j] - p->ox[i], dy = p->oy[j] - p->oy[i], dz = p->oz[j] - p->oz[i];
+      float di = (p->nx[i] * dx + p->ny[i] * dy) + p->nz[i] * dz;
+      float dj = -((p->nx[j] * dx + p->ny[j] * dy) + p->nz[j] * dz);
+      if (!(di > 0.0f && dj > 0.0f)) continue;
+      float s[3] = {p->ox[i] + c->eps * p->nx[i], p->oy[i] + c->eps * p->ny[i],
+                    p->oz[i] + c->eps * p->nz[i]};
+      float en[3] = {p->ox[j] + c->eps * p->nx[j], p->oy[j] + c->eps * p->ny[j],
+                     p->oz[j] + c->eps * p->nz[j]};
+      uint8_t vis;
+      testline_one(c->kd, c->M, c->tris, c->ntri, s, en, c->skip, &vis, NULL);
+      if (vis) {
+        /* row i written only by this thread; (j,i) bit set under a lock-free
+         * atomic since rows j belong to other threads */
+        __atomic_fetch_or(&c->bits[i * c->W + (j >> 5)], 1u << (j & 31), __ATOMIC_RELAXED);
+        __atomic_fetch_or(&c->bits[j * c->W + (i >> 5)], 1u << (i & 31), __ATOMIC_RELAXED);
+      }
+    }
+  }
+}
+
+static inline float or_ff(const or_patches_t *p, int64_t i, int64_t j) {
+  float dx = p->ox[j] - p->ox[i], dy = p->oy[j] - p->oy[i], dz = p->oz[j] - p->oz[i];
+  float di = (p->nx[i] * dx + p->ny[i] * dy) + p->nz[i] * dz;
+  float dj = -((p->nx[j] * dx + p->ny[j] * dy) + p->nz[j] * dz);
+  float r2 = (dx * dx + dy * dy) + dz * dz;
+  return ((di * dj) * p->area[j]) / ((OR_PI_F * r2) * r2);
+}
+
+int or_make_transfers(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                      const or_patches_t *p, int64_t np, float normal_offset,
+                      int32_t skip_id, int normalize, int nthreads, or_csr_t *out) {
+  memset(out, 0, sizeof *out);
+  int64_t W = (np + 31) / 32;
+  uint32_t *bits = (uint32_t *)calloc((size_t)(np * W > 0 ? np * W : 1), sizeof(uint32_t));
+  if (!bits) return -1;
+  mt_ctx_t c = {kd, tris, ntri, p, np, normal_offset, skip_id, bits, W,
+                kd ? kd_scene_maxabs(kd) : 1.0f};
+  /* rows have unequal work (j > i); interleave by splitting into many ranges */
+  int64_t chunks = np;
+  (void)chunks;
+  or_parallel(np, nthreads, mt_range, &c);
+  int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(np + 1));
+  if (!rp) { free(bits); return -1; }
+  rp[0] = 0;
+  for (int64_t i = 0; i < np; ++i) {
+    int64_t cnt = 0;
+    for (int64_t w = 0; w < W; ++w) cnt += __builtin_popcount(bits[i * W + w]);
+    rp[i + 1] = rp[i] + cnt;
+  }
+  int64_t nnz = rp[np];
+  int32_t *col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+  float *coef = (float *)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
+  double *rs = (double *)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+  if (!col || !coef || !rs) { free(bits); free(rp); free(col); free(coef); free(rs); return -1; }
+  for (int64_t i = 0; i < np; ++i) {
+    int64_t k = rp[i];
+    for (int64_t w = 0; w < W; ++w) {
+      uint32_t m = bits[i * W + w];
+      while (m) {
+        int bit = __builtin_ctz(m);
+        m &= m - 1;
+        int64_t j = w * 32 + bit;
+        col[k] = (int32_t)j;
+        coef[k] = or_ff(p, i, j);
+        ++k;
+      }
+    }
+    int64_t len = rp[i + 1] - rp[i];
+    double s = or_row_sum_f64(coef + rp[i], len);
+    rs[i] = s;
+    if (normalize && s > 1.0)
+      for (int64_t q = rp[i]; q < rp[i + 1]; ++q) coef[q] = (float)((double)coef[q] / s);
+  }
+  free(bits);
+  out->n_rows = np;
+  out->nnz = nnz;
+  out->row_ptr = rp;
+  out->col = col;
+  out->coef = coef;
+  out->row_sum = rs;
+  return 0;
+}
+
+void or_csr_free(or_csr_t *c) {
+  free(c->row_ptr);
+  free(c->col);
+  free(c->coef);
+  free(c->row_sum);
+  memset(c, 0, sizeof *c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* GatherSampleLight (absent; light records common/types/light.go:10-44,     */
+/* falloff parameters rad/lightmap/lights.go:259-341).  Frozen semantics in  */
+/* gather_one below; lights are summed in list order.                       */
+/* ------------------------------------------------------------------------ */
+void or_sky_dirs(float *dirs) {
+  /* 162 directions on a Fibonacci sphere (the reference's Anorms table,
+   * vmath/constants.go:15-44, carries only 28 of its 162 entries) */
+  const double ga = 3.14159265358979323846 * (3.0 - sqrt(5.0));
+  for (int k = 0; k < 162; ++k) {
+    double z = 1.0 - (2.0 * k + 1.0) / 162.0;
+    double r = sqrt(1.0 - z * z);
+    double ph = ga * k;
+    dirs[3 * k + 0] = (float)(r * cos(ph));
+    dirs[3 * k + 1] = (float)(r * sin(ph));
+    dirs[3 * k + 2] = (float)z;
+  }
+}
+
+typedef struct {
+  const or_kdtree_t *kd;
+  const or_tri_t *tris;
+  int64_t ntri;
+  const or_samples_t *s;
+  const or_light_t *L;
+  int32_t nl;
+  float eps;
+  int32_t skip;
+  float *rgb;
+  float M;
+  float dirs[162 * 3];
+} gl_ctx_t;
+
+static int seg_visible(const gl_ctx_t *c, const float s[3], const float e[3]) {
+  uint8_t v;
+  testline_one(c->kd, c->M, c->tris, c->ntri, s, e, c->skip, &v, NULL);
+  return v;
+}
+
+static void gather_one(const gl_ctx_t *c, int64_t i, float out[3]) {
+  const float p[3] = {c->s->px[i], c->s->py[i], c->s->pz[i]};
+  const float n[3] = {c->s->nx[i], c->s->ny[i], c->s->nz[i]};
+  const float st[3] = {p[0] + c->eps * n[0], p[1] + c->eps * n[1], p[2] + c->eps * n[2]};
+  const float MTL = (float)OR_MAX_TRACE_LENGTH;
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  for (int32_t li = 0; li < c->nl; ++li) {
+    const or_light_t *L = c->L + li;
+    float scale = 0.0f;
+    if (L->type == 0) {                          /* EMIT_SURFACE */
+      float lx = L->origin[0] - p[0], ly = L->origin[1] - p[1], lz = L->origin[2] - p[2];
+      float dist2 = (lx * lx + ly * ly) + lz * lz;
+      if (!(dist2 > 0.0f)) continue;
+      float dist = sqrtf(dist2);
+      float inv = 1.0f / dist;
+      float ux = lx * inv, uy = ly * inv, uz = lz * inv;
+      float dot = (n[0] * ux + n[1] * uy) + n[2] * uz;
+      if (!(dot > 0.0f)) continue;
+      float dot2 = -((L->normal[0] * ux + L->normal[1] * uy) + L->normal[2] * uz);
+      if (!(dot2 > 0.0f)) continue;
+      if (!seg_visible(c, st, L->origin)) continue;
+      scale = (dot * dot2) / dist2;
+    } else if (L->type == 1 || L->type == 2) {   /* EMIT_POINT / EMIT_SPOTLIGHT */
+      float lx = L->origin[0] - p[0], ly = L->origin[1] - p[1], lz = L->origin[2] - p[2];
+      float dist2 = (lx * lx + ly * ly) + lz * lz;
+      if (!(dist2 > 0.0f)) continue;
+      float dist = sqrtf(dist2);
+      float inv = 1.0f / dist;
+      float ux = lx * inv, uy = ly * inv, uz = lz * inv;
+      float dot = (n[0] * ux + n[1] * uy) + n[2] * uz;
+      if (!(dot > 0.0f)) continue;
+      float fd = dist < L->cap_distance ? dist : L->cap_distance;
+      float den = (L->constant_attn + L->linear_attn * fd) + L->quadratic_attn * (fd * fd);
+      if (!(den > 0.0f)) continue;
+      float fall = 1.0f / den;
+      if (L->type == 2) {
+        float dot2 = -((L->normal[0] * ux + L->normal[1] * uy) + L->normal[2] * uz);
+        if (!(dot2 > L->stopdot2)) continue;
+        if (dot2 <= L->stopdot)
+          fall = fall * ((dot2 - L->stopdot2) / (L->stopdot - L->stopdot2));
+        if (L->exponent != 0.0f && L->exponent != 1.0f)
+          fall = fall * powf(dot2, L->exponent);
+      }
+      if (L->end_fade > L->start_fade) {
+        if (dist >= L->end_fade) continue;
+        if (dist > L->start_fade) {
+          float x = (dist - L->start_fade) / (L->end_fade - L->start_fade);
+          float sm = (x * x) * (3.0f - 2.0f * x);
+          fall = fall * (1.0f - sm);
+        }
+      }
+      if (!seg_visible(c, st, L->origin)) continue;
+      scale = dot * fall;
+    } else if (L->type == 3) {                   /* EMIT_SKYLIGHT (sun) */
+      float dot = -((n[0] * L->normal[0] + n[1] * L->normal[1]) + n[2] * L->normal[2]);
+      if (!(dot > 0.0f)) continue;
+      float en[3] = {st[0] - MTL * L->normal[0], st[1] - MTL * L->normal[1],
+                     st[2] - MTL * L->normal[2]};
+      if (!seg_visible(c, st, en)) continue;
+      scale = dot;
+    } else if (L->type == 5) {                   /* EMIT_SKYAMBIENT */
+      float a = 0.0f, w = 0.0f;
+      for (int k = 0; k < 162; ++k) {
+        const float *dk = c->dirs + 3 * k;
+        float ck = (n[0] * dk[0] + n[1] * dk[1]) + n[2] * dk[2];
+        if (!(ck > 0.0f)) continue;
+        w = w + ck;
+        float en[3] = {st[0] + MTL * dk[0], st[1] + MTL * dk[1], st[2] + MTL * dk[2]};
+        if (seg_visible(c, st, en)) a = a + ck;
+      }
+      if (!(w > 0.0f)) continue;
+      scale = a / w;
+    } else {
+      continue;
+    }
+    acc[0] = acc[0] + L->intensity[0] * scale;
+    acc[1] = acc[1] + L->intensity[1] * scale;
+    acc[2] = acc[2] + L->intensity[2] * scale;
+  }
+  out[0] = acc[0];
+  out[1] = acc[1];
+  out[2] = acc[2];
+}
+
+static void gl_range(void *vc, int64_t b, int64_t e) {
+  gl_ctx_t *c = (gl_ctx_t *)vc;
+  for (int64_t i = b; i < e; ++i) gather_one(c, i, c->rgb + 3 * i);
+}
+
+void or_gather_light(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                     const or_samples_t *s, int64_t n, const or_light_t *lights,
+                     int32_t n_lights, float normal_offset, int32_t skip_id,
+                     float *rgb, int nthreads) {
+  gl_ctx_t *c = (gl_ctx_t *)malloc(sizeof(gl_ctx_t));
+  c->kd = kd; c->tris = tris; c->ntri = ntri; c->s = s; c->L = lights; c->nl = n_lights;
+  c->eps = normal_offset; c->skip = skip_id; c->rgb = rgb;
+  c->M = kd ? kd_scene_maxabs(kd) : 1.0f;
+  or_sky_dirs(c->dirs);
+  or_parallel(n, nthreads, gl_range, c);
+  free(c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP16 / BF16 conversion, round to nearest even (IEEE 754 binary16, bfloat16) */
+/* ------------------------------------------------------------------------ */
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+uint16_t or_f32_to_bf16(float f) {
+  uint32_t u = f2u(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+float or_bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
+
+uint16_t or_f32_to_f16(float f) {
+  uint32_t u = f2u(f);
+  uint32_t sign = (u >> 16) & 0x8000u;
+  uint32_t a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);       /* NaN */
+  if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);      /* overflow -> inf */
+  if (a < 0x38800000u) {                                        /* subnormal half */
+    if (a < 0x33000000u) return (uint16_t)sign;                 /* < 2^-25: +-0 */
+    uint32_t e = a >> 23, m = (a & 0x7fffffu) | 0x800000u;
+    /* half subnormal unit is 2^-24: value = m * 2^(e-150) = m*2^(e-126) units */
+    int sh = 126 - (int)e;              /* 14..24 */
+    uint32_t v = m >> sh;
+    uint32_t rem = m & ((1u << sh) - 1u), halfu = 1u << (sh - 1);
+    if (rem > halfu || (rem == halfu && (v & 1u))) ++v;
+    return (uint16_t)(sign | v);
+  }
+  uint32_t e = (a >> 23) - 112u, m = a & 0x7fffffu;
+  uint32_t v = (e << 10) | (m >> 13);
+  uint32_t rem = m & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (v & 1u))) ++v;
+  return (uint16_t)(sign | v);
+}
+
+float or_f16_to_f32(uint16_t h) {
+  uint32_t sign = ((uint32_t)h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  if (e == 0) {
+    if (m == 0) return u2f(sign);
+    float v = (float)m * 0x1p-24f;
+    return sign ? -v : v;
+  }
+  if (e == 31) return u2f(sign | 0x7f800000u | (m << 13));
+  return u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+/* ------------------------------------------------------------------------ */
+/* BounceLight (absent; state Patch.TotalLight/Reflectivity/Area,            */
+/* common/types/patch.go:33-42; -bounce cmd/args.go:66).  Frozen recurrence: */
+/*   add_i  = sum_t coef_t * emit[col_t]        (FP32, kernel order)         */
+/*   total_i += add_i ; emit_i = add_i * refl_i                              */
+/*   E_k = sum_i area_i * ((add.r + add.g) + add.b)   (FP64, fixed tree)    */
+/*   stop after bounce k when E_k <= stop_frac * sum_{m<=k} E_m, or k = max  */
+/* Kernel order: lane l of a 64-wide wave owns positions p with             */
+/* (p / G) % 64 == l (G = 4 for FP32, 8 for FP16/BF16), fmaf-accumulates    */
+/* them in increasing p, then xor-butterfly 32..1; FP16/BF16 rows multiply   */
+/* the reduced sum by the row scale.                                         */
+/* ------------------------------------------------------------------------ */
+double or_energy_reduce(const double *e, int64_t n) {
+  if (n <= 0) return 0.0;
+  int64_t m = n;
+  double *cur = (double *)malloc(sizeof(double) * (size_t)m);
+  memcpy(cur, e, sizeof(double) * (size_t)m);
+  while (m > 1) {
+    int64_t nb = (m + 2047) / 2048;
+    double *nx = (double *)malloc(sizeof(double) * (size_t)nb);
+    for (int64_t b = 0; b < nb; ++b) {
+      double v[256];
+      for (int t = 0; t < 256; ++t) {
+        double a = 0.0;
+        for (int j = 0; j < 8; ++j) {
+          int64_t k = b * 2048 + t + 256 * j;
+          if (k < m) a = a + cur[k];
+        }
+        v[t] = a;
+      }
+      for (int s = 128; s >= 1; s >>= 1)
+        for (int t = 0; t < s; ++t) v[t] = v[t] + v[t + s];
+      nx[b] = v[0];
+    }
+    free(cur);
+    cur = nx;
+    m = nb;
+  }
+  double r = cur[0];
+  free(cur);
+  return r;
+}
+
+typedef struct {
+  int64_t n;
+  const int64_t *rp;
+  const int32_t *col;
+  const void *coef;
+  int kind;
+  const float *scale;
+  const float *emit; /* 3n, current */
+  float *add;        /* 3n */
+} bg_ctx_t;
+
+static inline float coef_at(const bg_ctx_t *c, int64_t k) {
+  if (c->kind == 0) return ((const float *)c->coef)[k];
+  uint16_t h = ((const uint16_t *)c->coef)[k];
+  return c->kind == 1 ? or_f16_to_f32(h) : or_bf16_to_f32(h);
+}
+
+static void bg_range(void *vc, int64_t b, int64_t e) {
+  bg_ctx_t *c = (bg_ctx_t *)vc;
+  int G = c->kind == 0 ? 4 : 8;
+  for (int64_t i = b; i < e; ++i) {
+    float acc[64][3];
+    memset(acc, 0, sizeof acc);
+    int64_t s0 = c->rp[i], len = c->rp[i + 1] - s0;
+    for (int64_t p = 0; p < len; ++p) {
+      int l = (int)((p / G) % 64);
+      float w = coef_at(c, s0 + p);
+      const float *em = c->emit + 3 * (int64_t)c->col[s0 + p];
+      acc[l][0] = fmaf(w, em[0], acc[l][0]);
+      acc[l][1] = fmaf(w, em[1], acc[l][1]);
+      acc[l][2] = fmaf(w, em[2], acc[l][2]);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      float nw[64][3];
+      for (int l = 0; l < 64; ++l)
+        for (int q = 0; q < 3; ++q) nw[l][q] = acc[l][q] + acc[l ^ off][q];
+      memcpy(acc, nw, sizeof acc);
+    }
+    float sc = c->kind == 0 ? 1.0f : c->scale[i];
+    for (int q = 0; q < 3; ++q) c->add[3 * i + q] = c->kind == 0 ? acc[0][q] : acc[0][q] * sc;
+  }
+}
+
+int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
+                  const void *coef, int coef_kind, const float *row_scale,
+                  const float *emit0, const float *refl, const float *area,
+                  float *total, int max_bounces, double stop_frac, double *energy,
+                  int nthreads) {
+  float *emit = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+  float *add = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+  double *e = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  memcpy(emit, emit0, sizeof(float) * 3 * (size_t)n);
+  bg_ctx_t c = {n, row_ptr, col, coef, coef_kind, row_scale, emit, add};
+  double cum = 0.0;
+  int k = 0;
+  while (k < max_bounces) {
+    or_parallel(n, nthreads, bg_range, &c);
+    for (int64_t i = 0; i < n; ++i) {
+      float a0 = add[3 * i], a1 = add[3 * i + 1], a2 = add[3 * i + 2];
+      total[3 * i + 0] = total[3 * i + 0] + a0;
+      total[3 * i + 1] = total[3 * i + 1] + a1;
+      total[3 * i + 2] = total[3 * i + 2] + a2;
+      emit[3 * i + 0] = a0 * refl[3 * i + 0];
+      emit[3 * i + 1] = a1 * refl[3 * i + 1];
+      emit[3 * i + 2] = a2 * refl[3 * i + 2];
+      e[i] = (double)area[i] * (((double)a0 + (double)a1) + (double)a2);
+    }
+    double Ek = or_energy_reduce(e, n);
+    energy[k] = Ek;
+    cum = cum + Ek;
+    ++k;
+    if (Ek <= stop_frac * cum) break;
+  }
+  free(emit);
+  free(add);
+  free(e);
+  return k;
+}
+
+int or_bounce_f64(int64_t n, const int64_t *row_ptr, const int32_t *col,
+                  const float *coef, const float *emit0, const float *refl,
+                  const float *area, double *total, int max_bounces,
+                  double stop_frac, double *energy) {
+  double *emit = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+  double *add = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < 3 * n; ++i) emit[i] = emit0[i];
+  double cum = 0.0;
+  int k = 0;
+  while (k < max_bounces) {
+    for (int64_t i = 0; i < n; ++i) {
+      double a[3] = {0, 0, 0};
+      for (int64_t t = row_ptr[i]; t < row_ptr[i + 1]; ++t) {
+        const double *em = emit + 3 * (int64_t)col[t];
+        for (int q = 0; q < 3; ++q) a[q] += (double)coef[t] * em[q];
+      }
+      for (int q = 0; q < 3; ++q) add[3 * i + q] = a[q];
+    }
+    double Ek = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      for (int q = 0; q < 3; ++q) {
+        total[3 * i + q] += add[3 * i + q];
+        emit[3 * i + q] = add[3 * i + q] * (double)refl[3 * i + q];
+      }
+      Ek += (double)area[i] * (add[3 * i] + add[3 * i + 1] + add[3 * i + 2]);
+    }
+    energy[k] = Ek;
+    cum += Ek;
+    ++k;
+    if (Ek <= stop_frac * cum) break;
+  }
+  free(emit);
+  free(add);
+  return k;
+}

@@ -1,0 +1,152 @@
+/*
+ * oracle.h -- CPU restatement of the VRAD radiosity-bake hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code; it is used by tests/, by __graft_entry__.smoke() and by the
+ * cpu_baseline leg of bench.py, always as the checker.
+ *
+ * Every function below restates a reference function (Galaco/VRAD, a Go port
+ * of Source SDK VRAD) or, where the reference has only a stub or nothing, the
+ * semantics frozen for this path.  Citations are file:line in the reference.
+ *
+ * Arithmetic: IEEE FP32, no contraction (built with -ffp-contract=off, SSE
+ * scalar), correctly rounded division and sqrt; FP64 where stated.  The
+ * "kernel order" functions reproduce the exact summation order of the HIP
+ * kernels so that FP32 results are bit-identical.
+ */
+#ifndef VRAD_ORACLE_H
+#define VRAD_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* raytracer/constants.go:9-11 */
+#define OR_TRACE_ID_SKY 0x01000000
+#define OR_TRACE_ID_OPAQUE 0x02000000
+#define OR_TRACE_ID_STATICPROP 0x04000000
+/* raytracer/kdtree/constants.go:25-28 */
+#define OR_COST_OF_TRAVERSAL 75
+#define OR_COST_OF_INTERSECTION 167
+#define OR_MAX_TREE_DEPTH 21
+/* common/constants/constants.go:15-19 */
+#define OR_MAX_TRACE_LENGTH (1.732050807569 * 32768.0)
+
+/* raytracer/cache/triangle/triintersectdata.go:3-22 -- 48 bytes */
+typedef struct {
+  float nx, ny, nz, d;   /* FlNx, FlNy, FlNz, FlD */
+  int32_t id;            /* NTriangleID */
+  float e[6];            /* ProjectedEdgeEquations */
+  uint8_t c0, c1;        /* NCoordSelect0, NCoordSelect1 */
+  uint8_t flags, unused; /* NFlags, Unused0 */
+} or_tri_t;
+
+typedef struct {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+} or_rays_t;
+
+typedef struct {
+  const float *sx, *sy, *sz, *ex, *ey, *ez;
+} or_segs_t;
+
+typedef struct {
+  int64_t n_nodes;
+  uint32_t *children; /* OptimisedKDNode.Children: (index << 2) | type */
+  float *split;       /* SplittingPlaneValue (leaf: float(count)) */
+  int64_t n_idx;
+  int32_t *idx;       /* TriangleIndexList */
+  float bmin[3], bmax[3];
+  int max_depth;      /* deepest leaf (root = 0) */
+} or_kdtree_t;
+
+/* ---- triangle precompute: ChangeIntoIntersectionFormat ---- */
+void or_precompute(const float *v9, const int32_t *ids, const uint8_t *flags,
+                   int64_t n, or_tri_t *out);
+
+/* ---- closest hit, brute force over every triangle (the reference's
+ *      effective acceleration structure is one leaf holding all triangles) */
+void or_trace_brute(const or_tri_t *tris, int64_t ntri, const or_rays_t *r,
+                    int64_t n, int32_t skip_id, int32_t *hit, float *hit_t);
+/* FP64 brute force on the same vertices (stated-error reference) */
+void or_trace_brute_f64(const float *v9, const int32_t *ids, int64_t ntri,
+                        const or_rays_t *r, int64_t n, int32_t skip_id,
+                        int32_t *hit, double *hit_t);
+
+/* ---- SAH kd-tree (RefineNode intent) and traversal ---- */
+int or_kd_build(const float *v9, int64_t ntri, or_kdtree_t *out);
+void or_kd_free(or_kdtree_t *t);
+void or_trace_kd(const or_kdtree_t *kd, const or_tri_t *tris, const or_rays_t *r,
+                 int64_t n, int32_t skip_id, int32_t *hit, float *hit_t,
+                 int nthreads);
+
+/* ---- TestLineDoesHitSky intent ---- */
+void or_testline(const or_kdtree_t *kd /* NULL = brute force */,
+                 const or_tri_t *tris, int64_t ntri, const or_segs_t *s,
+                 int64_t n, int32_t skip_id, uint8_t *visible,
+                 uint8_t *hit_sky, int nthreads);
+
+/* ---- MakeTransfers (frozen semantics) ---- */
+typedef struct {
+  const float *ox, *oy, *oz, *nx, *ny, *nz, *area;
+} or_patches_t;
+typedef struct {
+  int64_t n_rows, nnz;
+  int64_t *row_ptr; /* n_rows + 1, unpadded */
+  int32_t *col;
+  float *coef;
+  double *row_sum;  /* FP64 row sums before normalisation */
+} or_csr_t;
+int or_make_transfers(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                      const or_patches_t *p, int64_t np, float normal_offset,
+                      int32_t skip_id, int normalize, int nthreads,
+                      or_csr_t *out);
+void or_csr_free(or_csr_t *c);
+/* FP64 row sum in the wave order of the kernels (see or_row_sum_f64) */
+double or_row_sum_f64(const float *coef, int64_t len);
+
+/* ---- direct light (GatherSampleLight, frozen semantics) ---- */
+typedef struct {
+  int32_t type;        /* emittype_t: 0 surface, 1 point, 2 spot, 3 skylight (sun), 5 sky ambient */
+  float origin[3];
+  float intensity[3];
+  float normal[3];     /* spot direction / direction the sun's light travels */
+  float constant_attn, linear_attn, quadratic_attn;
+  float stopdot, stopdot2, exponent;
+  float start_fade, end_fade, cap_distance;
+} or_light_t;
+typedef struct {
+  const float *px, *py, *pz, *nx, *ny, *nz;
+} or_samples_t;
+void or_gather_light(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                     const or_samples_t *s, int64_t n, const or_light_t *lights,
+                     int32_t n_lights, float normal_offset, int32_t skip_id,
+                     float *rgb, int nthreads);
+void or_sky_dirs(float *dirs /* 162*3 */);
+
+/* ---- BounceLight + FP64 energy ---- */
+/* FP32 in the exact order of the bounce kernel (coef_kind 0=f32, 1=f16, 2=bf16
+ * with per-row scale); energy reduced in the kernel's fixed tree order. */
+int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
+                  const void *coef, int coef_kind, const float *row_scale,
+                  const float *emit0, const float *refl, const float *area,
+                  float *total /* in/out, 3n */, int max_bounces,
+                  double stop_frac, double *energy, int nthreads);
+/* FP64 recurrence, sequential order, FP64 coefficients as given (f32 input) */
+int or_bounce_f64(int64_t n, const int64_t *row_ptr, const int32_t *col,
+                  const float *coef, const float *emit0, const float *refl,
+                  const float *area, double *total /* in/out, 3n */,
+                  int max_bounces, double stop_frac, double *energy);
+/* fixed-tree FP64 reduction used by the energy kernels */
+double or_energy_reduce(const double *e, int64_t n);
+
+/* FP16 / BF16 encode-decode (round to nearest even), used for the precision
+ * form's coefficient storage. */
+uint16_t or_f32_to_f16(float f);
+float or_f16_to_f32(uint16_t h);
+uint16_t or_f32_to_bf16(float f);
+float or_bf16_to_f32(uint16_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

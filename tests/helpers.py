@@ -1,0 +1,73 @@
+"""Test-side geometry helpers written from the reference's definitions."""
+import math
+
+import numpy as np
+
+
+def quad(id_, v1, v2, v3, v4):
+    """Environment.AddQuad (raytracer/environment.go:71-75)."""
+    return [(id_, v1, v2, v3), (id_ + 1, v1, v3, v4)]
+
+
+def box(mn, mx, id_=0x02000000):
+    """AddAxisAlignedRectangularSolid (raytracer/environment.go:77-117)."""
+    x0, y0, z0 = mn
+    x1, y1, z1 = mx
+    t = []
+    t += quad(id_, (x0, y1, z1), (x1, y1, z1), (x1, y0, z1), (x0, y0, z1))
+    t += quad(id_, (x0, y1, z0), (x1, y1, z0), (x1, y0, z0), (x0, y0, z0))
+    t += quad(id_, (x0, y1, z1), (x0, y1, z0), (x0, y0, z0), (x0, y0, z1))
+    t += quad(id_, (x1, y1, z1), (x1, y1, z0), (x1, y0, z0), (x1, y0, z1))
+    t += quad(id_, (x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (x0, y1, z0))
+    t += quad(id_, (x0, y0, z1), (x1, y0, z1), (x1, y0, z0), (x0, y0, z0))
+    return t
+
+
+def to_arrays(tris):
+    v9 = np.array([[*a, *b, *c] for _, a, b, c in tris], np.float32).reshape(-1, 9)
+    ids = np.array([i for i, *_ in tris], np.int32)
+    return v9, ids
+
+
+def square_patches(origin, u, v, normal, n):
+    """n x n patches of the parallelogram origin + a u + b v."""
+    o = np.asarray(origin, np.float64)
+    u = np.asarray(u, np.float64)
+    v = np.asarray(v, np.float64)
+    pts, nrm, area = [], [], []
+    a = np.linalg.norm(np.cross(u, v)) / (n * n)
+    for j in range(n):
+        for i in range(n):
+            pts.append(o + u * (i + 0.5) / n + v * (j + 0.5) / n)
+            nrm.append(normal)
+            area.append(a)
+    return (np.array(pts, np.float32), np.array(nrm, np.float32), np.array(area, np.float32))
+
+
+def howell_parallel(a, b, c):
+    """Parallel, directly opposed rectangles a x b at distance c (Howell C-11)."""
+    X, Y = a / c, b / c
+    s1, s2 = math.sqrt(1 + X * X), math.sqrt(1 + Y * Y)
+    return 2.0 / (math.pi * X * Y) * (
+        math.log(math.sqrt((1 + X * X) * (1 + Y * Y) / (1 + X * X + Y * Y)))
+        + X * s2 * math.atan(X / s2) + Y * s1 * math.atan(Y / s1)
+        - X * math.atan(X) - Y * math.atan(Y))
+
+
+def howell_perpendicular(w, h, l):
+    """Rectangles w x l and h x l at 90 degrees sharing the edge l (Howell C-14),
+    from the w-rectangle to the h-rectangle."""
+    W, H = w / l, h / l
+    A = (1 + W * W) * (1 + H * H) / (1 + W * W + H * H)
+    B = (W * W * (1 + W * W + H * H) / ((1 + W * W) * (W * W + H * H))) ** (W * W)
+    Cc = (H * H * (1 + H * H + W * W) / ((1 + H * H) * (H * H + W * W))) ** (H * H)
+    r = math.sqrt(H * H + W * W)
+    return 1.0 / (math.pi * W) * (W * math.atan(1 / W) + H * math.atan(1 / H)
+                                  - r * math.atan(1 / r) + 0.25 * math.log(A * B * Cc))
+
+
+def random_rays(rng, n, lo, hi):
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return o, d

@@ -1,0 +1,128 @@
+"""BounceLight on the GPU vs the CPU oracle (kernel-order FP32 bit-exact,
+FP64 stated error) and the furnace closed form."""
+import numpy as np
+import pytest
+
+import oracle
+from vrad_amd import _lib, lightmap
+from vrad_amd.patches import MakeTransfers, Transfers
+from vrad_amd.raytracer import Environment
+
+pytestmark = pytest.mark.gpu
+
+
+def _furnace(n=64, m=8, rho=0.5, seed=0):
+    rng = np.random.default_rng(seed)
+    rp = np.arange(n + 1, dtype=np.int64) * m
+    col = np.concatenate([np.sort(rng.choice(n, m, replace=False)) for _ in range(n)]).astype(np.int32)
+    coef = np.full(n * m, 1.0 / m, np.float32)
+    return rp, col, coef, np.ones((n, 3), np.float32), np.full((n, 3), rho, np.float32), \
+        np.ones(n, np.float32)
+
+
+@pytest.mark.parametrize("path", [_lib.VR_PATH_OPT, _lib.VR_PATH_PLAIN], ids=["opt", "plain"])
+def test_furnace_exact(path):
+    rp, col, coef, e0, refl, area = _furnace()
+    _lib.check(_lib.lib.vr_set_path(path))
+    try:
+        xf = Transfers.from_csr(rp, col, coef)
+        tot, en, k = lightmap.BounceLight(xf, e0, refl, area, max_bounces=100, stop_frac=1e-3)
+    finally:
+        _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_OPT))
+    assert k == 10
+    assert np.all(tot == np.float32(2.0 - 2.0 ** -9))
+    assert np.array_equal(en, 3.0 * 64 * 0.5 ** np.arange(10))
+
+
+def _room_bake(room_scene):
+    s = room_scene
+    env = Environment()
+    env.AddTriangles(s.verts9, s.ids)
+    env.SetupAccelerationStructure()
+    xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area)
+    direct = lightmap.GatherSampleLight(env, s.p_origin, s.p_normal, s.lights)
+    emit0 = (direct * s.p_refl).astype(np.float32)
+    return xf, direct, emit0
+
+
+def test_room_bounce_bitexact(room_scene):
+    s = room_scene
+    xf, direct, emit0 = _room_bake(s)
+    rp, col, coef = xf.download()
+    print(f"[gpu] room bounce nnz={len(col)}", flush=True)
+    tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct,
+                                      max_bounces=100, stop_frac=1e-3)
+    otot, oen, ok = oracle.bounce_f32(rp, col, coef, emit0, s.p_refl, s.p_area, total=direct,
+                                      max_bounces=100, stop_frac=1e-3, threads=8)
+    assert k == ok and 2 < k < 100
+    assert np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
+    assert np.array_equal(en.view(np.uint64), oen.view(np.uint64))
+    # stated error against the FP64 recurrence
+    t64, e64, k64 = oracle.bounce_f64(rp, col, coef, emit0, s.p_refl, s.p_area,
+                                      total=direct.astype(np.float64), max_bounces=k, stop_frac=0.0)
+    rel = np.max(np.abs(tot - t64) / np.maximum(np.abs(t64), 1e-30))
+    print(f"[gpu] FP32 vs FP64 max rel luxel error {rel:.3e}", flush=True)
+    assert rel < 1e-4
+
+
+def test_room_bounce_plain_vs_sequential(room_scene):
+    s = room_scene
+    xf, direct, emit0 = _room_bake(s)
+    rp, col, coef = xf.download()
+    _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_PLAIN))
+    try:
+        tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct)
+    finally:
+        _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_OPT))
+    otot, oen, ok = oracle.bounce_f32(rp, col, coef, emit0, s.p_refl, s.p_area, total=direct,
+                                      threads=8)
+    assert k == ok
+    assert np.max(np.abs(tot - otot) / np.abs(otot)) < 1e-5
+
+
+@pytest.mark.parametrize("prec", [_lib.VR_COEF_F16, _lib.VR_COEF_BF16], ids=["f16", "bf16"])
+def test_16bit_transfers(room_scene, prec):
+    s = room_scene
+    xf, direct, emit0 = _room_bake(s)
+    rp, col, coef32 = xf.download()
+    xf.set_precision(prec)
+    rp2, col2, dec, raw, scale = xf.download(raw16=True)
+    assert np.array_equal(rp, rp2) and np.array_equal(col, col2)
+    # encoding: q = rne(coef / rowmax), scale = rowmax
+    rowmax = np.maximum.reduceat(coef32, rp[:-1])
+    rowmax[np.diff(rp) == 0] = 0
+    assert np.array_equal(scale[np.diff(rp) > 0], rowmax[np.diff(rp) > 0])
+    q = (coef32 / np.repeat(scale, np.diff(rp))).astype(np.float32)
+    conv = oracle.f32_to_f16 if prec == _lib.VR_COEF_F16 else oracle.f32_to_bf16
+    assert np.array_equal(raw, np.array([conv(v) for v in q], np.uint16))
+    tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct)
+    otot, oen, ok = oracle.bounce_f32(rp, col, raw, emit0, s.p_refl, s.p_area, total=direct,
+                                      kind=prec, row_scale=scale, threads=8)
+    assert k == ok and np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
+    t64, _, _ = oracle.bounce_f64(rp, col, coef32, emit0, s.p_refl, s.p_area,
+                                  total=direct.astype(np.float64), max_bounces=k, stop_frac=0.0)
+    rel = np.max(np.abs(tot - t64) / np.abs(t64))
+    print(f"[gpu] prec={prec} max rel luxel error vs FP64 {rel:.3e}", flush=True)
+    assert rel < 1e-3
+
+
+def test_synthetic_csr_bitexact():
+    n = 3000
+    xf = Transfers.synthetic(seed=7, n_rows=n, mean_len=300.0, sigma=1.0, max_len=2000)
+    rp, col, coef = xf.download()
+    L = np.diff(rp)
+    assert L.min() >= 1 and L.max() <= 2000 and abs(L.mean() - 300) < 30
+    for i in range(0, n, 97):
+        c = col[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(c) > 0) and np.all(c != i) and c.min() >= 0 and c.max() < n
+    rs = np.add.reduceat(coef.astype(np.float64), rp[:-1])
+    assert np.all(rs < 0.9 + 1e-5) and np.all(rs > 0.3 - 1e-5)
+    rng = np.random.default_rng(3)
+    e0 = rng.random((n, 3)).astype(np.float32)
+    refl = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+    area = rng.uniform(1, 4, n).astype(np.float32)
+    tot, en, k = lightmap.BounceLight(xf, e0, refl, area, max_bounces=8, stop_frac=0.0)
+    otot, oen, ok = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=8,
+                                      stop_frac=0.0, threads=8)
+    assert k == ok == 8 and np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
+    assert np.array_equal(en, oen)

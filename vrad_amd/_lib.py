@@ -1,0 +1,168 @@
+"""ctypes binding of libvrad_hip.so (the C-ABI in include/vrad_hip.h).
+
+The product path has no fallback: if the shared library is missing the
+import of this module raises, and every entry point that returns a non-zero
+status raises VrError with the library's message.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvrad_hip.so")
+
+VR_OK = 0
+VR_E_INVALID, VR_E_NOMEM, VR_E_HIP, VR_E_UNSUPPORTED = -1, -2, -3, -4
+VR_E_STATE, VR_E_NODEVICE, VR_E_CAPACITY = -5, -6, -7
+VR_PATH_OPT, VR_PATH_PLAIN = 0, 1
+VR_COEF_F32, VR_COEF_F16, VR_COEF_BF16 = 0, 1, 2
+VR_EMIT_SURFACE, VR_EMIT_POINT, VR_EMIT_SPOTLIGHT = 0, 1, 2
+VR_EMIT_SKYLIGHT, VR_EMIT_SKYAMBIENT = 3, 5
+
+
+class VrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vrad_hip error {code}: {msg}")
+        self.code = code
+
+
+FP = C.POINTER(C.c_float)
+
+
+class Rays(C.Structure):
+    _fields_ = [(n, FP) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+
+
+class Segs(C.Structure):
+    _fields_ = [(n, FP) for n in ("sx", "sy", "sz", "ex", "ey", "ez")]
+
+
+class Patches(C.Structure):
+    _fields_ = [(n, FP) for n in ("ox", "oy", "oz", "nx", "ny", "nz", "area")]
+
+
+class Samples(C.Structure):
+    _fields_ = [(n, FP) for n in ("px", "py", "pz", "nx", "ny", "nz")]
+
+
+class XferParams(C.Structure):
+    _fields_ = [("normal_offset", C.c_float), ("skip_id", C.c_int32),
+                ("normalize", C.c_int32), ("batch_pairs", C.c_int64)]
+
+
+class XfersInfo(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("nnz", C.c_int64), ("stored_nnz", C.c_int64),
+                ("stored_bytes", C.c_int64), ("precision", C.c_int32),
+                ("pairs_considered", C.c_int64), ("segments_traced", C.c_int64)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_tri", C.c_int64), ("n_nodes", C.c_int64), ("n_leaf_refs", C.c_int64),
+                ("max_depth", C.c_int32), ("has_sky", C.c_int32),
+                ("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("device_bytes", C.c_int64)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("origin", C.c_float * 3), ("intensity", C.c_float * 3),
+                ("normal", C.c_float * 3), ("constant_attn", C.c_float),
+                ("linear_attn", C.c_float), ("quadratic_attn", C.c_float),
+                ("stopdot", C.c_float), ("stopdot2", C.c_float), ("exponent", C.c_float),
+                ("start_fade", C.c_float), ("end_fade", C.c_float),
+                ("cap_distance", C.c_float)]
+
+
+LIGHT_DTYPE = np.dtype([("type", "<i4"), ("origin", "<f4", 3), ("intensity", "<f4", 3),
+                        ("normal", "<f4", 3), ("constant_attn", "<f4"), ("linear_attn", "<f4"),
+                        ("quadratic_attn", "<f4"), ("stopdot", "<f4"), ("stopdot2", "<f4"),
+                        ("exponent", "<f4"), ("start_fade", "<f4"), ("end_fade", "<f4"),
+                        ("cap_distance", "<f4")])
+assert LIGHT_DTYPE.itemsize == C.sizeof(Light)
+
+
+class BounceParams(C.Structure):
+    _fields_ = [("max_bounces", C.c_int32), ("stop_frac", C.c_double)]
+
+
+# name -> (restype, argtypes); every function returns int status unless noted
+_P = C.c_void_p
+_SIGS = {
+    "vr_last_error": (C.c_char_p, []),
+    "vr_init": (C.c_int, [C.c_int]),
+    "vr_set_path": (C.c_int, [C.c_int]),
+    "vr_profile_enable": (C.c_int, [C.c_int]),
+    "vr_profile_reset": (C.c_int, []),
+    "vr_profile_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "vr_scene_create": (C.c_int, [C.POINTER(_P), _P, _P, _P, C.c_int64]),
+    "vr_scene_build": (C.c_int, [_P, C.c_int32]),
+    "vr_scene_build_host": (C.c_int, [_P, C.c_int32]),
+    "vr_scene_get_info": (C.c_int, [_P, C.POINTER(SceneInfo)]),
+    "vr_scene_get_kdtree": (C.c_int, [_P, _P, _P, _P]),
+    "vr_scene_get_triangles": (C.c_int, [_P, _P]),
+    "vr_scene_destroy": (C.c_int, [_P]),
+    "vr_trace_closest": (C.c_int, [_P, C.POINTER(Rays), C.c_int64, C.c_int32, _P, _P, _P]),
+    "vr_test_line": (C.c_int, [_P, C.POINTER(Segs), C.c_int64, C.c_int32, _P, _P]),
+    "vr_trace_closest_dev": (C.c_int, [_P, C.POINTER(Rays), C.c_int64, C.c_int32, _P, _P, _P, _P]),
+    "vr_test_line_dev": (C.c_int, [_P, C.POINTER(Segs), C.c_int64, C.c_int32, _P, _P, _P]),
+    "vr_make_transfers": (C.c_int, [_P, C.POINTER(Patches), C.c_int64, C.POINTER(XferParams),
+                                    C.POINTER(_P)]),
+    "vr_xfers_from_csr": (C.c_int, [C.POINTER(_P), C.c_int64, _P, _P, _P, C.c_int32]),
+    "vr_xfers_synthetic": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int64, C.c_double,
+                                     C.c_double, C.c_int64, C.c_int32]),
+    "vr_xfers_set_precision": (C.c_int, [_P, C.c_int32]),
+    "vr_xfers_get_info": (C.c_int, [_P, C.POINTER(XfersInfo)]),
+    "vr_xfers_download": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "vr_xfers_destroy": (C.c_int, [_P]),
+    "vr_gather_sample_light": (C.c_int, [_P, C.POINTER(Samples), C.c_int64, _P, C.c_int32,
+                                         C.c_float, C.c_int32, _P]),
+    "vr_bounce_light": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(BounceParams), _P,
+                                  C.POINTER(C.c_int32)]),
+    "vr_gen_room": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int32, C.c_float, C.c_float]),
+    "vr_gen_level": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int32, C.c_int32, C.c_float,
+                               C.c_float, C.c_int32, C.c_int32]),
+    "vr_gen_counts": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int32)]),
+    "vr_gen_copy": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vr_gen_destroy": (C.c_int, [_P]),
+}
+EXPORTED = tuple(_SIGS)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C vrad_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc != VR_OK:
+        raise VrError(rc, lib.vr_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a C-contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+def fptr(a):
+    return None if a is None else a.ctypes.data_as(FP)
+
+
+def f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a

@@ -1,0 +1,946 @@
+// C-ABI of libvrad_hip.so (declared in include/vrad_hip.h): scene residency,
+// visibility batches, transfer construction, direct light, bounce driver,
+// procedural inputs and kernel timing.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "vr_internal.h"
+
+namespace vr {
+int gen_room(uint64_t seed, int n_pillars, double chop, double tess, GenScene *out);
+int gen_level(uint64_t seed, int rx, int ry, double chop, double tess, int n_lights, int sun,
+              GenScene *out);
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+namespace {
+
+std::recursive_mutex g_mu;
+bool g_init = false;
+int g_device = 0;
+hipStream_t g_stream = nullptr;
+int g_path = VR_PATH_OPT;
+bool g_prof = false;
+
+struct ProfEntry {
+  std::string name;
+  hipEvent_t a, b;
+};
+std::vector<ProfEntry> g_pending;
+std::map<std::string, std::pair<double, int64_t>> g_prof_tot;
+
+void prof_drain() {
+  for (auto &p : g_pending) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto &t = g_prof_tot[p.name];
+      t.first += ms;
+      t.second += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  g_pending.clear();
+}
+
+// Brackets the launches in its scope with HIP events on `st` when enabled.
+struct Prof {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t st;
+  const char *name;
+  Prof(const char *nm, hipStream_t s) : st(s), name(nm) {
+    if (!g_prof) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      a = b = nullptr;
+      return;
+    }
+    (void)hipEventRecord(a, st);
+  }
+  ~Prof() {
+    if (!a) return;
+    (void)hipEventRecord(b, st);
+    g_pending.push_back(ProfEntry{name, a, b});
+  }
+};
+
+#define VR_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(VR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+int ensure_init() {
+  if (g_init) return VR_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(VR_E_NODEVICE, "no HIP device available");
+  VR_HIP(hipSetDevice(g_device));
+  VR_HIP(hipStreamCreate(&g_stream));
+  g_init = true;
+  return VR_OK;
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(VR_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return VR_OK;
+}
+
+// device buffer
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(VR_E_NOMEM, "hipMalloc(" + std::to_string(n) + "): " + hipGetErrorString(e));
+    }
+    bytes = n;
+    return VR_OK;
+  }
+  int upload(const void *h, size_t n, hipStream_t st) {
+    int rc = alloc(n);
+    if (rc) return rc;
+    if (n && h) VR_HIP(hipMemcpyAsync(p, h, n, hipMemcpyHostToDevice, st));
+    return VR_OK;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+template <class T>
+int upload_vec(DBuf &b, const std::vector<T> &v, hipStream_t st) {
+  return b.upload(v.data(), v.size() * sizeof(T), st);
+}
+
+}  // namespace
+}  // namespace vr
+
+using namespace vr;
+
+// ---------------------------------------------------------------------------
+struct vr_scene {
+  int64_t n_tri = 0;
+  std::vector<float> v9;
+  std::vector<int32_t> ids;
+  std::vector<uint8_t> flags;
+  bool built = false, host_built = false;
+  std::vector<TriIntersect> tris;
+  KdTree kd;
+  DBuf nodes, refs, planes;
+  DevScene dev{};
+  int has_sky = 0;
+  int64_t device_bytes = 0;
+};
+
+struct vr_xfers {
+  int64_t n = 0, nnz = 0, stored = 0;
+  int32_t precision = VR_COEF_F32;
+  int64_t pairs = 0, segments = 0;
+  std::vector<int64_t> rp_host;  // padded starts
+  std::vector<int32_t> cnt_host; // logical lengths
+  DBuf rp, col, coef32, coef16, scale, order, cnt;
+  DevXfers dev() const {
+    DevXfers x;
+    x.n = n;
+    x.rp = rp.as<int64_t>();
+    x.col = col.as<int32_t>();
+    x.coef = precision == VR_COEF_F32 ? coef32.p : coef16.p;
+    x.scale = scale.as<float>();
+    x.order = order.as<int32_t>();
+    x.kind = precision;
+    return x;
+  }
+};
+
+struct vr_gen {
+  GenScene g;
+};
+
+namespace vr {
+namespace {
+
+// rows longest-first (ties by index), padded starts
+void layout_rows(const std::vector<int32_t> &cnt, std::vector<int64_t> &rp,
+                 std::vector<int32_t> &order) {
+  const int64_t n = (int64_t)cnt.size();
+  rp.assign((size_t)n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + (((int64_t)cnt[i] + 7) & ~(int64_t)7);
+  order.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return cnt[a] > cnt[b]; });
+}
+
+int finish_layout(vr_xfers *x, hipStream_t st) {
+  std::vector<int32_t> order;
+  layout_rows(x->cnt_host, x->rp_host, order);
+  int rc = upload_vec(x->rp, x->rp_host, st);
+  if (!rc) rc = upload_vec(x->order, order, st);
+  if (!rc) rc = upload_vec(x->cnt, x->cnt_host, st);
+  x->stored = x->rp_host.back();
+  return rc;
+}
+
+std::vector<float> sky_dirs() {
+  std::vector<float> d(162 * 3);
+  const double ga = 3.14159265358979323846 * (3.0 - std::sqrt(5.0));
+  for (int k = 0; k < 162; ++k) {
+    double z = 1.0 - (2.0 * k + 1.0) / 162.0, r = std::sqrt(1.0 - z * z), ph = ga * k;
+    d[3 * k + 0] = (float)(r * std::cos(ph));
+    d[3 * k + 1] = (float)(r * std::sin(ph));
+    d[3 * k + 2] = (float)z;
+  }
+  return d;
+}
+
+// staging for host-pointer calls: one device block carved into arrays
+struct Staging {
+  DBuf buf;
+  std::vector<std::pair<size_t, size_t>> parts;
+  int make(const std::vector<size_t> &sizes) {
+    size_t off = 0;
+    parts.clear();
+    for (size_t s : sizes) {
+      parts.push_back({off, s});
+      off += (s + 255) & ~(size_t)255;
+    }
+    return buf.alloc(off);
+  }
+  template <class T>
+  T *ptr(int k) const {
+    return reinterpret_cast<T *>(buf.as<char>() + parts[(size_t)k].first);
+  }
+};
+
+}  // namespace
+}  // namespace vr
+
+#define LOCK std::lock_guard<std::recursive_mutex> lock_(g_mu)
+#define TRY(x)            \
+  do {                    \
+    int rc_ = (x);        \
+    if (rc_) return rc_;  \
+  } while (0)
+
+extern "C" {
+
+const char *vr_last_error(void) { return g_err.c_str(); }
+
+int vr_init(int device) {
+  LOCK;
+  if (g_init) {
+    if (device == g_device) return VR_OK;
+    return fail(VR_E_STATE, "library already initialised on another device");
+  }
+  g_device = device;
+  return ensure_init();
+}
+
+int vr_set_path(int path) {
+  LOCK;
+  if (path != VR_PATH_OPT && path != VR_PATH_PLAIN) return fail(VR_E_INVALID, "unknown path");
+  g_path = path;
+  return VR_OK;
+}
+
+int vr_profile_enable(int on) {
+  LOCK;
+  g_prof = on != 0;
+  return VR_OK;
+}
+
+int vr_profile_reset(void) {
+  LOCK;
+  prof_drain();
+  g_prof_tot.clear();
+  return VR_OK;
+}
+
+int vr_profile_get(const char *name, double *total_ms, int64_t *launches) {
+  LOCK;
+  if (!name) return fail(VR_E_INVALID, "name is NULL");
+  prof_drain();
+  auto it = g_prof_tot.find(name);
+  if (total_ms) *total_ms = it == g_prof_tot.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == g_prof_tot.end() ? 0 : it->second.second;
+  return VR_OK;
+}
+
+// ---- scene ----------------------------------------------------------------
+int vr_scene_create(vr_scene **out, const float *verts9, const int32_t *ids, const uint8_t *flags,
+                    int64_t n_tri) {
+  if (!out) return fail(VR_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (n_tri < 0) return fail(VR_E_INVALID, "n_tri < 0");
+  if (n_tri > 0 && (!verts9 || !ids)) return fail(VR_E_INVALID, "verts9/ids are NULL");
+  if (n_tri >= (int64_t)kTriMask) return fail(VR_E_CAPACITY, "more than 2^28 triangles");
+  std::unique_ptr<vr_scene> s(new (std::nothrow) vr_scene);
+  if (!s) return fail(VR_E_NOMEM, "scene");
+  s->n_tri = n_tri;
+  s->v9.assign(verts9, verts9 + 9 * n_tri);
+  s->ids.assign(ids, ids + n_tri);
+  if (flags) s->flags.assign(flags, flags + n_tri);
+  else s->flags.assign((size_t)n_tri, 0);
+  *out = s.release();
+  return VR_OK;
+}
+
+int vr_scene_build_host(vr_scene *s, int32_t n_threads) {
+  LOCK;
+  if (!s) return fail(VR_E_INVALID, "scene is NULL");
+  if (s->host_built) return VR_OK;
+  const int nt = host_threads(n_threads);
+  s->tris.resize((size_t)s->n_tri);
+  precompute_triangles(s->v9.data(), s->ids.data(), s->flags.data(), s->n_tri, s->tris.data(), nt);
+  s->kd = KdTree();
+  TRY(build_kdtree(s->v9.data(), s->n_tri, nt, &s->kd));
+  if (s->kd.max_depth >= kStackDepth)
+    return fail(VR_E_CAPACITY, "kd-tree deeper than the traversal stack");
+  s->has_sky = 0;
+  for (const TriIntersect &t : s->tris)
+    if (t.id & VR_TRACE_ID_SKY) s->has_sky = 1;
+  s->host_built = true;
+  return VR_OK;
+}
+
+int vr_scene_build(vr_scene *s, int32_t n_threads) {
+  LOCK;
+  if (!s) return fail(VR_E_INVALID, "scene is NULL");
+  TRY(vr_scene_build_host(s, n_threads));
+  TRY(ensure_init());
+  const KdTree &kd = s->kd;
+  std::vector<uint2> nodes(kd.children.size());
+  for (size_t k = 0; k < nodes.size(); ++k) {
+    uint32_t sb;
+    std::memcpy(&sb, &kd.split[k], 4);
+    nodes[k] = make_uint2(kd.children[k], sb);
+  }
+  std::vector<float4> refs(kd.idx.size() * 3);
+  for (size_t k = 0; k < kd.idx.size(); ++k) {
+    const TriIntersect &t = s->tris[(size_t)kd.idx[k]];
+    uint32_t meta = (uint32_t)kd.idx[k] | ((uint32_t)t.c0 << 28) | ((uint32_t)t.c1 << 30);
+    float idf, metaf;
+    std::memcpy(&idf, &t.id, 4);
+    std::memcpy(&metaf, &meta, 4);
+    refs[3 * k + 0] = make_float4(t.nx, t.ny, t.nz, t.d);
+    refs[3 * k + 1] = make_float4(t.e[0], t.e[1], t.e[2], t.e[3]);
+    refs[3 * k + 2] = make_float4(t.e[4], t.e[5], idf, metaf);
+  }
+  std::vector<float4> planes((size_t)s->n_tri);
+  for (int64_t i = 0; i < s->n_tri; ++i) {
+    const TriIntersect &t = s->tris[(size_t)i];
+    planes[(size_t)i] = make_float4(t.nx, t.ny, t.nz, t.d);
+  }
+  TRY(upload_vec(s->nodes, nodes, g_stream));
+  TRY(upload_vec(s->refs, refs, g_stream));
+  TRY(upload_vec(s->planes, planes, g_stream));
+  VR_HIP(hipStreamSynchronize(g_stream));
+  DevScene d{};
+  d.nodes = s->nodes.as<uint2>();
+  d.refs = s->refs.as<float4>();
+  d.planes = s->planes.as<float4>();
+  float M = 1.0f;
+  for (int c = 0; c < 3; ++c) {
+    d.bmin[c] = kd.bmin[c];
+    d.bmax[c] = kd.bmax[c];
+    M = std::max(M, std::max(std::fabs(kd.bmin[c]), std::fabs(kd.bmax[c])));
+  }
+  d.M = M;
+  d.has_sky = s->has_sky;
+  d.empty = s->n_tri == 0 ? 1 : 0;
+  s->dev = d;
+  s->device_bytes = (int64_t)(s->nodes.bytes + s->refs.bytes + s->planes.bytes);
+  s->built = true;
+  return VR_OK;
+}
+
+int vr_scene_get_info(const vr_scene *s, vr_scene_info *out) {
+  if (!s || !out) return fail(VR_E_INVALID, "NULL argument");
+  std::memset(out, 0, sizeof *out);
+  out->n_tri = s->n_tri;
+  out->n_nodes = (int64_t)s->kd.children.size();
+  out->n_leaf_refs = (int64_t)s->kd.idx.size();
+  out->max_depth = s->kd.max_depth;
+  out->has_sky = s->has_sky;
+  for (int c = 0; c < 3; ++c) {
+    out->bmin[c] = s->kd.bmin[c];
+    out->bmax[c] = s->kd.bmax[c];
+  }
+  out->device_bytes = s->device_bytes;
+  return VR_OK;
+}
+
+int vr_scene_get_kdtree(const vr_scene *s, uint32_t *children, float *split,
+                        int32_t *triangle_index_list) {
+  if (!s) return fail(VR_E_INVALID, "scene is NULL");
+  if (!s->host_built) return fail(VR_E_STATE, "scene not built");
+  if (children) std::memcpy(children, s->kd.children.data(), 4 * s->kd.children.size());
+  if (split) std::memcpy(split, s->kd.split.data(), 4 * s->kd.split.size());
+  if (triangle_index_list)
+    std::memcpy(triangle_index_list, s->kd.idx.data(), 4 * s->kd.idx.size());
+  return VR_OK;
+}
+
+int vr_scene_get_triangles(const vr_scene *s, void *out) {
+  if (!s || !out) return fail(VR_E_INVALID, "NULL argument");
+  if (!s->host_built) return fail(VR_E_STATE, "scene not built");
+  std::memcpy(out, s->tris.data(), sizeof(TriIntersect) * s->tris.size());
+  return VR_OK;
+}
+
+int vr_scene_destroy(vr_scene *s) {
+  LOCK;
+  delete s;
+  return VR_OK;
+}
+
+// ---- visibility ---------------------------------------------------------
+int vr_trace_closest_dev(vr_scene *s, const vr_rays *r, int64_t n, int32_t skip_id,
+                         int32_t *hit, float *hit_t, float *hit_n, void *stream) {
+  LOCK;
+  if (!s || !r) return fail(VR_E_INVALID, "NULL argument");
+  if (!s->built) return fail(VR_E_STATE, "scene not built");
+  if (n < 0) return fail(VR_E_INVALID, "n < 0");
+  if (n > 0 && (!hit || !hit_t)) return fail(VR_E_INVALID, "outputs are NULL");
+  TRY(ensure_init());
+  hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+  DevRays dr{r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->tmin, r->tmax};
+  {
+    Prof p("trace_closest", st);
+    launch_trace_closest(s->dev, dr, n, skip_id, hit, hit_t, hit_n, g_path, st);
+  }
+  return check_launch();
+}
+
+int vr_test_line_dev(vr_scene *s, const vr_segs *g, int64_t n, int32_t skip_id, uint8_t *vis,
+                     uint8_t *sky, void *stream) {
+  LOCK;
+  if (!s || !g) return fail(VR_E_INVALID, "NULL argument");
+  if (!s->built) return fail(VR_E_STATE, "scene not built");
+  if (n < 0) return fail(VR_E_INVALID, "n < 0");
+  if (n > 0 && !vis) return fail(VR_E_INVALID, "visible is NULL");
+  TRY(ensure_init());
+  hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+  DevSegs ds{g->sx, g->sy, g->sz, g->ex, g->ey, g->ez};
+  {
+    Prof p("test_line", st);
+    launch_test_line(s->dev, ds, n, skip_id, vis, sky, g_path, st);
+  }
+  return check_launch();
+}
+
+int vr_trace_closest(vr_scene *s, const vr_rays *r, int64_t n, int32_t skip_id, int32_t *hit,
+                     float *hit_t, float *hit_n) {
+  LOCK;
+  if (!s || !r) return fail(VR_E_INVALID, "NULL argument");
+  if (!s->built) return fail(VR_E_STATE, "scene not built");
+  if (n < 0) return fail(VR_E_INVALID, "n < 0");
+  if (n == 0) return VR_OK;
+  if (!hit || !hit_t || !r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz || !r->tmin ||
+      !r->tmax)
+    return fail(VR_E_INVALID, "NULL ray array or output");
+  TRY(ensure_init());
+  const size_t fb = 4 * (size_t)n;
+  Staging S;
+  TRY(S.make({fb, fb, fb, fb, fb, fb, fb, fb, fb, fb, hit_n ? 3 * fb : 0}));
+  const float *src[8] = {r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->tmin, r->tmax};
+  for (int k = 0; k < 8; ++k)
+    VR_HIP(hipMemcpyAsync(S.ptr<float>(k), src[k], fb, hipMemcpyHostToDevice, g_stream));
+  vr_rays dr{S.ptr<float>(0), S.ptr<float>(1), S.ptr<float>(2), S.ptr<float>(3),
+             S.ptr<float>(4), S.ptr<float>(5), S.ptr<float>(6), S.ptr<float>(7)};
+  TRY(vr_trace_closest_dev(s, &dr, n, skip_id, S.ptr<int32_t>(8), S.ptr<float>(9),
+                           hit_n ? S.ptr<float>(10) : nullptr, nullptr));
+  VR_HIP(hipMemcpyAsync(hit, S.ptr<int32_t>(8), fb, hipMemcpyDeviceToHost, g_stream));
+  VR_HIP(hipMemcpyAsync(hit_t, S.ptr<float>(9), fb, hipMemcpyDeviceToHost, g_stream));
+  if (hit_n) VR_HIP(hipMemcpyAsync(hit_n, S.ptr<float>(10), 3 * fb, hipMemcpyDeviceToHost, g_stream));
+  VR_HIP(hipStreamSynchronize(g_stream));
+  return VR_OK;
+}
+
+int vr_test_line(vr_scene *s, const vr_segs *g, int64_t n, int32_t skip_id, uint8_t *vis,
+                 uint8_t *sky) {
+  LOCK;
+  if (!s || !g) return fail(VR_E_INVALID, "NULL argument");
+  if (!s->built) return fail(VR_E_STATE, "scene not built");
+  if (n < 0) return fail(VR_E_INVALID, "n < 0");
+  if (n == 0) return VR_OK;
+  if (!vis || !g->sx || !g->sy || !g->sz || !g->ex || !g->ey || !g->ez)
+    return fail(VR_E_INVALID, "NULL segment array or output");
+  TRY(ensure_init());
+  const size_t fb = 4 * (size_t)n;
+  Staging S;
+  TRY(S.make({fb, fb, fb, fb, fb, fb, (size_t)n, (size_t)n}));
+  const float *src[6] = {g->sx, g->sy, g->sz, g->ex, g->ey, g->ez};
+  for (int k = 0; k < 6; ++k)
+    VR_HIP(hipMemcpyAsync(S.ptr<float>(k), src[k], fb, hipMemcpyHostToDevice, g_stream));
+  vr_segs ds{S.ptr<float>(0), S.ptr<float>(1), S.ptr<float>(2),
+             S.ptr<float>(3), S.ptr<float>(4), S.ptr<float>(5)};
+  TRY(vr_test_line_dev(s, &ds, n, skip_id, S.ptr<uint8_t>(6), sky ? S.ptr<uint8_t>(7) : nullptr,
+                       nullptr));
+  VR_HIP(hipMemcpyAsync(vis, S.ptr<uint8_t>(6), (size_t)n, hipMemcpyDeviceToHost, g_stream));
+  if (sky) VR_HIP(hipMemcpyAsync(sky, S.ptr<uint8_t>(7), (size_t)n, hipMemcpyDeviceToHost, g_stream));
+  VR_HIP(hipStreamSynchronize(g_stream));
+  return VR_OK;
+}
+
+// ---- transfers ------------------------------------------------------------
+__global__ void k_add_count(const uint32_t *c, uint32_t cap, unsigned long long *tot) {
+  *tot += (unsigned long long)min(*c, cap);
+}
+
+int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfer_params *prm,
+                      vr_xfers **out) {
+  LOCK;
+  if (!s || !p || !out) return fail(VR_E_INVALID, "NULL argument");
+  *out = nullptr;
+  if (!s->built) return fail(VR_E_STATE, "scene not built");
+  if (np < 0 || np >= (1ll << 31)) return fail(VR_E_INVALID, "bad patch count");
+  if (np > 0 && (!p->ox || !p->oy || !p->oz || !p->nx || !p->ny || !p->nz || !p->area))
+    return fail(VR_E_INVALID, "NULL patch array");
+  TRY(ensure_init());
+  vr_xfer_params P{0.1f, -1, 1, 0};
+  if (prm) P = *prm;
+  const int64_t cap = std::max<int64_t>(P.batch_pairs > 0 ? P.batch_pairs : (int64_t)8 << 20,
+                                        std::max<int64_t>(np - 1, 1));
+  if (cap >= (1ll << 31)) return fail(VR_E_INVALID, "batch_pairs too large");
+  hipStream_t st = g_stream;
+  std::unique_ptr<vr_xfers> x(new vr_xfers);
+  x->n = np;
+  const size_t fb = 4 * (size_t)std::max<int64_t>(np, 1);
+  Staging S;
+  TRY(S.make({fb, fb, fb, fb, fb, fb, fb}));
+  const float *src[7] = {p->ox, p->oy, p->oz, p->nx, p->ny, p->nz, p->area};
+  for (int k = 0; k < 7 && np > 0; ++k)
+    VR_HIP(hipMemcpyAsync(S.ptr<float>(k), src[k], 4 * (size_t)np, hipMemcpyHostToDevice, st));
+  DevPatches dp{S.ptr<float>(0), S.ptr<float>(1), S.ptr<float>(2), S.ptr<float>(3),
+                S.ptr<float>(4), S.ptr<float>(5), S.ptr<float>(6)};
+  const int64_t words = (np + 31) / 32;
+  DBuf bits, pairs, count, segtot;
+  TRY(bits.alloc((size_t)np * (size_t)words * 4));
+  VR_HIP(hipMemsetAsync(bits.p, 0, bits.bytes, st));
+  TRY(pairs.alloc((size_t)cap * sizeof(int2)));
+  TRY(count.alloc(4));
+  TRY(segtot.alloc(8));
+  VR_HIP(hipMemsetAsync(segtot.p, 0, 8, st));
+  {
+    Prof pr("make_transfers", st);
+    int64_t i0 = 0;
+    while (i0 + 1 < np) {
+      const int64_t rowlen = np - 1 - i0;
+      int64_t rows = std::max<int64_t>(1, cap / rowlen);
+      const int64_t i1 = std::min<int64_t>(np - 1, i0 + rows);
+      x->pairs += (2 * np - i0 - i1 - 1) * (i1 - i0) / 2;
+      VR_HIP(hipMemsetAsync(count.p, 0, 4, st));
+      launch_pair_cull(dp, np, i0, i1, pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap, st);
+      launch_pair_trace(s->dev, dp, pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap,
+                        P.normal_offset, P.skip_id, bits.as<uint32_t>(), words, g_path, st);
+      hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, st, count.as<uint32_t>(),
+                         (uint32_t)cap, segtot.as<unsigned long long>());
+      TRY(check_launch());
+      i0 = i1;
+    }
+    TRY(x->cnt.alloc(fb));
+    launch_row_count(bits.as<uint32_t>(), np, words, x->cnt.as<int32_t>(), st);
+    TRY(check_launch());
+    x->cnt_host.assign((size_t)np, 0);
+    if (np > 0)
+      VR_HIP(hipMemcpyAsync(x->cnt_host.data(), x->cnt.p, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
+    unsigned long long segs = 0;
+    VR_HIP(hipMemcpyAsync(&segs, segtot.p, 8, hipMemcpyDeviceToHost, st));
+    VR_HIP(hipStreamSynchronize(st));
+    x->segments = (int64_t)segs;
+    x->nnz = 0;
+    for (int32_t c : x->cnt_host) x->nnz += c;
+    TRY(finish_layout(x.get(), st));
+    TRY(x->col.alloc(4 * (size_t)x->stored));
+    TRY(x->coef32.alloc(4 * (size_t)x->stored));
+    launch_row_fill(bits.as<uint32_t>(), np, words, dp, x->rp.as<int64_t>(), x->cnt.as<int32_t>(),
+                    x->col.as<int32_t>(), x->coef32.as<float>(), st);
+    DBuf rowsum;
+    TRY(rowsum.alloc(8 * (size_t)std::max<int64_t>(np, 1)));
+    launch_row_normalize(np, x->rp.as<int64_t>(), x->coef32.as<float>(), rowsum.as<double>(),
+                         P.normalize, st);
+    TRY(check_launch());
+    VR_HIP(hipStreamSynchronize(st));
+  }
+  *out = x.release();
+  return VR_OK;
+}
+
+int vr_xfers_from_csr(vr_xfers **out, int64_t n, const int64_t *row_ptr, const int32_t *col,
+                      const float *coef, int32_t precision) {
+  LOCK;
+  if (!out) return fail(VR_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (n < 0 || (n > 0 && !row_ptr)) return fail(VR_E_INVALID, "bad rows");
+  TRY(ensure_init());
+  std::unique_ptr<vr_xfers> x(new vr_xfers);
+  x->n = n;
+  x->cnt_host.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t l = row_ptr[i + 1] - row_ptr[i];
+    if (l < 0 || l >= (1ll << 31)) return fail(VR_E_INVALID, "row_ptr not monotone");
+    x->cnt_host[(size_t)i] = (int32_t)l;
+  }
+  x->nnz = n > 0 ? row_ptr[n] - row_ptr[0] : 0;
+  if (x->nnz > 0 && (!col || !coef)) return fail(VR_E_INVALID, "col/coef NULL");
+  for (int64_t k = 0; k < x->nnz; ++k)
+    if (col[row_ptr[0] + k] < 0 || col[row_ptr[0] + k] >= n)
+      return fail(VR_E_INVALID, "column index out of range");
+  TRY(finish_layout(x.get(), g_stream));
+  std::vector<int32_t> pc((size_t)x->stored);
+  std::vector<float> pf((size_t)x->stored);
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t s = x->rp_host[i], l = x->cnt_host[i], e = x->rp_host[i + 1];
+    for (int64_t k = 0; k < l; ++k) {
+      pc[s + k] = col[row_ptr[i] + k];
+      pf[s + k] = coef[row_ptr[i] + k];
+    }
+    for (int64_t k = s + l; k < e; ++k) {
+      pc[k] = (int32_t)i;
+      pf[k] = 0.0f;
+    }
+  }
+  TRY(upload_vec(x->col, pc, g_stream));
+  TRY(upload_vec(x->coef32, pf, g_stream));
+  VR_HIP(hipStreamSynchronize(g_stream));
+  vr_xfers *raw = x.release();
+  if (precision != VR_COEF_F32) {
+    int rc = vr_xfers_set_precision(raw, precision);
+    if (rc) {
+      delete raw;
+      return rc;
+    }
+  }
+  *out = raw;
+  return VR_OK;
+}
+
+int vr_xfers_synthetic(vr_xfers **out, uint64_t seed, int64_t n, double mean_len, double sigma,
+                       int64_t max_len, int32_t precision) {
+  LOCK;
+  if (!out) return fail(VR_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (n < 2 || mean_len < 1 || sigma < 0 || max_len < 1)
+    return fail(VR_E_INVALID, "bad synthetic parameters");
+  TRY(ensure_init());
+  std::unique_ptr<vr_xfers> x(new vr_xfers);
+  x->n = n;
+  const int64_t cap = std::min<int64_t>(max_len, n - 1);
+  uint64_t st = seed ^ 0x243F6A8885A308D3ull;
+  auto next = [&]() {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const double mu = std::log(mean_len) - 0.5 * sigma * sigma;
+  std::vector<double> raw((size_t)n);
+  double sum = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    double u1 = ((double)(next() >> 11) + 1.0) * 0x1p-53, u2 = (double)(next() >> 11) * 0x1p-53;
+    double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * 3.14159265358979323846 * u2);
+    double L = std::min<double>((double)cap, std::max(1.0, std::exp(mu + sigma * z)));
+    raw[(size_t)i] = L;
+    sum += L;
+  }
+  const double f = mean_len * (double)n / sum;
+  x->cnt_host.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i)
+    x->cnt_host[(size_t)i] =
+        (int32_t)std::min<double>((double)cap, std::max(1.0, std::floor(raw[(size_t)i] * f + 0.5)));
+  x->nnz = 0;
+  for (int32_t c : x->cnt_host) x->nnz += c;
+  TRY(finish_layout(x.get(), g_stream));
+  TRY(x->col.alloc(4 * (size_t)x->stored));
+  TRY(x->coef32.alloc(4 * (size_t)x->stored));
+  launch_synthetic_fill(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), seed, x->col.as<int32_t>(),
+                        x->coef32.as<float>(), g_stream);
+  TRY(check_launch());
+  VR_HIP(hipStreamSynchronize(g_stream));
+  vr_xfers *rx = x.release();
+  if (precision != VR_COEF_F32) {
+    int rc = vr_xfers_set_precision(rx, precision);
+    if (rc) {
+      delete rx;
+      return rc;
+    }
+  }
+  *out = rx;
+  return VR_OK;
+}
+
+int vr_xfers_set_precision(vr_xfers *x, int32_t precision) {
+  LOCK;
+  if (!x) return fail(VR_E_INVALID, "xfers is NULL");
+  if (precision != VR_COEF_F32 && precision != VR_COEF_F16 && precision != VR_COEF_BF16)
+    return fail(VR_E_INVALID, "unknown precision");
+  TRY(ensure_init());
+  if (precision != VR_COEF_F32) {
+    TRY(x->coef16.alloc(2 * (size_t)x->stored));
+    TRY(x->scale.alloc(4 * (size_t)std::max<int64_t>(x->n, 1)));
+    launch_encode16(x->n, x->rp.as<int64_t>(), x->coef32.as<float>(), x->coef16.as<uint16_t>(),
+                    x->scale.as<float>(), precision, g_stream);
+    TRY(check_launch());
+    VR_HIP(hipStreamSynchronize(g_stream));
+  }
+  x->precision = precision;
+  return VR_OK;
+}
+
+int vr_xfers_get_info(const vr_xfers *x, vr_xfers_info *out) {
+  if (!x || !out) return fail(VR_E_INVALID, "NULL argument");
+  std::memset(out, 0, sizeof *out);
+  out->n_rows = x->n;
+  out->nnz = x->nnz;
+  out->stored_nnz = x->stored;
+  out->precision = x->precision;
+  const int64_t cb = x->precision == VR_COEF_F32 ? 4 : 2;
+  out->stored_bytes = x->stored * (4 + cb) + 8 * (x->n + 1) + (x->precision ? 4 * x->n : 0);
+  out->pairs_considered = x->pairs;
+  out->segments_traced = x->segments;
+  return VR_OK;
+}
+
+int vr_xfers_download(const vr_xfers *x, int64_t *row_ptr, int32_t *col, float *coef,
+                      uint16_t *raw16, float *row_scale) {
+  LOCK;
+  if (!x) return fail(VR_E_INVALID, "xfers is NULL");
+  TRY(ensure_init());
+  const int64_t n = x->n, T = x->stored;
+  if (row_ptr) {
+    row_ptr[0] = 0;
+    for (int64_t i = 0; i < n; ++i) row_ptr[i + 1] = row_ptr[i] + x->cnt_host[(size_t)i];
+  }
+  std::vector<int32_t> pc;
+  std::vector<float> pf;
+  std::vector<uint16_t> ph;
+  if (col) {
+    pc.resize((size_t)T);
+    VR_HIP(hipMemcpyAsync(pc.data(), x->col.p, 4 * (size_t)T, hipMemcpyDeviceToHost, g_stream));
+  }
+  if (x->precision == VR_COEF_F32) {
+    if (coef) {
+      pf.resize((size_t)T);
+      VR_HIP(hipMemcpyAsync(pf.data(), x->coef32.p, 4 * (size_t)T, hipMemcpyDeviceToHost, g_stream));
+    }
+    if (raw16) return fail(VR_E_STATE, "raw 16-bit coefficients requested from FP32 transfers");
+  } else {
+    if (coef || raw16) {
+      ph.resize((size_t)T);
+      VR_HIP(hipMemcpyAsync(ph.data(), x->coef16.p, 2 * (size_t)T, hipMemcpyDeviceToHost, g_stream));
+    }
+    if (coef) {
+      DBuf tmp;
+      TRY(tmp.alloc(4 * (size_t)T));
+      launch_decode16(T, x->coef16.as<uint16_t>(), tmp.as<float>(), x->precision, g_stream);
+      TRY(check_launch());
+      pf.resize((size_t)T);
+      VR_HIP(hipMemcpyAsync(pf.data(), tmp.p, 4 * (size_t)T, hipMemcpyDeviceToHost, g_stream));
+      VR_HIP(hipStreamSynchronize(g_stream));
+    }
+    if (row_scale && n > 0)
+      VR_HIP(hipMemcpyAsync(row_scale, x->scale.p, 4 * (size_t)n, hipMemcpyDeviceToHost, g_stream));
+  }
+  VR_HIP(hipStreamSynchronize(g_stream));
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t s = x->rp_host[(size_t)i];
+    for (int64_t q = 0; q < x->cnt_host[(size_t)i]; ++q, ++k) {
+      if (col) col[k] = pc[(size_t)(s + q)];
+      if (coef) coef[k] = pf[(size_t)(s + q)];
+      if (raw16) raw16[k] = ph[(size_t)(s + q)];
+    }
+  }
+  return VR_OK;
+}
+
+int vr_xfers_destroy(vr_xfers *x) {
+  LOCK;
+  delete x;
+  return VR_OK;
+}
+
+// ---- direct light -----------------------------------------------------------
+int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n, const vr_light *lights,
+                           int32_t n_lights, float normal_offset, int32_t skip_id, float *rgb) {
+  LOCK;
+  if (!s || !smp) return fail(VR_E_INVALID, "NULL argument");
+  if (!s->built) return fail(VR_E_STATE, "scene not built");
+  if (n < 0 || n_lights < 0) return fail(VR_E_INVALID, "negative count");
+  if (n == 0) return VR_OK;
+  if (!rgb || (n_lights > 0 && !lights)) return fail(VR_E_INVALID, "NULL lights/output");
+  TRY(ensure_init());
+  const size_t fb = 4 * (size_t)n;
+  Staging S;
+  TRY(S.make({fb, fb, fb, fb, fb, fb, 3 * fb, sizeof(vr_light) * (size_t)std::max(n_lights, 1),
+              162 * 3 * 4}));
+  const float *src[6] = {smp->px, smp->py, smp->pz, smp->nx, smp->ny, smp->nz};
+  for (int k = 0; k < 6; ++k) {
+    if (!src[k]) return fail(VR_E_INVALID, "NULL sample array");
+    VR_HIP(hipMemcpyAsync(S.ptr<float>(k), src[k], fb, hipMemcpyHostToDevice, g_stream));
+  }
+  if (n_lights > 0)
+    VR_HIP(hipMemcpyAsync(S.ptr<vr_light>(7), lights, sizeof(vr_light) * (size_t)n_lights,
+                          hipMemcpyHostToDevice, g_stream));
+  std::vector<float> dirs = sky_dirs();
+  VR_HIP(hipMemcpyAsync(S.ptr<float>(8), dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice,
+                        g_stream));
+  DevSamples ds{S.ptr<float>(0), S.ptr<float>(1), S.ptr<float>(2),
+                S.ptr<float>(3), S.ptr<float>(4), S.ptr<float>(5)};
+  {
+    Prof p("gather_light", g_stream);
+    launch_gather_light(s->dev, ds, n, S.ptr<vr_light>(7), n_lights, S.ptr<float>(8),
+                        normal_offset, skip_id, S.ptr<float>(6), g_path, g_stream);
+  }
+  TRY(check_launch());
+  VR_HIP(hipMemcpyAsync(rgb, S.ptr<float>(6), 3 * fb, hipMemcpyDeviceToHost, g_stream));
+  VR_HIP(hipStreamSynchronize(g_stream));
+  return VR_OK;
+}
+
+// ---- bounce ------------------------------------------------------------------
+int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const float *area,
+                    float *total, const vr_bounce_params *prm, double *energy,
+                    int32_t *n_bounces) {
+  LOCK;
+  if (!x || !prm) return fail(VR_E_INVALID, "NULL argument");
+  const int64_t n = x->n;
+  const int K = prm->max_bounces;
+  if (K < 0) return fail(VR_E_INVALID, "max_bounces < 0");
+  if (n > 0 && (!emit0 || !refl || !area || !total)) return fail(VR_E_INVALID, "NULL array");
+  if (K > 0 && !energy) return fail(VR_E_INVALID, "energy is NULL");
+  if (g_path == VR_PATH_PLAIN && x->precision != VR_COEF_F32)
+    return fail(VR_E_UNSUPPORTED, "plain path gathers FP32 transfers only");
+  TRY(ensure_init());
+  hipStream_t st = g_stream;
+  const size_t vb = 12 * (size_t)std::max<int64_t>(n, 1);
+  int64_t scratch_n = 2;
+  for (int64_t m = n; m > 1; m = (m + 2047) / 2048) scratch_n += (m + 2047) / 2048;
+  Staging S;
+  TRY(S.make({vb, vb, vb, vb, 4 * (size_t)std::max<int64_t>(n, 1),
+              8 * (size_t)std::max<int64_t>(n, 1), 8 * (size_t)scratch_n,
+              8 * (size_t)std::max(K, 1), 4 * (size_t)std::max(K, 1), sizeof(BounceState)}));
+  float *em[2] = {S.ptr<float>(0), S.ptr<float>(1)};
+  float *d_total = S.ptr<float>(2), *d_refl = S.ptr<float>(3), *d_area = S.ptr<float>(4);
+  double *e_row = S.ptr<double>(5), *scr = S.ptr<double>(6), *d_energy = S.ptr<double>(7);
+  uint32_t *counters = S.ptr<uint32_t>(8);
+  BounceState *state = S.ptr<BounceState>(9);
+  if (n > 0) {
+    VR_HIP(hipMemcpyAsync(em[0], emit0, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+    VR_HIP(hipMemcpyAsync(d_total, total, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+    VR_HIP(hipMemcpyAsync(d_refl, refl, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+    VR_HIP(hipMemcpyAsync(d_area, area, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  }
+  VR_HIP(hipMemsetAsync(counters, 0, 4 * (size_t)std::max(K, 1), st));
+  VR_HIP(hipMemsetAsync(state, 0, sizeof(BounceState), st));
+  const DevXfers dx = x->dev();
+  BounceState hs{0.0, 0, 0};
+  const int chunk = g_path == VR_PATH_PLAIN ? 1 : 8;
+  int k = 0;
+  while (k < K) {
+    const int kend = std::min(K, k + chunk);
+    for (; k < kend; ++k) {
+      {
+        Prof p("bounce_gather", st);
+        launch_bounce(dx, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row, counters + k,
+                      state, g_path, st);
+      }
+      {
+        Prof p("energy", st);
+        launch_energy(e_row, n, scr, d_energy, k, prm->stop_frac, state, st);
+      }
+      TRY(check_launch());
+    }
+    VR_HIP(hipMemcpyAsync(&hs, state, sizeof hs, hipMemcpyDeviceToHost, st));
+    VR_HIP(hipStreamSynchronize(st));
+    if (hs.converged) break;
+  }
+  const int done = hs.k_done;
+  if (done > 0) VR_HIP(hipMemcpyAsync(energy, d_energy, 8 * (size_t)done, hipMemcpyDeviceToHost, st));
+  if (n > 0) VR_HIP(hipMemcpyAsync(total, d_total, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  if (n_bounces) *n_bounces = done;
+  return VR_OK;
+}
+
+// ---- procedural inputs ---------------------------------------------------------
+int vr_gen_room(vr_gen **out, uint64_t seed, int32_t n_pillars, float chop, float tess) {
+  if (!out) return fail(VR_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!(chop > 0) || !(tess > 0) || n_pillars < 0) return fail(VR_E_INVALID, "bad parameters");
+  std::unique_ptr<vr_gen> g(new vr_gen);
+  TRY(gen_room(seed, n_pillars, chop, tess, &g->g));
+  *out = g.release();
+  return VR_OK;
+}
+
+int vr_gen_level(vr_gen **out, uint64_t seed, int32_t rx, int32_t ry, float chop, float tess,
+                 int32_t n_lights, int32_t sun) {
+  if (!out) return fail(VR_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (rx < 1 || ry < 1 || !(chop > 0) || !(tess > 0) || n_lights < 0)
+    return fail(VR_E_INVALID, "bad parameters");
+  std::unique_ptr<vr_gen> g(new vr_gen);
+  TRY(gen_level(seed, rx, ry, chop, tess, n_lights, sun, &g->g));
+  *out = g.release();
+  return VR_OK;
+}
+
+int vr_gen_counts(const vr_gen *g, int64_t *n_tri, int64_t *n_patches, int32_t *n_lights) {
+  if (!g) return fail(VR_E_INVALID, "gen is NULL");
+  if (n_tri) *n_tri = (int64_t)g->g.ids.size();
+  if (n_patches) *n_patches = (int64_t)g->g.parea.size();
+  if (n_lights) *n_lights = (int32_t)g->g.lights.size();
+  return VR_OK;
+}
+
+int vr_gen_copy(const vr_gen *g, float *verts9, int32_t *ids, uint8_t *flags, float *po, float *pn,
+                float *pa, float *pr, vr_light *lights) {
+  if (!g) return fail(VR_E_INVALID, "gen is NULL");
+  const GenScene &s = g->g;
+  if (verts9) std::memcpy(verts9, s.v9.data(), 4 * s.v9.size());
+  if (ids) std::memcpy(ids, s.ids.data(), 4 * s.ids.size());
+  if (flags) std::memcpy(flags, s.flags.data(), s.flags.size());
+  if (po) std::memcpy(po, s.po.data(), 4 * s.po.size());
+  if (pn) std::memcpy(pn, s.pn.data(), 4 * s.pn.size());
+  if (pa) std::memcpy(pa, s.parea.data(), 4 * s.parea.size());
+  if (pr) std::memcpy(pr, s.prefl.data(), 4 * s.prefl.size());
+  if (lights) std::memcpy(lights, s.lights.data(), sizeof(vr_light) * s.lights.size());
+  return VR_OK;
+}
+
+int vr_gen_destroy(vr_gen *g) {
+  delete g;
+  return VR_OK;
+}
+
+}  // extern "C"

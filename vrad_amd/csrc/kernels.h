@@ -1,0 +1,82 @@
+// Kernel launchers (defined in the .hip files), called by capi.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/vrad_hip.h"
+#include "trace_dev.h"
+
+namespace vr {
+
+constexpr int kStackDepth = 24;  // > max kd depth (21 + 1 leaf level)
+
+struct DevRays {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+};
+struct DevSegs {
+  const float *sx, *sy, *sz, *ex, *ey, *ez;
+};
+struct DevPatches {
+  const float *ox, *oy, *oz, *nx, *ny, *nz, *area;
+};
+
+// ---- visibility (trace.hip) ----
+void launch_trace_closest(const DevScene &sc, const DevRays &r, int64_t n, int32_t skip,
+                          int32_t *hit, float *hit_t, float *hit_n, int path, hipStream_t st);
+void launch_test_line(const DevScene &sc, const DevSegs &s, int64_t n, int32_t skip,
+                      uint8_t *vis, uint8_t *sky, int path, hipStream_t st);
+
+// ---- transfers (transfers.hip) ----
+void launch_pair_cull(const DevPatches &p, int64_t np, int64_t i0, int64_t i1, int2 *pairs,
+                      uint32_t *count, uint32_t cap, hipStream_t st);
+void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pairs,
+                       const uint32_t *count, uint32_t cap, float eps, int32_t skip,
+                       uint32_t *bits, int64_t words, int path, hipStream_t st);
+void launch_row_count(const uint32_t *bits, int64_t np, int64_t words, int32_t *cnt,
+                      hipStream_t st);
+// exclusive scan of padded lengths (multiple of 8) into rp[0..n]
+void launch_row_scan(const int32_t *cnt, int64_t n, int64_t *rp, hipStream_t st);
+void launch_row_fill(const uint32_t *bits, int64_t np, int64_t words, const DevPatches &p,
+                     const int64_t *rp, const int32_t *cnt, int32_t *col, float *coef,
+                     hipStream_t st);
+void launch_row_normalize(int64_t n, const int64_t *rp, float *coef, double *row_sum,
+                          int normalize, hipStream_t st);
+
+// ---- bounce (bounce.hip) ----
+struct DevXfers {
+  int64_t n;
+  const int64_t *rp;       // padded row starts (multiple of 8), n + 1
+  const int32_t *col;
+  const void *coef;        // float or uint16 (f16 / bf16)
+  const float *scale;      // per-row scale for 16-bit storage
+  const int32_t *order;    // rows sorted by length, longest first
+  int kind;                // VR_COEF_*
+};
+struct BounceState {       // device-resident bounce bookkeeping
+  double cum;
+  int32_t converged;
+  int32_t k_done;
+};
+void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, float *total,
+                   const float *refl, const float *area, double *e_row, uint32_t *row_counter,
+                   const BounceState *state, int path, hipStream_t st);
+// energy of the bounce just gathered; updates state / energy[k]
+void launch_energy(const double *e_row, int64_t n, double *scratch, double *energy, int k,
+                   double stop_frac, BounceState *state, hipStream_t st);
+void launch_synthetic_fill(int64_t n, const int64_t *rp, const int32_t *len, uint64_t seed,
+                           int32_t *col, float *coef, hipStream_t st);
+void launch_encode16(int64_t n, const int64_t *rp, const float *coef32, uint16_t *coef16,
+                     float *scale, int kind, hipStream_t st);
+void launch_decode16(int64_t total, const uint16_t *coef16, float *out, int kind,
+                     hipStream_t st);
+
+// ---- direct light (light.hip) ----
+struct DevSamples {
+  const float *px, *py, *pz, *nx, *ny, *nz;
+};
+void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
+                         const vr_light *lights, int32_t n_lights, const float *sky_dirs,
+                         float eps, int32_t skip, float *rgb, int path, hipStream_t st);
+
+}  // namespace vr

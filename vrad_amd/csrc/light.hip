@@ -1,0 +1,146 @@
+// GatherSampleLight: direct light at sample points from point, spot, surface,
+// sun and sky-ambient lights, each term gated by TestLine visibility.  The
+// per-light formulas and their operation order are those of the CPU
+// restatement; lights are summed in list order.
+#include "kernels.h"
+
+namespace vr {
+
+#define VR_MAX_TRACE_LENGTH_F ((float)(1.732050807569 * 32768.0))
+
+template <int D, class Stack>
+__device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, float pz, float nx,
+                           float ny, float nz, const vr_light *__restrict__ lights,
+                           int32_t n_lights, const float *__restrict__ dirs, float eps,
+                           int32_t skip, float &r, float &g, float &b) {
+  const float sx = px + eps * nx, sy = py + eps * ny, sz = pz + eps * nz;
+  const float MTL = VR_MAX_TRACE_LENGTH_F;
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+  bool sky;
+  for (int32_t li = 0; li < n_lights; ++li) {
+    const vr_light &L = lights[li];
+    float scale = 0.0f;
+    if (L.type == VR_EMIT_SURFACE) {
+      const float lx = L.origin[0] - px, ly = L.origin[1] - py, lz = L.origin[2] - pz;
+      const float dist2 = (lx * lx + ly * ly) + lz * lz;
+      if (!(dist2 > 0.0f)) continue;
+      const float dist = __builtin_sqrtf(dist2);
+      const float inv = 1.0f / dist;
+      const float ux = lx * inv, uy = ly * inv, uz = lz * inv;
+      const float dot = (nx * ux + ny * uy) + nz * uz;
+      if (!(dot > 0.0f)) continue;
+      const float dot2 = -((L.normal[0] * ux + L.normal[1] * uy) + L.normal[2] * uz);
+      if (!(dot2 > 0.0f)) continue;
+      if (!test_line<D>(sc, st, sx, sy, sz, L.origin[0], L.origin[1], L.origin[2], skip, sky))
+        continue;
+      scale = (dot * dot2) / dist2;
+    } else if (L.type == VR_EMIT_POINT || L.type == VR_EMIT_SPOTLIGHT) {
+      const float lx = L.origin[0] - px, ly = L.origin[1] - py, lz = L.origin[2] - pz;
+      const float dist2 = (lx * lx + ly * ly) + lz * lz;
+      if (!(dist2 > 0.0f)) continue;
+      const float dist = __builtin_sqrtf(dist2);
+      const float inv = 1.0f / dist;
+      const float ux = lx * inv, uy = ly * inv, uz = lz * inv;
+      const float dot = (nx * ux + ny * uy) + nz * uz;
+      if (!(dot > 0.0f)) continue;
+      const float fd = dist < L.cap_distance ? dist : L.cap_distance;
+      const float den = (L.constant_attn + L.linear_attn * fd) + L.quadratic_attn * (fd * fd);
+      if (!(den > 0.0f)) continue;
+      float fall = 1.0f / den;
+      if (L.type == VR_EMIT_SPOTLIGHT) {
+        const float dot2 = -((L.normal[0] * ux + L.normal[1] * uy) + L.normal[2] * uz);
+        if (!(dot2 > L.stopdot2)) continue;
+        if (dot2 <= L.stopdot) fall = fall * ((dot2 - L.stopdot2) / (L.stopdot - L.stopdot2));
+        if (L.exponent != 0.0f && L.exponent != 1.0f) fall = fall * powf(dot2, L.exponent);
+      }
+      if (L.end_fade > L.start_fade) {
+        if (dist >= L.end_fade) continue;
+        if (dist > L.start_fade) {
+          const float x = (dist - L.start_fade) / (L.end_fade - L.start_fade);
+          const float sm = (x * x) * (3.0f - 2.0f * x);
+          fall = fall * (1.0f - sm);
+        }
+      }
+      if (!test_line<D>(sc, st, sx, sy, sz, L.origin[0], L.origin[1], L.origin[2], skip, sky))
+        continue;
+      scale = dot * fall;
+    } else if (L.type == VR_EMIT_SKYLIGHT) {
+      const float dot = -((nx * L.normal[0] + ny * L.normal[1]) + nz * L.normal[2]);
+      if (!(dot > 0.0f)) continue;
+      const float ex = sx - MTL * L.normal[0], ey = sy - MTL * L.normal[1],
+                  ez = sz - MTL * L.normal[2];
+      if (!test_line<D>(sc, st, sx, sy, sz, ex, ey, ez, skip, sky)) continue;
+      scale = dot;
+    } else if (L.type == VR_EMIT_SKYAMBIENT) {
+      float a = 0.0f, w = 0.0f;
+      for (int k = 0; k < 162; ++k) {
+        const float dx = dirs[3 * k], dy = dirs[3 * k + 1], dz = dirs[3 * k + 2];
+        const float ck = (nx * dx + ny * dy) + nz * dz;
+        if (!(ck > 0.0f)) continue;
+        w = w + ck;
+        const float ex = sx + MTL * dx, ey = sy + MTL * dy, ez = sz + MTL * dz;
+        if (test_line<D>(sc, st, sx, sy, sz, ex, ey, ez, skip, sky)) a = a + ck;
+      }
+      if (!(w > 0.0f)) continue;
+      scale = a / w;
+    } else {
+      continue;
+    }
+    ar = ar + L.intensity[0] * scale;
+    ag = ag + L.intensity[1] * scale;
+    ab = ab + L.intensity[2] * scale;
+  }
+  r = ar;
+  g = ag;
+  b = ab;
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void k_gather_lds(DevScene sc, DevSamples s, int64_t n,
+                                                   const vr_light *__restrict__ lights,
+                                                   int32_t n_lights, const float *__restrict__ dirs,
+                                                   float eps, int32_t skip,
+                                                   float *__restrict__ rgb) {
+  __shared__ uint32_t s_node[D * 64];
+  __shared__ float2 s_t[D * 64];
+  const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  LdsStack st{s_node, s_t, (int)threadIdx.x};
+  float r, g, b;
+  gather_one<D>(sc, st, s.px[i], s.py[i], s.pz[i], s.nx[i], s.ny[i], s.nz[i], lights, n_lights,
+                dirs, eps, skip, r, g, b);
+  rgb[3 * i + 0] = r;
+  rgb[3 * i + 1] = g;
+  rgb[3 * i + 2] = b;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_gather_plain(DevScene sc, DevSamples s, int64_t n,
+                                                      const vr_light *__restrict__ lights,
+                                                      int32_t n_lights,
+                                                      const float *__restrict__ dirs, float eps,
+                                                      int32_t skip, float *__restrict__ rgb) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  ScratchStack<D> st;
+  float r, g, b;
+  gather_one<D>(sc, st, s.px[i], s.py[i], s.pz[i], s.nx[i], s.ny[i], s.nz[i], lights, n_lights,
+                dirs, eps, skip, r, g, b);
+  rgb[3 * i + 0] = r;
+  rgb[3 * i + 1] = g;
+  rgb[3 * i + 2] = b;
+}
+
+void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
+                         const vr_light *lights, int32_t n_lights, const float *sky_dirs,
+                         float eps, int32_t skip, float *rgb, int path, hipStream_t st) {
+  if (n <= 0) return;
+  if (path == VR_PATH_PLAIN)
+    hipLaunchKernelGGL(k_gather_plain<kStackDepth>, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                       0, st, sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb);
+  else
+    hipLaunchKernelGGL(k_gather_lds<kStackDepth>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
+                       sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb);
+}
+
+}  // namespace vr

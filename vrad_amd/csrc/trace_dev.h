@@ -1,0 +1,214 @@
+// Device-side kd-tree traversal and triangle test shared by the visibility,
+// transfer and direct-light kernels.  gfx950, wave64.
+//
+// Built with -ffp-contract=off: the triangle test below rounds once per
+// operation in the same association as the CPU restatement, so hit index and
+// hit t are bit-identical to it.  Traversal arithmetic only culls, with a
+// conservative slab margin, so its rounding never changes a result.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vr {
+
+struct DevScene {
+  const uint2 *nodes;     // x = children ((index << 2) | type), y = split bits
+  const float4 *refs;     // 3 float4 per leaf reference (leaf order)
+  const float4 *planes;   // per triangle (nx, ny, nz, d), original order
+  float bmin[3], bmax[3];
+  float M;                // max(1, max |scene coordinate|)
+  int has_sky;
+  int empty;              // no triangles
+};
+
+constexpr float kTiny = 1e-20f;
+constexpr uint32_t kTriMask = (1u << 28) - 1u;
+
+__device__ __forceinline__ float sel3(int c, float a, float b, float d) {
+  return c == 0 ? a : (c == 1 ? b : d);
+}
+
+// Projected-barycentric hit test (frozen semantics of Trace4Rays, which is a
+// stub at raytracer/environment.go:140-145): accept t in [tmin, tbest),
+// equal t resolved to the lower triangle index.
+__device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32_t k, float ox,
+                                         float oy, float oz, float dx, float dy, float dz,
+                                         float tmin, int32_t skip, float &tbest, int32_t &best,
+                                         int32_t &best_id) {
+  const float4 q0 = refs[3 * (size_t)k + 0];
+  const float4 q1 = refs[3 * (size_t)k + 1];
+  const float4 q2 = refs[3 * (size_t)k + 2];
+  const int32_t id = __float_as_int(q2.z);
+  if (id == skip) return;
+  const uint32_t meta = __float_as_uint(q2.w);
+  const int32_t tri = (int32_t)(meta & kTriMask);
+  const float den = (q0.x * dx + q0.y * dy) + q0.z * dz;
+  if (!(den != 0.0f)) return;
+  const float num = q0.w - ((q0.x * ox + q0.y * oy) + q0.z * oz);
+  const float t = num / den;
+  if (!(t >= tmin && (t < tbest || (t == tbest && tri < best)))) return;
+  const int c0 = (int)((meta >> 28) & 3u), c1 = (int)(meta >> 30);
+  const float h0 = sel3(c0, ox, oy, oz) + t * sel3(c0, dx, dy, dz);
+  const float h1 = sel3(c1, ox, oy, oz) + t * sel3(c1, dx, dy, dz);
+  const float b0 = (q1.x * h0 + q1.y * h1) + q1.z;
+  const float b1 = (q1.w * h0 + q2.x * h1) + q2.y;
+  if (b0 >= 0.0f && b1 >= 0.0f && (b0 + b1) <= 1.0f) {
+    tbest = t;
+    best = tri;
+    best_id = id;
+  }
+}
+
+// Per-lane traversal stack in LDS: entry e of lane l at [e * 64 + l].
+struct LdsStack {
+  uint32_t *node;
+  float2 *t;
+  int lane;
+  __device__ void push(int sp, uint32_t n, float a, float b) {
+    node[sp * 64 + lane] = n;
+    t[sp * 64 + lane] = make_float2(a, b);
+  }
+  __device__ void get(int sp, uint32_t &n, float &a, float &b) const {
+    n = node[sp * 64 + lane];
+    float2 v = t[sp * 64 + lane];
+    a = v.x;
+    b = v.y;
+  }
+};
+
+// Per-thread stack in private (scratch) memory: the plain path.
+template <int D>
+struct ScratchStack {
+  uint32_t node[D];
+  float a[D], b[D];
+  __device__ void push(int sp, uint32_t n, float x, float y) {
+    node[sp] = n;
+    a[sp] = x;
+    b[sp] = y;
+  }
+  __device__ void get(int sp, uint32_t &n, float &x, float &y) const {
+    n = node[sp];
+    x = a[sp];
+    y = b[sp];
+  }
+};
+
+// Closest hit in [tmin, tmax) (ANY: stop at the first hit).  D = stack
+// capacity; the host guarantees tree depth < D.
+template <int D, bool ANY, class Stack>
+__device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, float oz, float dx,
+                          float dy, float dz, float tmin, float tmax, int32_t skip, float &t_out,
+                          int32_t &best_out, int32_t &id_out) {
+  float tbest = tmax;
+  int32_t best = -1, best_id = 0;
+  t_out = 0.0f;
+  best_out = -1;
+  id_out = 0;
+  if (sc.empty) return;
+  const float om = fmaxf(fabsf(ox), fmaxf(fabsf(oy), fabsf(oz)));
+  const float dl = (sc.M + om) * 0x1p-15f;
+  const float ix = 1.0f / (fabsf(dx) < kTiny ? copysignf(kTiny, dx) : dx);
+  const float iy = 1.0f / (fabsf(dy) < kTiny ? copysignf(kTiny, dy) : dy);
+  const float iz = 1.0f / (fabsf(dz) < kTiny ? copysignf(kTiny, dz) : dz);
+  float ta = tmin, tb = tmax;
+  {
+    float t1 = ((sc.bmin[0] - dl) - ox) * ix, t2 = ((sc.bmax[0] + dl) - ox) * ix;
+    ta = fmaxf(ta, fminf(t1, t2));
+    tb = fminf(tb, fmaxf(t1, t2));
+    t1 = ((sc.bmin[1] - dl) - oy) * iy;
+    t2 = ((sc.bmax[1] + dl) - oy) * iy;
+    ta = fmaxf(ta, fminf(t1, t2));
+    tb = fminf(tb, fmaxf(t1, t2));
+    t1 = ((sc.bmin[2] - dl) - oz) * iz;
+    t2 = ((sc.bmax[2] + dl) - oz) * iz;
+    ta = fmaxf(ta, fminf(t1, t2));
+    tb = fminf(tb, fmaxf(t1, t2));
+  }
+  if (!(ta <= tb)) return;
+  int sp = 0;
+  uint32_t node = 0;
+  for (;;) {
+    const uint2 nd = sc.nodes[node];
+    const uint32_t type = nd.x & 3u;
+    if (type != 3u) {
+      const float sv = __uint_as_float(nd.y);
+      const float o_a = sel3((int)type, ox, oy, oz);
+      const float i_a = sel3((int)type, ix, iy, iz);
+      const float lo = ((sv - dl) - o_a) * i_a;
+      const float hi = ((sv + dl) - o_a) * i_a;
+      const uint32_t left = nd.x >> 2;
+      const bool fwd = i_a >= 0.0f;
+      const uint32_t near_n = fwd ? left : left + 1, far_n = fwd ? left + 1 : left;
+      const float near_end = fwd ? hi : lo, far_start = fwd ? lo : hi;
+      const float tbe = fminf(tb, tbest);
+      const bool near_ok = ta <= fminf(near_end, tbe);
+      const float fs = fmaxf(far_start, ta);
+      const bool far_ok = fs <= tbe;
+      if (near_ok) {
+        if (far_ok && sp < D) {  // depth < D is checked on the host
+          st.push(sp, far_n, fs, tb);
+          ++sp;
+        }
+        node = near_n;
+        tb = fminf(tb, near_end);
+        continue;
+      }
+      if (far_ok) {
+        node = far_n;
+        ta = fs;
+        continue;
+      }
+    } else {
+      const uint32_t start = nd.x >> 2;
+      const uint32_t cnt = (uint32_t)__uint_as_float(nd.y);
+      for (uint32_t k = start; k < start + cnt; ++k)
+        tri_test(sc.refs, k, ox, oy, oz, dx, dy, dz, tmin, skip, tbest, best, best_id);
+      if (ANY && best >= 0) break;
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      float a, b;
+      uint32_t n;
+      st.get(sp, n, a, b);
+      if (a > tbest) continue;
+      node = n;
+      ta = a;
+      tb = b;
+      found = true;
+      break;
+    }
+    if (!found) break;
+  }
+  if (best >= 0) {
+    t_out = tbest;
+    best_out = best;
+    id_out = best_id;
+  }
+}
+
+// TestLineDoesHitSky for one segment (raytracer/trace/testline.go:22-51):
+// dir = delta * (1/len), trace [0, len), occluded iff the closest hit is not
+// a TRACE_ID_SKY triangle.  Returns visible; *sky = closest hit is sky.
+template <int D, class Stack>
+__device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float sx, float sy,
+                                          float sz, float ex, float ey, float ez, int32_t skip,
+                                          bool &sky) {
+  const float vx = ex - sx, vy = ey - sy, vz = ez - sz;
+  const float len2 = (vx * vx + vy * vy) + vz * vz;
+  const float len = __builtin_sqrtf(len2);
+  const float inv = 1.0f / len;
+  const float dx = vx * inv, dy = vy * inv, dz = vz * inv;
+  float t;
+  int32_t best, id;
+  if (sc.has_sky)
+    trace_ray<D, false>(sc, st, sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
+  else
+    trace_ray<D, true>(sc, st, sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
+  const bool hit = best >= 0 && t < len;
+  sky = hit && (id & 0x01000000) != 0;
+  return !(hit && !sky);
+}
+
+}  // namespace vr

@@ -1,0 +1,251 @@
+// MakeTransfers: patch-pair visibility batches and CSR construction.
+//
+//  1. k_pair_cull   candidate pairs (i, j > i) of a row range that pass the
+//                   facing cull, compacted with a wave ballot + one atomic.
+//  2. k_pair_trace  TestLine(o_i + eps n_i -> o_j + eps n_j) per candidate;
+//                   visible pairs set bits (i,j) and (j,i) of an N x N
+//                   bit matrix (N^2/8 bytes: 10 GB at 280k patches, which an
+//                   MI355X holds with room to spare).
+//  3. k_row_count   popcount per row; k_row_scan pads lengths to 8 and scans.
+//  4. k_row_fill    columns in ascending order, F_ij in FP32.
+//  5. k_row_norm    FP64 row sum in the bounce-kernel wave order (G = 4);
+//                   rows with sum > 1 are scaled by 1/sum.
+#include "kernels.h"
+
+namespace vr {
+
+#define VR_PI_F 3.14159265358979323846f
+
+__global__ __launch_bounds__(256) void k_pair_cull(DevPatches p, int64_t np, int64_t i0,
+                                                   int64_t bpr, int2 *__restrict__ pairs,
+                                                   uint32_t *__restrict__ count, uint32_t cap) {
+  const int64_t row = i0 + (int64_t)blockIdx.x / bpr;
+  const int64_t j = row + 1 + ((int64_t)blockIdx.x % bpr) * 256 + threadIdx.x;
+  bool keep = false;
+  if (j < np) {
+    const float dx = p.ox[j] - p.ox[row], dy = p.oy[j] - p.oy[row], dz = p.oz[j] - p.oz[row];
+    const float di = (p.nx[row] * dx + p.ny[row] * dy) + p.nz[row] * dz;
+    const float dj = -((p.nx[j] * dx + p.ny[j] * dy) + p.nz[j] * dz);
+    keep = di > 0.0f && dj > 0.0f;
+  }
+  const uint64_t m = __ballot(keep);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, 0);
+  if (keep) {
+    const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (slot < cap) pairs[slot] = make_int2((int)row, (int)j);
+  }
+}
+
+void launch_pair_cull(const DevPatches &p, int64_t np, int64_t i0, int64_t i1, int2 *pairs,
+                      uint32_t *count, uint32_t cap, hipStream_t st) {
+  if (i1 <= i0) return;
+  const int64_t maxlen = np - 1 - i0;
+  if (maxlen <= 0) return;
+  const int64_t bpr = (maxlen + 255) / 256;
+  hipLaunchKernelGGL(k_pair_cull, dim3((unsigned)((i1 - i0) * bpr)), dim3(256), 0, st, p, np, i0,
+                     bpr, pairs, count, cap);
+}
+
+__device__ __forceinline__ void set_pair_bits(uint32_t *bits, int64_t words, int i, int j) {
+  atomicOr(bits + (int64_t)i * words + (j >> 5), 1u << (j & 31));
+  atomicOr(bits + (int64_t)j * words + (i >> 5), 1u << (i & 31));
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p,
+                                                       const int2 *__restrict__ pairs,
+                                                       const uint32_t *__restrict__ count,
+                                                       uint32_t cap, float eps, int32_t skip,
+                                                       uint32_t *__restrict__ bits,
+                                                       int64_t words) {
+  __shared__ uint32_t s_node[D * 64];
+  __shared__ float2 s_t[D * 64];
+  const uint32_t n = min(*count, cap);
+  const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= n) return;
+  const int2 ij = pairs[k];
+  const int i = ij.x, j = ij.y;
+  LdsStack st{s_node, s_t, (int)threadIdx.x};
+  bool sky;
+  const bool vis = test_line<D>(sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
+                                p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
+                                p.oy[j] + eps * p.ny[j], p.oz[j] + eps * p.nz[j], skip, sky);
+  if (vis) set_pair_bits(bits, words, i, j);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_pair_trace_plain(DevScene sc, DevPatches p,
+                                                          const int2 *__restrict__ pairs,
+                                                          const uint32_t *__restrict__ count,
+                                                          uint32_t cap, float eps, int32_t skip,
+                                                          uint32_t *__restrict__ bits,
+                                                          int64_t words) {
+  const uint32_t n = min(*count, cap);
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int2 ij = pairs[k];
+  const int i = ij.x, j = ij.y;
+  ScratchStack<D> st;
+  bool sky;
+  const bool vis = test_line<D>(sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
+                                p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
+                                p.oy[j] + eps * p.ny[j], p.oz[j] + eps * p.nz[j], skip, sky);
+  if (vis) set_pair_bits(bits, words, i, j);
+}
+
+void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pairs,
+                       const uint32_t *count, uint32_t cap, float eps, int32_t skip,
+                       uint32_t *bits, int64_t words, int path, hipStream_t st) {
+  if (cap == 0) return;
+  if (path == VR_PATH_PLAIN)
+    hipLaunchKernelGGL(k_pair_trace_plain<kStackDepth>, dim3((cap + 255) / 256), dim3(256), 0, st,
+                       sc, p, pairs, count, cap, eps, skip, bits, words);
+  else
+    hipLaunchKernelGGL(k_pair_trace_lds<kStackDepth>, dim3((cap + 63) / 64), dim3(64), 0, st, sc,
+                       p, pairs, count, cap, eps, skip, bits, words);
+}
+
+// one wave per row
+__global__ __launch_bounds__(256) void k_row_count(const uint32_t *__restrict__ bits, int64_t np,
+                                                   int64_t words, int32_t *__restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  if (row >= np) return;
+  const uint32_t *rb = bits + row * words;
+  int c = 0;
+  for (int64_t w = lane; w < words; w += 64) c += __popc(rb[w]);
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+  if (lane == 0) cnt[row] = c;
+}
+
+void launch_row_count(const uint32_t *bits, int64_t np, int64_t words, int32_t *cnt,
+                      hipStream_t st) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_row_count, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, bits, np,
+                     words, cnt);
+}
+
+// single-block exclusive scan of padded lengths: rp[i] = sum_{k<i} pad8(cnt[k])
+__global__ __launch_bounds__(1024) void k_row_scan(const int32_t *__restrict__ cnt, int64_t n,
+                                                   int64_t *__restrict__ rp) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t b = t * per, e = min(n, b + per);
+  int64_t s = 0;
+  for (int64_t k = b; k < e; ++k) s += ((int64_t)cnt[k] + 7) & ~(int64_t)7;
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t run = 0;
+    for (int k = 0; k < 1024; ++k) {
+      int64_t v = part[k];
+      part[k] = run;
+      run += v;
+    }
+    rp[n] = run;
+  }
+  __syncthreads();
+  int64_t run = part[t];
+  for (int64_t k = b; k < e; ++k) {
+    rp[k] = run;
+    run += ((int64_t)cnt[k] + 7) & ~(int64_t)7;
+  }
+}
+
+void launch_row_scan(const int32_t *cnt, int64_t n, int64_t *rp, hipStream_t st) {
+  hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, cnt, n, rp);
+}
+
+__device__ __forceinline__ float form_factor(const DevPatches &p, int64_t i, int64_t j) {
+  const float dx = p.ox[j] - p.ox[i], dy = p.oy[j] - p.oy[i], dz = p.oz[j] - p.oz[i];
+  const float di = (p.nx[i] * dx + p.ny[i] * dy) + p.nz[i] * dz;
+  const float dj = -((p.nx[j] * dx + p.ny[j] * dy) + p.nz[j] * dz);
+  const float r2 = (dx * dx + dy * dy) + dz * dz;
+  return ((di * dj) * p.area[j]) / ((VR_PI_F * r2) * r2);
+}
+
+// one wave per row: ascending columns from the bit matrix
+__global__ __launch_bounds__(256) void k_row_fill(const uint32_t *__restrict__ bits, int64_t np,
+                                                  int64_t words, DevPatches p,
+                                                  const int64_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ cnt,
+                                                  int32_t *__restrict__ col,
+                                                  float *__restrict__ coef) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  if (row >= np) return;
+  const uint32_t *rb = bits + row * words;
+  const int64_t s = rp[row];
+  int64_t run = 0;
+  for (int64_t w0 = 0; w0 < words; w0 += 64) {
+    const int64_t w = w0 + lane;
+    uint32_t m = w < words ? rb[w] : 0u;
+    int pc = __popc(m);
+    // inclusive wave scan of pc
+    int inc = pc;
+    for (int off = 1; off < 64; off <<= 1) {
+      int v = __shfl_up(inc, off);
+      if (lane >= off) inc += v;
+    }
+    const int tot = __shfl(inc, 63);
+    int64_t pos = s + run + (inc - pc);
+    while (m) {
+      const int bit = __ffs(m) - 1;
+      m &= m - 1u;
+      const int64_t j = w * 32 + bit;
+      col[pos] = (int32_t)j;
+      coef[pos] = form_factor(p, row, j);
+      ++pos;
+    }
+    run += tot;
+  }
+  const int64_t c = cnt[row], e = rp[row + 1];
+  for (int64_t k = s + c + lane; k < e; k += 64) {
+    col[k] = (int32_t)row;
+    coef[k] = 0.0f;
+  }
+}
+
+void launch_row_fill(const uint32_t *bits, int64_t np, int64_t words, const DevPatches &p,
+                     const int64_t *rp, const int32_t *cnt, int32_t *col, float *coef,
+                     hipStream_t st) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_row_fill, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, bits, np, words,
+                     p, rp, cnt, col, coef);
+}
+
+// FP64 row sum in wave order (lane owns p with (p/4)%64 == lane), butterfly
+__global__ __launch_bounds__(256) void k_row_norm(int64_t n, const int64_t *__restrict__ rp,
+                                                  float *__restrict__ coef,
+                                                  double *__restrict__ row_sum, int normalize) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  if (row >= n) return;
+  const int64_t s = rp[row], e = rp[row + 1];
+  double a = 0.0;
+  for (int64_t k = s + 4 * lane; k < e; k += 256) {
+    const float4 w = *reinterpret_cast<const float4 *>(coef + k);
+    a = a + (double)w.x;
+    a = a + (double)w.y;
+    a = a + (double)w.z;
+    a = a + (double)w.w;
+  }
+  for (int off = 32; off >= 1; off >>= 1) a = a + __shfl_xor(a, off);
+  if (lane == 0) row_sum[row] = a;
+  if (normalize && a > 1.0)
+    for (int64_t k = s + lane; k < e; k += 64) coef[k] = (float)((double)coef[k] / a);
+}
+
+void launch_row_normalize(int64_t n, const int64_t *rp, float *coef, double *row_sum,
+                          int normalize, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_row_norm, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, n, rp, coef,
+                     row_sum, normalize);
+}
+
+}  // namespace vr

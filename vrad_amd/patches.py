@@ -1,0 +1,97 @@
+"""rad/patches on the MI355X path: MakeTransfers.
+
+The reference declares the result records (Transfer, common/types/transfer.go
+:3-6; Patch.NumTransfers / Transfers, common/types/patch.go:60-61) but never
+builds them.  Here the transfers stay resident on the GPU as a padded CSR
+(Transfers), with download() for inspection.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr, fptr, f32
+
+ON_EPSILON = 0.1  # vmath/constants.go:8
+
+
+class Transfers:
+    def __init__(self, handle):
+        self._h = handle
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        inf = _lib.XfersInfo()
+        check(lib.vr_xfers_get_info(self._h, C.byref(inf)))
+        return inf
+
+    def set_precision(self, precision):
+        check(lib.vr_xfers_set_precision(self._h, int(precision)))
+
+    def download(self, raw16=False):
+        """Logical CSR: (row_ptr int64, col int32, coef float32[, raw uint16,
+        row_scale float32]).  16-bit coefficients come back decoded and
+        without the row scale."""
+        inf = self.info()
+        rp = np.zeros(inf.n_rows + 1, np.int64)
+        col = np.zeros(inf.nnz, np.int32)
+        coef = np.zeros(inf.nnz, np.float32)
+        if inf.precision != _lib.VR_COEF_F32:
+            raw = np.zeros(inf.nnz, np.uint16)
+            sc = np.zeros(inf.n_rows, np.float32)
+            check(lib.vr_xfers_download(self._h, ptr(rp), ptr(col), ptr(coef), ptr(raw), ptr(sc)))
+            return (rp, col, coef, raw, sc) if raw16 else (rp, col, coef)
+        check(lib.vr_xfers_download(self._h, ptr(rp), ptr(col), ptr(coef), None, None))
+        return rp, col, coef
+
+    def close(self):
+        if self._h is not None:
+            lib.vr_xfers_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def from_csr(row_ptr, col, coef, precision=_lib.VR_COEF_F32):
+        rp = np.ascontiguousarray(row_ptr, np.int64)
+        c = np.ascontiguousarray(col, np.int32)
+        w = f32(coef)
+        h = C.c_void_p()
+        check(lib.vr_xfers_from_csr(C.byref(h), len(rp) - 1, ptr(rp), ptr(c), ptr(w),
+                                    int(precision)))
+        return Transfers(h)
+
+    @staticmethod
+    def synthetic(seed, n_rows, mean_len=2000.0, sigma=1.0, max_len=40000,
+                  precision=_lib.VR_COEF_F32):
+        h = C.c_void_p()
+        check(lib.vr_xfers_synthetic(C.byref(h), int(seed), int(n_rows), float(mean_len),
+                                     float(sigma), int(max_len), int(precision)))
+        return Transfers(h)
+
+
+def patch_struct(origin, normal, area):
+    o = f32(origin).reshape(-1, 3)
+    nrm = f32(normal).reshape(-1, 3)
+    a = f32(area).reshape(-1)
+    cols = [np.ascontiguousarray(o[:, k]) for k in range(3)] + \
+           [np.ascontiguousarray(nrm[:, k]) for k in range(3)] + [a]
+    return _lib.Patches(*[fptr(c) for c in cols]), cols
+
+
+def MakeTransfers(env, origin, normal, area, normal_offset=ON_EPSILON, skip_id=-1,
+                  normalize=True, batch_pairs=0):
+    """Leaf patches (origin (n,3), normal (n,3), area (n,)) -> Transfers."""
+    p, keep = patch_struct(origin, normal, area)
+    prm = _lib.XferParams(float(normal_offset), int(skip_id), 1 if normalize else 0,
+                          int(batch_pairs))
+    h = C.c_void_p()
+    check(lib.vr_make_transfers(env.handle, C.byref(p), len(keep[-1]), C.byref(prm), C.byref(h)))
+    return Transfers(h)
