@@ -33,7 +33,8 @@ def test_room_direct_light_bitexact(room_scene, path):
     kd = oracle.KdTree(s.verts9)
     want = oracle.gather_light(kd, tri, s.p_origin, s.p_normal, s.lights, threads=8)
     assert np.array_equal(rgb.view(np.uint32), want.view(np.uint32))
-    assert (rgb[:, 0] > 0).mean() > 0.5
+    lit = (rgb[:, 0] > 0).mean()          # one low light among 40 full-height pillars
+    assert 0.05 < lit < 0.95
 
 
 def test_closed_forms_and_light_kinds():

@@ -55,6 +55,15 @@ __device__ __forceinline__ void gather_chunk(const int32_t *__restrict__ col, co
   }
 }
 
+// One counter increment per wave, returned to every lane as a scalar.  All
+// lanes issue the atomic (lane 0 adds 1, the others 0): a lane-0-only branch
+// in the loop header lets the optimiser send the other lanes straight back to
+// the broadcast, where they would read a stale value and never leave the loop.
+__device__ __forceinline__ uint32_t wave_fetch_add(uint32_t *ctr, int lane) {
+  const uint32_t v = atomicAdd(ctr, lane == 0 ? 1u : 0u);
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(256) void k_bounce_wave(DevXfers x, const float *__restrict__ emit_in,
                                                      float *__restrict__ emit_out,
@@ -68,9 +77,7 @@ __global__ __launch_bounds__(256) void k_bounce_wave(DevXfers x, const float *__
   constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
   const int lane = threadIdx.x & 63;
   for (;;) {
-    uint32_t slot = 0;
-    if (lane == 0) slot = atomicAdd(row_counter, 1u);
-    slot = __shfl(slot, 0);
+    const uint32_t slot = wave_fetch_add(row_counter, lane);
     if ((int64_t)slot >= x.n) break;
     const int64_t i = x.order[slot];
     const int64_t s = x.rp[i], e = x.rp[i + 1];
@@ -226,8 +233,9 @@ __global__ __launch_bounds__(256) void k_synthetic_fill(int64_t n, const int64_t
                                                         uint64_t seed, int32_t *__restrict__ col,
                                                         float *__restrict__ coef) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * 256) >> 6;
+  const int64_t wave =
+      (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
   for (int64_t i = wave; i < n; i += nwaves) {
     const int64_t L = len[i], s = rp[i], e = rp[i + 1];
     double sum = 0.0;
@@ -274,8 +282,9 @@ __global__ __launch_bounds__(256) void k_encode16(int64_t n, const int64_t *__re
                                                   uint16_t *__restrict__ c16,
                                                   float *__restrict__ scale, int kind) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * 256) >> 6;
+  const int64_t wave =
+      (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
   for (int64_t i = wave; i < n; i += nwaves) {
     const int64_t s = rp[i], e = rp[i + 1];
     float mx = 0.0f;

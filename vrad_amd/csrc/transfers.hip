@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void k_pair_cull(DevPatches p, int64_t np, int
   const int lane = threadIdx.x & 63;
   uint32_t base = 0;
   if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
-  base = __shfl(base, 0);
+  base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
   if (keep) {
     const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     if (slot < cap) pairs[slot] = make_int2((int)row, (int)j);
