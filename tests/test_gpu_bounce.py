@@ -112,7 +112,8 @@ def test_16bit_transfers(room_scene, prec):
                                   total=direct.astype(np.float64), max_bounces=k, stop_frac=0.0)
     rel = max_rel(tot, t64)
     print(f"[gpu] prec={prec} max rel luxel error vs FP64 {rel:.3e}", flush=True)
-    assert rel < 1e-3
+    # FP16 meets the 1e-3 luxel target; BF16 (8-bit mantissa) is stated at 5e-3
+    assert rel < (1e-3 if prec == _lib.VR_COEF_F16 else 5e-3)
 
 
 def test_synthetic_csr_bitexact():
