@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -25,10 +27,34 @@ HBM_PEAK_GBPS = 8000.0
 SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
 
 
-def pick_segments(s, n_seg, seed, eps=0.1):
-    """Random patch pairs (i < j) passing the facing cull of MakeTransfers,
-    turned into its TestLine segments (o + eps n)."""
-    import numpy as np
+def pair_segments(o, nrm, i, j, eps=0.1):
+    e = np.float32(eps)
+    return (o[i] + e * nrm[i]).astype(np.float32), (o[j] + e * nrm[j]).astype(np.float32)
+
+
+def row_batch(s, n_seg, i0):
+    """One MakeTransfers visibility batch: rows i = i0, i0+1, ... with every
+    j > i that passes the facing cull, in the order the pair kernel emits."""
+    o, nrm = s.p_origin, s.p_normal
+    n = len(o)
+    out_i, out_j, have, i = [], [], 0, i0
+    while have < n_seg and i < n - 1:
+        j = np.arange(i + 1, n)
+        dv = o[j] - o[i]
+        di = dv @ nrm[i]
+        dj = -np.einsum("ij,ij->i", nrm[j], dv)
+        j = j[(di > 0) & (dj > 0)]
+        out_i.append(np.full(len(j), i))
+        out_j.append(j)
+        have += len(j)
+        i += 1
+    i = np.concatenate(out_i)[:n_seg]
+    j = np.concatenate(out_j)[:n_seg]
+    return pair_segments(o, nrm, i, j)
+
+
+def random_batch(s, n_seg, seed):
+    """Random patch pairs passing the facing cull (no ray coherence)."""
     rng = np.random.default_rng(seed)
     o, nrm = s.p_origin, s.p_normal
     n = len(o)
@@ -46,12 +72,7 @@ def pick_segments(s, n_seg, seed, eps=0.1):
         out_i.append(lo[keep])
         out_j.append(hi[keep])
         have += int(keep.sum())
-    i = np.concatenate(out_i)[:n_seg]
-    j = np.concatenate(out_j)[:n_seg]
-    e = np.float32(eps)
-    st = (o[i] + e * nrm[i]).astype(np.float32)
-    en = (o[j] + e * nrm[j]).astype(np.float32)
-    return st, en
+    return pair_segments(o, nrm, np.concatenate(out_i)[:n_seg], np.concatenate(out_j)[:n_seg])
 
 
 def main():
@@ -72,7 +93,6 @@ def main():
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     from vrad_amd import _lib, scenes
@@ -93,38 +113,68 @@ def main():
     env.SetupAccelerationStructure()
     build_s = time.time() - tb
     info = env.info()
-    st, en = pick_segments(s, args.segments, seed=20)
+    st, en = row_batch(s, args.segments, i0=len(s.p_area) // 2)
     n = len(st)
-    cols = [torch.from_numpy(np.ascontiguousarray(a[:, k])).to(dev) for a in (st, en) for k in range(3)]
-    vis = torch.empty(n, dtype=torch.uint8, device=dev)
-    sky = torch.empty(n, dtype=torch.uint8, device=dev)
-    segs = _lib.Segs(*[_lib.C.cast(c.data_ptr(), _lib.FP) for c in cols])
     stream = torch.cuda.current_stream(dev)
     sptr = _lib.C.c_void_p(stream.cuda_stream)
-
-    def step():
-        check(lib.vr_test_line_dev(env.handle, _lib.C.byref(segs), n, -1,
-                                   _lib.C.c_void_p(vis.data_ptr()),
-                                   _lib.C.c_void_p(sky.data_ptr()), sptr))
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    check(lib.vr_profile_reset())
-    check(lib.vr_profile_enable(1))
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    check(lib.vr_profile_enable(0))
     ms, launches = _lib.C.c_double(), _lib.C.c_int64()
-    check(lib.vr_profile_get(b"test_line", _lib.C.byref(ms), _lib.C.byref(launches)))
-    kern_ms = ms.value / max(launches.value, 1)
+
+    class Batch:
+        def __init__(self, a, b):
+            self.cols = [torch.from_numpy(np.ascontiguousarray(x[:, k])).to(dev)
+                         for x in (a, b) for k in range(3)]
+            self.n = len(a)
+            self.vis = torch.empty(self.n, dtype=torch.uint8, device=dev)
+            self.sky = torch.empty(self.n, dtype=torch.uint8, device=dev)
+            self.segs = _lib.Segs(*[_lib.C.cast(c.data_ptr(), _lib.FP) for c in self.cols])
+
+        def step(self):
+            check(lib.vr_test_line_dev(env.handle, _lib.C.byref(self.segs), self.n, -1,
+                                       _lib.C.c_void_p(self.vis.data_ptr()),
+                                       _lib.C.c_void_p(self.sky.data_ptr()), sptr))
+
+        def run(self, steps, warmup):
+            for _ in range(warmup):
+                self.step()
+            torch.cuda.synchronize()
+            check(lib.vr_profile_reset())
+            check(lib.vr_profile_enable(1))
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(steps):
+                self.step()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t
+            check(lib.vr_profile_enable(0))
+            check(lib.vr_profile_get(b"test_line", _lib.C.byref(ms), _lib.C.byref(launches)))
+            return el, ms.value / max(launches.value, 1)
+
+    batch = Batch(st, en)
+    # autotune the LDS share of the traversal stack on this batch
+    sweep = {}
+    for S in (4, 8, 12, 16, 24):
+        check(lib.vr_set_tuning(b"trace_lds_entries", S))
+        el, km = batch.run(3, 1)
+        sweep[S] = round(n / (km * 1e-3) / 1e6, 1)
+    best_s = max(sweep, key=sweep.get)
+    check(lib.vr_set_tuning(b"trace_lds_entries", best_s))
+
+    elapsed, kern_ms = batch.run(args.steps, args.warmup)
+    vis = batch.vis
     vis_frac = float(vis.float().mean().item())
     mrays = n * args.steps / elapsed / 1e6
     achieved = SEG_BYTES * n / (kern_ms * 1e-3) / 1e9
+
+    # the plain path (scratch stacks, no XCD remap) and an incoherent batch
+    check(lib.vr_set_path(_lib.VR_PATH_PLAIN))
+    _, plain_ms = batch.run(3, 1)
+    check(lib.vr_set_path(_lib.VR_PATH_OPT))
+    rb = Batch(*random_batch(s, args.segments, seed=20))
+    _, rand_ms = rb.run(3, 1)
+    del rb
+    visibility = {"lds_entries_sweep_Mrays": sweep, "lds_entries": best_s,
+                  "plain_path_Mrays": round(n / (plain_ms * 1e-3) / 1e6, 1),
+                  "random_pairs_Mrays": round(n / (rand_ms * 1e-3) / 1e6, 1)}
 
     # ---- CPU restatement on a bounded sample: the same batch, repeated ----
     cpu = None
@@ -134,7 +184,7 @@ def main():
         tri = oracle.precompute(s.verts9, s.ids)
         kd = oracle.KdTree(s.verts9)
         passes, cpu_s = 0, 0.0
-        while passes < 32 and (passes == 0 or cpu_s < args.cpu_seconds):
+        while passes < 512 and (passes == 0 or cpu_s < args.cpu_seconds):
             c0 = time.perf_counter()
             cv, _ = oracle.testline(kd, tri, st, en, threads=threads)
             cpu_s += time.perf_counter() - c0
@@ -194,7 +244,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "plain GPU path at scale: 8x8-room procedural level, "
-                               "4M MakeTransfers TestLine segments per batch",
+                               "4M MakeTransfers TestLine segments per batch (row order)",
                    "triangles": int(info.n_tri), "patches": int(len(s.p_area)),
                    "kd_nodes": int(info.n_nodes), "kd_leaf_refs": int(info.n_leaf_refs),
                    "kd_build_s": round(build_s, 2), "segments_per_step": n,
@@ -204,6 +254,7 @@ def main():
                      "traffic": None, "kernel": "test_line", "kernel_ms": round(kern_ms, 4),
                      "algorithmic_bytes_per_launch": SEG_BYTES * n},
         "cpu_baseline": cpu,
+        "visibility": visibility,
         "bounce": bounce,
     }
     print(json.dumps(out), flush=True)

@@ -72,6 +72,13 @@ const char *vr_last_error(void);
 int vr_init(int device);
 /* Select VR_PATH_OPT / VR_PATH_PLAIN for the calls that follow. */
 int vr_set_path(int path);
+/* Launch-shape knobs; results do not depend on them.
+ *   "trace_lds_entries"  traversal-stack entries per lane held in LDS
+ *                        (4, 8, 12, 16, 24; the rest spill to scratch)
+ *   "bounce_waves"       persistent waves of the bounce gather
+ * No reference counterpart (the autotuner's interface). */
+int vr_set_tuning(const char *name, int value);
+int vr_get_tuning(const char *name, int *value);
 /* Kernel timing with HIP events on the launching stream.  name = kernel
  * family ("trace_closest", "trace_occluded", "bounce_gather", "energy",
  * "make_transfers", "gather_light"). */

@@ -25,12 +25,16 @@ def _env(v9, ids):
     return env
 
 
-@pytest.fixture(params=[_lib.VR_PATH_OPT, _lib.VR_PATH_PLAIN], ids=["opt", "plain"])
+@pytest.fixture(params=[(_lib.VR_PATH_OPT, 8), (_lib.VR_PATH_OPT, 4), (_lib.VR_PATH_OPT, 24),
+                        (_lib.VR_PATH_PLAIN, 8)], ids=["opt-lds8", "opt-lds4", "opt-lds24", "plain"])
 def path(request):
-    print(f"[gpu] path={request.param}", flush=True)
-    _lib.check(_lib.lib.vr_set_path(request.param))
-    yield request.param
+    p, lds = request.param
+    print(f"[gpu] path={p} trace_lds_entries={lds}", flush=True)
+    _lib.check(_lib.lib.vr_set_path(p))
+    _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", lds))
+    yield p
     _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_OPT))
+    _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", 8))
 
 
 @pytest.fixture(scope="module")

@@ -90,6 +90,8 @@ _SIGS = {
     "vr_last_error": (C.c_char_p, []),
     "vr_init": (C.c_int, [C.c_int]),
     "vr_set_path": (C.c_int, [C.c_int]),
+    "vr_set_tuning": (C.c_int, [C.c_char_p, C.c_int]),
+    "vr_get_tuning": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
     "vr_profile_enable": (C.c_int, [C.c_int]),
     "vr_profile_reset": (C.c_int, []),
     "vr_profile_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

@@ -146,7 +146,8 @@ void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, flo
     return;
   }
   // persistent: enough waves to fill 256 CUs several times over
-  int64_t waves = x.n < 256 * 32 ? x.n : 256 * 32;
+  const int64_t tw = tuning().bounce_waves;
+  int64_t waves = x.n < tw ? x.n : tw;
   unsigned blocks = (unsigned)((waves + 3) / 4);
   if (x.kind == VR_COEF_F32)
     hipLaunchKernelGGL(k_bounce_wave<VR_COEF_F32>, dim3(blocks), dim3(256), 0, st, x, emit_in,

@@ -17,6 +17,11 @@ int gen_room(uint64_t seed, int n_pillars, double chop, double tess, GenScene *o
 int gen_level(uint64_t seed, int rx, int ry, double chop, double tess, int n_lights, int sun,
               GenScene *out);
 
+Tuning &tuning() {
+  static Tuning t;
+  return t;
+}
+
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
 int fail(int code, const std::string &m) {
@@ -268,6 +273,33 @@ int vr_set_path(int path) {
   LOCK;
   if (path != VR_PATH_OPT && path != VR_PATH_PLAIN) return fail(VR_E_INVALID, "unknown path");
   g_path = path;
+  return VR_OK;
+}
+
+int vr_set_tuning(const char *name, int value) {
+  LOCK;
+  if (!name) return fail(VR_E_INVALID, "name is NULL");
+  const std::string k(name);
+  if (k == "trace_lds_entries") {
+    if (value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
+      return fail(VR_E_INVALID, "trace_lds_entries must be 4, 8, 12, 16 or 24");
+    tuning().trace_lds_entries = value;
+  } else if (k == "bounce_waves") {
+    if (value < 1 || value > (1 << 20)) return fail(VR_E_INVALID, "bounce_waves out of range");
+    tuning().bounce_waves = value;
+  } else {
+    return fail(VR_E_INVALID, "unknown tuning knob " + k);
+  }
+  return VR_OK;
+}
+
+int vr_get_tuning(const char *name, int *value) {
+  LOCK;
+  if (!name || !value) return fail(VR_E_INVALID, "NULL argument");
+  const std::string k(name);
+  if (k == "trace_lds_entries") *value = tuning().trace_lds_entries;
+  else if (k == "bounce_waves") *value = tuning().bounce_waves;
+  else return fail(VR_E_INVALID, "unknown tuning knob " + k);
   return VR_OK;
 }
 

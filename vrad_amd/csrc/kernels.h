@@ -11,6 +11,13 @@ namespace vr {
 
 constexpr int kStackDepth = 24;  // > max kd depth (21 + 1 leaf level)
 
+// Launch-shape knobs (vr_set_tuning); every value gives identical results.
+struct Tuning {
+  int trace_lds_entries = 8;   // traversal-stack entries per lane kept in LDS: 4/8/12/16/24
+  int bounce_waves = 8192;     // persistent waves of the bounce gather
+};
+Tuning &tuning();
+
 struct DevRays {
   const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
 };

@@ -95,17 +95,20 @@ __device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, fl
   b = ab;
 }
 
-template <int D>
+template <int S, int D>
 __global__ __launch_bounds__(64) void k_gather_lds(DevScene sc, DevSamples s, int64_t n,
                                                    const vr_light *__restrict__ lights,
                                                    int32_t n_lights, const float *__restrict__ dirs,
                                                    float eps, int32_t skip,
                                                    float *__restrict__ rgb) {
-  __shared__ uint32_t s_node[D * 64];
-  __shared__ float2 s_t[D * 64];
-  const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  __shared__ uint32_t s_node[S * 64];
+  __shared__ float2 s_t[S * 64];
+  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (i >= n) return;
-  LdsStack st{s_node, s_t, (int)threadIdx.x};
+  HybridStack<S, D> st;
+  st.node = s_node;
+  st.t = s_t;
+  st.lane = (int)threadIdx.x;
   float r, g, b;
   gather_one<D>(sc, st, s.px[i], s.py[i], s.pz[i], s.nx[i], s.ny[i], s.nz[i], lights, n_lights,
                 dirs, eps, skip, r, g, b);
@@ -139,7 +142,7 @@ void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
     hipLaunchKernelGGL(k_gather_plain<kStackDepth>, dim3((unsigned)((n + 255) / 256)), dim3(256),
                        0, st, sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb);
   else
-    hipLaunchKernelGGL(k_gather_lds<kStackDepth>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
+    hipLaunchKernelGGL((k_gather_lds<8, kStackDepth>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
                        sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb);
 }
 

@@ -1,23 +1,27 @@
 // Visibility kernels: batched Trace4Rays (closest hit) and TestLine
 // (segment visibility) over the resident kd-tree.
 //
-// Optimised path: one wave per block, per-lane traversal stacks in LDS
-// (24 entries x 12 B x 64 lanes = 18 KB per wave).  Plain path: one thread
-// per ray, 256-thread blocks, stack in private scratch memory.
+// Optimised path: one wave per block, XCD-aware ray ranges, per-lane
+// traversal stacks with the top S entries in LDS (S x 12 B x 64 lanes per
+// wave) and the rest in scratch.  Plain path: one thread per ray, 256-thread
+// blocks, the whole stack in private scratch memory.
 #include "kernels.h"
 
 namespace vr {
 
-template <int D>
+template <int S, int D>
 __global__ __launch_bounds__(64) void k_trace_closest_lds(DevScene sc, DevRays r, int64_t n,
                                                           int32_t skip, int32_t *__restrict__ hit,
                                                           float *__restrict__ hit_t,
                                                           float *__restrict__ hit_n) {
-  __shared__ uint32_t s_node[D * 64];
-  __shared__ float2 s_t[D * 64];
-  const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  __shared__ uint32_t s_node[S * 64];
+  __shared__ float2 s_t[S * 64];
+  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (i >= n) return;
-  LdsStack st{s_node, s_t, (int)threadIdx.x};
+  HybridStack<S, D> st;
+  st.node = s_node;
+  st.t = s_t;
+  st.lane = (int)threadIdx.x;
   float t;
   int32_t best, id;
   trace_ray<D, false>(sc, st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.tmin[i],
@@ -36,7 +40,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_trace_closest_plain(DevScene sc, DevRays r, int64_t n,
                                                              int32_t skip, int32_t *hit,
                                                              float *hit_t, float *hit_n) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (i >= n) return;
   ScratchStack<D> st;
   float t;
@@ -53,15 +57,18 @@ __global__ __launch_bounds__(256) void k_trace_closest_plain(DevScene sc, DevRay
   }
 }
 
-template <int D>
+template <int S, int D>
 __global__ __launch_bounds__(64) void k_test_line_lds(DevScene sc, DevSegs s, int64_t n,
                                                       int32_t skip, uint8_t *__restrict__ vis,
                                                       uint8_t *__restrict__ sky) {
-  __shared__ uint32_t s_node[D * 64];
-  __shared__ float2 s_t[D * 64];
-  const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  __shared__ uint32_t s_node[S * 64];
+  __shared__ float2 s_t[S * 64];
+  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (i >= n) return;
-  LdsStack st{s_node, s_t, (int)threadIdx.x};
+  HybridStack<S, D> st;
+  st.node = s_node;
+  st.t = s_t;
+  st.lane = (int)threadIdx.x;
   bool hs;
   bool v = test_line<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i], s.ez[i], skip, hs);
   vis[i] = v ? 1 : 0;
@@ -72,7 +79,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_test_line_plain(DevScene sc, DevSegs s, int64_t n,
                                                          int32_t skip, uint8_t *vis,
                                                          uint8_t *sky) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (i >= n) return;
   ScratchStack<D> st;
   bool hs;
@@ -81,26 +88,46 @@ __global__ __launch_bounds__(256) void k_test_line_plain(DevScene sc, DevSegs s,
   if (sky) sky[i] = hs ? 1 : 0;
 }
 
+#define VR_LDS_VARIANTS(X) X(4) X(8) X(12) X(16) X(24)
+
 void launch_trace_closest(const DevScene &sc, const DevRays &r, int64_t n, int32_t skip,
                           int32_t *hit, float *hit_t, float *hit_n, int path, hipStream_t st) {
   if (n <= 0) return;
-  if (path == VR_PATH_PLAIN)
+  if (path == VR_PATH_PLAIN) {
     hipLaunchKernelGGL(k_trace_closest_plain<kStackDepth>, dim3((unsigned)((n + 255) / 256)),
                        dim3(256), 0, st, sc, r, n, skip, hit, hit_t, hit_n);
-  else
-    hipLaunchKernelGGL(k_trace_closest_lds<kStackDepth>, dim3((unsigned)((n + 63) / 64)), dim3(64),
-                       0, st, sc, r, n, skip, hit, hit_t, hit_n);
+    return;
+  }
+  const dim3 g((unsigned)((n + 63) / 64));
+  switch (tuning().trace_lds_entries) {
+#define X(S)                                                                                  \
+  case S:                                                                                     \
+    hipLaunchKernelGGL((k_trace_closest_lds<S, kStackDepth>), g, dim3(64), 0, st, sc, r, n, skip, \
+                       hit, hit_t, hit_n);                                                    \
+    break;
+    VR_LDS_VARIANTS(X)
+#undef X
+  }
 }
 
 void launch_test_line(const DevScene &sc, const DevSegs &s, int64_t n, int32_t skip,
                       uint8_t *vis, uint8_t *sky, int path, hipStream_t st) {
   if (n <= 0) return;
-  if (path == VR_PATH_PLAIN)
+  if (path == VR_PATH_PLAIN) {
     hipLaunchKernelGGL(k_test_line_plain<kStackDepth>, dim3((unsigned)((n + 255) / 256)),
                        dim3(256), 0, st, sc, s, n, skip, vis, sky);
-  else
-    hipLaunchKernelGGL(k_test_line_lds<kStackDepth>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
-                       st, sc, s, n, skip, vis, sky);
+    return;
+  }
+  const dim3 g((unsigned)((n + 63) / 64));
+  switch (tuning().trace_lds_entries) {
+#define X(S)                                                                                   \
+  case S:                                                                                      \
+    hipLaunchKernelGGL((k_test_line_lds<S, kStackDepth>), g, dim3(64), 0, st, sc, s, n, skip, vis, \
+                       sky);                                                                   \
+    break;
+    VR_LDS_VARIANTS(X)
+#undef X
+  }
 }
 
 }  // namespace vr

@@ -23,6 +23,15 @@ struct DevScene {
 };
 
 constexpr float kTiny = 1e-20f;
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Remap workgroup b of nb so that XCD x takes one
+// contiguous range of work chunks: neighbouring rays (same origin patch in a
+// MakeTransfers batch) then share one L2.  A bijection on [0, nb).
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, k = b >> 3, per = nb >> 3, rem = nb & 7u;
+  return x * per + min(x, rem) + k;
+}
 constexpr uint32_t kTriMask = (1u << 28) - 1u;
 
 __device__ __forceinline__ float sel3(int c, float a, float b, float d) {
@@ -60,24 +69,40 @@ __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32
   }
 }
 
-// Per-lane traversal stack in LDS: entry e of lane l at [e * 64 + l].
-struct LdsStack {
+// Per-lane traversal stack: the first S entries in LDS (entry e of lane l at
+// [e * 64 + l], conflict-free), deeper entries in private scratch.  S = D
+// keeps the whole stack in LDS; S = 0 is the plain path's scratch stack.
+template <int S, int D>
+struct HybridStack {
   uint32_t *node;
   float2 *t;
   int lane;
+  uint32_t pn[D - S > 0 ? D - S : 1];
+  float pa[D - S > 0 ? D - S : 1], pb[D - S > 0 ? D - S : 1];
   __device__ void push(int sp, uint32_t n, float a, float b) {
-    node[sp * 64 + lane] = n;
-    t[sp * 64 + lane] = make_float2(a, b);
+    if (S == D || sp < S) {
+      node[sp * 64 + lane] = n;
+      t[sp * 64 + lane] = make_float2(a, b);
+    } else {
+      pn[sp - S] = n;
+      pa[sp - S] = a;
+      pb[sp - S] = b;
+    }
   }
   __device__ void get(int sp, uint32_t &n, float &a, float &b) const {
-    n = node[sp * 64 + lane];
-    float2 v = t[sp * 64 + lane];
-    a = v.x;
-    b = v.y;
+    if (S == D || sp < S) {
+      n = node[sp * 64 + lane];
+      const float2 v = t[sp * 64 + lane];
+      a = v.x;
+      b = v.y;
+    } else {
+      n = pn[sp - S];
+      a = pa[sp - S];
+      b = pb[sp - S];
+    }
   }
 };
 
-// Per-thread stack in private (scratch) memory: the plain path.
 template <int D>
 struct ScratchStack {
   uint32_t node[D];

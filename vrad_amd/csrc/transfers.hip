@@ -55,21 +55,24 @@ __device__ __forceinline__ void set_pair_bits(uint32_t *bits, int64_t words, int
   atomicOr(bits + (int64_t)j * words + (i >> 5), 1u << (i & 31));
 }
 
-template <int D>
+template <int S, int D>
 __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p,
                                                        const int2 *__restrict__ pairs,
                                                        const uint32_t *__restrict__ count,
                                                        uint32_t cap, float eps, int32_t skip,
                                                        uint32_t *__restrict__ bits,
                                                        int64_t words) {
-  __shared__ uint32_t s_node[D * 64];
-  __shared__ float2 s_t[D * 64];
+  __shared__ uint32_t s_node[S * 64];
+  __shared__ float2 s_t[S * 64];
   const uint32_t n = min(*count, cap);
-  const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (k >= n) return;
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
-  LdsStack st{s_node, s_t, (int)threadIdx.x};
+  HybridStack<S, D> st;
+  st.node = s_node;
+  st.t = s_t;
+  st.lane = (int)threadIdx.x;
   bool sky;
   const bool vis = test_line<D>(sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
                                 p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(256) void k_pair_trace_plain(DevScene sc, DevPatche
                                                           uint32_t *__restrict__ bits,
                                                           int64_t words) {
   const uint32_t n = min(*count, cap);
-  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (k >= n) return;
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
@@ -104,9 +107,12 @@ void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pair
   if (path == VR_PATH_PLAIN)
     hipLaunchKernelGGL(k_pair_trace_plain<kStackDepth>, dim3((cap + 255) / 256), dim3(256), 0, st,
                        sc, p, pairs, count, cap, eps, skip, bits, words);
+  else if (tuning().trace_lds_entries == 24)
+    hipLaunchKernelGGL((k_pair_trace_lds<24, kStackDepth>), dim3((cap + 63) / 64), dim3(64), 0, st,
+                       sc, p, pairs, count, cap, eps, skip, bits, words);
   else
-    hipLaunchKernelGGL(k_pair_trace_lds<kStackDepth>, dim3((cap + 63) / 64), dim3(64), 0, st, sc,
-                       p, pairs, count, cap, eps, skip, bits, words);
+    hipLaunchKernelGGL((k_pair_trace_lds<8, kStackDepth>), dim3((cap + 63) / 64), dim3(64), 0, st,
+                       sc, p, pairs, count, cap, eps, skip, bits, words);
 }
 
 // one wave per row
