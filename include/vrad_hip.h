@@ -72,11 +72,20 @@ const char *vr_last_error(void);
 int vr_init(int device);
 /* Select VR_PATH_OPT / VR_PATH_PLAIN for the calls that follow. */
 int vr_set_path(int path);
-/* Launch-shape knobs; results do not depend on them.
+/* Kernel-shape knobs (the autotuner's interface; no reference counterpart).
+ * Results do not depend on:
  *   "trace_lds_entries"  traversal-stack entries per lane held in LDS
  *                        (4, 8, 12, 16, 24; the rest spill to scratch)
- *   "bounce_waves"       persistent waves of the bounce gather
- * No reference counterpart (the autotuner's interface). */
+ *   "bounce_waves"       persistent waves of the wave-per-row gather
+ * These select the FP32 summation order of BounceLight (the CPU
+ * restatement takes the same values):
+ *   "bounce_kernel"      0 = LDS-tiled gather (default), 1 = wave per row
+ *   "bounce_tile_cols"   2048 / 4096 patches per LDS tile
+ *   "bounce_group"       8 / 16 lanes per row segment
+ *   "bounce_passes"      1 / 2 passes of a row segment loaded one step ahead
+ * The tile width and group fix the summation order; the other bounce knobs
+ * change only the schedule.  vr_bounce_light fails with VR_E_UNSUPPORTED for a
+ * combination that has no compiled kernel. */
 int vr_set_tuning(const char *name, int value);
 int vr_get_tuning(const char *name, int *value);
 /* Kernel timing with HIP events on the launching stream.  name = kernel
@@ -189,7 +198,8 @@ typedef struct {
  * get F_ij = cos_i cos_j A_j / (pi r^2) and the CSR stays resident. */
 int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t n_patches,
                       const vr_xfer_params *params, vr_xfers **out);
-/* Upload a host CSR (row_ptr n_rows+1, col/coef nnz) as transfers. */
+/* Upload a host CSR (row_ptr n_rows+1, col/coef nnz) as transfers; each row
+ * is stored with its columns in ascending order (stable). */
 int vr_xfers_from_csr(vr_xfers **out, int64_t n_rows, const int64_t *row_ptr,
                       const int32_t *col, const float *coef, int32_t precision);
 /* Synthetic transfers on the device: row lengths lognormal

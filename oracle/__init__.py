@@ -76,7 +76,7 @@ def _load():
                                    P, C.c_int32, C.c_float, C.c_int32, P, C.c_int]),
         "or_sky_dirs": (None, [P]),
         "or_bounce_f32": (C.c_int, [C.c_int64, P, P, P, C.c_int, P, P, P, P, P, C.c_int,
-                                    C.c_double, P, C.c_int]),
+                                    C.c_double, P, C.c_int, C.c_int, C.c_int, C.c_int]),
         "or_bounce_f64": (C.c_int, [C.c_int64, P, P, P, P, P, P, P, C.c_int, C.c_double, P]),
         "or_energy_reduce": (C.c_double, [P, C.c_int64]),
         "or_f32_to_f16": (C.c_uint16, [C.c_float]),
@@ -219,8 +219,13 @@ def sky_dirs():
     return d
 
 
+ORDERS = {"wave": 0, "sequential": 1, "tiled": 2}
+
+
 def bounce_f32(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, stop_frac=1e-3,
-               kind=0, row_scale=None, threads=1):
+               kind=0, row_scale=None, threads=1, order="tiled", tile_cols=4096, group=8):
+    """FP32 bounce in a kernel's summation order (default: the LDS-tiled
+    gather with 4096-patch tiles and 8-lane groups)."""
     n = len(rp) - 1
     rp = np.ascontiguousarray(rp, np.int64)
     col = np.ascontiguousarray(col, np.int32)
@@ -233,7 +238,8 @@ def bounce_f32(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, st
         np.ascontiguousarray(total, np.float32).reshape(n, 3).copy()
     en = np.zeros(max(max_bounces, 1), np.float64)
     k = lib.or_bounce_f32(n, _p(rp), _p(col), _p(coef), kind, _p(sc), _p(e0), _p(rf), _p(ar),
-                          _p(tot), max_bounces, stop_frac, _p(en), threads)
+                          _p(tot), max_bounces, stop_frac, _p(en), ORDERS[order], tile_cols, group,
+                          threads)
     return tot, en[:k], k
 
 

@@ -979,10 +979,15 @@ float or_f16_to_f32(uint16_t h) {
 /*   total_i += add_i ; emit_i = add_i * refl_i                              */
 /*   E_k = sum_i area_i * ((add.r + add.g) + add.b)   (FP64, fixed tree)    */
 /*   stop after bounce k when E_k <= stop_frac * sum_{m<=k} E_m, or k = max  */
-/* Kernel order: lane l of a 64-wide wave owns positions p with             */
-/* (p / G) % 64 == l (G = 4 for FP32, 8 for FP16/BF16), fmaf-accumulates    */
-/* them in increasing p, then xor-butterfly 32..1; FP16/BF16 rows multiply   */
-/* the reduced sum by the row scale.                                         */
+/* Kernel orders (G = 4 for FP32, 8 for FP16/BF16; FP16/BF16 rows multiply   */
+/* the reduced sum by the row scale):                                        */
+/*  0 wave per row: lane l owns positions p with (p / G) % 64 == l, fmaf in  */
+/*    increasing p, xor-butterfly 32..1;                                     */
+/*  1 sequential (one thread per row);                                       */
+/*  2 LDS-tiled: the row splits into segments by column tile of width C; in  */
+/*    a segment [sa, sb) lane l of a GS-lane group owns positions p with     */
+/*    ((p - sa) / G) % GS == l, butterfly GS/2..1, and segment sums are      */
+/*    added to the row total in tile order (empty ones add 0).              */
 /* ------------------------------------------------------------------------ */
 double or_energy_reduce(const double *e, int64_t n) {
   if (n <= 0) return 0.0;
@@ -1021,6 +1026,8 @@ typedef struct {
   const int32_t *col;
   const void *coef;
   int kind;
+  int order;         /* 0 wave per row, 1 sequential, 2 LDS-tiled */
+  int tile_cols, group;
   const float *scale;
   const float *emit; /* 3n, current */
   float *add;        /* 3n */
@@ -1032,29 +1039,63 @@ static inline float coef_at(const bg_ctx_t *c, int64_t k) {
   return c->kind == 1 ? or_f16_to_f32(h) : or_bf16_to_f32(h);
 }
 
+/* xor butterfly over the first `lanes` accumulators */
+static void butterfly3(float (*acc)[3], int lanes) {
+  for (int off = lanes / 2; off >= 1; off >>= 1) {
+    float nw[64][3];
+    for (int l = 0; l < lanes; ++l)
+      for (int q = 0; q < 3; ++q) nw[l][q] = acc[l][q] + acc[l ^ off][q];
+    for (int l = 0; l < lanes; ++l)
+      for (int q = 0; q < 3; ++q) acc[l][q] = nw[l][q];
+  }
+}
+
 static void bg_range(void *vc, int64_t b, int64_t e) {
   bg_ctx_t *c = (bg_ctx_t *)vc;
-  int G = c->kind == 0 ? 4 : 8;
+  const int G = c->kind == 0 ? 4 : 8;
   for (int64_t i = b; i < e; ++i) {
-    float acc[64][3];
-    memset(acc, 0, sizeof acc);
-    int64_t s0 = c->rp[i], len = c->rp[i + 1] - s0;
-    for (int64_t p = 0; p < len; ++p) {
-      int l = (int)((p / G) % 64);
-      float w = coef_at(c, s0 + p);
-      const float *em = c->emit + 3 * (int64_t)c->col[s0 + p];
-      acc[l][0] = fmaf(w, em[0], acc[l][0]);
-      acc[l][1] = fmaf(w, em[1], acc[l][1]);
-      acc[l][2] = fmaf(w, em[2], acc[l][2]);
+    const int64_t s0 = c->rp[i], len = c->rp[i + 1] - s0;
+    float out[3] = {0.0f, 0.0f, 0.0f};
+    if (c->order == 2) {
+      /* LDS-tiled: segments per column tile, GS-lane groups of G-entry
+       * chunks aligned to G, butterfly per segment, tile-ordered row sum */
+      const int64_t C = c->tile_cols;
+      const int GS = c->group;
+      const int64_t nt = (c->n + C - 1) / C;
+      int64_t q = 0;
+      for (int64_t t = 0; t < nt; ++t) {
+        const int64_t sa = q;
+        while (q < len && (int64_t)c->col[s0 + q] < (t + 1) * C) ++q;
+        const int64_t sb = q, base = sa;
+        float acc[64][3];
+        memset(acc, 0, sizeof acc);
+        for (int64_t p = sa; p < sb; ++p) {
+          const int l = (int)(((p - base) / G) % GS);
+          const float w = coef_at(c, s0 + p);
+          const float *em = c->emit + 3 * (int64_t)c->col[s0 + p];
+          acc[l][0] = fmaf(w, em[0], acc[l][0]);
+          acc[l][1] = fmaf(w, em[1], acc[l][1]);
+          acc[l][2] = fmaf(w, em[2], acc[l][2]);
+        }
+        butterfly3(acc, GS);
+        for (int k = 0; k < 3; ++k) out[k] = out[k] + acc[0][k];
+      }
+    } else {
+      float acc[64][3];
+      memset(acc, 0, sizeof acc);
+      for (int64_t p = 0; p < len; ++p) {
+        const int l = c->order == 1 ? 0 : (int)((p / G) % 64);
+        const float w = coef_at(c, s0 + p);
+        const float *em = c->emit + 3 * (int64_t)c->col[s0 + p];
+        acc[l][0] = fmaf(w, em[0], acc[l][0]);
+        acc[l][1] = fmaf(w, em[1], acc[l][1]);
+        acc[l][2] = fmaf(w, em[2], acc[l][2]);
+      }
+      butterfly3(acc, 64);
+      for (int k = 0; k < 3; ++k) out[k] = acc[0][k];
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-      float nw[64][3];
-      for (int l = 0; l < 64; ++l)
-        for (int q = 0; q < 3; ++q) nw[l][q] = acc[l][q] + acc[l ^ off][q];
-      memcpy(acc, nw, sizeof acc);
-    }
-    float sc = c->kind == 0 ? 1.0f : c->scale[i];
-    for (int q = 0; q < 3; ++q) c->add[3 * i + q] = c->kind == 0 ? acc[0][q] : acc[0][q] * sc;
+    const float sc = c->kind == 0 ? 1.0f : c->scale[i];
+    for (int q = 0; q < 3; ++q) c->add[3 * i + q] = c->kind == 0 ? out[q] : out[q] * sc;
   }
 }
 
@@ -1062,12 +1103,12 @@ int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
                   const void *coef, int coef_kind, const float *row_scale,
                   const float *emit0, const float *refl, const float *area,
                   float *total, int max_bounces, double stop_frac, double *energy,
-                  int nthreads) {
+                  int order, int tile_cols, int group, int nthreads) {
   float *emit = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
   float *add = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
   double *e = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
   memcpy(emit, emit0, sizeof(float) * 3 * (size_t)n);
-  bg_ctx_t c = {n, row_ptr, col, coef, coef_kind, row_scale, emit, add};
+  bg_ctx_t c = {n, row_ptr, col, coef, coef_kind, order, tile_cols, group, row_scale, emit, add};
   double cum = 0.0;
   int k = 0;
   while (k < max_bounces) {

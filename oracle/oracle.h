@@ -124,13 +124,16 @@ void or_gather_light(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
 void or_sky_dirs(float *dirs /* 162*3 */);
 
 /* ---- BounceLight + FP64 energy ---- */
-/* FP32 in the exact order of the bounce kernel (coef_kind 0=f32, 1=f16, 2=bf16
- * with per-row scale); energy reduced in the kernel's fixed tree order. */
+/* FP32 in the exact order of a bounce kernel (coef_kind 0=f32, 1=f16, 2=bf16
+ * with per-row scale; order 0 = wave per row, 1 = sequential, 2 = LDS-tiled
+ * with tile_cols / group); energy reduced in the kernels' fixed tree order.
+ * Order 2 needs each row's columns in ascending order. */
 int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
                   const void *coef, int coef_kind, const float *row_scale,
                   const float *emit0, const float *refl, const float *area,
                   float *total /* in/out, 3n */, int max_bounces,
-                  double stop_frac, double *energy, int nthreads);
+                  double stop_frac, double *energy, int order, int tile_cols,
+                  int group, int nthreads);
 /* FP64 recurrence, sequential order, FP64 coefficients as given (f32 input) */
 int or_bounce_f64(int64_t n, const int64_t *row_ptr, const int32_t *col,
                   const float *coef, const float *emit0, const float *refl,

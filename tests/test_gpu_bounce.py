@@ -54,15 +54,36 @@ def _room_bake(room_scene):
     return xf, direct, emit0
 
 
-def test_room_bounce_bitexact(room_scene):
+# (kernel, tile cols, group, passes loaded one step ahead)
+KERNELS = [(0, 4096, 8, 2), (0, 4096, 16, 1), (0, 2048, 16, 2), (0, 2048, 8, 2), (1, 4096, 8, 2)]
+KERNEL_IDS = ["tiled4096x8", "tiled4096x16-p1", "tiled2048x16", "tiled2048x8", "wave"]
+KNOBS = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes")
+
+
+def _set_kernel(*cfg):
+    for k, v in zip(KNOBS, cfg):
+        _lib.check(_lib.lib.vr_set_tuning(k, v))
+
+
+def _oracle_kw(kernel, cols, group, *schedule):
+    return dict(order="tiled" if kernel == 0 else "wave", tile_cols=cols, group=group)
+
+
+@pytest.mark.parametrize("cfg", KERNELS, ids=KERNEL_IDS)
+def test_room_bounce_bitexact(room_scene, cfg):
     s = room_scene
     xf, direct, emit0 = _room_bake(s)
     rp, col, coef = xf.download()
-    print(f"[gpu] room bounce nnz={len(col)}", flush=True)
-    tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct,
-                                      max_bounces=100, stop_frac=1e-3)
+    print(f"[gpu] room bounce nnz={len(col)} kernel={cfg}", flush=True)
+    _set_kernel(*cfg)
+    try:
+        tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct,
+                                          max_bounces=100, stop_frac=1e-3)
+    finally:
+        _set_kernel(*KERNELS[0])
     otot, oen, ok = oracle.bounce_f32(rp, col, coef, emit0, s.p_refl, s.p_area, total=direct,
-                                      max_bounces=100, stop_frac=1e-3, threads=8)
+                                      max_bounces=100, stop_frac=1e-3, threads=8,
+                                      **_oracle_kw(*cfg))
     assert k == ok and 2 < k < 100
     assert np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
     assert np.array_equal(en.view(np.uint64), oen.view(np.uint64))
@@ -84,9 +105,9 @@ def test_room_bounce_plain_vs_sequential(room_scene):
     finally:
         _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_OPT))
     otot, oen, ok = oracle.bounce_f32(rp, col, coef, emit0, s.p_refl, s.p_area, total=direct,
-                                      threads=8)
+                                      threads=8, order="sequential")
     assert k == ok
-    assert max_rel(tot, otot) < 1e-5
+    assert np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
 
 
 @pytest.mark.parametrize("prec", [_lib.VR_COEF_F16, _lib.VR_COEF_BF16], ids=["f16", "bf16"])
@@ -114,6 +135,33 @@ def test_16bit_transfers(room_scene, prec):
     print(f"[gpu] prec={prec} max rel luxel error vs FP64 {rel:.3e}", flush=True)
     # FP16 meets the 1e-3 luxel target; BF16 (8-bit mantissa) is stated at 5e-3
     assert rel < (1e-3 if prec == _lib.VR_COEF_F16 else 5e-3)
+
+
+@pytest.mark.parametrize("cfg", KERNELS, ids=KERNEL_IDS)
+def test_synthetic_multitile_bitexact(cfg):
+    n = 20000
+    xf = Transfers.synthetic(seed=9, n_rows=n, mean_len=200.0, sigma=1.0, max_len=4000)
+    rp, col, coef = xf.download()
+    rng = np.random.default_rng(4)
+    e0 = rng.random((n, 3)).astype(np.float32)
+    refl = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+    area = rng.uniform(1, 4, n).astype(np.float32)
+    print(f"[gpu] synthetic n={n} nnz={len(col)} kernel={cfg}", flush=True)
+    _set_kernel(*cfg)
+    try:
+        tot, en, k = lightmap.BounceLight(xf, e0, refl, area, max_bounces=4, stop_frac=0.0)
+        xf.set_precision(_lib.VR_COEF_F16)
+        tot16, _, _ = lightmap.BounceLight(xf, e0, refl, area, max_bounces=4, stop_frac=0.0)
+        _, _, _, raw, sc = xf.download(raw16=True)
+    finally:
+        _set_kernel(*KERNELS[0])
+    otot, oen, ok = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=4, stop_frac=0.0,
+                                      threads=8, **_oracle_kw(*cfg))
+    assert k == ok == 4 and np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
+    assert np.array_equal(en, oen)
+    o16, _, _ = oracle.bounce_f32(rp, col, raw, e0, refl, area, max_bounces=4, stop_frac=0.0,
+                                  kind=_lib.VR_COEF_F16, row_scale=sc, threads=8, **_oracle_kw(*cfg))
+    assert np.array_equal(tot16.view(np.uint32), o16.view(np.uint32))
 
 
 def test_synthetic_csr_bitexact():

@@ -362,6 +362,28 @@ def test_bounce_matches_linear_solve():
     assert np.max(np.abs(tot32 - tot) / np.abs(tot)) < 1e-5
 
 
+def test_bounce_orders_agree():
+    """The kernel summation orders differ only by FP32 rounding."""
+    rng = np.random.default_rng(8)
+    n = 3000
+    L = rng.integers(1, 400, n)
+    rp = np.concatenate([[0], np.cumsum(L)]).astype(np.int64)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in L]).astype(np.int32)
+    coef = (rng.random(rp[-1]) / 400).astype(np.float32)
+    e0 = rng.random((n, 3)).astype(np.float32)
+    refl = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+    area = np.ones(n, np.float32)
+    ref, _, _ = oracle.bounce_f64(rp, col, coef, e0, refl, area, max_bounces=5, stop_frac=0.0)
+    outs = []
+    for kw in (dict(order="wave"), dict(order="sequential"), dict(order="tiled"),
+               dict(order="tiled", tile_cols=512, group=16)):
+        t, _, k = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=5, stop_frac=0.0,
+                                    **kw)
+        assert k == 5 and np.max(np.abs(t - ref) / ref) < 2e-6
+        outs.append(t)
+    assert not np.array_equal(outs[0], outs[1])      # the orders really differ
+
+
 def test_energy_reduce_exact():
     rng = np.random.default_rng(2)
     for n in (0, 1, 5, 2048, 2049, 100000):

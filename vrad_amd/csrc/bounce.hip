@@ -160,6 +160,390 @@ void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, flo
                        emit_out, total, refl, area, e_row, row_counter, state);
 }
 
+// ---- LDS-tiled gather ------------------------------------------------------
+// The emit vector is cut into column tiles of C patches that are staged in
+// LDS.  A workgroup of W waves owns 64 W rows: chunk c takes rows
+// order[c + k * n_chunks], a stride through the longest-first order, so every
+// chunk and every wave gets the same mix of lengths; slot j of wave w is
+// chunk row k = W j + w.  For each tile, the wave's 64 lanes form 64/GS groups
+// of GS lanes; in step s group gi takes slot j = s (64/GS) + gi, i.e. that
+// row's segment (its entries with columns in the tile, a contiguous range
+// because columns are sorted).  Lane l of the group owns the G-entry chunks at
+// segment positions G l + G GS i, FMA-accumulates them in order, the group
+// butterflies (GS/2 .. 1), and lane s of the group, which owns slot j, adds
+// the segment sum to its row total; tiles in order.  Coefficients come from a
+// tile-blocked copy: block (c, t) holds the segments of chunk c for tile t
+// back to back in slot order (u = 64 w + j), each padded to a multiple of G
+// with (col = tile start, coef = 0), so every 16-byte load is full and the
+// loads of one step cover a few contiguous lines.  Each lane loads its first
+// P passes (P GS G entries of the segment) one step ahead.
+// One G-entry chunk of a row: raw 16-byte loads, decoded when consumed so the
+// loads of the next step stay in flight meanwhile.
+template <int KIND>
+struct Chunk;
+template <>
+struct Chunk<VR_COEF_F32> {
+  int4 c0;
+  float4 w;
+  __device__ void load(const DevXfers &x, int64_t k) {
+    c0 = *reinterpret_cast<const int4 *>(x.col + k);
+    w = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(x.coef) + k);
+  }
+  __device__ int col(int m) const { return m == 0 ? c0.x : m == 1 ? c0.y : m == 2 ? c0.z : c0.w; }
+  __device__ float coef(int m) const { return m == 0 ? w.x : m == 1 ? w.y : m == 2 ? w.z : w.w; }
+};
+template <int KIND>
+struct Chunk {
+  int4 c0, c1;
+  uint4 h;
+  __device__ void load(const DevXfers &x, int64_t k) {
+    c0 = *reinterpret_cast<const int4 *>(x.col + k);
+    c1 = *reinterpret_cast<const int4 *>(x.col + k + 4);
+    h = *reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(x.coef) + k);
+  }
+  __device__ int col(int m) const {
+    const int4 &c = m < 4 ? c0 : c1;
+    const int mm = m & 3;
+    return mm == 0 ? c.x : mm == 1 ? c.y : mm == 2 ? c.z : c.w;
+  }
+  __device__ float coef(int m) const {
+    const uint32_t wd = (m >> 1) == 0 ? h.x : (m >> 1) == 1 ? h.y : (m >> 1) == 2 ? h.z : h.w;
+    const uint16_t v = (uint16_t)(wd >> (16 * (m & 1)));
+    return KIND == VR_COEF_F16 ? f16_to_f32(v) : bf16_to_f32(v);
+  }
+};
+
+// Exact xor-lane exchange inside 16-lane rows with DPP (no LDS traffic):
+// quad_perm for 1 and 2, bank-masked row shifts for 4, row rotate for 8.
+template <int OFF>
+__device__ __forceinline__ float dpp_xor(float v) {
+  const int x = __float_as_int(v);
+  int y;
+  if constexpr (OFF == 1) {
+    y = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (OFF == 2) {
+    y = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (OFF == 4) {
+    y = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0x5, false);  // row_shl:4, banks 0,2
+    y = __builtin_amdgcn_update_dpp(y, x, 0x114, 0xF, 0xA, false);  // row_shr:4, banks 1,3
+  } else {
+    static_assert(OFF == 8, "xor offset");
+    y = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+  }
+  return __int_as_float(y);
+}
+
+template <int GS>
+__device__ __forceinline__ float group_sum(float v) {
+  if constexpr (GS >= 16) v = v + dpp_xor<8>(v);
+  if constexpr (GS >= 8) v = v + dpp_xor<4>(v);
+  v = v + dpp_xor<2>(v);
+  v = v + dpp_xor<1>(v);
+  return v;
+}
+
+// Asynchronous global -> LDS copy of 16-byte pieces (gfx950 LDS DMA): lane l
+// of the wave writes its piece at lds_base + 16 l; the wave's pieces are
+// consecutive in LDS.  Completion is tracked by vmcnt like any global load.
+__device__ __forceinline__ void dma16(const void *src, void *lds_base) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds_base, 16, 0,
+                                   0);
+}
+
+template <int KIND, int C>
+__device__ __forceinline__ void gather_chunk(const Chunk<KIND> &ch, const float *__restrict__ te,
+                                             int p0, float &r, float &g, float &b) {
+  constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
+#pragma unroll
+  for (int m = 0; m < G; ++m) {
+    const float *e = te + 3 * (ch.col(m) - p0);
+    const float w = ch.coef(m);
+    r = __builtin_fmaf(w, e[0], r);
+    g = __builtin_fmaf(w, e[1], g);
+    b = __builtin_fmaf(w, e[2], b);
+  }
+}
+
+template <int KIND, int P>
+struct StepLoad {
+  Chunk<KIND> ch[P];
+  int q = 0, sb = 0;
+};
+
+// One workgroup of 16 waves per chunk of 1024 rows; emit tiles (r, g, b per
+// patch, as in global memory) and the block's segment offsets are copied into
+// two LDS buffers by DMA, tile t + 1 while tile t is gathered.  The copy reads
+// emit_in in whole 16-byte pieces, so emit_in must stay readable up to the
+// next 16-byte boundary past its 12 n bytes (the bounce buffers are padded to
+// 256 bytes).
+template <int KIND, int C, int GS, int P>
+__global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
+                                                       const float *__restrict__ emit_in,
+                                                       float *__restrict__ emit_out,
+                                                       float *__restrict__ total,
+                                                       const float *__restrict__ refl,
+                                                       const float *__restrict__ area,
+                                                       double *__restrict__ e_row,
+                                                       const BounceState *__restrict__ state) {
+  if (state->converged) return;
+  constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
+  constexpr int T = 1024;              // threads
+  constexpr int R = 1024;              // rows per chunk
+  constexpr int RS = R + 4;            // stride of a block's offsets (16-byte multiple)
+  constexpr int NG = 64 / GS;          // rows per step (one per lane group)
+  constexpr int NS = GS;               // steps per tile: 64 slots / NG
+  constexpr int PASS = GS * G;         // entries per pass of a group
+  constexpr int EP = 3 * C / 4;        // 16-byte pieces of a full emit tile
+  constexpr int OP = (R + 1 + 3) / 4;  // 16-byte pieces of a block's offsets
+  constexpr int ER = (EP + T - 1) / T;  // copy rounds per tile
+  __shared__ __attribute__((aligned(16))) float s_e[2][3 * C];
+  __shared__ __attribute__((aligned(16))) int32_t s_off[2][4 * OP];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int gi = lane / GS, gl = lane % GS;
+  const int64_t c = blockIdx.x;
+  const int j_own = gl * NG + gi;  // the slot whose row total this lane keeps
+  const int64_t oi = c + ((int64_t)j_own * 16 + wave) * tl.n_chunks;
+  const int64_t row = oi < x.n ? (int64_t)x.order[oi] : -1;
+  DevXfers xb = x;  // blocked arrays, same chunk decoders
+  xb.col = tl.bcol;
+  xb.coef = tl.bcoef;
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+
+  auto fetch_tile = [&](int t, int buf) {
+    const int64_t p1 = (int64_t)t * C;
+    const int np = (int)min((int64_t)C, x.n - p1);
+    const int pieces = (12 * np + 15) / 16;
+    const char *src = reinterpret_cast<const char *>(emit_in + 3 * p1);
+#pragma unroll
+    for (int m = 0; m < ER; ++m) {
+      const int k = (int)threadIdx.x + T * m;
+      if (k < pieces) dma16(src + 16 * k, &s_e[buf][4 * (T * m + 64 * wave)]);
+    }
+    const char *osrc = reinterpret_cast<const char *>(tl.boff + (c * tl.nt + t) * (int64_t)RS);
+    if ((int)threadIdx.x < OP) dma16(osrc + 16 * threadIdx.x, &s_off[buf][4 * 64 * wave]);
+  };
+  fetch_tile(0, 0);
+
+  for (int t = 0; t < tl.nt; ++t) {
+    const int buf = t & 1;
+    const int p0 = t * C;
+    const int64_t bs = tl.bstart[c * tl.nt + t];
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of tile t have landed
+    __syncthreads();                // ... and every wave's; tile t - 1 is done
+    if (t + 1 < tl.nt) fetch_tile(t + 1, buf ^ 1);
+    const float *te = s_e[buf];
+    const int32_t *so = s_off[buf];
+
+    auto issue = [&](int s) {
+      StepLoad<KIND, P> L;
+      const int u = wave * 64 + s * NG + gi;
+      L.q = so[u] + gl * G;
+      L.sb = so[u + 1];
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (L.q + k * PASS < L.sb) L.ch[k].load(xb, bs + L.q + k * PASS);
+      return L;
+    };
+    StepLoad<KIND, P> cur = issue(0);
+#pragma unroll 1
+    for (int st = 0; st < NS; ++st) {
+      StepLoad<KIND, P> nxt;
+      if (st + 1 < NS) nxt = issue(st + 1);
+      float r = 0.0f, g = 0.0f, b = 0.0f;
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (cur.q + k * PASS < cur.sb) gather_chunk<KIND, C>(cur.ch[k], te, p0, r, g, b);
+      for (int q = cur.q + P * PASS; q < cur.sb; q += PASS) {
+        Chunk<KIND> more;
+        more.load(xb, bs + q);
+        gather_chunk<KIND, C>(more, te, p0, r, g, b);
+      }
+      r = group_sum<GS>(r);
+      g = group_sum<GS>(g);
+      b = group_sum<GS>(b);
+      if (gl == st) {
+        ar = ar + r;
+        ag = ag + g;
+        ab = ab + b;
+      }
+      cur = nxt;
+    }
+  }
+  if (row >= 0) {
+    if constexpr (KIND != VR_COEF_F32) {
+      const float sc = x.scale[row];
+      ar = ar * sc;
+      ag = ag * sc;
+      ab = ab * sc;
+    }
+    total[3 * row + 0] = total[3 * row + 0] + ar;
+    total[3 * row + 1] = total[3 * row + 1] + ag;
+    total[3 * row + 2] = total[3 * row + 2] + ab;
+    emit_out[3 * row + 0] = ar * refl[3 * row + 0];
+    emit_out[3 * row + 1] = ag * refl[3 * row + 1];
+    emit_out[3 * row + 2] = ab * refl[3 * row + 2];
+    e_row[row] = (double)area[row] * (((double)ar + (double)ag) + (double)ab);
+  }
+}
+
+// Blocked layout, pass 1: one workgroup per block (c, t): padded segment
+// lengths of the 1024 owner slots, exclusive scan -> boff (stride 1028),
+// total -> btot.
+__global__ __launch_bounds__(1024) void k_block_offsets(DevXfers x, DevTiles tl, int G,
+                                                        int64_t *__restrict__ btot,
+                                                        int32_t *__restrict__ boff) {
+  __shared__ int64_t sc[1024];
+  const int R = 1024, RS = R + 4;
+  const int64_t blk = blockIdx.x;  // c * nt + t
+  const int64_t c = blk / tl.nt;
+  const int t = (int)(blk % tl.nt);
+  const int u = threadIdx.x;
+  int64_t len = 0;
+  if (u < R) {
+    const int w = u / 64, j = u % 64;
+    const int64_t oi = c + ((int64_t)j * 16 + w) * tl.n_chunks;
+    if (oi < x.n) {
+      const int64_t row = x.order[oi];
+      const int32_t *tp = tl.tp + row * (int64_t)(tl.nt + 1);
+      len = tp[t + 1] - tp[t];
+      len = (len + G - 1) / G * G;
+    }
+  }
+  sc[u] = len;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    const int64_t v = u >= off ? sc[u - off] : 0;
+    __syncthreads();
+    sc[u] += v;
+    __syncthreads();
+  }
+  int32_t *bo = boff + blk * RS;
+  if (u < R) bo[u + 1] = (int32_t)sc[u];
+  if (u == 0) {
+    bo[0] = 0;
+    btot[blk] = sc[R - 1];
+  }
+}
+
+// Blocked layout, pass 2: copy each owner's segment, pad with (col = tile
+// start, coef = 0).  One wave per (block, owner slot).
+__global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int G,
+                                                    int coef_bytes, int32_t *__restrict__ bcol,
+                                                    void *__restrict__ bcoef) {
+  const int lane = threadIdx.x & 63;
+  const int R = 1024, RS = R + 4;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t nblk = tl.n_chunks * (int64_t)tl.nt;
+  if (gw >= nblk * R) return;
+  const int64_t blk = gw / R;
+  const int u = (int)(gw % R), w = u / 64, j = u % 64;
+  const int64_t c = blk / tl.nt;
+  const int t = (int)(blk % tl.nt);
+  const int64_t oi = c + ((int64_t)j * 16 + w) * tl.n_chunks;
+  if (oi >= x.n) return;
+  const int64_t row = x.order[oi];
+  const int32_t *tp = tl.tp + row * (int64_t)(tl.nt + 1);
+  const int64_t src = x.rp[row] + tp[t];
+  const int len = tp[t + 1] - tp[t];
+  const int32_t *bo = tl.boff + blk * RS;
+  const int64_t dst = tl.bstart[blk] + bo[u];
+  const int plen = bo[u + 1] - bo[u];
+  const int pad_col = t * tl.cols;
+  for (int q = lane; q < plen; q += 64) {
+    const bool real = q < len;
+    bcol[dst + q] = real ? x.col[src + q] : pad_col;
+    if (coef_bytes == 4)
+      reinterpret_cast<float *>(bcoef)[dst + q] =
+          real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;
+    else
+      reinterpret_cast<uint16_t *>(bcoef)[dst + q] =
+          real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
+  }
+}
+
+void launch_block_offsets(const DevXfers &x, const DevTiles &tl, int G, int64_t *btot,
+                          int32_t *boff, hipStream_t st) {
+  const int64_t nblk = tl.n_chunks * (int64_t)tl.nt;
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_block_offsets, dim3((unsigned)nblk), dim3(1024), 0, st, x, tl, G, btot,
+                     boff);
+}
+
+void launch_block_fill(const DevXfers &x, const DevTiles &tl, int G, int coef_bytes,
+                       int32_t *bcol, void *bcoef, hipStream_t st) {
+  const int64_t waves = tl.n_chunks * (int64_t)tl.nt * 1024;
+  if (waves <= 0) return;
+  hipLaunchKernelGGL(k_block_fill, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, x, tl, G,
+                     coef_bytes, bcol, bcoef);
+}
+
+// tile boundaries per row: tp[r][t] = first row position with column >= t*C
+__global__ __launch_bounds__(256) void k_tile_ptrs(int64_t n, const int64_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ cnt,
+                                                   const int32_t *__restrict__ col, int cols,
+                                                   int nt, int32_t *__restrict__ tp) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (row >= n) return;
+  const int32_t *c = col + rp[row];
+  const int L = cnt[row];
+  for (int t = lane; t <= nt; t += 64) {
+    const int64_t key = (int64_t)t * cols;
+    int lo = 0, hi = L;  // first index with c[idx] >= key
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int64_t)c[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    tp[row * (int64_t)(nt + 1) + t] = t == nt ? L : lo;
+  }
+}
+
+void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const int32_t *col,
+                      int cols, int nt, int32_t *tp, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_tile_ptrs, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, n, rp, cnt, col,
+                     cols, nt, tp);
+}
+
+// Compiled shapes (C, GS, P) of the LDS-tiled gather.
+#define VR_TILED_SHAPES(X) \
+  X(4096, 8, 2)            \
+  X(4096, 8, 1)            \
+  X(4096, 16, 2)           \
+  X(4096, 16, 1)           \
+  X(2048, 8, 2)            \
+  X(2048, 16, 2)
+
+bool bounce_tiled_variant(int cols, int group, int passes) {
+#define VR_HAS(CC, GG, PP) \
+  if (cols == CC && group == GG && passes == PP) return true;
+  VR_TILED_SHAPES(VR_HAS)
+#undef VR_HAS
+  return false;
+}
+
+void launch_bounce_tiled(int passes, const DevXfers &x, const DevTiles &tl, const float *emit_in,
+                         float *emit_out, float *total, const float *refl, const float *area,
+                         double *e_row, const BounceState *state, hipStream_t st) {
+  if (x.n <= 0) return;
+  const dim3 g((unsigned)tl.n_chunks);
+#define VR_GO(K, CC, GG, PP)                                                                  \
+  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP>), g, dim3(1024), 0, st, x, tl, emit_in,   \
+                     emit_out, total, refl, area, e_row, state)
+#define VR_SHAPE(CC, GG, PP)                                                \
+  if (tl.cols == CC && tl.group == GG && passes == PP) {                    \
+    if (x.kind == VR_COEF_F32) VR_GO(VR_COEF_F32, CC, GG, PP);              \
+    else if (x.kind == VR_COEF_F16) VR_GO(VR_COEF_F16, CC, GG, PP);         \
+    else VR_GO(VR_COEF_BF16, CC, GG, PP);                                   \
+    return;                                                                 \
+  }
+  VR_TILED_SHAPES(VR_SHAPE)
+#undef VR_SHAPE
+#undef VR_GO
+}
+
 // ---- FP64 energy: fixed tree.  Each block of 256 threads reduces 2048
 // values: thread t sums [base + t + 256 j] for j = 0..7 from 0.0, then an
 // LDS tree halves 128..1.  Stages repeat until one value is left.

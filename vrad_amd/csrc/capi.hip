@@ -172,6 +172,11 @@ struct vr_xfers {
   std::vector<int64_t> rp_host;  // padded starts
   std::vector<int32_t> cnt_host; // logical lengths
   DBuf rp, col, coef32, coef16, scale, order, cnt;
+  DBuf tp;            // tile boundaries for the LDS-tiled gather
+  int tp_cols = 0;
+  // tile-blocked copy of the coefficients for the LDS-tiled gather
+  DBuf bcol, bcoef, bstart, boff;
+  int blk_cols = 0, blk_prec = -1;
   DevXfers dev() const {
     DevXfers x;
     x.n = n;
@@ -287,6 +292,20 @@ int vr_set_tuning(const char *name, int value) {
   } else if (k == "bounce_waves") {
     if (value < 1 || value > (1 << 20)) return fail(VR_E_INVALID, "bounce_waves out of range");
     tuning().bounce_waves = value;
+  } else if (k == "bounce_kernel") {
+    if (value != 0 && value != 1) return fail(VR_E_INVALID, "bounce_kernel must be 0 or 1");
+    tuning().bounce_kernel = value;
+  } else if (k == "bounce_tile_cols") {
+    if (value != 2048 && value != 4096)
+      return fail(VR_E_INVALID, "bounce_tile_cols must be 2048 or 4096");
+    tuning().bounce_tile_cols = value;
+  } else if (k == "bounce_group") {
+    if (value != 8 && value != 16) return fail(VR_E_INVALID, "bounce_group must be 8 or 16");
+    tuning().bounce_group = value;
+  } else if (k == "bounce_passes") {
+    if (value != 1 && value != 2) return fail(VR_E_INVALID, "bounce_passes must be 1 or 2");
+    tuning().bounce_passes = value;
+
   } else {
     return fail(VR_E_INVALID, "unknown tuning knob " + k);
   }
@@ -299,6 +318,10 @@ int vr_get_tuning(const char *name, int *value) {
   const std::string k(name);
   if (k == "trace_lds_entries") *value = tuning().trace_lds_entries;
   else if (k == "bounce_waves") *value = tuning().bounce_waves;
+  else if (k == "bounce_kernel") *value = tuning().bounce_kernel;
+  else if (k == "bounce_tile_cols") *value = tuning().bounce_tile_cols;
+  else if (k == "bounce_group") *value = tuning().bounce_group;
+  else if (k == "bounce_passes") *value = tuning().bounce_passes;
   else return fail(VR_E_INVALID, "unknown tuning knob " + k);
   return VR_OK;
 }
@@ -650,11 +673,18 @@ int vr_xfers_from_csr(vr_xfers **out, int64_t n, const int64_t *row_ptr, const i
   TRY(finish_layout(x.get(), g_stream));
   std::vector<int32_t> pc((size_t)x->stored);
   std::vector<float> pf((size_t)x->stored);
+  std::vector<int64_t> perm;
   for (int64_t i = 0; i < n; ++i) {
     int64_t s = x->rp_host[i], l = x->cnt_host[i], e = x->rp_host[i + 1];
+    // rows are stored with ascending columns (the tiled gather reads each
+    // column tile of a row as one contiguous segment)
+    perm.resize((size_t)l);
+    for (int64_t k = 0; k < l; ++k) perm[(size_t)k] = row_ptr[i] + k;
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int64_t a, int64_t b) { return col[a] < col[b]; });
     for (int64_t k = 0; k < l; ++k) {
-      pc[s + k] = col[row_ptr[i] + k];
-      pf[s + k] = coef[row_ptr[i] + k];
+      pc[s + k] = col[perm[(size_t)k]];
+      pf[s + k] = coef[perm[(size_t)k]];
     }
     for (int64_t k = s + l; k < e; ++k) {
       pc[k] = (int32_t)i;
@@ -895,6 +925,61 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
   VR_HIP(hipMemsetAsync(counters, 0, 4 * (size_t)std::max(K, 1), st));
   VR_HIP(hipMemsetAsync(state, 0, sizeof(BounceState), st));
   const DevXfers dx = x->dev();
+  const Tuning tu = tuning();
+  const bool tiled = g_path == VR_PATH_OPT && tu.bounce_kernel == 0;
+  DevTiles tl{};
+  if (tiled) {
+    const int cols = tu.bounce_tile_cols;
+    const int nt = (int)((n + cols - 1) / cols);
+    tl.nt = nt;
+    tl.cols = cols;
+    tl.group = tu.bounce_group;
+    if (!bounce_tiled_variant(cols, tl.group, tu.bounce_passes))
+      return fail(VR_E_UNSUPPORTED, "no compiled LDS-tiled gather for this tuning shape");
+    tl.n_chunks = (n + 1023) / 1024;
+    if (x->tp_cols != cols || !x->tp.p) {
+      TRY(x->tp.alloc(4 * (size_t)std::max<int64_t>(n, 1) * (size_t)(nt + 1)));
+      launch_tile_ptrs(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), x->col.as<int32_t>(), cols,
+                       nt, x->tp.as<int32_t>(), st);
+      TRY(check_launch());
+      x->tp_cols = cols;
+      x->blk_cols = 0;
+    }
+    tl.tp = x->tp.as<int32_t>();
+    if (x->blk_cols != cols || x->blk_prec != x->precision) {
+      // build the tile-blocked copy for this tile width and precision
+      const int G = x->precision == VR_COEF_F32 ? 4 : 8;
+      const int cb = x->precision == VR_COEF_F32 ? 4 : 2;
+      const int64_t nblk = tl.n_chunks * nt;
+      DBuf btot;
+      TRY(btot.alloc(8 * (size_t)nblk));
+      TRY(x->boff.alloc(4 * (size_t)nblk * 1028));
+      tl.boff = x->boff.as<int32_t>();
+      launch_block_offsets(dx, tl, G, btot.as<int64_t>(), x->boff.as<int32_t>(), st);
+      TRY(check_launch());
+      std::vector<int64_t> tot((size_t)nblk), start((size_t)nblk);
+      VR_HIP(hipMemcpyAsync(tot.data(), btot.p, 8 * (size_t)nblk, hipMemcpyDeviceToHost, st));
+      VR_HIP(hipStreamSynchronize(st));
+      int64_t run = 0;
+      for (int64_t b = 0; b < nblk; ++b) {
+        start[(size_t)b] = run;
+        run += tot[(size_t)b];
+      }
+      TRY(upload_vec(x->bstart, start, st));
+      TRY(x->bcol.alloc(4 * (size_t)std::max<int64_t>(run, 1) + 64));
+      TRY(x->bcoef.alloc((size_t)cb * (size_t)std::max<int64_t>(run, 1) + 64));
+      tl.bstart = x->bstart.as<int64_t>();
+      launch_block_fill(dx, tl, G, cb, x->bcol.as<int32_t>(), x->bcoef.p, st);
+      TRY(check_launch());
+      VR_HIP(hipStreamSynchronize(st));
+      x->blk_cols = cols;
+      x->blk_prec = x->precision;
+    }
+    tl.bcol = x->bcol.as<int32_t>();
+    tl.bcoef = x->bcoef.p;
+    tl.bstart = x->bstart.as<int64_t>();
+    tl.boff = x->boff.as<int32_t>();
+  }
   BounceState hs{0.0, 0, 0};
   const int chunk = g_path == VR_PATH_PLAIN ? 1 : 8;
   int k = 0;
@@ -903,8 +988,12 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
     for (; k < kend; ++k) {
       {
         Prof p("bounce_gather", st);
-        launch_bounce(dx, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row, counters + k,
-                      state, g_path, st);
+        if (tiled)
+          launch_bounce_tiled(tu.bounce_passes, dx, tl, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,
+                              state, st);
+        else
+          launch_bounce(dx, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,
+                        counters + k, state, g_path, st);
       }
       {
         Prof p("energy", st);

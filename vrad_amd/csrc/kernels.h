@@ -14,7 +14,13 @@ constexpr int kStackDepth = 24;  // > max kd depth (21 + 1 leaf level)
 // Launch-shape knobs (vr_set_tuning); every value gives identical results.
 struct Tuning {
   int trace_lds_entries = 8;   // traversal-stack entries per lane kept in LDS: 4/8/12/16/24
-  int bounce_waves = 8192;     // persistent waves of the bounce gather
+  int bounce_waves = 8192;     // persistent waves of the wave-per-row gather
+  // Bounce gather kernel and its shape.  Unlike the knobs above these fix the
+  // FP32 summation order (the CPU restatement follows the same settings).
+  int bounce_kernel = 0;       // 0 = LDS-tiled, 1 = wave per row
+  int bounce_tile_cols = 4096; // 2048 / 4096 patches per LDS tile
+  int bounce_group = 8;        // 8 / 16 lanes per row segment
+  int bounce_passes = 2;       // passes of a segment each lane loads one step ahead
 };
 Tuning &tuning();
 
@@ -60,6 +66,18 @@ struct DevXfers {
   const int32_t *order;    // rows sorted by length, longest first
   int kind;                // VR_COEF_*
 };
+struct DevTiles {
+  const int32_t *tp;       // n x (nt + 1) row positions of the tile boundaries
+  int nt, cols, group;
+  int64_t n_chunks;        // workgroups of 1024 rows
+  // tile-blocked copy: block (c, t) holds the segments of chunk c's rows for
+  // column tile t back to back in owner order u = 64 wave + lane, each padded
+  // to a multiple of G with zero coefficients
+  const int32_t *bcol;
+  const void *bcoef;
+  const int64_t *bstart;   // n_chunks * nt block starts
+  const int32_t *boff;     // n_chunks * nt blocks of 1025 segment offsets, stride 1028
+};
 struct BounceState {       // device-resident bounce bookkeeping
   double cum;
   int32_t converged;
@@ -68,6 +86,17 @@ struct BounceState {       // device-resident bounce bookkeeping
 void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, float *total,
                    const float *refl, const float *area, double *e_row, uint32_t *row_counter,
                    const BounceState *state, int path, hipStream_t st);
+bool bounce_tiled_variant(int cols, int group, int passes);
+void launch_bounce_tiled(int passes, const DevXfers &x, const DevTiles &tl, const float *emit_in,
+                         float *emit_out, float *total, const float *refl, const float *area,
+                         double *e_row, const BounceState *state, hipStream_t st);
+void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const int32_t *col,
+                      int cols, int nt, int32_t *tp, hipStream_t st);
+// blocked layout: offsets (boff) and per-block totals, then the copy
+void launch_block_offsets(const DevXfers &x, const DevTiles &tl, int G, int64_t *btot,
+                          int32_t *boff, hipStream_t st);
+void launch_block_fill(const DevXfers &x, const DevTiles &tl, int G, int coef_bytes,
+                       int32_t *bcol, void *bcoef, hipStream_t st);
 // energy of the bounce just gathered; updates state / energy[k]
 void launch_energy(const double *e_row, int64_t n, double *scratch, double *energy, int k,
                    double stop_frac, BounceState *state, hipStream_t st);
