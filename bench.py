@@ -8,7 +8,7 @@ resident in HBM.  One step = one 4M-segment batch through the visibility
 kernel.
 
 Also reported (same run): the traversal-stack autotune sweep, the plain path
-and an incoherent (random-pair) batch; BounceLight gather bandwidth on the
+and, with --random-pairs, an incoherent batch; BounceLight gather bandwidth on the
 precision-form transfer shape (250k rows, lognormal lengths, mean 2000, FP32
 and FP16) after a kernel-shape sweep; and the CPU restatement timed on the
 same segments.
@@ -155,14 +155,15 @@ def run_visibility(cx, args):
     check(lib.vr_set_path(L.VR_PATH_PLAIN))
     _, plain_ms = batch.run(3, 1)
     check(lib.vr_set_path(L.VR_PATH_OPT))
-    rb = Batch(*random_batch(s, args.segments, seed=20))
-    _, rand_ms = rb.run(3, 1)
-    del rb
+    extra = {"lds_entries_sweep_Mrays": sweep, "lds_entries": best_s,
+             "plain_path_Mrays": round(n / (plain_ms * 1e-3) / 1e6, 1)}
+    if args.random_pairs:
+        rb = Batch(*random_batch(s, args.segments, seed=20))
+        _, rand_ms = rb.run(3, 1)
+        del rb
+        extra["random_pairs_Mrays"] = round(n / (rand_ms * 1e-3) / 1e6, 1)
     return dict(s=s, st=st, en=en, n=n, vis=vis, elapsed=elapsed, kern_ms=kern_ms,
-                info=info, build_s=build_s,
-                extra={"lds_entries_sweep_Mrays": sweep, "lds_entries": best_s,
-                       "plain_path_Mrays": round(n / (plain_ms * 1e-3) / 1e6, 1),
-                       "random_pairs_Mrays": round(n / (rand_ms * 1e-3) / 1e6, 1)})
+                info=info, build_s=build_s, extra=extra)
 
 
 def run_cpu(v, args):
@@ -210,32 +211,53 @@ def run_bounce(cx, args):
         check(lib.vr_profile_enable(0))
         return cx.kernel_ms(b"bounce_gather")
 
-    # (kernel, tile cols, group, passes loaded one step ahead)
-    shapes = [(0, 4096, 8, 2), (0, 4096, 8, 1), (0, 4096, 16, 2), (0, 4096, 16, 1),
+    # (kernel, tile cols, group, passes loaded ahead of use)
+    shapes = [(0, 4096, 8, 2), (0, 4096, 16, 2), (0, 4096, 16, 1), (0, 4096, 8, 3),
               (0, 2048, 8, 2), (0, 2048, 16, 2), (1, 4096, 8, 2)]
 
     def shape_name(shp):
         return "tiled%dx%d-p%d" % shp[1:] if shp[0] == 0 else "wave"
 
+    # best shapes of earlier sweeps, used when the sweep is skipped
+    default_shape = {"fp32": (0, 4096, 16, 2), "fp16": (0, 4096, 8, 2)}
+
     out = {}
     for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4)):
         xf.set_precision(prec)
-        B = T * (4 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
+        # stored layout read by the tiled gather: 16-bit column (LDS offset)
+        # + coefficient per transfer; per patch row_ptr 8, emit read 12,
+        # reflectivity 12, emit write 12, total read+write 24, scale
+        B = T * (2 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
+        B_i32 = T * (4 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
         sweep = {}
-        for shp in (shapes if args.bounce_sweep else shapes[:1]):
+        for shp in (shapes if args.bounce_sweep else [default_shape[name]]):
             set_bounce(*shp)
             key = shape_name(shp)
             sweep[key] = round(bounce_ms(3), 4)
         best = min(sweep, key=sweep.get)
-        set_bounce(*shapes[list(sweep).index(best)])
+        set_bounce(*{shape_name(x): x for x in shapes}[best])
         per = bounce_ms(8)
         gbps = B / (per * 1e-3) / 1e9
         out[name] = {"ms_per_bounce": round(per, 4), "GBps": round(gbps, 1),
                      "frac": round(gbps / HBM_PEAK_GBPS, 4), "bytes_per_bounce": B,
-                     "transfers": T, "rows": N, "kernel": best, "sweep_ms": sweep}
+                     "bytes_per_bounce_int32_index": B_i32,
+                     "transfers": T, "rows": N, "kernel": best, "sweep_ms": sweep,
+                     "traffic": measured_traffic("bounce_gather_" + name)[0]}
     set_bounce(*shapes[0])
     xf.close()
     return out
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/traffic.json, written from the PMC runs and
+    scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+        return t[kernel]["bytes_per_launch"], t[kernel]["source"]
+    except (OSError, KeyError, ValueError):
+        return None, None
 
 
 def main():
@@ -249,14 +271,16 @@ def main():
     ap.add_argument("--no-bounce", action="store_true")
     ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--random-pairs", action="store_true",
+                    help="also time an incoherent batch of random patch pairs")
     ap.add_argument("--bounce-sweep", type=int, default=1,
                     help="sweep the bounce kernel shapes (autotune) before timing")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="finish with a device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; runs on more than one GPU are not part of "
+              "this benchmark (the multi-GPU sharding is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
     cx = Ctx()
@@ -289,7 +313,7 @@ def main():
             "ms_per_step": round(v["elapsed"] / args.steps * 1e3, 4),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
-                         "traffic": None, "kernel": "test_line",
+                         "traffic": measured_traffic("test_line")[0], "kernel": "test_line",
                          "kernel_ms": round(v["kern_ms"], 4),
                          "algorithmic_bytes_per_launch": SEG_BYTES * n},
             "visibility": v["extra"],

@@ -8,7 +8,8 @@
  *
  * Each declaration names the reference (Galaco/VRAD, Go) interface it
  * replaces as file:line; where the reference has a stub or nothing, the
- * comment says so.  The Go-side binding is shown in INTEGRATION.md.
+ * comment says so.  A Go caller binds these with cgo, one thin wrapper per
+ * reference function that turns the status into a Go error.
  *
  * Threading: calls are serialised by one library lock; the host-pointer
  * forms are synchronous, the *_dev forms are asynchronous on the stream
@@ -82,7 +83,7 @@ int vr_set_path(int path);
  *   "bounce_kernel"      0 = LDS-tiled gather (default), 1 = wave per row
  *   "bounce_tile_cols"   2048 / 4096 patches per LDS tile
  *   "bounce_group"       8 / 16 lanes per row segment
- *   "bounce_passes"      1 / 2 passes of a row segment loaded one step ahead
+ *   "bounce_passes"      1 / 2 / 3 passes of a row segment loaded ahead of use
  * The tile width and group fix the summation order; the other bounce knobs
  * change only the schedule.  vr_bounce_light fails with VR_E_UNSUPPORTED for a
  * combination that has no compiled kernel. */

@@ -54,9 +54,11 @@ def _room_bake(room_scene):
     return xf, direct, emit0
 
 
-# (kernel, tile cols, group, passes loaded one step ahead)
-KERNELS = [(0, 4096, 8, 2), (0, 4096, 16, 1), (0, 2048, 16, 2), (0, 2048, 8, 2), (1, 4096, 8, 2)]
-KERNEL_IDS = ["tiled4096x8", "tiled4096x16-p1", "tiled2048x16", "tiled2048x8", "wave"]
+# (kernel, tile cols, group, passes loaded ahead of use)
+KERNELS = [(0, 4096, 8, 2), (0, 4096, 16, 2), (0, 4096, 16, 1), (0, 4096, 8, 3), (0, 2048, 16, 2),
+           (0, 2048, 8, 2), (1, 4096, 8, 2)]
+KERNEL_IDS = ["tiled4096x8", "tiled4096x16", "tiled4096x16-p1", "tiled4096x8-p3", "tiled2048x16",
+              "tiled2048x8", "wave"]
 KNOBS = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes")
 
 

@@ -162,49 +162,59 @@ void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, flo
 
 // ---- LDS-tiled gather ------------------------------------------------------
 // The emit vector is cut into column tiles of C patches that are staged in
-// LDS.  A workgroup of W waves owns 64 W rows: chunk c takes rows
+// LDS.  A workgroup of 16 waves owns up to 1024 rows: chunk c takes rows
 // order[c + k * n_chunks], a stride through the longest-first order, so every
 // chunk and every wave gets the same mix of lengths; slot j of wave w is
-// chunk row k = W j + w.  For each tile, the wave's 64 lanes form 64/GS groups
-// of GS lanes; in step s group gi takes slot j = s (64/GS) + gi, i.e. that
-// row's segment (its entries with columns in the tile, a contiguous range
-// because columns are sorted).  Lane l of the group owns the G-entry chunks at
-// segment positions G l + G GS i, FMA-accumulates them in order, the group
-// butterflies (GS/2 .. 1), and lane s of the group, which owns slot j, adds
-// the segment sum to its row total; tiles in order.  Coefficients come from a
-// tile-blocked copy: block (c, t) holds the segments of chunk c for tile t
-// back to back in slot order (u = 64 w + j), each padded to a multiple of G
-// with (col = tile start, coef = 0), so every 16-byte load is full and the
-// loads of one step cover a few contiguous lines.  Each lane loads its first
-// P passes (P GS G entries of the segment) one step ahead.
-// One G-entry chunk of a row: raw 16-byte loads, decoded when consumed so the
-// loads of the next step stay in flight meanwhile.
+// chunk row k = 16 j + w.  For each tile the wave's 64 lanes form 64/GS
+// groups of GS lanes; in step s group gi takes slot j = s (64/GS) + gi, i.e.
+// that row's segment (its entries with columns in the tile, a contiguous
+// range because columns are sorted).  Lane l of the group owns the G-entry
+// chunks at segment positions G l + G GS i, FMA-accumulates them in order,
+// the group butterflies (GS/2 .. 1), and lane s of the group, which owns slot
+// j, adds the segment sum to its row total; tiles in order.
+//
+// Coefficients come from a tile-blocked copy: block (c, t) holds the segments
+// of chunk c for tile t back to back in slot order (u = 64 w + j), each
+// padded to a multiple of G with (offset 0, coef 0), so every load is a whole
+// vector and the loads of one step cover a few contiguous lines.  Columns are
+// stored as 16-bit byte offsets into the LDS emit tile.
+//
+// Pipeline (per wave): the loads of step s + 1 are issued before step s is
+// gathered, into the other of two register slots, and every load is issued
+// unconditionally (a lane with nothing to load reads the block's first chunk
+// and ignores it), so the in-order memory counter waits only for the step it
+// needs.  The wave's segment offsets of tile t + 1 are loaded into registers
+// during tile t, so tile t + 1's first loads leave before the tile barrier.
+// The emit tile t + 1 is copied into the second LDS buffer by DMA at the start
+// of tile t.
+// One G-entry chunk of the tile-blocked copy: 16-bit LDS offsets and the
+// coefficients, loaded with 32-bit offsets from wave-uniform block bases.
 template <int KIND>
-struct Chunk;
+struct BChunk;
 template <>
-struct Chunk<VR_COEF_F32> {
-  int4 c0;
+struct BChunk<VR_COEF_F32> {
+  uint2 o;
   float4 w;
-  __device__ void load(const DevXfers &x, int64_t k) {
-    c0 = *reinterpret_cast<const int4 *>(x.col + k);
-    w = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(x.coef) + k);
+  __device__ void load(const char *ib, const char *cb, uint32_t q) {
+    o = *reinterpret_cast<const uint2 *>(ib + 2u * q);
+    w = *reinterpret_cast<const float4 *>(cb + 4u * q);
   }
-  __device__ int col(int m) const { return m == 0 ? c0.x : m == 1 ? c0.y : m == 2 ? c0.z : c0.w; }
+  __device__ uint32_t off(int m) const {
+    const uint32_t d = m < 2 ? o.x : o.y;
+    return (m & 1) ? (d >> 16) : (d & 0xffffu);
+  }
   __device__ float coef(int m) const { return m == 0 ? w.x : m == 1 ? w.y : m == 2 ? w.z : w.w; }
 };
 template <int KIND>
-struct Chunk {
-  int4 c0, c1;
-  uint4 h;
-  __device__ void load(const DevXfers &x, int64_t k) {
-    c0 = *reinterpret_cast<const int4 *>(x.col + k);
-    c1 = *reinterpret_cast<const int4 *>(x.col + k + 4);
-    h = *reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(x.coef) + k);
+struct BChunk {
+  uint4 o, h;
+  __device__ void load(const char *ib, const char *cb, uint32_t q) {
+    o = *reinterpret_cast<const uint4 *>(ib + 2u * q);
+    h = *reinterpret_cast<const uint4 *>(cb + 2u * q);
   }
-  __device__ int col(int m) const {
-    const int4 &c = m < 4 ? c0 : c1;
-    const int mm = m & 3;
-    return mm == 0 ? c.x : mm == 1 ? c.y : mm == 2 ? c.z : c.w;
+  __device__ uint32_t off(int m) const {
+    const uint32_t d = (m >> 1) == 0 ? o.x : (m >> 1) == 1 ? o.y : (m >> 1) == 2 ? o.z : o.w;
+    return (m & 1) ? (d >> 16) : (d & 0xffffu);
   }
   __device__ float coef(int m) const {
     const uint32_t wd = (m >> 1) == 0 ? h.x : (m >> 1) == 1 ? h.y : (m >> 1) == 2 ? h.z : h.w;
@@ -250,13 +260,13 @@ __device__ __forceinline__ void dma16(const void *src, void *lds_base) {
                                    0);
 }
 
-template <int KIND, int C>
-__device__ __forceinline__ void gather_chunk(const Chunk<KIND> &ch, const float *__restrict__ te,
-                                             int p0, float &r, float &g, float &b) {
+template <int KIND>
+__device__ __forceinline__ void gather_bchunk(const BChunk<KIND> &ch, const char *te, float &r,
+                                              float &g, float &b) {
   constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
 #pragma unroll
   for (int m = 0; m < G; ++m) {
-    const float *e = te + 3 * (ch.col(m) - p0);
+    const float *e = reinterpret_cast<const float *>(te + ch.off(m));
     const float w = ch.coef(m);
     r = __builtin_fmaf(w, e[0], r);
     g = __builtin_fmaf(w, e[1], g);
@@ -266,18 +276,23 @@ __device__ __forceinline__ void gather_chunk(const Chunk<KIND> &ch, const float 
 
 template <int KIND, int P>
 struct StepLoad {
-  Chunk<KIND> ch[P];
-  int q = 0, sb = 0;
+  BChunk<KIND> ch[P];
+  int q, sb;
 };
 
-// One workgroup of 16 waves per chunk of 1024 rows; emit tiles (r, g, b per
-// patch, as in global memory) and the block's segment offsets are copied into
-// two LDS buffers by DMA, tile t + 1 while tile t is gathered.  The copy reads
-// emit_in in whole 16-byte pieces, so emit_in must stay readable up to the
-// next 16-byte boundary past its 12 n bytes (the bounce buffers are padded to
-// 256 bytes).
+template <int V>
+struct IntC {
+  static constexpr int value = V;
+};
+
+// One workgroup of 16 waves per chunk; emit tiles (r, g, b per patch, as in
+// global memory) are copied into two LDS buffers by DMA, tile t + 1 while
+// tile t is gathered.  The copy reads emit_in in whole 16-byte pieces, so
+// emit_in must stay readable up to the next 16-byte boundary past its 12 n
+// bytes (the bounce buffers are padded to 256 bytes).
 template <int KIND, int C, int GS, int P>
 __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
+                                                       const int64_t *__restrict__ bstart,
                                                        const float *__restrict__ emit_in,
                                                        float *__restrict__ emit_out,
                                                        float *__restrict__ total,
@@ -288,76 +303,100 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
   if (state->converged) return;
   constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
   constexpr int T = 1024;              // threads
-  constexpr int R = 1024;              // rows per chunk
-  constexpr int RS = R + 4;            // stride of a block's offsets (16-byte multiple)
+  constexpr int RS = 1028;             // stride of a block's segment offsets
   constexpr int NG = 64 / GS;          // rows per step (one per lane group)
   constexpr int NS = GS;               // steps per tile: 64 slots / NG
   constexpr int PASS = GS * G;         // entries per pass of a group
   constexpr int EP = 3 * C / 4;        // 16-byte pieces of a full emit tile
-  constexpr int OP = (R + 1 + 3) / 4;  // 16-byte pieces of a block's offsets
-  constexpr int ER = (EP + T - 1) / T;  // copy rounds per tile
-  __shared__ __attribute__((aligned(16))) float s_e[2][3 * C];
-  __shared__ __attribute__((aligned(16))) int32_t s_off[2][4 * OP];
+  constexpr int ER = (EP + T - 1) / T; // copy rounds per tile
+  static_assert(NS % 2 == 0, "steps alternate between two load slots");
+  __shared__ __attribute__((aligned(16))) float s_e[2][4 * T * ER];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int gi = lane / GS, gl = lane % GS;
   const int64_t c = blockIdx.x;
+  const int nt = tl.nt;
   const int j_own = gl * NG + gi;  // the slot whose row total this lane keeps
   const int64_t oi = c + ((int64_t)j_own * 16 + wave) * tl.n_chunks;
   const int64_t row = oi < x.n ? (int64_t)x.order[oi] : -1;
-  DevXfers xb = x;  // blocked arrays, same chunk decoders
-  xb.col = tl.bcol;
-  xb.coef = tl.bcoef;
+  const char *bidx = reinterpret_cast<const char *>(tl.bidx);
+  const char *bcoef = reinterpret_cast<const char *>(tl.bcoef);
+  const int cbytes = KIND == VR_COEF_F32 ? 4 : 2;
   float ar = 0.0f, ag = 0.0f, ab = 0.0f;
 
-  auto fetch_tile = [&](int t, int buf) {
-    const int64_t p1 = (int64_t)t * C;
+  // emit tile tn -> LDS buffer buf, every lane issuing ER pieces (lanes past
+  // the tile's end copy its last piece again into their own slots)
+  auto fetch_emit = [&](int tn, int buf) {
+    const int64_t p1 = (int64_t)tn * C;
     const int np = (int)min((int64_t)C, x.n - p1);
-    const int pieces = (12 * np + 15) / 16;
+    const int last = (12 * np + 15) / 16 - 1;
     const char *src = reinterpret_cast<const char *>(emit_in + 3 * p1);
 #pragma unroll
     for (int m = 0; m < ER; ++m) {
-      const int k = (int)threadIdx.x + T * m;
-      if (k < pieces) dma16(src + 16 * k, &s_e[buf][4 * (T * m + 64 * wave)]);
+      const int k = min((int)threadIdx.x + T * m, last);
+      dma16(src + 16 * k, &s_e[buf][4 * (T * m + 64 * wave)]);
     }
-    const char *osrc = reinterpret_cast<const char *>(tl.boff + (c * tl.nt + t) * (int64_t)RS);
-    if ((int)threadIdx.x < OP) dma16(osrc + 16 * threadIdx.x, &s_off[buf][4 * 64 * wave]);
   };
-  fetch_tile(0, 0);
-
-  for (int t = 0; t < tl.nt; ++t) {
-    const int buf = t & 1;
-    const int p0 = t * C;
-    const int64_t bs = tl.bstart[c * tl.nt + t];
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of tile t have landed
-    __syncthreads();                // ... and every wave's; tile t - 1 is done
-    if (t + 1 < tl.nt) fetch_tile(t + 1, buf ^ 1);
-    const float *te = s_e[buf];
-    const int32_t *so = s_off[buf];
-
-    auto issue = [&](int s) {
-      StepLoad<KIND, P> L;
-      const int u = wave * 64 + s * NG + gi;
-      L.q = so[u] + gl * G;
-      L.sb = so[u + 1];
+  // this lane's slot offsets (start, end) in block (c, tn)
+  auto slot_offsets = [&](int tn, int &sa, int &sb) {
+    const int32_t *bo = tl.boff + (c * nt + tn) * (int64_t)RS + 64 * wave + lane;
+    sa = bo[0];
+    sb = bo[1];
+  };
+  // loads of step s of a tile with block start bs, slot offsets (oa, ob)
+  auto issue = [&](int s, int64_t bs, int oa, int ob, StepLoad<KIND, P> &L) {
+    const int j = s * NG + gi;
+    const int sa = __shfl(oa, j);
+    L.sb = __shfl(ob, j);
+    L.q = sa + gl * G;
+    const char *ib = bidx + 2 * bs;
+    const char *cb = bcoef + cbytes * bs;
 #pragma unroll
-      for (int k = 0; k < P; ++k)
-        if (L.q + k * PASS < L.sb) L.ch[k].load(xb, bs + L.q + k * PASS);
-      return L;
-    };
-    StepLoad<KIND, P> cur = issue(0);
-#pragma unroll 1
-    for (int st = 0; st < NS; ++st) {
-      StepLoad<KIND, P> nxt;
-      if (st + 1 < NS) nxt = issue(st + 1);
+    for (int k = 0; k < P; ++k) {
+      const int qk = L.q + k * PASS;
+      L.ch[k].load(ib, cb, qk < L.sb ? (uint32_t)qk : 0u);
+    }
+  };
+
+  int oa, ob;
+  slot_offsets(0, oa, ob);
+  int64_t bs = bstart[c * nt];
+  StepLoad<KIND, P> A, B;
+  fetch_emit(0, 0);
+  issue(0, bs, oa, ob, A);
+  __builtin_amdgcn_s_waitcnt(0);  // the first tile's copy (fewer loads follow it)
+
+  // Before a tile's barrier each wave needs only its own copy of the tile to
+  // have landed (and its LDS reads of the previous tile done).  The copy was
+  // issued before the 2 P NS chunk loads of the previous tile's steps, and the
+  // memory counter retires in order, so vmcnt <= min(63, 2 P NS) implies it.
+  constexpr int WAIT_N = 2 * P * NS < 63 ? 2 * P * NS : 63;
+  constexpr int WAIT_IMM = (WAIT_N & 15) | ((WAIT_N >> 4) << 14) | (7 << 4);  // lgkmcnt 0
+  auto tile = [&](auto bufc, int t) {
+    constexpr int buf = decltype(bufc)::value;
+    __builtin_amdgcn_s_waitcnt(WAIT_IMM);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();  // every wave's copy; tile t - 1 is done
+    __asm__ volatile("" ::: "memory");
+    const int tn = min(t + 1, nt - 1);
+    fetch_emit(tn, buf ^ 1);       // on the last tile a harmless re-copy
+    __asm__ volatile("" ::: "memory");  // the step loads below stay after the copy
+    int oa_n, ob_n;
+    slot_offsets(tn, oa_n, ob_n);
+    const int64_t bs_n = bstart[c * nt + tn];
+    const char *te = reinterpret_cast<const char *>(s_e[buf]);
+
+    auto step = [&](int st, StepLoad<KIND, P> &cur, StepLoad<KIND, P> &nxt) {
+      if (st + 1 < NS) issue(st + 1, bs, oa, ob, nxt);
+      else issue(0, bs_n, oa_n, ob_n, nxt);  // next tile's first step
       float r = 0.0f, g = 0.0f, b = 0.0f;
 #pragma unroll
       for (int k = 0; k < P; ++k)
-        if (cur.q + k * PASS < cur.sb) gather_chunk<KIND, C>(cur.ch[k], te, p0, r, g, b);
+        if (cur.q + k * PASS < cur.sb) gather_bchunk<KIND>(cur.ch[k], te, r, g, b);
       for (int q = cur.q + P * PASS; q < cur.sb; q += PASS) {
-        Chunk<KIND> more;
-        more.load(xb, bs + q);
-        gather_chunk<KIND, C>(more, te, p0, r, g, b);
+        BChunk<KIND> more;
+        more.load(bidx + 2 * bs, bcoef + cbytes * bs, (uint32_t)q);
+        gather_bchunk<KIND>(more, te, r, g, b);
       }
       r = group_sum<GS>(r);
       g = group_sum<GS>(g);
@@ -367,9 +406,22 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
         ag = ag + g;
         ab = ab + b;
       }
-      cur = nxt;
+    };
+    // straight-line steps: every wait on the in-order memory counter is exact
+#pragma unroll
+    for (int st = 0; st < NS; st += 2) {
+      step(st, A, B);
+      step(st + 1, B, A);
     }
+    oa = oa_n;
+    ob = ob_n;
+    bs = bs_n;
+  };
+  for (int t = 0; t < nt; t += 2) {
+    tile(IntC<0>{}, t);
+    if (t + 1 < nt) tile(IntC<1>{}, t + 1);
   }
+  __builtin_amdgcn_s_waitcnt(0);  // the last (unused) copies and loads
   if (row >= 0) {
     if constexpr (KIND != VR_COEF_F32) {
       const float sc = x.scale[row];
@@ -426,10 +478,11 @@ __global__ __launch_bounds__(1024) void k_block_offsets(DevXfers x, DevTiles tl,
   }
 }
 
-// Blocked layout, pass 2: copy each owner's segment, pad with (col = tile
-// start, coef = 0).  One wave per (block, owner slot).
+// Blocked layout, pass 2: copy each owner's segment with its columns turned
+// into LDS byte offsets 12 (col - t C); pad with (offset 0, coef 0).  One
+// wave per (block, owner slot).
 __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int G,
-                                                    int coef_bytes, int32_t *__restrict__ bcol,
+                                                    int coef_bytes, uint16_t *__restrict__ bidx,
                                                     void *__restrict__ bcoef) {
   const int lane = threadIdx.x & 63;
   const int R = 1024, RS = R + 4;
@@ -449,10 +502,10 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
   const int32_t *bo = tl.boff + blk * RS;
   const int64_t dst = tl.bstart[blk] + bo[u];
   const int plen = bo[u + 1] - bo[u];
-  const int pad_col = t * tl.cols;
+  const int p0 = t * tl.cols;
   for (int q = lane; q < plen; q += 64) {
     const bool real = q < len;
-    bcol[dst + q] = real ? x.col[src + q] : pad_col;
+    bidx[dst + q] = real ? (uint16_t)(12 * (x.col[src + q] - p0)) : (uint16_t)0;
     if (coef_bytes == 4)
       reinterpret_cast<float *>(bcoef)[dst + q] =
           real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;
@@ -471,11 +524,11 @@ void launch_block_offsets(const DevXfers &x, const DevTiles &tl, int G, int64_t 
 }
 
 void launch_block_fill(const DevXfers &x, const DevTiles &tl, int G, int coef_bytes,
-                       int32_t *bcol, void *bcoef, hipStream_t st) {
+                       uint16_t *bidx, void *bcoef, hipStream_t st) {
   const int64_t waves = tl.n_chunks * (int64_t)tl.nt * 1024;
   if (waves <= 0) return;
   hipLaunchKernelGGL(k_block_fill, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, x, tl, G,
-                     coef_bytes, bcol, bcoef);
+                     coef_bytes, bidx, bcoef);
 }
 
 // tile boundaries per row: tp[r][t] = first row position with column >= t*C
@@ -510,9 +563,9 @@ void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const in
 // Compiled shapes (C, GS, P) of the LDS-tiled gather.
 #define VR_TILED_SHAPES(X) \
   X(4096, 8, 2)            \
-  X(4096, 8, 1)            \
   X(4096, 16, 2)           \
   X(4096, 16, 1)           \
+  X(4096, 8, 3)            \
   X(2048, 8, 2)            \
   X(2048, 16, 2)
 
@@ -530,8 +583,8 @@ void launch_bounce_tiled(int passes, const DevXfers &x, const DevTiles &tl, cons
   if (x.n <= 0) return;
   const dim3 g((unsigned)tl.n_chunks);
 #define VR_GO(K, CC, GG, PP)                                                                  \
-  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP>), g, dim3(1024), 0, st, x, tl, emit_in,   \
-                     emit_out, total, refl, area, e_row, state)
+  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP>), g, dim3(1024), 0, st, x, tl, tl.bstart, \
+                     emit_in, emit_out, total, refl, area, e_row, state)
 #define VR_SHAPE(CC, GG, PP)                                                \
   if (tl.cols == CC && tl.group == GG && passes == PP) {                    \
     if (x.kind == VR_COEF_F32) VR_GO(VR_COEF_F32, CC, GG, PP);              \

@@ -86,6 +86,8 @@ struct Prof {
       return fail(VR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
   } while (0)
 
+int g_cus = 256;
+
 int ensure_init() {
   if (g_init) return VR_OK;
   int n = 0;
@@ -93,6 +95,10 @@ int ensure_init() {
     return fail(VR_E_NODEVICE, "no HIP device available");
   VR_HIP(hipSetDevice(g_device));
   VR_HIP(hipStreamCreate(&g_stream));
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device) == hipSuccess &&
+      cus > 0)
+    g_cus = cus;
   g_init = true;
   return VR_OK;
 }
@@ -175,8 +181,9 @@ struct vr_xfers {
   DBuf tp;            // tile boundaries for the LDS-tiled gather
   int tp_cols = 0;
   // tile-blocked copy of the coefficients for the LDS-tiled gather
-  DBuf bcol, bcoef, bstart, boff;
+  DBuf bidx, bcoef, bstart, boff;
   int blk_cols = 0, blk_prec = -1;
+  int64_t blk_chunks = 0;
   DevXfers dev() const {
     DevXfers x;
     x.n = n;
@@ -303,7 +310,7 @@ int vr_set_tuning(const char *name, int value) {
     if (value != 8 && value != 16) return fail(VR_E_INVALID, "bounce_group must be 8 or 16");
     tuning().bounce_group = value;
   } else if (k == "bounce_passes") {
-    if (value != 1 && value != 2) return fail(VR_E_INVALID, "bounce_passes must be 1 or 2");
+    if (value < 1 || value > 3) return fail(VR_E_INVALID, "bounce_passes must be 1, 2 or 3");
     tuning().bounce_passes = value;
 
   } else {
@@ -936,7 +943,9 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
     tl.group = tu.bounce_group;
     if (!bounce_tiled_variant(cols, tl.group, tu.bounce_passes))
       return fail(VR_E_UNSUPPORTED, "no compiled LDS-tiled gather for this tuning shape");
-    tl.n_chunks = (n + 1023) / 1024;
+    // chunks of at most 1024 rows, and at least one per compute unit while a
+    // chunk keeps >= 256 rows (the chunk count changes the schedule only)
+    tl.n_chunks = std::max<int64_t>((n + 1023) / 1024, std::min<int64_t>(g_cus, (n + 255) / 256));
     if (x->tp_cols != cols || !x->tp.p) {
       TRY(x->tp.alloc(4 * (size_t)std::max<int64_t>(n, 1) * (size_t)(nt + 1)));
       launch_tile_ptrs(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), x->col.as<int32_t>(), cols,
@@ -946,7 +955,7 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
       x->blk_cols = 0;
     }
     tl.tp = x->tp.as<int32_t>();
-    if (x->blk_cols != cols || x->blk_prec != x->precision) {
+    if (x->blk_cols != cols || x->blk_prec != x->precision || x->blk_chunks != tl.n_chunks) {
       // build the tile-blocked copy for this tile width and precision
       const int G = x->precision == VR_COEF_F32 ? 4 : 8;
       const int cb = x->precision == VR_COEF_F32 ? 4 : 2;
@@ -966,16 +975,17 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
         run += tot[(size_t)b];
       }
       TRY(upload_vec(x->bstart, start, st));
-      TRY(x->bcol.alloc(4 * (size_t)std::max<int64_t>(run, 1) + 64));
+      TRY(x->bidx.alloc(2 * (size_t)std::max<int64_t>(run, 1) + 64));
       TRY(x->bcoef.alloc((size_t)cb * (size_t)std::max<int64_t>(run, 1) + 64));
       tl.bstart = x->bstart.as<int64_t>();
-      launch_block_fill(dx, tl, G, cb, x->bcol.as<int32_t>(), x->bcoef.p, st);
+      launch_block_fill(dx, tl, G, cb, x->bidx.as<uint16_t>(), x->bcoef.p, st);
       TRY(check_launch());
       VR_HIP(hipStreamSynchronize(st));
       x->blk_cols = cols;
       x->blk_prec = x->precision;
+      x->blk_chunks = tl.n_chunks;
     }
-    tl.bcol = x->bcol.as<int32_t>();
+    tl.bidx = x->bidx.as<uint16_t>();
     tl.bcoef = x->bcoef.p;
     tl.bstart = x->bstart.as<int64_t>();
     tl.boff = x->boff.as<int32_t>();

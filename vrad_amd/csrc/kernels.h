@@ -20,7 +20,7 @@ struct Tuning {
   int bounce_kernel = 0;       // 0 = LDS-tiled, 1 = wave per row
   int bounce_tile_cols = 4096; // 2048 / 4096 patches per LDS tile
   int bounce_group = 8;        // 8 / 16 lanes per row segment
-  int bounce_passes = 2;       // passes of a segment each lane loads one step ahead
+  int bounce_passes = 2;       // passes of a segment each lane loads ahead of use
 };
 Tuning &tuning();
 
@@ -72,8 +72,9 @@ struct DevTiles {
   int64_t n_chunks;        // workgroups of 1024 rows
   // tile-blocked copy: block (c, t) holds the segments of chunk c's rows for
   // column tile t back to back in owner order u = 64 wave + lane, each padded
-  // to a multiple of G with zero coefficients
-  const int32_t *bcol;
+  // to a multiple of G with zero coefficients; an entry's column is stored as
+  // its 16-bit byte offset in the LDS emit tile, 12 (col - t C)
+  const uint16_t *bidx;
   const void *bcoef;
   const int64_t *bstart;   // n_chunks * nt block starts
   const int32_t *boff;     // n_chunks * nt blocks of 1025 segment offsets, stride 1028
@@ -96,7 +97,7 @@ void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const in
 void launch_block_offsets(const DevXfers &x, const DevTiles &tl, int G, int64_t *btot,
                           int32_t *boff, hipStream_t st);
 void launch_block_fill(const DevXfers &x, const DevTiles &tl, int G, int coef_bytes,
-                       int32_t *bcol, void *bcoef, hipStream_t st);
+                       uint16_t *bidx, void *bcoef, hipStream_t st);
 // energy of the bounce just gathered; updates state / energy[k]
 void launch_energy(const double *e_row, int64_t n, double *scratch, double *energy, int k,
                    double stop_frac, BounceState *state, hipStream_t st);
