@@ -198,7 +198,8 @@ def run_bounce(cx, args):
     xf = Transfers.synthetic(seed=3, n_rows=N, mean_len=2000.0, sigma=1.0, max_len=40000)
     T = xf.info().nnz
 
-    knobs = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes")
+    knobs = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes",
+             b"bounce_prefetch")
 
     def set_bounce(*shape):
         for k, val in zip(knobs, shape):
@@ -211,15 +212,16 @@ def run_bounce(cx, args):
         check(lib.vr_profile_enable(0))
         return cx.kernel_ms(b"bounce_gather")
 
-    # (kernel, tile cols, group, passes loaded ahead of use)
-    shapes = [(0, 4096, 8, 2), (0, 4096, 16, 2), (0, 4096, 16, 1), (0, 4096, 8, 3),
-              (0, 2048, 8, 2), (0, 2048, 16, 2), (1, 4096, 8, 2)]
+    # (kernel, tile cols, group, passes loaded ahead of use, steps loads run ahead)
+    shapes = [(0, 4096, 16, 1, 1), (0, 4096, 16, 1, 3), (0, 4096, 16, 2, 1), (0, 4096, 8, 2, 1),
+              (0, 4096, 8, 2, 3), (0, 4096, 8, 1, 3), (0, 2048, 8, 2, 1), (0, 2048, 16, 2, 1),
+              (1, 4096, 8, 2, 1)]
 
     def shape_name(shp):
-        return "tiled%dx%d-p%d" % shp[1:] if shp[0] == 0 else "wave"
+        return "tiled%dx%d-p%d-d%d" % shp[1:] if shp[0] == 0 else "wave"
 
     # best shapes of earlier sweeps, used when the sweep is skipped
-    default_shape = {"fp32": (0, 4096, 16, 2), "fp16": (0, 4096, 8, 2)}
+    default_shape = {"fp32": (0, 4096, 16, 1, 3), "fp16": (0, 4096, 8, 2, 3)}
 
     out = {}
     for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4)):

@@ -82,8 +82,11 @@ int vr_set_path(int path);
  * restatement takes the same values):
  *   "bounce_kernel"      0 = LDS-tiled gather (default), 1 = wave per row
  *   "bounce_tile_cols"   2048 / 4096 patches per LDS tile
- *   "bounce_group"       8 / 16 lanes per row segment
- *   "bounce_passes"      1 / 2 / 3 passes of a row segment loaded ahead of use
+ *   "bounce_group"       8 / 16 lanes per row segment; 0 (default) = 16 for
+ *                        FP32 coefficients, 8 for FP16/BF16
+ *   "bounce_passes"      1 / 2 passes of a row segment loaded ahead of use;
+ *                        0 (default) = 1
+ *   "bounce_prefetch"    1 / 3 (default) steps the gather's loads run ahead
  * The tile width and group fix the summation order; the other bounce knobs
  * change only the schedule.  vr_bounce_light fails with VR_E_UNSUPPORTED for a
  * combination that has no compiled kernel. */

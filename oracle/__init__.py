@@ -223,9 +223,12 @@ ORDERS = {"wave": 0, "sequential": 1, "tiled": 2}
 
 
 def bounce_f32(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, stop_frac=1e-3,
-               kind=0, row_scale=None, threads=1, order="tiled", tile_cols=4096, group=8):
+               kind=0, row_scale=None, threads=1, order="tiled", tile_cols=4096, group=None):
     """FP32 bounce in a kernel's summation order (default: the LDS-tiled
-    gather with 4096-patch tiles and 8-lane groups)."""
+    gather with 4096-patch tiles, 16-lane groups for FP32 coefficients and
+    8-lane groups for FP16/BF16)."""
+    if group is None:
+        group = 16 if kind == 0 else 8
     n = len(rp) - 1
     rp = np.ascontiguousarray(rp, np.int64)
     col = np.ascontiguousarray(col, np.int32)

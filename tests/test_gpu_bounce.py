@@ -54,12 +54,14 @@ def _room_bake(room_scene):
     return xf, direct, emit0
 
 
-# (kernel, tile cols, group, passes loaded ahead of use)
-KERNELS = [(0, 4096, 8, 2), (0, 4096, 16, 2), (0, 4096, 16, 1), (0, 4096, 8, 3), (0, 2048, 16, 2),
-           (0, 2048, 8, 2), (1, 4096, 8, 2)]
-KERNEL_IDS = ["tiled4096x8", "tiled4096x16", "tiled4096x16-p1", "tiled4096x8-p3", "tiled2048x16",
-              "tiled2048x8", "wave"]
-KNOBS = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes")
+# (kernel, tile cols, group, passes loaded ahead of use, steps loads run ahead)
+KERNELS = [(0, 4096, 8, 2, 1), (0, 4096, 16, 1, 3), (0, 4096, 16, 2, 1), (0, 4096, 8, 2, 3),
+           (0, 2048, 16, 2, 1), (0, 2048, 8, 2, 1), (1, 4096, 8, 2, 1)]
+KERNEL_IDS = ["tiled4096x8", "tiled4096x16-p1-d3", "tiled4096x16", "tiled4096x8-d3",
+              "tiled2048x16", "tiled2048x8", "wave"]
+KNOBS = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes",
+         b"bounce_prefetch")
+DEFAULT_KNOBS = (0, 4096, 0, 0, 3)  # the library defaults (group/passes by precision)
 
 
 def _set_kernel(*cfg):
@@ -82,7 +84,7 @@ def test_room_bounce_bitexact(room_scene, cfg):
         tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct,
                                           max_bounces=100, stop_frac=1e-3)
     finally:
-        _set_kernel(*KERNELS[0])
+        _set_kernel(*DEFAULT_KNOBS)
     otot, oen, ok = oracle.bounce_f32(rp, col, coef, emit0, s.p_refl, s.p_area, total=direct,
                                       max_bounces=100, stop_frac=1e-3, threads=8,
                                       **_oracle_kw(*cfg))
@@ -156,7 +158,7 @@ def test_synthetic_multitile_bitexact(cfg):
         tot16, _, _ = lightmap.BounceLight(xf, e0, refl, area, max_bounces=4, stop_frac=0.0)
         _, _, _, raw, sc = xf.download(raw16=True)
     finally:
-        _set_kernel(*KERNELS[0])
+        _set_kernel(*DEFAULT_KNOBS)
     otot, oen, ok = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=4, stop_frac=0.0,
                                       threads=8, **_oracle_kw(*cfg))
     assert k == ok == 4 and np.array_equal(tot.view(np.uint32), otot.view(np.uint32))

@@ -179,8 +179,8 @@ void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, flo
 // vector and the loads of one step cover a few contiguous lines.  Columns are
 // stored as 16-bit byte offsets into the LDS emit tile.
 //
-// Pipeline (per wave): the loads of step s + 1 are issued before step s is
-// gathered, into the other of two register slots, and every load is issued
+// Pipeline (per wave): the loads of step s + D are issued before step s is
+// gathered, into a ring of D + 1 register slots, and every load is issued
 // unconditionally (a lane with nothing to load reads the block's first chunk
 // and ignores it), so the in-order memory counter waits only for the step it
 // needs.  The wave's segment offsets of tile t + 1 are loaded into registers
@@ -290,7 +290,7 @@ struct IntC {
 // tile t is gathered.  The copy reads emit_in in whole 16-byte pieces, so
 // emit_in must stay readable up to the next 16-byte boundary past its 12 n
 // bytes (the bounce buffers are padded to 256 bytes).
-template <int KIND, int C, int GS, int P>
+template <int KIND, int C, int GS, int P, int D>
 __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
                                                        const int64_t *__restrict__ bstart,
                                                        const float *__restrict__ emit_in,
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
   constexpr int PASS = GS * G;         // entries per pass of a group
   constexpr int EP = 3 * C / 4;        // 16-byte pieces of a full emit tile
   constexpr int ER = (EP + T - 1) / T; // copy rounds per tile
-  static_assert(NS % 2 == 0, "steps alternate between two load slots");
+  static_assert(NS % (D + 1) == 0, "steps rotate through D + 1 load slots");
   __shared__ __attribute__((aligned(16))) float s_e[2][4 * T * ER];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -361,9 +361,10 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
   int oa, ob;
   slot_offsets(0, oa, ob);
   int64_t bs = bstart[c * nt];
-  StepLoad<KIND, P> A, B;
+  StepLoad<KIND, P> ring[D + 1];  // step s of a tile lives in slot s % (D + 1)
   fetch_emit(0, 0);
-  issue(0, bs, oa, ob, A);
+#pragma unroll
+  for (int k = 0; k < D; ++k) issue(k, bs, oa, ob, ring[k]);
   __builtin_amdgcn_s_waitcnt(0);  // the first tile's copy (fewer loads follow it)
 
   // Before a tile's barrier each wave needs only its own copy of the tile to
@@ -386,9 +387,13 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
     const int64_t bs_n = bstart[c * nt + tn];
     const char *te = reinterpret_cast<const char *>(s_e[buf]);
 
-    auto step = [&](int st, StepLoad<KIND, P> &cur, StepLoad<KIND, P> &nxt) {
-      if (st + 1 < NS) issue(st + 1, bs, oa, ob, nxt);
-      else issue(0, bs_n, oa_n, ob_n, nxt);  // next tile's first step
+    // straight-line steps: every wait on the in-order memory counter is exact
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+      StepLoad<KIND, P> &cur = ring[st % (D + 1)];
+      StepLoad<KIND, P> &ahead = ring[(st + D) % (D + 1)];
+      if (st + D < NS) issue(st + D, bs, oa, ob, ahead);
+      else issue(st + D - NS, bs_n, oa_n, ob_n, ahead);  // next tile's first steps
       float r = 0.0f, g = 0.0f, b = 0.0f;
 #pragma unroll
       for (int k = 0; k < P; ++k)
@@ -406,12 +411,6 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
         ag = ag + g;
         ab = ab + b;
       }
-    };
-    // straight-line steps: every wait on the in-order memory counter is exact
-#pragma unroll
-    for (int st = 0; st < NS; st += 2) {
-      step(st, A, B);
-      step(st + 1, B, A);
     }
     oa = oa_n;
     ob = ob_n;
@@ -560,37 +559,40 @@ void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const in
                      cols, nt, tp);
 }
 
-// Compiled shapes (C, GS, P) of the LDS-tiled gather.
+// Compiled shapes (C, GS, P, D) of the LDS-tiled gather.
 #define VR_TILED_SHAPES(X) \
-  X(4096, 8, 2)            \
-  X(4096, 16, 2)           \
-  X(4096, 16, 1)           \
-  X(4096, 8, 3)            \
-  X(2048, 8, 2)            \
-  X(2048, 16, 2)
+  X(4096, 16, 1, 1)        \
+  X(4096, 16, 1, 3)        \
+  X(4096, 16, 2, 1)        \
+  X(4096, 8, 2, 1)         \
+  X(4096, 8, 2, 3)         \
+  X(4096, 8, 1, 3)         \
+  X(2048, 8, 2, 1)         \
+  X(2048, 16, 2, 1)
 
-bool bounce_tiled_variant(int cols, int group, int passes) {
-#define VR_HAS(CC, GG, PP) \
-  if (cols == CC && group == GG && passes == PP) return true;
+bool bounce_tiled_variant(int cols, int group, int passes, int prefetch) {
+#define VR_HAS(CC, GG, PP, DD) \
+  if (cols == CC && group == GG && passes == PP && prefetch == DD) return true;
   VR_TILED_SHAPES(VR_HAS)
 #undef VR_HAS
   return false;
 }
 
-void launch_bounce_tiled(int passes, const DevXfers &x, const DevTiles &tl, const float *emit_in,
-                         float *emit_out, float *total, const float *refl, const float *area,
-                         double *e_row, const BounceState *state, hipStream_t st) {
+void launch_bounce_tiled(int passes, int prefetch, const DevXfers &x, const DevTiles &tl,
+                         const float *emit_in, float *emit_out, float *total, const float *refl,
+                         const float *area, double *e_row, const BounceState *state,
+                         hipStream_t st) {
   if (x.n <= 0) return;
   const dim3 g((unsigned)tl.n_chunks);
-#define VR_GO(K, CC, GG, PP)                                                                  \
-  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP>), g, dim3(1024), 0, st, x, tl, tl.bstart, \
-                     emit_in, emit_out, total, refl, area, e_row, state)
-#define VR_SHAPE(CC, GG, PP)                                                \
-  if (tl.cols == CC && tl.group == GG && passes == PP) {                    \
-    if (x.kind == VR_COEF_F32) VR_GO(VR_COEF_F32, CC, GG, PP);              \
-    else if (x.kind == VR_COEF_F16) VR_GO(VR_COEF_F16, CC, GG, PP);         \
-    else VR_GO(VR_COEF_BF16, CC, GG, PP);                                   \
-    return;                                                                 \
+#define VR_GO(K, CC, GG, PP, DD)                                                           \
+  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP, DD>), g, dim3(1024), 0, st, x, tl,     \
+                     tl.bstart, emit_in, emit_out, total, refl, area, e_row, state)
+#define VR_SHAPE(CC, GG, PP, DD)                                                  \
+  if (tl.cols == CC && tl.group == GG && passes == PP && prefetch == DD) {        \
+    if (x.kind == VR_COEF_F32) VR_GO(VR_COEF_F32, CC, GG, PP, DD);                \
+    else if (x.kind == VR_COEF_F16) VR_GO(VR_COEF_F16, CC, GG, PP, DD);           \
+    else VR_GO(VR_COEF_BF16, CC, GG, PP, DD);                                     \
+    return;                                                                       \
   }
   VR_TILED_SHAPES(VR_SHAPE)
 #undef VR_SHAPE

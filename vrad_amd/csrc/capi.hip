@@ -307,10 +307,15 @@ int vr_set_tuning(const char *name, int value) {
       return fail(VR_E_INVALID, "bounce_tile_cols must be 2048 or 4096");
     tuning().bounce_tile_cols = value;
   } else if (k == "bounce_group") {
-    if (value != 8 && value != 16) return fail(VR_E_INVALID, "bounce_group must be 8 or 16");
+    if (value != 0 && value != 8 && value != 16)
+      return fail(VR_E_INVALID, "bounce_group must be 0 (by precision), 8 or 16");
     tuning().bounce_group = value;
+  } else if (k == "bounce_prefetch") {
+    if (value != 1 && value != 3) return fail(VR_E_INVALID, "bounce_prefetch must be 1 or 3");
+    tuning().bounce_prefetch = value;
   } else if (k == "bounce_passes") {
-    if (value < 1 || value > 3) return fail(VR_E_INVALID, "bounce_passes must be 1, 2 or 3");
+    if (value < 0 || value > 2)
+      return fail(VR_E_INVALID, "bounce_passes must be 0 (by precision), 1 or 2");
     tuning().bounce_passes = value;
 
   } else {
@@ -329,6 +334,7 @@ int vr_get_tuning(const char *name, int *value) {
   else if (k == "bounce_tile_cols") *value = tuning().bounce_tile_cols;
   else if (k == "bounce_group") *value = tuning().bounce_group;
   else if (k == "bounce_passes") *value = tuning().bounce_passes;
+  else if (k == "bounce_prefetch") *value = tuning().bounce_prefetch;
   else return fail(VR_E_INVALID, "unknown tuning knob " + k);
   return VR_OK;
 }
@@ -935,13 +941,16 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
   const Tuning tu = tuning();
   const bool tiled = g_path == VR_PATH_OPT && tu.bounce_kernel == 0;
   DevTiles tl{};
+  int passes = 1;
   if (tiled) {
     const int cols = tu.bounce_tile_cols;
     const int nt = (int)((n + cols - 1) / cols);
     tl.nt = nt;
     tl.cols = cols;
-    tl.group = tu.bounce_group;
-    if (!bounce_tiled_variant(cols, tl.group, tu.bounce_passes))
+    // defaults by precision: 16-lane groups for FP32, 8 for FP16/BF16 chunks
+    tl.group = tu.bounce_group ? tu.bounce_group : (x->precision == VR_COEF_F32 ? 16 : 8);
+    passes = tu.bounce_passes ? tu.bounce_passes : 1;
+    if (!bounce_tiled_variant(cols, tl.group, passes, tu.bounce_prefetch))
       return fail(VR_E_UNSUPPORTED, "no compiled LDS-tiled gather for this tuning shape");
     // chunks of at most 1024 rows, and at least one per compute unit while a
     // chunk keeps >= 256 rows (the chunk count changes the schedule only)
@@ -999,7 +1008,7 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
       {
         Prof p("bounce_gather", st);
         if (tiled)
-          launch_bounce_tiled(tu.bounce_passes, dx, tl, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,
+          launch_bounce_tiled(passes, tu.bounce_prefetch, dx, tl, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,
                               state, st);
         else
           launch_bounce(dx, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,

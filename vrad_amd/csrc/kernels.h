@@ -19,8 +19,9 @@ struct Tuning {
   // FP32 summation order (the CPU restatement follows the same settings).
   int bounce_kernel = 0;       // 0 = LDS-tiled, 1 = wave per row
   int bounce_tile_cols = 4096; // 2048 / 4096 patches per LDS tile
-  int bounce_group = 8;        // 8 / 16 lanes per row segment
-  int bounce_passes = 2;       // passes of a segment each lane loads ahead of use
+  int bounce_group = 0;        // 8 / 16 lanes per row segment, 0 = by precision
+  int bounce_passes = 0;       // passes of a segment loaded ahead of use, 0 = by precision
+  int bounce_prefetch = 3;     // steps the gather's loads run ahead (1 or 3)
 };
 Tuning &tuning();
 
@@ -87,8 +88,8 @@ struct BounceState {       // device-resident bounce bookkeeping
 void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, float *total,
                    const float *refl, const float *area, double *e_row, uint32_t *row_counter,
                    const BounceState *state, int path, hipStream_t st);
-bool bounce_tiled_variant(int cols, int group, int passes);
-void launch_bounce_tiled(int passes, const DevXfers &x, const DevTiles &tl, const float *emit_in,
+bool bounce_tiled_variant(int cols, int group, int passes, int prefetch);
+void launch_bounce_tiled(int passes, int prefetch, const DevXfers &x, const DevTiles &tl, const float *emit_in,
                          float *emit_out, float *total, const float *refl, const float *area,
                          double *e_row, const BounceState *state, hipStream_t st);
 void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const int32_t *col,
