@@ -261,6 +261,23 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl,
                     const vr_bounce_params *params, double *energy,
                     int32_t *n_bounces);
 
+/* Batched BounceLight (no reference counterpart; the batching form of the
+ * bake): every job is an independent transfer set with its own host arrays;
+ * one launch per bounce covers all jobs' rows (the LDS-tiled gather), each
+ * job keeps its own FP64 energy and stop test on the device.  Jobs share the
+ * coefficient precision.  use_graph != 0 captures 8 bounces as one hipGraph
+ * and replays it until every job has stopped.  Per job the results equal
+ * vr_bounce_light's bit for bit. */
+typedef struct {
+  vr_xfers *xfers;
+  const float *emit0, *refl, *area; /* 3n, 3n, n */
+  float *total;                     /* 3n, in/out */
+  double *energy;                   /* max_bounces, out */
+  int32_t n_bounces;                /* out */
+} vr_bounce_job;
+int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs,
+                          const vr_bounce_params *params, int32_t use_graph);
+
 /* ---- procedural scenes (inputs for tests and benchmarks) --------------- */
 typedef struct vr_gen vr_gen;
 /* Closed room 1024x1024x256 with n_pillars seeded axis-aligned pillars,

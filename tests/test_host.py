@@ -121,6 +121,13 @@ def test_bad_arguments_return_status():
     assert lib.vr_set_path(7) == _lib.VR_E_INVALID
     g = C.c_void_p()
     assert lib.vr_gen_room(C.byref(g), 1, 4, -1.0, 32.0) == _lib.VR_E_INVALID
+    # batched bounce: argument checks come before any device work
+    prm = _lib.BounceParams(8, 0.0)
+    jobs = (_lib.BounceJob * 2)()
+    assert lib.vr_bounce_light_batch(jobs, 0, C.byref(prm), 0) == _lib.VR_OK
+    assert lib.vr_bounce_light_batch(jobs, 2, C.byref(prm), 0) == _lib.VR_E_INVALID
+    assert b"transfers are NULL" in lib.vr_last_error()
+    assert lib.vr_bounce_light_batch(None, 1, C.byref(prm), 0) == _lib.VR_E_INVALID
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device error path")

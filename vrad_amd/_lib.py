@@ -84,6 +84,11 @@ class BounceParams(C.Structure):
     _fields_ = [("max_bounces", C.c_int32), ("stop_frac", C.c_double)]
 
 
+class BounceJob(C.Structure):
+    _fields_ = [("xfers", C.c_void_p), ("emit0", FP), ("refl", FP), ("area", FP),
+                ("total", FP), ("energy", C.POINTER(C.c_double)), ("n_bounces", C.c_int32)]
+
+
 # name -> (restype, argtypes); every function returns int status unless noted
 _P = C.c_void_p
 _SIGS = {
@@ -119,6 +124,8 @@ _SIGS = {
                                          C.c_float, C.c_int32, _P]),
     "vr_bounce_light": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(BounceParams), _P,
                                   C.POINTER(C.c_int32)]),
+    "vr_bounce_light_batch": (C.c_int, [C.POINTER(BounceJob), C.c_int32, C.POINTER(BounceParams),
+                                        C.c_int32]),
     "vr_gen_room": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int32, C.c_float, C.c_float]),
     "vr_gen_level": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int32, C.c_int32, C.c_float,
                                C.c_float, C.c_int32, C.c_int32]),

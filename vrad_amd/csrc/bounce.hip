@@ -285,22 +285,44 @@ struct IntC {
   static constexpr int value = V;
 };
 
-// One workgroup of 16 waves per chunk; emit tiles (r, g, b per patch, as in
-// global memory) are copied into two LDS buffers by DMA, tile t + 1 while
-// tile t is gathered.  The copy reads emit_in in whole 16-byte pieces, so
-// emit_in must stay readable up to the next 16-byte boundary past its 12 n
-// bytes (the bounce buffers are padded to 256 bytes).
+// Index of the job whose range [begin_j, begin_{j+1}) holds b (begin_0 = 0,
+// ascending); `begin` is the member of BounceJob named by the pointer-to-member.
+template <int64_t BounceJob::*begin>
+__device__ __forceinline__ int find_job(const BounceJob *__restrict__ jobs, int n_jobs,
+                                        int64_t b) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].*begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool job_done(const BounceJob &jb) {
+  return jb.state->converged || jb.state->k_done >= jb.max_bounces;
+}
+
+// One workgroup of 16 waves per chunk of a job; emit tiles (r, g, b per
+// patch, as in global memory) are copied into two LDS buffers by DMA, tile
+// t + 1 while tile t is gathered.  The copy reads emit in whole 16-byte
+// pieces, so emit must stay readable up to the next 16-byte boundary past its
+// 12 n bytes (the bounce buffers are padded to 256 bytes).
 template <int KIND, int C, int GS, int P, int D>
-__global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
-                                                       const int64_t *__restrict__ bstart,
-                                                       const float *__restrict__ emit_in,
-                                                       float *__restrict__ emit_out,
-                                                       float *__restrict__ total,
-                                                       const float *__restrict__ refl,
-                                                       const float *__restrict__ area,
-                                                       double *__restrict__ e_row,
-                                                       const BounceState *__restrict__ state) {
-  if (state->converged) return;
+__global__ __launch_bounds__(1024) void k_bounce_tiled(const BounceJob *__restrict__ jobs,
+                                                       int n_jobs) {
+  const int jid = find_job<&BounceJob::chunk_begin>(jobs, n_jobs, blockIdx.x);
+  const BounceJob &jb = jobs[jid];
+  if (job_done(jb)) return;
+  const int kb = jb.state->k_done;
+  const DevXfers x = jb.x;
+  const DevTiles tl = jb.tl;
+  const float *__restrict__ emit_in = jb.emit[kb & 1];
+  float *__restrict__ emit_out = jb.emit[(kb + 1) & 1];
+  float *__restrict__ total = jb.total;
+  const float *__restrict__ refl = jb.refl;
+  const float *__restrict__ area = jb.area;
+  double *__restrict__ e_row = jb.e_row;
   constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
   constexpr int T = 1024;              // threads
   constexpr int RS = 1028;             // stride of a block's segment offsets
@@ -311,11 +333,14 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
   constexpr int ER = (EP + T - 1) / T; // copy rounds per tile
   static_assert(NS % (D + 1) == 0, "steps rotate through D + 1 load slots");
   __shared__ __attribute__((aligned(16))) float s_e[2][4 * T * ER];
+  __shared__ int64_t s_bs[kMaxTilesPerJob];  // this chunk's block starts
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int gi = lane / GS, gl = lane % GS;
-  const int64_t c = blockIdx.x;
+  const int64_t c = (int64_t)blockIdx.x - jb.chunk_begin;
   const int nt = tl.nt;
+  for (int t = (int)threadIdx.x; t < nt; t += T) s_bs[t] = tl.bstart[c * nt + t];
+  __syncthreads();
   const int j_own = gl * NG + gi;  // the slot whose row total this lane keeps
   const int64_t oi = c + ((int64_t)j_own * 16 + wave) * tl.n_chunks;
   const int64_t row = oi < x.n ? (int64_t)x.order[oi] : -1;
@@ -360,7 +385,7 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
 
   int oa, ob;
   slot_offsets(0, oa, ob);
-  int64_t bs = bstart[c * nt];
+  int64_t bs = s_bs[0];
   StepLoad<KIND, P> ring[D + 1];  // step s of a tile lives in slot s % (D + 1)
   fetch_emit(0, 0);
 #pragma unroll
@@ -384,7 +409,7 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(DevXfers x, DevTiles tl,
     __asm__ volatile("" ::: "memory");  // the step loads below stay after the copy
     int oa_n, ob_n;
     slot_offsets(tn, oa_n, ob_n);
-    const int64_t bs_n = bstart[c * nt + tn];
+    const int64_t bs_n = s_bs[tn];
     const char *te = reinterpret_cast<const char *>(s_e[buf]);
 
     // straight-line steps: every wait on the in-order memory counter is exact
@@ -578,19 +603,16 @@ bool bounce_tiled_variant(int cols, int group, int passes, int prefetch) {
   return false;
 }
 
-void launch_bounce_tiled(int passes, int prefetch, const DevXfers &x, const DevTiles &tl,
-                         const float *emit_in, float *emit_out, float *total, const float *refl,
-                         const float *area, double *e_row, const BounceState *state,
-                         hipStream_t st) {
-  if (x.n <= 0) return;
-  const dim3 g((unsigned)tl.n_chunks);
-#define VR_GO(K, CC, GG, PP, DD)                                                           \
-  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP, DD>), g, dim3(1024), 0, st, x, tl,     \
-                     tl.bstart, emit_in, emit_out, total, refl, area, e_row, state)
+void launch_gather_jobs(int kind, int cols, int group, int passes, int prefetch,
+                        const BounceJob *jobs, int n_jobs, int64_t total_chunks, hipStream_t st) {
+  if (n_jobs <= 0 || total_chunks <= 0) return;
+  const dim3 g((unsigned)total_chunks);
+#define VR_GO(K, CC, GG, PP, DD) \
+  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP, DD>), g, dim3(1024), 0, st, jobs, n_jobs)
 #define VR_SHAPE(CC, GG, PP, DD)                                                  \
-  if (tl.cols == CC && tl.group == GG && passes == PP && prefetch == DD) {        \
-    if (x.kind == VR_COEF_F32) VR_GO(VR_COEF_F32, CC, GG, PP, DD);                \
-    else if (x.kind == VR_COEF_F16) VR_GO(VR_COEF_F16, CC, GG, PP, DD);           \
+  if (cols == CC && group == GG && passes == PP && prefetch == DD) {              \
+    if (kind == VR_COEF_F32) VR_GO(VR_COEF_F32, CC, GG, PP, DD);                  \
+    else if (kind == VR_COEF_F16) VR_GO(VR_COEF_F16, CC, GG, PP, DD);             \
     else VR_GO(VR_COEF_BF16, CC, GG, PP, DD);                                     \
     return;                                                                       \
   }
@@ -653,6 +675,81 @@ void launch_energy(const double *e_row, int64_t n, double *scratch, double *ener
     m = nb;
   }
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(1), 0, st, in, energy, k, stop_frac, state);
+}
+
+// ---- FP64 energy of batched jobs: the same tree as launch_energy for
+// n <= 2048^2 (stage 1: blocks of 2048 over e_row; stage 2: one block over
+// the stage-1 partials when there is more than one), then the stop test.
+__device__ __forceinline__ double reduce_block2048(const double *__restrict__ in, int64_t m,
+                                                   int64_t blk, double *v) {
+  const int t = threadIdx.x;
+  const int64_t base = blk * 2048;
+  double a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int64_t k = base + t + 256 * j;
+    if (k < m) a = a + in[k];
+  }
+  v[t] = a;
+  __syncthreads();
+  for (int s = 128; s >= 1; s >>= 1) {
+    if (t < s) v[t] = v[t] + v[t + s];
+    __syncthreads();
+  }
+  return v[0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_jobs1(const BounceJob *__restrict__ jobs,
+                                                      int n_jobs) {
+  __shared__ double v[256];
+  const int jid = find_job<&BounceJob::eblk_begin>(jobs, n_jobs, blockIdx.x);
+  const BounceJob &jb = jobs[jid];
+  if (job_done(jb)) return;
+  const int64_t lb = (int64_t)blockIdx.x - jb.eblk_begin;
+  const double r = reduce_block2048(jb.e_row, jb.x.n, lb, v);
+  if (threadIdx.x == 0) jb.scratch[lb] = r;
+}
+
+__global__ __launch_bounds__(256) void k_energy_jobs2(const BounceJob *__restrict__ jobs) {
+  __shared__ double v[256];
+  const BounceJob &jb = jobs[blockIdx.x];
+  if (job_done(jb) || jb.e_blocks <= 1) return;
+  const double r = reduce_block2048(jb.scratch, jb.e_blocks, 0, v);
+  if (threadIdx.x == 0) jb.scratch[jb.e_blocks] = r;
+}
+
+__global__ __launch_bounds__(64) void k_energy_jobs_final(const BounceJob *__restrict__ jobs,
+                                                          int n_jobs) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= n_jobs) return;
+  const BounceJob &jb = jobs[j];
+  if (job_done(jb)) return;
+  const double ek = jb.e_blocks > 1 ? jb.scratch[jb.e_blocks]
+                    : jb.e_blocks == 1 ? jb.scratch[0] : 0.0;
+  BounceState *stt = jb.state;
+  const int k = stt->k_done;
+  jb.energy[k] = ek;
+  const double cum = stt->cum + ek;
+  stt->cum = cum;
+  stt->k_done = k + 1;
+  if (ek <= jb.stop_frac * cum) stt->converged = 1;
+}
+
+void launch_energy_jobs(const BounceJob *jobs, int n_jobs, int64_t total_eblocks, hipStream_t st) {
+  if (n_jobs <= 0) return;
+  if (total_eblocks > 0)
+    hipLaunchKernelGGL(k_energy_jobs1, dim3((unsigned)total_eblocks), dim3(256), 0, st, jobs,
+                       n_jobs);
+  hipLaunchKernelGGL(k_energy_jobs2, dim3((unsigned)n_jobs), dim3(256), 0, st, jobs);
+  hipLaunchKernelGGL(k_energy_jobs_final, dim3((unsigned)((n_jobs + 63) / 64)), dim3(64), 0, st,
+                     jobs, n_jobs);
+}
+
+void launch_bounce_jobs(int kind, int cols, int group, int passes, int prefetch,
+                        const BounceJob *jobs, int n_jobs, int64_t total_chunks,
+                        int64_t total_eblocks, hipStream_t st) {
+  launch_gather_jobs(kind, cols, group, passes, prefetch, jobs, n_jobs, total_chunks, st);
+  launch_energy_jobs(jobs, n_jobs, total_eblocks, st);
 }
 
 // ---- synthetic transfers (config of the precision form) -------------------

@@ -903,6 +903,183 @@ int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n, const 
 }
 
 // ---- bounce ------------------------------------------------------------------
+namespace vr {
+namespace {
+
+// Gather shape of the LDS-tiled kernel for x at the current tuning.
+struct TiledShape {
+  int cols, group, passes, prefetch;
+};
+int tiled_shape(const vr_xfers *x, const Tuning &tu, TiledShape &sh) {
+  sh.cols = tu.bounce_tile_cols;
+  // defaults by precision: 16-lane groups for FP32, 8 for FP16/BF16 chunks
+  sh.group = tu.bounce_group ? tu.bounce_group : (x->precision == VR_COEF_F32 ? 16 : 8);
+  sh.passes = tu.bounce_passes ? tu.bounce_passes : 1;
+  sh.prefetch = tu.bounce_prefetch;
+  if (!bounce_tiled_variant(sh.cols, sh.group, sh.passes, sh.prefetch))
+    return fail(VR_E_UNSUPPORTED, "no compiled LDS-tiled gather for this tuning shape");
+  return VR_OK;
+}
+
+// Tile pointers and the tile-blocked copy of x for the tile width, built once
+// and kept with the transfers; fills tl.
+int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t st) {
+  const int64_t n = x->n;
+  const int cols = sh.cols;
+  const int nt = (int)((n + cols - 1) / cols);
+  if (nt > kMaxTilesPerJob)
+    return fail(VR_E_CAPACITY, "more than " + std::to_string(kMaxTilesPerJob) +
+                                   " column tiles; use a wider tile");
+  tl = DevTiles{};
+  tl.nt = nt;
+  tl.cols = cols;
+  tl.group = sh.group;
+  // chunks of at most 1024 rows, and at least one per compute unit while a
+  // chunk keeps >= 256 rows (the chunk count changes the schedule only)
+  tl.n_chunks = n > 0 ? std::max<int64_t>((n + 1023) / 1024, std::min<int64_t>(g_cus, (n + 255) / 256))
+                      : 0;
+  if (n == 0) return VR_OK;
+  const DevXfers dx = x->dev();
+  if (x->tp_cols != cols || !x->tp.p) {
+    TRY(x->tp.alloc(4 * (size_t)n * (size_t)(nt + 1)));
+    launch_tile_ptrs(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), x->col.as<int32_t>(), cols, nt,
+                     x->tp.as<int32_t>(), st);
+    TRY(check_launch());
+    x->tp_cols = cols;
+    x->blk_cols = 0;
+  }
+  tl.tp = x->tp.as<int32_t>();
+  if (x->blk_cols != cols || x->blk_prec != x->precision || x->blk_chunks != tl.n_chunks) {
+    const int G = x->precision == VR_COEF_F32 ? 4 : 8;
+    const int cb = x->precision == VR_COEF_F32 ? 4 : 2;
+    const int64_t nblk = tl.n_chunks * nt;
+    DBuf btot;
+    TRY(btot.alloc(8 * (size_t)nblk));
+    TRY(x->boff.alloc(4 * (size_t)nblk * 1028));
+    tl.boff = x->boff.as<int32_t>();
+    launch_block_offsets(dx, tl, G, btot.as<int64_t>(), x->boff.as<int32_t>(), st);
+    TRY(check_launch());
+    std::vector<int64_t> tot((size_t)nblk), start((size_t)nblk);
+    VR_HIP(hipMemcpyAsync(tot.data(), btot.p, 8 * (size_t)nblk, hipMemcpyDeviceToHost, st));
+    VR_HIP(hipStreamSynchronize(st));
+    int64_t run = 0;
+    for (int64_t b = 0; b < nblk; ++b) {
+      start[(size_t)b] = run;
+      run += tot[(size_t)b];
+    }
+    TRY(upload_vec(x->bstart, start, st));
+    TRY(x->bidx.alloc(2 * (size_t)std::max<int64_t>(run, 1) + 64));
+    TRY(x->bcoef.alloc((size_t)cb * (size_t)std::max<int64_t>(run, 1) + 64));
+    tl.bstart = x->bstart.as<int64_t>();
+    launch_block_fill(dx, tl, G, cb, x->bidx.as<uint16_t>(), x->bcoef.p, st);
+    TRY(check_launch());
+    VR_HIP(hipStreamSynchronize(st));
+    x->blk_cols = cols;
+    x->blk_prec = x->precision;
+    x->blk_chunks = tl.n_chunks;
+  }
+  tl.bidx = x->bidx.as<uint16_t>();
+  tl.bcoef = x->bcoef.p;
+  tl.bstart = x->bstart.as<int64_t>();
+  tl.boff = x->boff.as<int32_t>();
+  return VR_OK;
+}
+
+// Device buffers of one job of a tiled bounce (emit x2, total, refl, area,
+// e_row, energy scratch, energy); the state lives in a shared array.
+struct JobBuffers {
+  Staging S;
+  int make(int64_t n, int K) {
+    const size_t vb = 12 * (size_t)std::max<int64_t>(n, 1);
+    const int64_t eb = (n + 2047) / 2048;
+    return S.make({vb, vb, vb, vb, 4 * (size_t)std::max<int64_t>(n, 1),
+                   8 * (size_t)std::max<int64_t>(n, 1), 8 * (size_t)(eb + 2),
+                   8 * (size_t)std::max(K, 1)});
+  }
+};
+
+void fill_job(BounceJob &jb, const vr_xfers *x, const DevTiles &tl, JobBuffers &b,
+              BounceState *state, int K, double stop_frac) {
+  jb.x = x->dev();
+  jb.tl = tl;
+  jb.emit[0] = b.S.ptr<float>(0);
+  jb.emit[1] = b.S.ptr<float>(1);
+  jb.total = b.S.ptr<float>(2);
+  jb.refl = b.S.ptr<float>(3);
+  jb.area = b.S.ptr<float>(4);
+  jb.e_row = b.S.ptr<double>(5);
+  jb.scratch = b.S.ptr<double>(6);
+  jb.energy = b.S.ptr<double>(7);
+  jb.state = state;
+  jb.e_blocks = (x->n + 2047) / 2048;
+  jb.max_bounces = K;
+  jb.stop_frac = stop_frac;
+}
+
+// Runs the bounces of a set of prepared jobs (device table `d_jobs`, states
+// `d_states`) in chunks of 8, optionally as a captured graph of one chunk
+// replayed until every job has stopped.
+int run_jobs(const TiledShape &sh, int kind, const BounceJob *d_jobs, int n_jobs,
+             int64_t total_chunks, int64_t total_eblocks, BounceState *d_states, int K,
+             bool use_graph, hipStream_t st) {
+  std::vector<BounceState> hs((size_t)n_jobs);
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int rc = VR_OK;
+  if (use_graph && K > 0) {
+    VR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < 8; ++i)
+      launch_bounce_jobs(kind, sh.cols, sh.group, sh.passes, sh.prefetch, d_jobs, n_jobs,
+                         total_chunks, total_eblocks, st);
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) return fail(VR_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(VR_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+  }
+  for (int done = 0; done < K;) {
+    if (exec) {
+      hipError_t e = hipGraphLaunch(exec, st);
+      if (e != hipSuccess) {
+        rc = fail(VR_E_HIP, std::string("graph launch: ") + hipGetErrorString(e));
+        break;
+      }
+    } else {
+      for (int i = 0; i < std::min(8, K - done); ++i) {
+        {
+          Prof p("bounce_gather", st);
+          launch_gather_jobs(kind, sh.cols, sh.group, sh.passes, sh.prefetch, d_jobs, n_jobs,
+                             total_chunks, st);
+        }
+        {
+          Prof p("energy", st);
+          launch_energy_jobs(d_jobs, n_jobs, total_eblocks, st);
+        }
+      }
+      if ((rc = check_launch())) break;
+    }
+    done += 8;
+    hipError_t e = hipMemcpyAsync(hs.data(), d_states, sizeof(BounceState) * (size_t)n_jobs,
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      rc = fail(VR_E_HIP, std::string("bounce: ") + hipGetErrorString(e));
+      break;
+    }
+    bool all = true;
+    for (const BounceState &b : hs) all = all && (b.converged || b.k_done >= K);
+    if (all) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  return rc;
+}
+
+}  // namespace
+}  // namespace vr
+
 int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const float *area,
                     float *total, const vr_bounce_params *prm, double *energy,
                     int32_t *n_bounces) {
@@ -917,6 +1094,14 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
     return fail(VR_E_UNSUPPORTED, "plain path gathers FP32 transfers only");
   TRY(ensure_init());
   hipStream_t st = g_stream;
+  const Tuning tu = tuning();
+  if (g_path == VR_PATH_OPT && tu.bounce_kernel == 0) {
+    // LDS-tiled gather: a batch of one job
+    vr_bounce_job jb{x, emit0, refl, area, total, energy, 0};
+    const int rc = vr_bounce_light_batch(&jb, 1, prm, 0);
+    if (n_bounces) *n_bounces = jb.n_bounces;
+    return rc;
+  }
   const size_t vb = 12 * (size_t)std::max<int64_t>(n, 1);
   int64_t scratch_n = 2;
   for (int64_t m = n; m > 1; m = (m + 2047) / 2048) scratch_n += (m + 2047) / 2048;
@@ -938,67 +1123,6 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
   VR_HIP(hipMemsetAsync(counters, 0, 4 * (size_t)std::max(K, 1), st));
   VR_HIP(hipMemsetAsync(state, 0, sizeof(BounceState), st));
   const DevXfers dx = x->dev();
-  const Tuning tu = tuning();
-  const bool tiled = g_path == VR_PATH_OPT && tu.bounce_kernel == 0;
-  DevTiles tl{};
-  int passes = 1;
-  if (tiled) {
-    const int cols = tu.bounce_tile_cols;
-    const int nt = (int)((n + cols - 1) / cols);
-    tl.nt = nt;
-    tl.cols = cols;
-    // defaults by precision: 16-lane groups for FP32, 8 for FP16/BF16 chunks
-    tl.group = tu.bounce_group ? tu.bounce_group : (x->precision == VR_COEF_F32 ? 16 : 8);
-    passes = tu.bounce_passes ? tu.bounce_passes : 1;
-    if (!bounce_tiled_variant(cols, tl.group, passes, tu.bounce_prefetch))
-      return fail(VR_E_UNSUPPORTED, "no compiled LDS-tiled gather for this tuning shape");
-    // chunks of at most 1024 rows, and at least one per compute unit while a
-    // chunk keeps >= 256 rows (the chunk count changes the schedule only)
-    tl.n_chunks = std::max<int64_t>((n + 1023) / 1024, std::min<int64_t>(g_cus, (n + 255) / 256));
-    if (x->tp_cols != cols || !x->tp.p) {
-      TRY(x->tp.alloc(4 * (size_t)std::max<int64_t>(n, 1) * (size_t)(nt + 1)));
-      launch_tile_ptrs(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), x->col.as<int32_t>(), cols,
-                       nt, x->tp.as<int32_t>(), st);
-      TRY(check_launch());
-      x->tp_cols = cols;
-      x->blk_cols = 0;
-    }
-    tl.tp = x->tp.as<int32_t>();
-    if (x->blk_cols != cols || x->blk_prec != x->precision || x->blk_chunks != tl.n_chunks) {
-      // build the tile-blocked copy for this tile width and precision
-      const int G = x->precision == VR_COEF_F32 ? 4 : 8;
-      const int cb = x->precision == VR_COEF_F32 ? 4 : 2;
-      const int64_t nblk = tl.n_chunks * nt;
-      DBuf btot;
-      TRY(btot.alloc(8 * (size_t)nblk));
-      TRY(x->boff.alloc(4 * (size_t)nblk * 1028));
-      tl.boff = x->boff.as<int32_t>();
-      launch_block_offsets(dx, tl, G, btot.as<int64_t>(), x->boff.as<int32_t>(), st);
-      TRY(check_launch());
-      std::vector<int64_t> tot((size_t)nblk), start((size_t)nblk);
-      VR_HIP(hipMemcpyAsync(tot.data(), btot.p, 8 * (size_t)nblk, hipMemcpyDeviceToHost, st));
-      VR_HIP(hipStreamSynchronize(st));
-      int64_t run = 0;
-      for (int64_t b = 0; b < nblk; ++b) {
-        start[(size_t)b] = run;
-        run += tot[(size_t)b];
-      }
-      TRY(upload_vec(x->bstart, start, st));
-      TRY(x->bidx.alloc(2 * (size_t)std::max<int64_t>(run, 1) + 64));
-      TRY(x->bcoef.alloc((size_t)cb * (size_t)std::max<int64_t>(run, 1) + 64));
-      tl.bstart = x->bstart.as<int64_t>();
-      launch_block_fill(dx, tl, G, cb, x->bidx.as<uint16_t>(), x->bcoef.p, st);
-      TRY(check_launch());
-      VR_HIP(hipStreamSynchronize(st));
-      x->blk_cols = cols;
-      x->blk_prec = x->precision;
-      x->blk_chunks = tl.n_chunks;
-    }
-    tl.bidx = x->bidx.as<uint16_t>();
-    tl.bcoef = x->bcoef.p;
-    tl.bstart = x->bstart.as<int64_t>();
-    tl.boff = x->boff.as<int32_t>();
-  }
   BounceState hs{0.0, 0, 0};
   const int chunk = g_path == VR_PATH_PLAIN ? 1 : 8;
   int k = 0;
@@ -1007,12 +1131,8 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
     for (; k < kend; ++k) {
       {
         Prof p("bounce_gather", st);
-        if (tiled)
-          launch_bounce_tiled(passes, tu.bounce_prefetch, dx, tl, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,
-                              state, st);
-        else
-          launch_bounce(dx, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,
-                        counters + k, state, g_path, st);
+        launch_bounce(dx, em[k & 1], em[(k + 1) & 1], d_total, d_refl, d_area, e_row,
+                      counters + k, state, g_path, st);
       }
       {
         Prof p("energy", st);
@@ -1029,6 +1149,77 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
   if (n > 0) VR_HIP(hipMemcpyAsync(total, d_total, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
   VR_HIP(hipStreamSynchronize(st));
   if (n_bounces) *n_bounces = done;
+  return VR_OK;
+}
+
+int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs, const vr_bounce_params *prm,
+                          int32_t use_graph) {
+  LOCK;
+  if (!jobs || !prm || n_jobs < 0) return fail(VR_E_INVALID, "NULL argument or n_jobs < 0");
+  const int K = prm->max_bounces;
+  if (K < 0) return fail(VR_E_INVALID, "max_bounces < 0");
+  if (n_jobs == 0) return VR_OK;
+  const Tuning tu = tuning();
+  if (g_path != VR_PATH_OPT || tu.bounce_kernel != 0)
+    return fail(VR_E_UNSUPPORTED, "batched bounces run on the LDS-tiled gather (optimised path)");
+  const int kind = jobs[0].xfers ? jobs[0].xfers->precision : -1;
+  for (int j = 0; j < n_jobs; ++j) {
+    const vr_bounce_job &b = jobs[j];
+    if (!b.xfers) return fail(VR_E_INVALID, "job " + std::to_string(j) + ": transfers are NULL");
+    if (b.xfers->precision != kind)
+      return fail(VR_E_INVALID, "jobs of one batch must share the coefficient precision");
+    if (b.xfers->n > 0 && (!b.emit0 || !b.refl || !b.area || !b.total))
+      return fail(VR_E_INVALID, "job " + std::to_string(j) + ": NULL array");
+    if (K > 0 && !b.energy) return fail(VR_E_INVALID, "job " + std::to_string(j) + ": energy is NULL");
+    if (b.xfers->n > (int64_t)2048 * 2048)
+      return fail(VR_E_CAPACITY, "a batched job holds at most 2048^2 rows");
+  }
+  TRY(ensure_init());
+  hipStream_t st = g_stream;
+  TiledShape sh;
+  TRY(tiled_shape(jobs[0].xfers, tu, sh));
+  std::vector<BounceJob> hj((size_t)n_jobs);
+  std::vector<JobBuffers> bufs((size_t)n_jobs);
+  DBuf d_states, d_jobs;
+  TRY(d_states.alloc(sizeof(BounceState) * (size_t)n_jobs));
+  VR_HIP(hipMemsetAsync(d_states.p, 0, sizeof(BounceState) * (size_t)n_jobs, st));
+  int64_t chunks = 0, eblocks = 0;
+  for (int j = 0; j < n_jobs; ++j) {
+    vr_xfers *x = jobs[j].xfers;
+    const int64_t n = x->n;
+    DevTiles tl;
+    TRY(prepare_tiles(x, sh, tl, st));
+    TRY(bufs[(size_t)j].make(n, K));
+    BounceJob &jb = hj[(size_t)j];
+    fill_job(jb, x, tl, bufs[(size_t)j], d_states.as<BounceState>() + j, K, prm->stop_frac);
+    jb.chunk_begin = chunks;
+    jb.eblk_begin = eblocks;
+    chunks += tl.n_chunks;
+    eblocks += jb.e_blocks;
+    if (n > 0) {
+      VR_HIP(hipMemcpyAsync(jb.emit[0], jobs[j].emit0, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+      VR_HIP(hipMemcpyAsync(jb.total, jobs[j].total, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+      VR_HIP(hipMemcpyAsync((void *)jb.refl, jobs[j].refl, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+      VR_HIP(hipMemcpyAsync((void *)jb.area, jobs[j].area, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    }
+  }
+  TRY(d_jobs.upload(hj.data(), sizeof(BounceJob) * (size_t)n_jobs, st));
+  TRY(run_jobs(sh, kind, d_jobs.as<BounceJob>(), n_jobs, chunks, eblocks,
+               d_states.as<BounceState>(), K, use_graph != 0, st));
+  std::vector<BounceState> hs((size_t)n_jobs);
+  VR_HIP(hipMemcpyAsync(hs.data(), d_states.p, sizeof(BounceState) * (size_t)n_jobs,
+                        hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  for (int j = 0; j < n_jobs; ++j) {
+    const BounceJob &jb = hj[(size_t)j];
+    const int done = hs[(size_t)j].k_done;
+    const int64_t n = jobs[j].xfers->n;
+    if (done > 0)
+      VR_HIP(hipMemcpyAsync(jobs[j].energy, jb.energy, 8 * (size_t)done, hipMemcpyDeviceToHost, st));
+    if (n > 0) VR_HIP(hipMemcpyAsync(jobs[j].total, jb.total, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
+    jobs[j].n_bounces = done;
+  }
+  VR_HIP(hipStreamSynchronize(st));
   return VR_OK;
 }
 

@@ -85,13 +85,37 @@ struct BounceState {       // device-resident bounce bookkeeping
   int32_t converged;
   int32_t k_done;
 };
+// One transfer set's bounce in a batched launch of the LDS-tiled gather and
+// the FP64 energy.  The bounce index is state->k_done: the gather reads
+// emit[k & 1] and writes emit[(k + 1) & 1]; every kernel returns at once for
+// a job that has converged or done max_bounces.
+struct BounceJob {
+  DevXfers x;
+  DevTiles tl;
+  float *emit[2];
+  float *total;
+  const float *refl, *area;
+  double *e_row, *scratch, *energy;
+  BounceState *state;
+  int64_t chunk_begin;     // first gather workgroup of the job
+  int64_t eblk_begin;      // first stage-1 energy block of the job
+  int64_t e_blocks;        // ceil(n / 2048) stage-1 blocks (n <= 2048^2)
+  int32_t max_bounces;
+  double stop_frac;
+};
+constexpr int kMaxTilesPerJob = 1024;  // block starts of a chunk cached in LDS
 void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, float *total,
                    const float *refl, const float *area, double *e_row, uint32_t *row_counter,
                    const BounceState *state, int path, hipStream_t st);
 bool bounce_tiled_variant(int cols, int group, int passes, int prefetch);
-void launch_bounce_tiled(int passes, int prefetch, const DevXfers &x, const DevTiles &tl, const float *emit_in,
-                         float *emit_out, float *total, const float *refl, const float *area,
-                         double *e_row, const BounceState *state, hipStream_t st);
+// one bounce of every job: gather (jobs share precision and shape), then the
+// energy tree and stop test of each job
+void launch_bounce_jobs(int kind, int cols, int group, int passes, int prefetch,
+                        const BounceJob *jobs, int n_jobs, int64_t total_chunks,
+                        int64_t total_eblocks, hipStream_t st);
+void launch_gather_jobs(int kind, int cols, int group, int passes, int prefetch,
+                        const BounceJob *jobs, int n_jobs, int64_t total_chunks, hipStream_t st);
+void launch_energy_jobs(const BounceJob *jobs, int n_jobs, int64_t total_eblocks, hipStream_t st);
 void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const int32_t *col,
                       int cols, int nt, int32_t *tp, hipStream_t st);
 // blocked layout: offsets (boff) and per-block totals, then the copy

@@ -115,3 +115,25 @@ def BounceLight(transfers, emit0, refl, area, total=None, max_bounces=100, stop_
     check(lib.vr_bounce_light(transfers.handle, ptr(e0), ptr(rf), ptr(ar), ptr(tot),
                               C.byref(prm), ptr(energy), C.byref(nb)))
     return tot, energy[:nb.value], nb.value
+
+
+def BounceLightBatch(jobs, max_bounces=100, stop_frac=1e-3, use_graph=True):
+    """Batched BounceLight over independent transfer sets in one launch per
+    bounce (optionally a replayed hipGraph of 8 bounces).  `jobs` is a list of
+    (transfers, emit0, refl, area, total_or_None); returns a list of
+    (total (n,3), energy per bounce (k,), k), equal per job to BounceLight."""
+    keep, arr = [], (_lib.BounceJob * max(len(jobs), 1))()
+    for i, (xf, emit0, refl, area, total) in enumerate(jobs):
+        e0 = f32(emit0).reshape(-1, 3)
+        n = e0.shape[0]
+        rf = f32(refl).reshape(n, 3)
+        ar = f32(area).reshape(n)
+        tot = np.zeros((n, 3), np.float32) if total is None else f32(total).reshape(n, 3).copy()
+        energy = np.zeros(max(max_bounces, 1), np.float64)
+        keep.append((e0, rf, ar, tot, energy))
+        h = xf.handle
+        arr[i] = _lib.BounceJob(h.value if isinstance(h, C.c_void_p) else h, _lib.fptr(e0), _lib.fptr(rf), _lib.fptr(ar),
+                                _lib.fptr(tot), energy.ctypes.data_as(C.POINTER(C.c_double)), 0)
+    prm = _lib.BounceParams(int(max_bounces), float(stop_frac))
+    check(lib.vr_bounce_light_batch(arr, len(jobs), C.byref(prm), 1 if use_graph else 0))
+    return [(k[3], k[4][:arr[i].n_bounces], arr[i].n_bounces) for i, k in enumerate(keep)]
