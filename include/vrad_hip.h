@@ -278,6 +278,58 @@ typedef struct {
 int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs,
                           const vr_bounce_params *params, int32_t use_graph);
 
+/* ---- serving: re-bake requests against resident scenes ----------------- */
+/* No reference counterpart (the serving form of the bake).  A server keeps
+ * each scene's tree, patches, transfers and the transfers' tile-blocked copy
+ * resident in HBM and runs re-bake requests: the scene's lights with 1..k of
+ * them moved, direct light re-gathered, then `bounces` bounces from it.
+ *
+ * VR_SERVE_BATCHED: continuous batching.  A scheduler thread admits queued
+ * requests in arrival order while a slot (max_active) and the request's
+ * working memory (carved from an hbm_budget arena) are free; every iteration
+ * runs one bounce of every admitted request in one batched launch, so
+ * requests that arrive while others run join at the next bounce.  The
+ * scenes share one coefficient precision (FP16 for the serving form).
+ * VR_SERVE_PLAIN: the comparison path, one request at a time in arrival
+ * order with the plain kernels (scratch-stack gather, one thread per CSR row
+ * on the FP32 coefficients, a host read of the energy after every bounce).
+ * A request's result does not depend on what else runs with it.
+ * Scenes are added before the first request; the scene and transfer handles
+ * must outlive the server. */
+#define VR_SERVE_BATCHED 0
+#define VR_SERVE_PLAIN 1
+typedef struct vr_server vr_server;
+typedef struct {
+  int32_t mode;        /* VR_SERVE_BATCHED / VR_SERVE_PLAIN */
+  int32_t max_active;  /* requests on the device at once (1..256; plain: 1) */
+  int64_t hbm_budget;  /* bytes of request working memory (<= 0: 8 GiB) */
+  int32_t bounces;     /* bounces per request */
+  float normal_offset; /* sample offset along the normal for visibility */
+} vr_server_params;
+typedef struct {
+  int64_t submitted, completed, iterations, max_active_seen, pending, active;
+} vr_server_stats;
+int vr_server_create(vr_server **out, const vr_server_params *params);
+/* patches: sample points and normals (n = the transfers' rows), reflectivity
+ * 3n, area n; lights: the scene's base light list.  *scene_id is its index. */
+int vr_server_add_scene(vr_server *sv, vr_scene *scene, vr_xfers *xfers,
+                        const vr_samples *patches, const float *refl,
+                        const float *area, const vr_light *lights,
+                        int32_t n_lights, int32_t *scene_id);
+/* Queue a re-bake of scene_id with lights light_index[i] moved by
+ * offset[3i..3i+2]; the request's clock starts here. */
+int vr_server_submit(vr_server *sv, int32_t scene_id, const int32_t *light_index,
+                     const float *offset, int32_t n_moved, int64_t *ticket);
+/* Wait for a request (timeout_ms < 0: no limit; VR_E_STATE on timeout) and
+ * take its result: total 3n, energy per bounce, bounces done, latency
+ * (completion - submit) in ms; any output may be NULL.  A ticket is taken
+ * once. */
+int vr_server_wait(vr_server *sv, int64_t ticket, int32_t timeout_ms, float *total,
+                   double *energy, int32_t *n_bounces, double *latency_ms);
+int vr_server_get_stats(vr_server *sv, vr_server_stats *out);
+/* Finishes the queued requests, then stops the scheduler. */
+int vr_server_destroy(vr_server *sv);
+
 /* ---- procedural scenes (inputs for tests and benchmarks) --------------- */
 typedef struct vr_gen vr_gen;
 /* Closed room 1024x1024x256 with n_pillars seeded axis-aligned pillars,

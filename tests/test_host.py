@@ -128,6 +128,13 @@ def test_bad_arguments_return_status():
     assert lib.vr_bounce_light_batch(jobs, 2, C.byref(prm), 0) == _lib.VR_E_INVALID
     assert b"transfers are NULL" in lib.vr_last_error()
     assert lib.vr_bounce_light_batch(None, 1, C.byref(prm), 0) == _lib.VR_E_INVALID
+    # serving: parameters are checked before any device work
+    sv = C.c_void_p()
+    bad = _lib.ServerParams(7, 4, 0, 8, 0.1)
+    assert lib.vr_server_create(C.byref(sv), C.byref(bad)) == _lib.VR_E_INVALID
+    bad = _lib.ServerParams(_lib.VR_SERVE_BATCHED, 0, 0, 8, 0.1)
+    assert lib.vr_server_create(C.byref(sv), C.byref(bad)) == _lib.VR_E_INVALID
+    assert lib.vr_server_destroy(None) == _lib.VR_OK
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device error path")

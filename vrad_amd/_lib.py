@@ -19,6 +19,7 @@ VR_PATH_OPT, VR_PATH_PLAIN = 0, 1
 VR_COEF_F32, VR_COEF_F16, VR_COEF_BF16 = 0, 1, 2
 VR_EMIT_SURFACE, VR_EMIT_POINT, VR_EMIT_SPOTLIGHT = 0, 1, 2
 VR_EMIT_SKYLIGHT, VR_EMIT_SKYAMBIENT = 3, 5
+VR_SERVE_BATCHED, VR_SERVE_PLAIN = 0, 1
 
 
 class VrError(RuntimeError):
@@ -84,6 +85,16 @@ class BounceParams(C.Structure):
     _fields_ = [("max_bounces", C.c_int32), ("stop_frac", C.c_double)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("max_active", C.c_int32), ("hbm_budget", C.c_int64),
+                ("bounces", C.c_int32), ("normal_offset", C.c_float)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "iterations",
+                                         "max_active_seen", "pending", "active")]
+
+
 class BounceJob(C.Structure):
     _fields_ = [("xfers", C.c_void_p), ("emit0", FP), ("refl", FP), ("area", FP),
                 ("total", FP), ("energy", C.POINTER(C.c_double)), ("n_bounces", C.c_int32)]
@@ -133,6 +144,14 @@ _SIGS = {
                                 C.POINTER(C.c_int32)]),
     "vr_gen_copy": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "vr_gen_destroy": (C.c_int, [_P]),
+    "vr_server_create": (C.c_int, [C.POINTER(_P), C.POINTER(ServerParams)]),
+    "vr_server_add_scene": (C.c_int, [_P, _P, _P, C.POINTER(Samples), _P, _P, _P, C.c_int32,
+                                      C.POINTER(C.c_int32)]),
+    "vr_server_submit": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int32, C.POINTER(C.c_int64)]),
+    "vr_server_wait": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_double)]),
+    "vr_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "vr_server_destroy": (C.c_int, [_P]),
 }
 EXPORTED = tuple(_SIGS)
 

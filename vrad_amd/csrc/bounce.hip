@@ -752,6 +752,26 @@ void launch_bounce_jobs(int kind, int cols, int group, int passes, int prefetch,
   launch_energy_jobs(jobs, n_jobs, total_eblocks, st);
 }
 
+// ---- first bounce input of a re-bake: total = direct, emit = direct * refl
+__global__ __launch_bounds__(256) void k_emit_init(int64_t n3, const float *__restrict__ direct,
+                                                   const float *__restrict__ refl,
+                                                   float *__restrict__ emit,
+                                                   float *__restrict__ total) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n3) return;
+  const float d = direct[i];
+  emit[i] = d * refl[i];
+  total[i] = d;
+}
+
+void launch_emit_init(int64_t n, const float *direct, const float *refl, float *emit,
+                      float *total, hipStream_t st) {
+  const int64_t n3 = 3 * n;
+  if (n3 <= 0) return;
+  hipLaunchKernelGGL(k_emit_init, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, n3, direct,
+                     refl, emit, total);
+}
+
 // ---- synthetic transfers (config of the precision form) -------------------
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -2,11 +2,16 @@
 // visibility batches, transfer construction, direct light, bounce driver,
 // procedural inputs and kernel timing.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -1220,6 +1225,483 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs, const vr_bounce_p
     jobs[j].n_bounces = done;
   }
   VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
+}
+
+// ---- serving: resident scenes, re-bake requests, continuous batching ------------
+namespace vr {
+namespace {
+
+// First-fit allocator over one device arena (offsets in bytes, 256-aligned).
+struct Arena {
+  DBuf mem;
+  size_t cap = 0;
+  std::map<size_t, size_t> free_;  // offset -> size
+  int init(size_t bytes) {
+    cap = (bytes + 255) & ~(size_t)255;
+    TRY(mem.alloc(cap));
+    free_.clear();
+    free_[0] = cap;
+    return VR_OK;
+  }
+  bool alloc(size_t bytes, size_t *off) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      *off = it->first;
+      const size_t rest = it->second - bytes, at = it->first + bytes;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  char *ptr(size_t off) const { return mem.as<char>() + off; }
+};
+
+struct ServeScene {
+  vr_scene *scene = nullptr;
+  vr_xfers *xfers = nullptr;
+  int64_t n = 0;
+  DBuf samples;  // px py pz nx ny nz (6 n floats)
+  DBuf refl, area;
+  std::vector<vr_light> lights;
+  DevTiles tl{};
+};
+
+// Parts of a request's block in the arena, each padded to 256 bytes.
+struct ReqLayout {
+  size_t emit0, emit1, total, e_row, scratch, energy, lights, counters, bytes;
+  ReqLayout(int64_t n, int bounces, int n_lights) {
+    auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t vb = pad(12 * (size_t)std::max<int64_t>(n, 1) + 16);
+    const int64_t eb = (n + 2047) / 2048;
+    size_t o = 0;
+    emit0 = o; o += vb;
+    emit1 = o; o += vb;
+    total = o; o += vb;
+    e_row = o; o += pad(8 * (size_t)std::max<int64_t>(n, 1));
+    scratch = o; o += pad(8 * (size_t)(eb + 2));
+    energy = o; o += pad(8 * (size_t)std::max(bounces, 1));
+    lights = o; o += pad(sizeof(vr_light) * (size_t)std::max(n_lights, 1));
+    counters = o; o += pad(4 * (size_t)std::max(bounces, 1));
+    bytes = o;
+  }
+};
+
+struct ServeReq {
+  int64_t ticket = 0;
+  int scene = 0;
+  std::vector<vr_light> lights;
+  std::chrono::steady_clock::time_point t_submit, t_done;
+  size_t off = 0, bytes = 0;
+  int slot = -1;
+  int status = VR_OK;
+  std::string err;
+  bool done = false;
+  std::vector<float> total;
+  std::vector<double> energy;
+  int n_bounces = 0;
+};
+
+}  // namespace
+}  // namespace vr
+
+struct vr_server {
+  vr_server_params prm{};
+  std::vector<std::unique_ptr<vr::ServeScene>> scenes;
+  vr::Arena arena;
+  DBuf sky, states, jobs_dev;
+  vr::BounceJob *jobs_host = nullptr;  // pinned mirror of the job table
+  hipStream_t stream = nullptr;
+  vr::TiledShape shape{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<vr::ServeReq *> pending;
+  std::vector<vr::ServeReq *> active;
+  std::map<int64_t, std::unique_ptr<vr::ServeReq>> reqs;
+  std::vector<int> free_slots;
+  int64_t next_ticket = 1;
+  bool stop = false, started = false;
+  std::thread worker;
+  vr_server_stats st{};
+};
+
+namespace vr {
+namespace {
+
+int serve_request_setup(vr_server *sv, ServeReq *r) {
+  // re-gather the direct light with the moved lights, then emit = direct * refl
+  const ServeScene &sc = *sv->scenes[(size_t)r->scene];
+  const ReqLayout L(sc.n, sv->prm.bounces, (int)r->lights.size());
+  char *base = sv->arena.ptr(r->off);
+  hipStream_t st = sv->stream;
+  if (!r->lights.empty())
+    VR_HIP(hipMemcpyAsync(base + L.lights, r->lights.data(), sizeof(vr_light) * r->lights.size(),
+                          hipMemcpyHostToDevice, st));
+  const float *p = sc.samples.as<float>();
+  const int64_t n = sc.n;
+  DevSamples ds{p, p + n, p + 2 * n, p + 3 * n, p + 4 * n, p + 5 * n};
+  float *direct = reinterpret_cast<float *>(base + L.emit1);  // free until the first bounce
+  const int path = sv->prm.mode == VR_SERVE_PLAIN ? VR_PATH_PLAIN : VR_PATH_OPT;
+  launch_gather_light(sc.scene->dev, ds, n, reinterpret_cast<vr_light *>(base + L.lights),
+                      (int32_t)r->lights.size(), sv->sky.as<float>(), sv->prm.normal_offset, -1,
+                      direct, path, st);
+  launch_emit_init(n, direct, sc.refl.as<float>(), reinterpret_cast<float *>(base + L.emit0),
+                   reinterpret_cast<float *>(base + L.total), st);
+  VR_HIP(hipMemsetAsync(sv->states.as<BounceState>() + r->slot, 0, sizeof(BounceState), st));
+  return check_launch();
+}
+
+void serve_job(vr_server *sv, ServeReq *r, BounceJob &jb) {
+  const ServeScene &sc = *sv->scenes[(size_t)r->scene];
+  const ReqLayout L(sc.n, sv->prm.bounces, (int)r->lights.size());
+  char *base = sv->arena.ptr(r->off);
+  jb.x = sc.xfers->dev();
+  jb.tl = sc.tl;
+  jb.emit[0] = reinterpret_cast<float *>(base + L.emit0);
+  jb.emit[1] = reinterpret_cast<float *>(base + L.emit1);
+  jb.total = reinterpret_cast<float *>(base + L.total);
+  jb.refl = sc.refl.as<float>();
+  jb.area = sc.area.as<float>();
+  jb.e_row = reinterpret_cast<double *>(base + L.e_row);
+  jb.scratch = reinterpret_cast<double *>(base + L.scratch);
+  jb.energy = reinterpret_cast<double *>(base + L.energy);
+  jb.state = sv->states.as<BounceState>() + r->slot;
+  jb.e_blocks = (sc.n + 2047) / 2048;
+  jb.max_bounces = sv->prm.bounces;
+  jb.stop_frac = 0.0;
+}
+
+int serve_finish(vr_server *sv, ServeReq *r, int n_bounces) {
+  const ServeScene &sc = *sv->scenes[(size_t)r->scene];
+  const ReqLayout L(sc.n, sv->prm.bounces, (int)r->lights.size());
+  char *base = sv->arena.ptr(r->off);
+  r->total.resize(3 * (size_t)sc.n);
+  r->energy.resize((size_t)std::max(n_bounces, 0));
+  if (sc.n > 0)
+    VR_HIP(hipMemcpyAsync(r->total.data(), base + L.total, 12 * (size_t)sc.n,
+                          hipMemcpyDeviceToHost, sv->stream));
+  if (n_bounces > 0)
+    VR_HIP(hipMemcpyAsync(r->energy.data(), base + L.energy, 8 * (size_t)n_bounces,
+                          hipMemcpyDeviceToHost, sv->stream));
+  r->n_bounces = n_bounces;
+  return VR_OK;
+}
+
+// Plain sequential re-bake of one request: scratch-stack gather, one thread
+// per CSR row in FP32, a host read of the energy after every bounce.
+int serve_plain(vr_server *sv, ServeReq *r) {
+  TRY(serve_request_setup(sv, r));
+  const ServeScene &sc = *sv->scenes[(size_t)r->scene];
+  const ReqLayout L(sc.n, sv->prm.bounces, (int)r->lights.size());
+  char *base = sv->arena.ptr(r->off);
+  hipStream_t st = sv->stream;
+  DevXfers dx = sc.xfers->dev();
+  dx.coef = sc.xfers->coef32.p;
+  dx.kind = VR_COEF_F32;
+  float *em[2] = {reinterpret_cast<float *>(base + L.emit0), reinterpret_cast<float *>(base + L.emit1)};
+  BounceState *state = sv->states.as<BounceState>() + r->slot;
+  BounceState hs{};
+  int k = 0;
+  for (; k < sv->prm.bounces; ++k) {
+    launch_bounce(dx, em[k & 1], em[(k + 1) & 1], reinterpret_cast<float *>(base + L.total),
+                  sc.refl.as<float>(), sc.area.as<float>(), reinterpret_cast<double *>(base + L.e_row),
+                  reinterpret_cast<uint32_t *>(base + L.counters) + k, state, VR_PATH_PLAIN, st);
+    launch_energy(reinterpret_cast<double *>(base + L.e_row), sc.n,
+                  reinterpret_cast<double *>(base + L.scratch),
+                  reinterpret_cast<double *>(base + L.energy), k, 0.0, state, st);
+    TRY(check_launch());
+    VR_HIP(hipMemcpyAsync(&hs, state, sizeof hs, hipMemcpyDeviceToHost, st));
+    VR_HIP(hipStreamSynchronize(st));
+    if (hs.converged) {
+      ++k;
+      break;
+    }
+  }
+  TRY(serve_finish(sv, r, hs.k_done));
+  VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
+}
+
+void serve_loop(vr_server *sv) {
+  (void)hipSetDevice(g_device);
+  std::vector<BounceState> hs((size_t)sv->prm.max_active);
+  for (;;) {
+    std::vector<ServeReq *> admitted;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
+      if (sv->stop && sv->pending.empty() && sv->active.empty()) return;
+      // admission: FIFO, while a slot and the request's footprint are free
+      while (!sv->pending.empty() && !sv->free_slots.empty()) {
+        ServeReq *r = sv->pending.front();
+        size_t off = 0;
+        if (!sv->arena.alloc(r->bytes, &off)) break;
+        r->off = off;
+        r->slot = sv->free_slots.back();
+        sv->free_slots.pop_back();
+        sv->pending.pop_front();
+        admitted.push_back(r);
+      }
+    }
+    std::vector<ServeReq *> finished;
+    int rc = VR_OK;
+    if (sv->prm.mode == VR_SERVE_PLAIN) {
+      for (ServeReq *r : admitted) {
+        rc = serve_plain(sv, r);
+        r->status = rc;
+        if (rc) r->err = vr_last_error();
+        finished.push_back(r);
+      }
+    } else {
+      for (ServeReq *r : admitted) {
+        rc = serve_request_setup(sv, r);
+        if (rc) {
+          r->status = rc;
+          r->err = vr_last_error();
+          finished.push_back(r);
+        } else {
+          sv->active.push_back(r);
+        }
+      }
+      const int na = (int)sv->active.size();
+      if (na > 0) {
+        // one bounce of every active request in one batched launch
+        int64_t chunks = 0, eblocks = 0;
+        for (int j = 0; j < na; ++j) {
+          BounceJob &jb = sv->jobs_host[j];
+          serve_job(sv, sv->active[(size_t)j], jb);
+          jb.chunk_begin = chunks;
+          jb.eblk_begin = eblocks;
+          chunks += jb.tl.n_chunks;
+          eblocks += jb.e_blocks;
+        }
+        hipStream_t st = sv->stream;
+        hipError_t e = hipMemcpyAsync(sv->jobs_dev.p, sv->jobs_host, sizeof(BounceJob) * (size_t)na,
+                                      hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+          launch_bounce_jobs(sv->scenes[0]->xfers->precision, sv->shape.cols, sv->shape.group, sv->shape.passes,
+                             sv->shape.prefetch, sv->jobs_dev.as<BounceJob>(), na, chunks,
+                             eblocks, st);
+          e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(hs.data(), sv->states.p, sizeof(BounceState) * hs.size(),
+                             hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        std::vector<ServeReq *> still;
+        for (ServeReq *r : sv->active) {
+          const BounceState &b = hs[(size_t)r->slot];
+          if (e != hipSuccess) {
+            r->status = fail(VR_E_HIP, std::string("serving: ") + hipGetErrorString(e));
+            r->err = vr_last_error();
+            finished.push_back(r);
+          } else if (b.converged || b.k_done >= sv->prm.bounces) {
+            r->status = serve_finish(sv, r, b.k_done);
+            if (r->status) r->err = vr_last_error();
+            finished.push_back(r);
+          } else {
+            still.push_back(r);
+          }
+        }
+        sv->active.swap(still);
+        (void)hipStreamSynchronize(st);
+        sv->st.iterations += 1;
+        sv->st.max_active_seen = std::max<int64_t>(sv->st.max_active_seen, na);
+      }
+    }
+    const auto now = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (ServeReq *r : finished) {
+      r->t_done = now;
+      r->done = true;
+      sv->arena.release(r->off, r->bytes);
+      sv->free_slots.push_back(r->slot);
+      sv->st.completed += 1;
+    }
+    sv->st.active = (int64_t)sv->active.size();
+    if (!finished.empty()) sv->cv_done.notify_all();
+  }
+}
+
+}  // namespace
+}  // namespace vr
+
+int vr_server_create(vr_server **out, const vr_server_params *p) {
+  LOCK;
+  if (!out || !p) return fail(VR_E_INVALID, "NULL argument");
+  *out = nullptr;
+  if (p->mode != VR_SERVE_BATCHED && p->mode != VR_SERVE_PLAIN)
+    return fail(VR_E_INVALID, "unknown serving mode");
+  if (p->max_active < 1 || p->max_active > 256) return fail(VR_E_INVALID, "max_active not in [1, 256]");
+  if (p->bounces < 1 || p->bounces > 1000) return fail(VR_E_INVALID, "bounces not in [1, 1000]");
+  TRY(ensure_init());
+  std::unique_ptr<vr_server> sv(new vr_server);
+  sv->prm = *p;
+  if (sv->prm.mode == VR_SERVE_PLAIN) sv->prm.max_active = 1;
+  if (sv->prm.hbm_budget <= 0) sv->prm.hbm_budget = (int64_t)8 << 30;
+  if (!(sv->prm.normal_offset >= 0)) sv->prm.normal_offset = 0.1f;
+  TRY(sv->arena.init((size_t)sv->prm.hbm_budget));
+  TRY(sv->states.alloc(sizeof(BounceState) * (size_t)sv->prm.max_active));
+  TRY(sv->jobs_dev.alloc(sizeof(BounceJob) * (size_t)sv->prm.max_active));
+  VR_HIP(hipHostMalloc((void **)&sv->jobs_host, sizeof(BounceJob) * (size_t)sv->prm.max_active,
+                       hipHostMallocDefault));
+  VR_HIP(hipStreamCreate(&sv->stream));
+  std::vector<float> dirs = sky_dirs();
+  TRY(sv->sky.upload(dirs.data(), dirs.size() * 4, sv->stream));
+  for (int i = sv->prm.max_active - 1; i >= 0; --i) sv->free_slots.push_back(i);
+  VR_HIP(hipStreamSynchronize(sv->stream));
+  *out = sv.release();
+  return VR_OK;
+}
+
+int vr_server_add_scene(vr_server *sv, vr_scene *scene, vr_xfers *x, const vr_samples *smp,
+                        const float *refl, const float *area, const vr_light *lights,
+                        int32_t n_lights, int32_t *scene_id) {
+  LOCK;
+  if (!sv || !scene || !x || !smp || !scene_id) return fail(VR_E_INVALID, "NULL argument");
+  if (!scene->built) return fail(VR_E_STATE, "scene not built");
+  if (n_lights < 0 || (n_lights > 0 && !lights)) return fail(VR_E_INVALID, "bad lights");
+  const int64_t n = x->n;
+  if (n > 0 && (!refl || !area || !smp->px || !smp->py || !smp->pz || !smp->nx || !smp->ny || !smp->nz))
+    return fail(VR_E_INVALID, "NULL patch array");
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->started) return fail(VR_E_STATE, "scenes are added before the first request");
+  }
+  if (sv->prm.mode == VR_SERVE_BATCHED) {
+    if (!sv->scenes.empty() && sv->scenes[0]->xfers->precision != x->precision)
+      return fail(VR_E_INVALID, "the scenes of a server share the coefficient precision");
+    if (n > (int64_t)2048 * 2048) return fail(VR_E_CAPACITY, "a served scene holds at most 2048^2 patches");
+  }
+  std::unique_ptr<vr::ServeScene> sc(new vr::ServeScene);
+  sc->scene = scene;
+  sc->xfers = x;
+  sc->n = n;
+  sc->lights.assign(lights, lights + n_lights);
+  const size_t fb = 4 * (size_t)std::max<int64_t>(n, 1);
+  TRY(sc->samples.alloc(6 * fb));
+  TRY(sc->refl.alloc(3 * fb));
+  TRY(sc->area.alloc(fb));
+  const float *src[6] = {smp->px, smp->py, smp->pz, smp->nx, smp->ny, smp->nz};
+  if (n > 0) {
+    for (int k = 0; k < 6; ++k)
+      VR_HIP(hipMemcpyAsync(sc->samples.as<float>() + k * n, src[k], 4 * (size_t)n,
+                            hipMemcpyHostToDevice, g_stream));
+    VR_HIP(hipMemcpyAsync(sc->refl.p, refl, 12 * (size_t)n, hipMemcpyHostToDevice, g_stream));
+    VR_HIP(hipMemcpyAsync(sc->area.p, area, 4 * (size_t)n, hipMemcpyHostToDevice, g_stream));
+  }
+  if (sv->prm.mode == VR_SERVE_BATCHED) {
+    TRY(vr::tiled_shape(x, tuning(), sv->shape));
+    TRY(vr::prepare_tiles(x, sv->shape, sc->tl, g_stream));
+  } else if (!x->coef32.p) {
+    return fail(VR_E_STATE, "plain serving needs the FP32 coefficients");
+  }
+  VR_HIP(hipStreamSynchronize(g_stream));
+  *scene_id = (int32_t)sv->scenes.size();
+  sv->scenes.push_back(std::move(sc));
+  return VR_OK;
+}
+
+int vr_server_submit(vr_server *sv, int32_t scene_id, const int32_t *light_index,
+                     const float *offset, int32_t n_moved, int64_t *ticket) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!sv || !ticket) return fail(VR_E_INVALID, "NULL argument");
+  if (scene_id < 0 || scene_id >= (int32_t)sv->scenes.size()) return fail(VR_E_INVALID, "unknown scene");
+  if (n_moved < 0 || (n_moved > 0 && (!light_index || !offset))) return fail(VR_E_INVALID, "bad moves");
+  const vr::ServeScene &sc = *sv->scenes[(size_t)scene_id];
+  std::unique_ptr<vr::ServeReq> r(new vr::ServeReq);
+  r->scene = scene_id;
+  r->lights = sc.lights;
+  for (int i = 0; i < n_moved; ++i) {
+    const int li = light_index[i];
+    if (li < 0 || li >= (int)r->lights.size()) return fail(VR_E_INVALID, "light index out of range");
+    for (int c = 0; c < 3; ++c) r->lights[(size_t)li].origin[c] += offset[3 * i + c];
+  }
+  r->bytes = vr::ReqLayout(sc.n, sv->prm.bounces, (int)r->lights.size()).bytes;
+  if (r->bytes > sv->arena.cap)
+    return fail(VR_E_CAPACITY, "request needs more device memory than the serving budget");
+  r->t_submit = t0;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (!sv->started) {
+    sv->started = true;
+    sv->worker = std::thread(vr::serve_loop, sv);
+  }
+  r->ticket = sv->next_ticket++;
+  *ticket = r->ticket;
+  sv->pending.push_back(r.get());
+  sv->reqs[r->ticket] = std::move(r);
+  sv->st.submitted += 1;
+  sv->cv_work.notify_one();
+  return VR_OK;
+}
+
+int vr_server_wait(vr_server *sv, int64_t ticket, int32_t timeout_ms, float *total, double *energy,
+                   int32_t *n_bounces, double *latency_ms) {
+  if (!sv) return fail(VR_E_INVALID, "NULL server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return fail(VR_E_INVALID, "unknown ticket");
+  vr::ServeReq *r = it->second.get();
+  auto ready = [&] { return r->done; };
+  if (timeout_ms < 0) {
+    sv->cv_done.wait(lk, ready);
+  } else if (!sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+    return fail(VR_E_STATE, "request not finished");
+  }
+  if (r->status) {
+    const int rc = fail(r->status, r->err);
+    sv->reqs.erase(it);
+    return rc;
+  }
+  if (total && !r->total.empty()) std::memcpy(total, r->total.data(), 4 * r->total.size());
+  if (energy && !r->energy.empty()) std::memcpy(energy, r->energy.data(), 8 * r->energy.size());
+  if (n_bounces) *n_bounces = r->n_bounces;
+  if (latency_ms)
+    *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  sv->reqs.erase(it);
+  return VR_OK;
+}
+
+int vr_server_get_stats(vr_server *sv, vr_server_stats *out) {
+  if (!sv || !out) return fail(VR_E_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  *out = sv->st;
+  out->pending = (int64_t)sv->pending.size();
+  return VR_OK;
+}
+
+int vr_server_destroy(vr_server *sv) {
+  if (!sv) return VR_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+    sv->cv_work.notify_all();
+  }
+  if (sv->worker.joinable()) sv->worker.join();
+  if (sv->stream) {
+    (void)hipStreamSynchronize(sv->stream);
+    (void)hipStreamDestroy(sv->stream);
+  }
+  if (sv->jobs_host) (void)hipHostFree(sv->jobs_host);
+  delete sv;
   return VR_OK;
 }
 

@@ -116,6 +116,9 @@ void launch_bounce_jobs(int kind, int cols, int group, int passes, int prefetch,
 void launch_gather_jobs(int kind, int cols, int group, int passes, int prefetch,
                         const BounceJob *jobs, int n_jobs, int64_t total_chunks, hipStream_t st);
 void launch_energy_jobs(const BounceJob *jobs, int n_jobs, int64_t total_eblocks, hipStream_t st);
+// total = direct; emit = direct * refl (3n floats, no aliasing)
+void launch_emit_init(int64_t n, const float *direct, const float *refl, float *emit,
+                      float *total, hipStream_t st);
 void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const int32_t *col,
                       int cols, int nt, int32_t *tp, hipStream_t st);
 // blocked layout: offsets (boff) and per-block totals, then the copy
