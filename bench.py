@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
 LEVEL = dict(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
 
@@ -98,6 +102,7 @@ class Ctx:
 
 
 def run_visibility(cx, args):
+    progress("visibility")
     from vrad_amd import scenes
     from vrad_amd.raytracer import Environment
     torch, lib, check, L = cx.torch, cx.lib, cx.check, cx.L
@@ -187,6 +192,7 @@ def run_cpu(v, args):
 
 
 def run_bounce(cx, args):
+    progress("bounce gather sweep")
     from vrad_amd import lightmap
     from vrad_amd.patches import Transfers
     lib, check, L = cx.lib, cx.check, cx.L
@@ -250,6 +256,222 @@ def run_bounce(cx, args):
     return out
 
 
+def run_room_bake(cx, args):
+    """The CPU-runnable case end to end: closed room (40 pillars), patches
+    chopped at 32 units, one point light; MakeTransfers, direct light and
+    FP32 bounces to the energy stop rule (1e-3, at most 100), on the GPU and
+    with the CPU restatement on the host cores; results compared bit for bit."""
+    import oracle
+    from vrad_amd import lightmap, scenes
+    from vrad_amd.patches import MakeTransfers
+    from vrad_amd.raytracer import Environment
+    progress("closed-room full bake")
+    s = scenes.room(seed=1, n_pillars=40, chop=32.0, tess=32.0)
+
+    def gpu_bake():
+        t = {}
+        t0 = time.perf_counter()
+        env = Environment()
+        env.AddTriangles(s.verts9, s.ids)
+        env.SetupAccelerationStructure()
+        t["tree_ms"] = (time.perf_counter() - t0) * 1e3
+        t1 = time.perf_counter()
+        xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area)
+        t["make_transfers_ms"] = (time.perf_counter() - t1) * 1e3
+        t2 = time.perf_counter()
+        direct = lightmap.GatherSampleLight(env, s.p_origin, s.p_normal, s.lights)
+        emit0 = (direct * s.p_refl).astype(np.float32)
+        t["direct_ms"] = (time.perf_counter() - t2) * 1e3
+        t3 = time.perf_counter()
+        tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct)
+        t["bounce_ms"] = (time.perf_counter() - t3) * 1e3
+        t["total_ms"] = (time.perf_counter() - t0) * 1e3
+        nnz = xf.info().nnz
+        xf.close()
+        return t, tot, k, nnz
+
+    gpu_bake()                       # first call pays one-time set-up
+    tg, tot, k, nnz = gpu_bake()
+    threads = min(16, os.cpu_count() or 1)
+    tc = {}
+    c0 = time.perf_counter()
+    tri = oracle.precompute(s.verts9, s.ids)
+    kd = oracle.KdTree(s.verts9)
+    tc["tree_ms"] = (time.perf_counter() - c0) * 1e3
+    c1 = time.perf_counter()
+    rp, col, coef, _ = oracle.make_transfers(kd, tri, s.p_origin, s.p_normal, s.p_area,
+                                             threads=threads)
+    tc["make_transfers_ms"] = (time.perf_counter() - c1) * 1e3
+    c2 = time.perf_counter()
+    direct = oracle.gather_light(kd, tri, s.p_origin, s.p_normal, s.lights, threads=threads)
+    emit0 = (direct * s.p_refl).astype(np.float32)
+    tc["direct_ms"] = (time.perf_counter() - c2) * 1e3
+    c3 = time.perf_counter()
+    otot, oen, ok = oracle.bounce_f32(rp, col, coef, emit0, s.p_refl, s.p_area, total=direct,
+                                      threads=threads)
+    tc["bounce_ms"] = (time.perf_counter() - c3) * 1e3
+    tc["total_ms"] = (time.perf_counter() - c0) * 1e3
+    r = lambda d: {kk: round(v, 2) for kk, v in d.items()}
+    return {"triangles": int(len(s.ids)), "patches": int(len(s.p_area)), "transfers": int(nnz),
+            "bounces": int(k), "gpu": r(tg), "cpu": r(tc), "cpu_threads": threads,
+            "speedup_total": round(tc["total_ms"] / tg["total_ms"], 1),
+            "bitexact_vs_cpu": bool(k == ok and np.array_equal(tot.view(np.uint32),
+                                                                otot.view(np.uint32)))}
+
+
+def run_batch_bake(cx, args):
+    """The batching form: 64 procedural scenes of uneven size baked together.
+    Per scene MakeTransfers and direct light, then every scene's FP32 bounces
+    to the stop rule in one batched launch per bounce (a replayed hipGraph of
+    8 bounces), against the same bounces one scene at a time (optimised
+    kernels) and on the plain path; per-scene results compared bit for bit."""
+    from vrad_amd import lightmap, scenes
+    from vrad_amd.patches import MakeTransfers
+    from vrad_amd.raytracer import Environment
+    lib, check, L = cx.lib, cx.check, cx.L
+    progress(f"batched bake of {args.batch_scenes} scenes")
+    rng = np.random.default_rng(64)
+    baked, t_xf, t_direct, tris = [], 0.0, 0.0, []
+    for i in range(args.batch_scenes):
+        # uneven sizes, about 1k to 60k patches: log-uniform target, then a room
+        # grid and chop that reach it (about 5300 patches per room at chop 32)
+        target = float(np.exp(rng.uniform(np.log(1000.0), np.log(60000.0))))
+        rooms = int(np.clip(round(target / 5300.0 * 2.0), 1, 16))
+        rx = int(np.clip(round(np.sqrt(rooms)), 1, 4))
+        ry = int(np.clip(round(rooms / rx), 1, 4))
+        chop = float(np.clip(32.0 * np.sqrt(5300.0 * rx * ry / target), 24.0, 256.0))
+        s = scenes.level(seed=500 + i, rooms_x=rx, rooms_y=ry, chop=chop, tess=64.0,
+                         n_lights=int(rng.integers(1, 17)), sun=bool(rng.integers(0, 2)))
+        env = Environment()
+        env.AddTriangles(s.verts9, s.ids)
+        env.SetupAccelerationStructure()
+        t0 = time.perf_counter()
+        xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area)
+        t_xf += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        direct = lightmap.GatherSampleLight(env, s.p_origin, s.p_normal, s.lights)
+        t_direct += time.perf_counter() - t0
+        emit0 = (direct * s.p_refl).astype(np.float32)
+        baked.append((xf, emit0, s.p_refl, s.p_area, direct))
+        tris.append(len(s.ids))
+    jobs = [(xf, e0, rf, ar, d) for xf, e0, rf, ar, d in baked]
+    lightmap.BounceLightBatch(jobs[:2], max_bounces=2)          # warm the kernels
+    t0 = time.perf_counter()
+    batch = lightmap.BounceLightBatch(jobs, max_bounces=100, stop_frac=1e-3, use_graph=True)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    single = [lightmap.BounceLight(xf, e0, rf, ar, total=d) for xf, e0, rf, ar, d in jobs]
+    t_single = time.perf_counter() - t0
+    check(lib.vr_set_path(L.VR_PATH_PLAIN))
+    try:
+        t0 = time.perf_counter()
+        for xf, e0, rf, ar, d in jobs:
+            lightmap.BounceLight(xf, e0, rf, ar, total=d)
+        t_plain = time.perf_counter() - t0
+    finally:
+        check(lib.vr_set_path(L.VR_PATH_OPT))
+    same = all(k1 == k2 and np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
+               for (t1, _, k1), (t2, _, k2) in zip(batch, single))
+    n = [len(j[2]) for j in jobs]
+    nnz = [j[0].info().nnz for j in jobs]
+    ks = [b[2] for b in batch]
+    for j in jobs:
+        j[0].close()
+    return {"scenes": len(jobs), "triangles": [int(min(tris)), int(max(tris))],
+            "patches": [int(min(n)), int(max(n))], "patches_total": int(sum(n)),
+            "transfers_total": int(sum(nnz)), "bounces": [int(min(ks)), int(max(ks))],
+            "make_transfers_s": round(t_xf, 3), "direct_s": round(t_direct, 3),
+            "bounce_batched_graph_s": round(t_batch, 4), "bounce_one_by_one_s": round(t_single, 4),
+            "bounce_plain_s": round(t_plain, 4),
+            "batched_vs_plain": round(t_plain / t_batch, 1),
+            "batched_equals_one_by_one": bool(same)}
+
+
+def run_serving(cx, args):
+    """The serving form: resident scenes from the precision-form generator
+    (mixed sizes, synthetic lognormal transfers, FP16 with per-row scale), a
+    seeded Poisson stream of re-bake requests (1-4 lights moved by
+    U[-64,64]^3, direct re-gather + 8 bounces); p50/p99 latency (completion -
+    arrival) and throughput of the batched server and of the plain sequential
+    server on the same arrival trace; a sample of requests re-run alone must
+    equal its results in the mix."""
+    from vrad_amd import scenes
+    from vrad_amd.patches import Transfers
+    from vrad_amd.raytracer import Environment
+    from vrad_amd.serve import BATCHED, PLAIN, Server
+    L = cx.L
+    progress(f"serving: building {args.serve_scenes} resident scenes")
+    rng = np.random.default_rng(2026)
+    resident = []
+    for i in range(args.serve_scenes):
+        rx, ry = (int(v) for v in rng.integers(2, 7, 2))
+        s = scenes.level(seed=900 + i, rooms_x=rx, rooms_y=ry, chop=32.0, tess=64.0,
+                         n_lights=int(rng.integers(4, 17)), sun=True)
+        env = Environment()
+        env.AddTriangles(s.verts9, s.ids)
+        env.SetupAccelerationStructure()
+        n = len(s.p_area)
+        xf = Transfers.synthetic(seed=7000 + i, n_rows=n, mean_len=2000.0, sigma=1.0,
+                                 max_len=min(40000, n - 1), precision=L.VR_COEF_F16)
+        resident.append((s, env, xf))
+    trace, t = [], 0.0
+    for _ in range(args.serve_requests):
+        t += float(rng.exponential(1.0 / args.serve_rate))
+        sid = int(rng.integers(len(resident)))
+        nl = len(resident[sid][0].lights)
+        k = int(rng.integers(1, min(4, nl) + 1))
+        moves = [(int(li), rng.uniform(-64, 64, 3).astype(np.float32))
+                 for li in rng.choice(nl, size=k, replace=False)]
+        trace.append((t, sid, moves))
+
+    def serve(mode, max_active):
+        sv = Server(mode=mode, max_active=max_active, hbm_budget=16 << 30, bounces=8)
+        for s, env, xf in resident:
+            sv.add_scene(env, xf, s.p_origin, s.p_normal, s.p_refl, s.p_area, s.lights)
+        sv.wait(sv.submit(0, []))                       # first request pays set-up
+        progress(f"serving trace, mode {mode}: {len(trace)} requests")
+        tickets, t0 = [], time.perf_counter()
+        for i, (at, sid, moves) in enumerate(trace):
+            if i % 200 == 199:
+                progress(f"  {i + 1} submitted, {sv.stats()['completed']} done")
+            dt = at - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(sv.submit(sid, moves))
+        res = [sv.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+        lat = np.array([r[2] for r in res])
+        out = {"p50_ms": round(float(np.percentile(lat, 50)), 2),
+               "p99_ms": round(float(np.percentile(lat, 99)), 2),
+               "mean_ms": round(float(lat.mean()), 2),
+               "throughput_rps": round(len(res) / wall, 2), "wall_s": round(wall, 2),
+               "max_active_seen": int(st["max_active_seen"]), "iterations": int(st["iterations"])}
+        return sv, out, res
+
+    sv, batched, res_b = serve(BATCHED, 16)
+    # alone vs mix: a sample of the trace's requests re-run one at a time
+    same = True
+    for j in range(0, len(trace), max(1, len(trace) // 16)):
+        _, sid, moves = trace[j]
+        tot, en, _ = sv.wait(sv.submit(sid, moves))
+        same = same and np.array_equal(tot.view(np.uint32), res_b[j][0].view(np.uint32)) \
+            and np.array_equal(en, res_b[j][1])
+    sv.close()
+    del res_b
+    svp, plain, _ = serve(PLAIN, 1)
+    svp.close()
+    sizes = [len(r[0].p_area) for r in resident]
+    nnz = [r[2].info().nnz for r in resident]
+    for _, _, xf in resident:
+        xf.close()
+    return {"requests": len(trace), "rate_rps": args.serve_rate, "scenes": len(resident),
+            "patches": [int(min(sizes)), int(max(sizes))], "transfers_total": int(sum(nnz)),
+            "batched": batched, "plain_sequential": plain,
+            "p99_ratio_plain_over_batched": round(plain["p99_ms"] / batched["p99_ms"], 1),
+            "alone_equals_mix": bool(same)}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (profiles/traffic.json, written from the PMC runs and
@@ -273,6 +495,13 @@ def main():
     ap.add_argument("--no-bounce", action="store_true")
     ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-room", action="store_true", help="skip the closed-room full bake")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched bake")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--serve-scenes", type=int, default=32)
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
     ap.add_argument("--random-pairs", action="store_true",
                     help="also time an incoherent batch of random patch pairs")
     ap.add_argument("--bounce-sweep", type=int, default=1,
@@ -289,6 +518,9 @@ def main():
     v = None if args.no_trace else run_visibility(cx, args)
     cpu = None if (v is None or args.no_cpu) else run_cpu(v, args)
     bounce = None if args.no_bounce else run_bounce(cx, args)
+    room = None if args.no_room else run_room_bake(cx, args)
+    batch = None if args.no_batch else run_batch_bake(cx, args)
+    serving = None if args.no_serve else run_serving(cx, args)
 
     if args.calib_copy_mb > 0:
         torch = cx.torch
@@ -306,6 +538,7 @@ def main():
         "config": {"workload": "plain GPU path at scale: 8x8-room procedural level, "
                                "4M MakeTransfers TestLine segments per batch (row order)"},
         "roofline": None, "cpu_baseline": cpu, "visibility": None, "bounce": bounce,
+        "room_bake": room, "batch_bake": batch, "serving": serving,
     }
     if v is not None:
         n, info = v["n"], v["info"]
