@@ -170,7 +170,9 @@ struct vr_scene {
   bool built = false, host_built = false;
   std::vector<TriIntersect> tris;
   KdTree kd;
-  DBuf nodes, refs, planes;
+  KdTree kd_sky;                   // over the TRACE_ID_SKY triangles only
+  std::vector<int32_t> sky_index;  // kd_sky triangle -> triangle index
+  DBuf nodes, refs, planes, sky_nodes, sky_refs;
   DevScene dev{};
   int has_sky = 0;
   int64_t device_bytes = 0;
@@ -400,6 +402,21 @@ int vr_scene_build_host(vr_scene *s, int32_t n_threads) {
   s->has_sky = 0;
   for (const TriIntersect &t : s->tris)
     if (t.id & VR_TRACE_ID_SKY) s->has_sky = 1;
+  // a second tree over the sky triangles alone: TestLine finds the closest
+  // sky hit there, then stops the full tree at the first opaque hit before it
+  s->kd_sky = KdTree();
+  s->sky_index.clear();
+  if (s->has_sky) {
+    std::vector<float> v9s;
+    for (int64_t i = 0; i < s->n_tri; ++i)
+      if (s->tris[(size_t)i].id & VR_TRACE_ID_SKY) {
+        s->sky_index.push_back((int32_t)i);
+        v9s.insert(v9s.end(), s->v9.begin() + 9 * i, s->v9.begin() + 9 * i + 9);
+      }
+    TRY(build_kdtree(v9s.data(), (int64_t)s->sky_index.size(), nt, &s->kd_sky));
+    if (s->kd_sky.max_depth >= kStackDepth)
+      return fail(VR_E_CAPACITY, "sky kd-tree deeper than the traversal stack");
+  }
   s->host_built = true;
   return VR_OK;
 }
@@ -410,22 +427,37 @@ int vr_scene_build(vr_scene *s, int32_t n_threads) {
   TRY(vr_scene_build_host(s, n_threads));
   TRY(ensure_init());
   const KdTree &kd = s->kd;
-  std::vector<uint2> nodes(kd.children.size());
-  for (size_t k = 0; k < nodes.size(); ++k) {
-    uint32_t sb;
-    std::memcpy(&sb, &kd.split[k], 4);
-    nodes[k] = make_uint2(kd.children[k], sb);
-  }
-  std::vector<float4> refs(kd.idx.size() * 3);
-  for (size_t k = 0; k < kd.idx.size(); ++k) {
-    const TriIntersect &t = s->tris[(size_t)kd.idx[k]];
-    uint32_t meta = (uint32_t)kd.idx[k] | ((uint32_t)t.c0 << 28) | ((uint32_t)t.c1 << 30);
-    float idf, metaf;
-    std::memcpy(&idf, &t.id, 4);
-    std::memcpy(&metaf, &meta, 4);
-    refs[3 * k + 0] = make_float4(t.nx, t.ny, t.nz, t.d);
-    refs[3 * k + 1] = make_float4(t.e[0], t.e[1], t.e[2], t.e[3]);
-    refs[3 * k + 2] = make_float4(t.e[4], t.e[5], idf, metaf);
+  // device records of a tree: nodes, and 3 float4 per leaf reference whose
+  // meta holds the triangle's index in OptimizedTriangleList (`map` turns a
+  // tree's own triangle numbering into that index)
+  auto flatten = [&](const KdTree &t, const std::vector<int32_t> *map, std::vector<uint2> &nodes,
+                     std::vector<float4> &refs) {
+    nodes.resize(t.children.size());
+    for (size_t k = 0; k < nodes.size(); ++k) {
+      uint32_t sb;
+      std::memcpy(&sb, &t.split[k], 4);
+      nodes[k] = make_uint2(t.children[k], sb);
+    }
+    refs.resize(t.idx.size() * 3);
+    for (size_t k = 0; k < t.idx.size(); ++k) {
+      const int32_t ti = map ? (*map)[(size_t)t.idx[k]] : t.idx[k];
+      const TriIntersect &tr = s->tris[(size_t)ti];
+      uint32_t meta = (uint32_t)ti | ((uint32_t)tr.c0 << 28) | ((uint32_t)tr.c1 << 30);
+      float idf, metaf;
+      std::memcpy(&idf, &tr.id, 4);
+      std::memcpy(&metaf, &meta, 4);
+      refs[3 * k + 0] = make_float4(tr.nx, tr.ny, tr.nz, tr.d);
+      refs[3 * k + 1] = make_float4(tr.e[0], tr.e[1], tr.e[2], tr.e[3]);
+      refs[3 * k + 2] = make_float4(tr.e[4], tr.e[5], idf, metaf);
+    }
+  };
+  std::vector<uint2> nodes, sky_nodes;
+  std::vector<float4> refs, sky_refs;
+  flatten(kd, nullptr, nodes, refs);
+  if (s->has_sky) {
+    flatten(s->kd_sky, &s->sky_index, sky_nodes, sky_refs);
+    TRY(upload_vec(s->sky_nodes, sky_nodes, g_stream));
+    TRY(upload_vec(s->sky_refs, sky_refs, g_stream));
   }
   std::vector<float4> planes((size_t)s->n_tri);
   for (int64_t i = 0; i < s->n_tri; ++i) {
@@ -449,8 +481,17 @@ int vr_scene_build(vr_scene *s, int32_t n_threads) {
   d.M = M;
   d.has_sky = s->has_sky;
   d.empty = s->n_tri == 0 ? 1 : 0;
+  if (s->has_sky) {
+    d.sky_nodes = s->sky_nodes.as<uint2>();
+    d.sky_refs = s->sky_refs.as<float4>();
+    for (int c = 0; c < 3; ++c) {
+      d.sky_bmin[c] = s->kd_sky.bmin[c];
+      d.sky_bmax[c] = s->kd_sky.bmax[c];
+    }
+  }
   s->dev = d;
-  s->device_bytes = (int64_t)(s->nodes.bytes + s->refs.bytes + s->planes.bytes);
+  s->device_bytes = (int64_t)(s->nodes.bytes + s->refs.bytes + s->planes.bytes +
+                              s->sky_nodes.bytes + s->sky_refs.bytes);
   s->built = true;
   return VR_OK;
 }

@@ -20,6 +20,10 @@ struct DevScene {
   float M;                // max(1, max |scene coordinate|)
   int has_sky;
   int empty;              // no triangles
+  // a second tree over the TRACE_ID_SKY triangles alone (when has_sky)
+  const uint2 *sky_nodes;
+  const float4 *sky_refs;
+  float sky_bmin[3], sky_bmax[3];
 };
 
 constexpr float kTiny = 1e-20f;
@@ -40,7 +44,9 @@ __device__ __forceinline__ float sel3(int c, float a, float b, float d) {
 
 // Projected-barycentric hit test (frozen semantics of Trace4Rays, which is a
 // stub at raytracer/environment.go:140-145): accept t in [tmin, tbest),
-// equal t resolved to the lower triangle index.
+// equal t resolved to the lower triangle index.  SKIPSKY ignores
+// TRACE_ID_SKY triangles.
+template <bool SKIPSKY = false>
 __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32_t k, float ox,
                                          float oy, float oz, float dx, float dy, float dz,
                                          float tmin, int32_t skip, float &tbest, int32_t &best,
@@ -50,6 +56,7 @@ __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32
   const float4 q2 = refs[3 * (size_t)k + 2];
   const int32_t id = __float_as_int(q2.z);
   if (id == skip) return;
+  if (SKIPSKY && (id & 0x01000000)) return;
   const uint32_t meta = __float_as_uint(q2.w);
   const int32_t tri = (int32_t)(meta & kTriMask);
   const float den = (q0.x * dx + q0.y * dy) + q0.z * dz;
@@ -119,42 +126,43 @@ struct ScratchStack {
   }
 };
 
-// Closest hit in [tmin, tmax) (ANY: stop at the first hit).  D = stack
+// Walks one kd-tree (nodes/refs/bounds) for hits with tmin <= t < tmax that
+// come before (tbest, best) in (t, triangle index) order, updating them; the
+// result is the first of the hits in that order, bit for bit the brute-force
+// result.  ANY: return at the first accepted hit.  SKIPSKY: ignore
+// TRACE_ID_SKY triangles.  Returns whether a hit was accepted.  D = stack
 // capacity; the host guarantees tree depth < D.
-template <int D, bool ANY, class Stack>
-__device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, float oz, float dx,
-                          float dy, float dz, float tmin, float tmax, int32_t skip, float &t_out,
-                          int32_t &best_out, int32_t &id_out) {
-  float tbest = tmax;
-  int32_t best = -1, best_id = 0;
-  t_out = 0.0f;
-  best_out = -1;
-  id_out = 0;
-  if (sc.empty) return;
+template <int D, bool ANY, bool SKIPSKY, class Stack>
+__device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restrict__ refs,
+                          const float *bmin, const float *bmax, float M, Stack &st, float ox,
+                          float oy, float oz, float dx, float dy, float dz, float tmin,
+                          float tmax, int32_t skip, float &tbest, int32_t &best,
+                          int32_t &best_id) {
+  const int32_t best0 = best;
   const float om = fmaxf(fabsf(ox), fmaxf(fabsf(oy), fabsf(oz)));
-  const float dl = (sc.M + om) * 0x1p-15f;
+  const float dl = (M + om) * 0x1p-15f;
   const float ix = 1.0f / (fabsf(dx) < kTiny ? copysignf(kTiny, dx) : dx);
   const float iy = 1.0f / (fabsf(dy) < kTiny ? copysignf(kTiny, dy) : dy);
   const float iz = 1.0f / (fabsf(dz) < kTiny ? copysignf(kTiny, dz) : dz);
   float ta = tmin, tb = tmax;
   {
-    float t1 = ((sc.bmin[0] - dl) - ox) * ix, t2 = ((sc.bmax[0] + dl) - ox) * ix;
+    float t1 = ((bmin[0] - dl) - ox) * ix, t2 = ((bmax[0] + dl) - ox) * ix;
     ta = fmaxf(ta, fminf(t1, t2));
     tb = fminf(tb, fmaxf(t1, t2));
-    t1 = ((sc.bmin[1] - dl) - oy) * iy;
-    t2 = ((sc.bmax[1] + dl) - oy) * iy;
+    t1 = ((bmin[1] - dl) - oy) * iy;
+    t2 = ((bmax[1] + dl) - oy) * iy;
     ta = fmaxf(ta, fminf(t1, t2));
     tb = fminf(tb, fmaxf(t1, t2));
-    t1 = ((sc.bmin[2] - dl) - oz) * iz;
-    t2 = ((sc.bmax[2] + dl) - oz) * iz;
+    t1 = ((bmin[2] - dl) - oz) * iz;
+    t2 = ((bmax[2] + dl) - oz) * iz;
     ta = fmaxf(ta, fminf(t1, t2));
     tb = fminf(tb, fmaxf(t1, t2));
   }
-  if (!(ta <= tb)) return;
+  if (!(ta <= tb)) return false;
   int sp = 0;
   uint32_t node = 0;
   for (;;) {
-    const uint2 nd = sc.nodes[node];
+    const uint2 nd = nodes[node];
     const uint32_t type = nd.x & 3u;
     if (type != 3u) {
       const float sv = __uint_as_float(nd.y);
@@ -188,8 +196,8 @@ __device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, flo
       const uint32_t start = nd.x >> 2;
       const uint32_t cnt = (uint32_t)__uint_as_float(nd.y);
       for (uint32_t k = start; k < start + cnt; ++k)
-        tri_test(sc.refs, k, ox, oy, oz, dx, dy, dz, tmin, skip, tbest, best, best_id);
-      if (ANY && best >= 0) break;
+        tri_test<SKIPSKY>(refs, k, ox, oy, oz, dx, dy, dz, tmin, skip, tbest, best, best_id);
+      if (ANY && best != best0) return true;
     }
     bool found = false;
     while (sp > 0) {
@@ -206,7 +214,22 @@ __device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, flo
     }
     if (!found) break;
   }
-  if (best >= 0) {
+  return best != best0;
+}
+
+// Closest hit in [tmin, tmax) (ANY: stop at the first hit).
+template <int D, bool ANY, class Stack>
+__device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, float oz, float dx,
+                          float dy, float dz, float tmin, float tmax, int32_t skip, float &t_out,
+                          int32_t &best_out, int32_t &id_out) {
+  float tbest = tmax;
+  int32_t best = -1, best_id = 0;
+  t_out = 0.0f;
+  best_out = -1;
+  id_out = 0;
+  if (sc.empty) return;
+  if (walk_tree<D, ANY, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, sc.M, st, ox, oy, oz, dx, dy,
+                               dz, tmin, tmax, skip, tbest, best, best_id)) {
     t_out = tbest;
     best_out = best;
     id_out = best_id;
@@ -216,6 +239,10 @@ __device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, flo
 // TestLineDoesHitSky for one segment (raytracer/trace/testline.go:22-51):
 // dir = delta * (1/len), trace [0, len), occluded iff the closest hit is not
 // a TRACE_ID_SKY triangle.  Returns visible; *sky = closest hit is sky.
+// With sky triangles present this is evaluated in two walks with the same
+// result: the closest sky hit (ts, is) on the sky-only tree, then the full
+// tree stops at the first non-sky hit before (ts, is) in (t, index) order;
+// the segment is occluded iff there is one.
 template <int D, class Stack>
 __device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float sx, float sy,
                                           float sz, float ex, float ey, float ez, int32_t skip,
@@ -225,15 +252,18 @@ __device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float s
   const float len = __builtin_sqrtf(len2);
   const float inv = 1.0f / len;
   const float dx = vx * inv, dy = vy * inv, dz = vz * inv;
-  float t;
-  int32_t best, id;
+  sky = false;
+  if (sc.empty) return true;
+  float t = len;
+  int32_t best = -1, id = 0;
   if (sc.has_sky)
-    trace_ray<D, false>(sc, st, sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
-  else
-    trace_ray<D, true>(sc, st, sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
-  const bool hit = best >= 0 && t < len;
-  sky = hit && (id & 0x01000000) != 0;
-  return !(hit && !sky);
+    walk_tree<D, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, sc.M, st, sx,
+                               sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
+  const int32_t sky_best = best;
+  const bool occluded = walk_tree<D, true, true>(sc.nodes, sc.refs, sc.bmin, sc.bmax, sc.M, st, sx,
+                                                 sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
+  sky = !occluded && sky_best >= 0;
+  return !occluded;
 }
 
 }  // namespace vr
