@@ -187,6 +187,18 @@ void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, flo
 // during tile t, so tile t + 1's first loads leave before the tile barrier.
 // The emit tile t + 1 is copied into the second LDS buffer by DMA at the start
 // of tile t.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+// vector load through a global-address-space pointer, so it is issued as a
+// global (not flat) load: flat loads also count against lgkmcnt, and the LDS
+// gather's waits would then wait for them
+template <class V>
+__device__ __forceinline__ V load_global(const char *p) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) V *>(
+      (const __attribute__((address_space(1))) char *)p);
+}
+
 // One G-entry chunk of the tile-blocked copy: 16-bit LDS offsets and the
 // coefficients, loaded with 32-bit offsets from wave-uniform block bases.
 template <int KIND>
@@ -196,8 +208,10 @@ struct BChunk<VR_COEF_F32> {
   uint2 o;
   float4 w;
   __device__ void load(const char *ib, const char *cb, uint32_t q) {
-    o = *reinterpret_cast<const uint2 *>(ib + 2u * q);
-    w = *reinterpret_cast<const float4 *>(cb + 4u * q);
+    const u32x2 a = load_global<u32x2>(ib + 2u * q);
+    const f32x4 b = load_global<f32x4>(cb + 4u * q);
+    o = make_uint2(a.x, a.y);
+    w = make_float4(b.x, b.y, b.z, b.w);
   }
   __device__ uint32_t off(int m) const {
     const uint32_t d = m < 2 ? o.x : o.y;
@@ -209,8 +223,10 @@ template <int KIND>
 struct BChunk {
   uint4 o, h;
   __device__ void load(const char *ib, const char *cb, uint32_t q) {
-    o = *reinterpret_cast<const uint4 *>(ib + 2u * q);
-    h = *reinterpret_cast<const uint4 *>(cb + 2u * q);
+    const u32x4 a = load_global<u32x4>(ib + 2u * q);
+    const u32x4 b = load_global<u32x4>(cb + 2u * q);
+    o = make_uint4(a.x, a.y, a.z, a.w);
+    h = make_uint4(b.x, b.y, b.z, b.w);
   }
   __device__ uint32_t off(int m) const {
     const uint32_t d = (m >> 1) == 0 ? o.x : (m >> 1) == 1 ? o.y : (m >> 1) == 2 ? o.z : o.w;
@@ -230,23 +246,34 @@ __device__ __forceinline__ float dpp_xor(float v) {
   const int x = __float_as_int(v);
   int y;
   if constexpr (OFF == 1) {
-    y = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    y = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
   } else if constexpr (OFF == 2) {
-    y = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    y = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
   } else if constexpr (OFF == 4) {
     y = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0x5, false);  // row_shl:4, banks 0,2
     y = __builtin_amdgcn_update_dpp(y, x, 0x114, 0xF, 0xA, false);  // row_shr:4, banks 1,3
   } else {
     static_assert(OFF == 8, "xor offset");
-    y = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+    y = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, true);  // row_ror:8
   }
   return __int_as_float(y);
 }
 
+// Row rotate by 4 (row_ror:4).  After an xor-8 stage every 16-lane row has
+// period 8, so lane l's rotate partner holds the value of lane l ^ 4.
+__device__ __forceinline__ float dpp_ror4(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x124, 0xF, 0xF, true));
+}
+
+// Butterfly GS/2 .. 1 over groups of GS lanes (each lane ends with the sum).
 template <int GS>
 __device__ __forceinline__ float group_sum(float v) {
-  if constexpr (GS >= 16) v = v + dpp_xor<8>(v);
-  if constexpr (GS >= 8) v = v + dpp_xor<4>(v);
+  if constexpr (GS >= 16) {
+    v = v + dpp_xor<8>(v);
+    v = v + dpp_ror4(v);
+  } else if constexpr (GS >= 8) {
+    v = v + dpp_xor<4>(v);
+  }
   v = v + dpp_xor<2>(v);
   v = v + dpp_xor<1>(v);
   return v;
