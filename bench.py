@@ -7,11 +7,22 @@ MakeTransfers order (rows of the upper triangle that pass the facing cull),
 resident in HBM.  One step = one 4M-segment batch through the visibility
 kernel.
 
-Also reported (same run): the traversal-stack autotune sweep, the plain path
-and, with --random-pairs, an incoherent batch; BounceLight gather bandwidth on the
-precision-form transfer shape (250k rows, lognormal lengths, mean 2000, FP32
-and FP16) after a kernel-shape sweep; and the CPU restatement timed on the
-same segments.
+Also reported (same run):
+  visibility   the traversal-stack autotune sweep, the plain path and, with
+               --random-pairs, an incoherent batch;
+  cpu_baseline the CPU restatement on the same segments (host cores);
+  bounce       BounceLight gather bandwidth on the precision-form transfer
+               shape (250k rows, lognormal lengths, mean 2000; FP32 and FP16)
+               after a kernel-shape sweep;
+  room_bake    the CPU-runnable closed room baked end to end on the GPU and
+               by the CPU restatement, compared bit for bit;
+  batch_bake   64 procedural scenes of uneven size: bounces of all of them in
+               one launch per bounce (replayed hipGraph) vs one by one and vs
+               the plain path;
+  serving      re-bake requests (Poisson arrivals) against 32 resident scenes:
+               p50/p99 latency and throughput of continuous batching vs the
+               plain sequential re-bake, alone-vs-mix equality, FP16 luxel
+               error vs the FP64 recurrence.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -27,12 +38,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
+LEVEL = dict(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
 
 
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
-LEVEL = dict(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
 
 
 def pair_segments(o, nrm, i, j, eps=0.1):
@@ -219,15 +230,15 @@ def run_bounce(cx, args):
         return cx.kernel_ms(b"bounce_gather")
 
     # (kernel, tile cols, group, passes loaded ahead of use, steps loads run ahead)
-    shapes = [(0, 4096, 16, 1, 1), (0, 4096, 16, 1, 3), (0, 4096, 16, 2, 1), (0, 4096, 8, 2, 1),
-              (0, 4096, 8, 2, 3), (0, 4096, 8, 1, 3), (0, 2048, 8, 2, 1), (0, 2048, 16, 2, 1),
-              (1, 4096, 8, 2, 1)]
+    shapes = [(0, 4096, 16, 1, 1), (0, 4096, 16, 1, 3), (0, 4096, 16, 1, 7), (0, 4096, 16, 2, 3),
+              (0, 4096, 8, 2, 1), (0, 4096, 8, 2, 3), (0, 4096, 8, 1, 3), (0, 4096, 8, 1, 7),
+              (0, 2048, 8, 2, 1), (0, 2048, 16, 2, 1), (1, 4096, 8, 2, 1)]
 
     def shape_name(shp):
         return "tiled%dx%d-p%d-d%d" % shp[1:] if shp[0] == 0 else "wave"
 
     # best shapes of earlier sweeps, used when the sweep is skipped
-    default_shape = {"fp32": (0, 4096, 16, 1, 3), "fp16": (0, 4096, 8, 2, 3)}
+    default_shape = {"fp32": (0, 4096, 16, 1, 3), "fp16": (0, 4096, 8, 1, 3)}
 
     out = {}
     for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4)):
@@ -414,6 +425,13 @@ def run_serving(cx, args):
         xf = Transfers.synthetic(seed=7000 + i, n_rows=n, mean_len=2000.0, sigma=1.0,
                                  max_len=min(40000, n - 1), precision=L.VR_COEF_F16)
         resident.append((s, env, xf))
+    # the FP32 coefficients of the smallest scene, for the FP64 error check
+    # (read before any server holds the scenes' tile-blocked copies)
+    small = int(np.argmin([len(r[0].p_area) for r in resident]))
+    xs = resident[small][2]
+    xs.set_precision(L.VR_COEF_F32)
+    csr32 = xs.download()
+    xs.set_precision(L.VR_COEF_F16)
     trace, t = [], 0.0
     for _ in range(args.serve_requests):
         t += float(rng.exponential(1.0 / args.serve_rate))
@@ -458,6 +476,24 @@ def run_serving(cx, args):
         same = same and np.array_equal(tot.view(np.uint32), res_b[j][0].view(np.uint32)) \
             and np.array_equal(en, res_b[j][1])
     sv.close()
+    # luxel error of a served FP16 re-bake against the FP64 recurrence (same
+    # direct light, FP32 coefficients), on the smallest scene
+    import oracle
+    js = next(j for j, (_, sid, _) in enumerate(trace) if sid == small)
+    s_sm = resident[small][0]
+    lights = s_sm.lights.copy()
+    for li, d in trace[js][2]:
+        lights["origin"][li] += d
+    direct = oracle.gather_light(oracle.KdTree(s_sm.verts9), oracle.precompute(s_sm.verts9, s_sm.ids),
+                                 s_sm.p_origin, s_sm.p_normal, lights, threads=16)
+    emit0 = (direct * s_sm.p_refl).astype(np.float32)
+    rp, col, coef = csr32
+    t64, _, _ = oracle.bounce_f64(rp, col, coef, emit0, s_sm.p_refl, s_sm.p_area,
+                                  total=direct.astype(np.float64), max_bounces=8, stop_frac=0.0)
+    got = res_b[js][0].astype(np.float64)
+    nz = t64 != 0
+    lux_err = float(np.max(np.abs(got[nz] - t64[nz]) / np.abs(t64[nz]))) if nz.any() else 0.0
+    lux_err_zero_ok = bool(np.all(got[~nz] == 0))
     del res_b
     svp, plain, _ = serve(PLAIN, 1)
     svp.close()
@@ -469,7 +505,10 @@ def run_serving(cx, args):
             "patches": [int(min(sizes)), int(max(sizes))], "transfers_total": int(sum(nnz)),
             "batched": batched, "plain_sequential": plain,
             "p99_ratio_plain_over_batched": round(plain["p99_ms"] / batched["p99_ms"], 1),
-            "alone_equals_mix": bool(same)}
+            "alone_equals_mix": bool(same),
+            "luxel_rel_err_fp16_vs_fp64": lux_err, "luxel_zero_where_fp64_zero": lux_err_zero_ok,
+            "luxel_err_sample": f"request {js} of the trace on the smallest scene "
+                                f"({len(s_sm.p_area)} patches), 8 bounces"}
 
 
 def measured_traffic(kernel):

@@ -86,7 +86,7 @@ int vr_set_path(int path);
  *                        FP32 coefficients, 8 for FP16/BF16
  *   "bounce_passes"      1 / 2 passes of a row segment loaded ahead of use;
  *                        0 (default) = 1
- *   "bounce_prefetch"    1 / 3 (default) steps the gather's loads run ahead
+ *   "bounce_prefetch"    1 / 3 (default) / 7 steps the gather's loads run ahead
  * The tile width and group fix the summation order; the other bounce knobs
  * change only the schedule.  vr_bounce_light fails with VR_E_UNSUPPORTED for a
  * combination that has no compiled kernel. */

@@ -615,10 +615,13 @@ void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const in
 #define VR_TILED_SHAPES(X) \
   X(4096, 16, 1, 1)        \
   X(4096, 16, 1, 3)        \
+  X(4096, 16, 1, 7)        \
   X(4096, 16, 2, 1)        \
+  X(4096, 16, 2, 3)        \
   X(4096, 8, 2, 1)         \
   X(4096, 8, 2, 3)         \
   X(4096, 8, 1, 3)         \
+  X(4096, 8, 1, 7)         \
   X(2048, 8, 2, 1)         \
   X(2048, 16, 2, 1)
 

@@ -318,7 +318,8 @@ int vr_set_tuning(const char *name, int value) {
       return fail(VR_E_INVALID, "bounce_group must be 0 (by precision), 8 or 16");
     tuning().bounce_group = value;
   } else if (k == "bounce_prefetch") {
-    if (value != 1 && value != 3) return fail(VR_E_INVALID, "bounce_prefetch must be 1 or 3");
+    if (value != 1 && value != 3 && value != 7)
+      return fail(VR_E_INVALID, "bounce_prefetch must be 1, 3 or 7");
     tuning().bounce_prefetch = value;
   } else if (k == "bounce_passes") {
     if (value < 0 || value > 2)

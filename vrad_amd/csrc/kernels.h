@@ -21,7 +21,7 @@ struct Tuning {
   int bounce_tile_cols = 4096; // 2048 / 4096 patches per LDS tile
   int bounce_group = 0;        // 8 / 16 lanes per row segment, 0 = by precision
   int bounce_passes = 0;       // passes of a segment loaded ahead of use, 0 = by precision
-  int bounce_prefetch = 3;     // steps the gather's loads run ahead (1 or 3)
+  int bounce_prefetch = 3;     // steps the gather's loads run ahead (1, 3 or 7)
 };
 Tuning &tuning();
 
