@@ -16,27 +16,53 @@ namespace vr {
 
 #define VR_PI_F 3.14159265358979323846f
 
+// A block of 256 threads tests 256 * kCullIt candidates (row, j) of one row;
+// the waves keep their ballots, the block reserves its survivors' slots with
+// one atomic, and every wave writes its pairs in j order.
+constexpr int kCullIt = 16;
+
 __global__ __launch_bounds__(256) void k_pair_cull(DevPatches p, int64_t np, int64_t i0,
                                                    int64_t bpr, int2 *__restrict__ pairs,
                                                    uint32_t *__restrict__ count, uint32_t cap) {
+  __shared__ uint32_t s_cnt[4];
+  __shared__ uint32_t s_base;
   const int64_t row = i0 + (int64_t)blockIdx.x / bpr;
-  const int64_t j = row + 1 + ((int64_t)blockIdx.x % bpr) * 256 + threadIdx.x;
-  bool keep = false;
-  if (j < np) {
-    const float dx = p.ox[j] - p.ox[row], dy = p.oy[j] - p.oy[row], dz = p.oz[j] - p.oz[row];
-    const float di = (p.nx[row] * dx + p.ny[row] * dy) + p.nz[row] * dz;
-    const float dj = -((p.nx[j] * dx + p.ny[j] * dy) + p.nz[j] * dz);
-    keep = di > 0.0f && dj > 0.0f;
+  const int64_t jb = row + 1 + ((int64_t)blockIdx.x % bpr) * (256 * kCullIt);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float rx = p.ox[row], ry = p.oy[row], rz = p.oz[row];
+  const float qx = p.nx[row], qy = p.ny[row], qz = p.nz[row];
+  uint64_t m[kCullIt];
+  uint32_t wtot = 0;
+#pragma unroll
+  for (int it = 0; it < kCullIt; ++it) {
+    const int64_t j = jb + it * 256 + threadIdx.x;
+    bool keep = false;
+    if (j < np) {
+      const float dx = p.ox[j] - rx, dy = p.oy[j] - ry, dz = p.oz[j] - rz;
+      const float di = (qx * dx + qy * dy) + qz * dz;
+      const float dj = -((p.nx[j] * dx + p.ny[j] * dy) + p.nz[j] * dz);
+      keep = di > 0.0f && dj > 0.0f;
+    }
+    m[it] = __ballot(keep);
+    wtot += (uint32_t)__popcll(m[it]);
   }
-  const uint64_t m = __ballot(keep);
-  if (m == 0) return;
-  const int lane = threadIdx.x & 63;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
-  base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-  if (keep) {
-    const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (slot < cap) pairs[slot] = make_int2((int)row, (int)j);
+  if (lane == 0) s_cnt[wave] = wtot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    s_base = tot ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = s_base;
+  for (int w = 0; w < wave; ++w) base += s_cnt[w];
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int it = 0; it < kCullIt; ++it) {
+    if ((m[it] >> lane) & 1ull) {
+      const uint32_t slot = base + (uint32_t)__popcll(m[it] & below);
+      if (slot < cap) pairs[slot] = make_int2((int)row, (int)(jb + it * 256 + threadIdx.x));
+    }
+    base += (uint32_t)__popcll(m[it]);
   }
 }
 
@@ -45,7 +71,7 @@ void launch_pair_cull(const DevPatches &p, int64_t np, int64_t i0, int64_t i1, i
   if (i1 <= i0) return;
   const int64_t maxlen = np - 1 - i0;
   if (maxlen <= 0) return;
-  const int64_t bpr = (maxlen + 255) / 256;
+  const int64_t bpr = (maxlen + 256 * kCullIt - 1) / (256 * kCullIt);
   hipLaunchKernelGGL(k_pair_cull, dim3((unsigned)((i1 - i0) * bpr)), dim3(256), 0, st, p, np, i0,
                      bpr, pairs, count, cap);
 }
