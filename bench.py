@@ -195,6 +195,13 @@ def run_cpu(v, args):
         cv, _ = oracle.testline(kd, tri, st, en, threads=threads)
         cpu_s += time.perf_counter() - c0
         passes += 1
+    # the traversal's work on this batch (closest-hit rule, untimed pass)
+    wv, visits, tests = oracle.testline_stats(kd, tri, st, en, threads=threads)
+    v["work"] = {"kd_node_visits_per_segment": round(visits / n, 2),
+                 "triangle_tests_per_segment": round(tests / n, 2),
+                 "gpu_giga_node_visits_per_s": round(visits / (v["kern_ms"] * 1e-3) / 1e9, 2),
+                 "counted_by": "oracle kd traversal (closest-hit order)",
+                 "visible_matches": bool(np.array_equal(wv, cv))}
     return {"value": round(n * passes / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
             "sample": f"the {n}-segment batch x {passes} passes, kd-tree TestLine "
@@ -590,7 +597,7 @@ def main():
                          "traffic": measured_traffic("test_line")[0], "kernel": "test_line",
                          "kernel_ms": round(v["kern_ms"], 4),
                          "algorithmic_bytes_per_launch": SEG_BYTES * n},
-            "visibility": v["extra"],
+            "visibility": dict(v["extra"], **({"work": v["work"]} if "work" in v else {})),
         })
         out["config"].update({"triangles": int(info.n_tri), "patches": int(len(v["s"].p_area)),
                               "kd_nodes": int(info.n_nodes), "kd_leaf_refs": int(info.n_leaf_refs),

@@ -67,6 +67,8 @@ def _load():
                                C.c_int]),
         "or_testline": (None, [C.POINTER(_Kd), P, C.c_int64, C.POINTER(_Segs), C.c_int64,
                                C.c_int32, P, P, C.c_int]),
+        "or_testline_stats": (None, [C.POINTER(_Kd), P, C.POINTER(_Segs), C.c_int64, C.c_int32,
+                                     P, C.c_int, P, P]),
         "or_make_transfers": (C.c_int, [C.POINTER(_Kd), P, C.c_int64, C.POINTER(_Patches),
                                         C.c_int64, C.c_float, C.c_int32, C.c_int, C.c_int,
                                         C.POINTER(_Csr)]),
@@ -180,6 +182,19 @@ def testline(kd, tris, s, e, skip=-1, threads=1):
     lib.or_testline(C.byref(kd.kd) if kd is not None else None, _p(tris), len(tris), C.byref(g),
                     n, skip, _p(vis), _p(sky), threads)
     return vis, sky
+
+
+def testline_stats(kd, tris, s, e, skip=-1, threads=1):
+    """testline on a kd-tree plus the work it took: (visible, node visits,
+    triangle tests), the counts summed over all segments."""
+    cols = _cols(s, 3) + _cols(e, 3)
+    g = _Segs(*[_f(c) for c in cols])
+    n = len(cols[0])
+    vis = np.zeros(n, np.uint8)
+    cnt = np.zeros(2, np.int64)
+    lib.or_testline_stats(C.byref(kd.kd), _p(tris), C.byref(g), n, skip, _p(vis), threads,
+                          _p(cnt[:1]), _p(cnt[1:]))
+    return vis, int(cnt[0]), int(cnt[1])
 
 
 def make_transfers(kd, tris, origin, normal, area, eps=0.1, skip=-1, normalize=True, threads=1):

@@ -480,9 +480,12 @@ static float kd_scene_maxabs(const or_kdtree_t *kd) {
 }
 
 #define OR_KD_STACK 64
+/* nv / nt (nullable) count node visits and triangle tests (the work measure
+ * of a traversal; not part of any result) */
 static void kd_trace_one(const or_kdtree_t *kd, float M, const or_tri_t *tris,
                          const float o[3], const float d[3], float tmin, float tmax,
-                         int32_t skip, int anyhit, int32_t *best_out, float *t_out) {
+                         int32_t skip, int anyhit, int32_t *best_out, float *t_out,
+                         int64_t *nv, int64_t *nt) {
   float tbest = tmax;
   int32_t best = -1;
   float om = f_max(fabsf(o[0]), f_max(fabsf(o[1]), fabsf(o[2])));
@@ -512,6 +515,7 @@ static void kd_trace_one(const or_kdtree_t *kd, float M, const or_tri_t *tris,
   for (;;) {
     uint32_t ch = kd->children[node];
     uint32_t type = ch & 3u;
+    if (nv) ++*nv;
     if (type != 3u) {
       float sv = kd->split[node];
       float lo = ((sv - dl) - o[type]) * inv[type];
@@ -540,6 +544,7 @@ static void kd_trace_one(const or_kdtree_t *kd, float M, const or_tri_t *tris,
       for (uint32_t k = start; k < start + cnt; ++k) {
         int32_t t = kd->idx[k];
         if (tris[t].id == skip) continue;
+        if (nt) ++*nt;
         or_hit(tris + t, t, o, d, tmin, &tbest, &best);
       }
       if (anyhit && best >= 0) break;
@@ -575,7 +580,7 @@ static void kd_trace_range(void *vc, int64_t b, int64_t e) {
     float o[3] = {c->r->ox[i], c->r->oy[i], c->r->oz[i]};
     float d[3] = {c->r->dx[i], c->r->dy[i], c->r->dz[i]};
     kd_trace_one(c->kd, c->M, c->tris, o, d, c->r->tmin[i], c->r->tmax[i], c->skip,
-                 0, c->hit + i, c->hit_t + i);
+                 0, c->hit + i, c->hit_t + i, NULL, NULL);
   }
 }
 
@@ -594,7 +599,8 @@ void or_trace_kd(const or_kdtree_t *kd, const or_tri_t *tris, const or_rays_t *r
 /* ------------------------------------------------------------------------ */
 static void testline_one(const or_kdtree_t *kd, float M, const or_tri_t *tris,
                          int64_t ntri, const float s[3], const float e[3],
-                         int32_t skip, uint8_t *vis, uint8_t *sky) {
+                         int32_t skip, uint8_t *vis, uint8_t *sky, int64_t *nv,
+                         int64_t *nt) {
   float dv[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
   float len2 = (dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2];
   float len = sqrtf(len2);
@@ -603,7 +609,7 @@ static void testline_one(const or_kdtree_t *kd, float M, const or_tri_t *tris,
   int32_t best = -1;
   float tb = 0.0f;
   if (kd) {
-    kd_trace_one(kd, M, tris, s, d, 0.0f, len, skip, 0, &best, &tb);
+    kd_trace_one(kd, M, tris, s, d, 0.0f, len, skip, 0, &best, &tb, nv, nt);
   } else {
     float tbest = len;
     for (int64_t k = 0; k < ntri; ++k) {
@@ -626,23 +632,36 @@ typedef struct {
   int32_t skip;
   uint8_t *vis, *sky;
   float M;
+  int64_t *visits, *tests; /* nullable: totals of the work measure */
 } tl_ctx_t;
 
 static void tl_range(void *vc, int64_t b, int64_t e) {
   tl_ctx_t *c = (tl_ctx_t *)vc;
+  int64_t nv = 0, nt = 0;
   for (int64_t i = b; i < e; ++i) {
     float s[3] = {c->s->sx[i], c->s->sy[i], c->s->sz[i]};
     float en[3] = {c->s->ex[i], c->s->ey[i], c->s->ez[i]};
     testline_one(c->kd, c->M, c->tris, c->ntri, s, en, c->skip, c->vis + i,
-                 c->sky ? c->sky + i : NULL);
+                 c->sky ? c->sky + i : NULL, c->visits ? &nv : NULL, c->tests ? &nt : NULL);
   }
+  if (c->visits) __atomic_fetch_add(c->visits, nv, __ATOMIC_RELAXED);
+  if (c->tests) __atomic_fetch_add(c->tests, nt, __ATOMIC_RELAXED);
 }
 
 void or_testline(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
                  const or_segs_t *s, int64_t n, int32_t skip_id, uint8_t *visible,
                  uint8_t *hit_sky, int nthreads) {
   tl_ctx_t c = {kd, tris, ntri, s, skip_id, visible, hit_sky,
-                kd ? kd_scene_maxabs(kd) : 1.0f};
+                kd ? kd_scene_maxabs(kd) : 1.0f, NULL, NULL};
+  or_parallel(n, nthreads, tl_range, &c);
+}
+
+void or_testline_stats(const or_kdtree_t *kd, const or_tri_t *tris, const or_segs_t *s,
+                       int64_t n, int32_t skip_id, uint8_t *visible, int nthreads,
+                       int64_t *visits, int64_t *tests) {
+  *visits = 0;
+  *tests = 0;
+  tl_ctx_t c = {kd, tris, 0, s, skip_id, visible, NULL, kd_scene_maxabs(kd), visits, tests};
   or_parallel(n, nthreads, tl_range, &c);
 }
 
@@ -701,7 +720,7 @@ static void mt_range(void *vc, int64_t b, int64_t e) {
       float en[3] = {p->ox[j] + c->eps * p->nx[j], p->oy[j] + c->eps * p->ny[j],
                      p->oz[j] + c->eps * p->nz[j]};
       uint8_t vis;
-      testline_one(c->kd, c->M, c->tris, c->ntri, s, en, c->skip, &vis, NULL);
+      testline_one(c->kd, c->M, c->tris, c->ntri, s, en, c->skip, &vis, NULL, NULL, NULL);
       if (vis) {
         /* row i written only by this thread; (j,i) bit set under a lock-free
          * atomic since rows j belong to other threads */
@@ -818,7 +837,7 @@ typedef struct {
 
 static int seg_visible(const gl_ctx_t *c, const float s[3], const float e[3]) {
   uint8_t v;
-  testline_one(c->kd, c->M, c->tris, c->ntri, s, e, c->skip, &v, NULL);
+  testline_one(c->kd, c->M, c->tris, c->ntri, s, e, c->skip, &v, NULL, NULL, NULL);
   return v;
 }
 

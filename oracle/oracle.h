@@ -84,6 +84,11 @@ void or_testline(const or_kdtree_t *kd /* NULL = brute force */,
                  const or_tri_t *tris, int64_t ntri, const or_segs_t *s,
                  int64_t n, int32_t skip_id, uint8_t *visible,
                  uint8_t *hit_sky, int nthreads);
+/* the same closest-hit query on a kd-tree, also returning the total node
+ * visits and triangle tests over the n segments (the traversal's work) */
+void or_testline_stats(const or_kdtree_t *kd, const or_tri_t *tris, const or_segs_t *s,
+                       int64_t n, int32_t skip_id, uint8_t *visible, int nthreads,
+                       int64_t *visits, int64_t *tests);
 
 /* ---- MakeTransfers (frozen semantics) ---- */
 typedef struct {

@@ -195,6 +195,40 @@ def test_testline_occluder_and_sky():
         assert list(sky) == [0, 0, 1, 0, 0]
 
 
+def test_testline_work_counts():
+    # one leaf of two triangles (x in [0,6]): a crossing segment visits the leaf
+    # once and tests both; one that misses the scene box visits nothing; a
+    # skipped id is not tested
+    v9 = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0], [5, 0, 0, 6, 0, 0, 5, 1, 0]], np.float32)
+    ids = np.array([7, 9], np.int32)
+    tri = oracle.precompute(v9, ids)
+    kd = oracle.KdTree(v9)
+    cross = (np.array([[0.2, 0.2, -1]], np.float32), np.array([[0.2, 0.2, 1]], np.float32))
+    miss = (np.array([[0, 50, -1]], np.float32), np.array([[0, 50, 1]], np.float32))
+    vis, nv, nt = oracle.testline_stats(kd, tri, *cross)
+    assert (list(vis), nv, nt) == ([0], 1, 2)
+    vis, nv, nt = oracle.testline_stats(kd, tri, *miss)
+    assert (list(vis), nv, nt) == ([1], 0, 0)
+    vis, nv, nt = oracle.testline_stats(kd, tri, *cross, skip=7)
+    assert (list(vis), nv, nt) == ([1], 1, 1)
+
+
+def test_testline_work_counts_real_tree(room_scene):
+    # on a deep tree the counts agree with testline's answers and stay far below
+    # the brute-force work (every triangle for every segment)
+    s = room_scene
+    tri = oracle.precompute(s.verts9, s.ids)
+    kd = oracle.KdTree(s.verts9)
+    rng = np.random.default_rng(3)
+    a = rng.uniform([0, 0, 0], [1024, 1024, 256], (2000, 3)).astype(np.float32)
+    b = rng.uniform([0, 0, 0], [1024, 1024, 256], (2000, 3)).astype(np.float32)
+    vis, nv, nt = oracle.testline_stats(kd, tri, a, b, threads=4)
+    vis2, _ = oracle.testline(kd, tri, a, b, threads=4)
+    assert np.array_equal(vis, vis2)
+    assert 2000 <= nv <= 2000 * len(kd.children)
+    assert 0 < nt < 2000 * len(s.ids) // 10
+
+
 # ---- transfers -------------------------------------------------------------------
 def test_form_factor_parallel_squares():
     n = 16
