@@ -234,7 +234,7 @@ def sky_dirs():
     return d
 
 
-ORDERS = {"wave": 0, "sequential": 1, "tiled": 2}
+ORDERS = {"wave": 0, "sequential": 1, "tiled": 2, "stream": 3}
 
 
 def bounce_f32(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, stop_frac=1e-3,

@@ -1007,6 +1007,11 @@ float or_f16_to_f32(uint16_t h) {
 /*    a segment [sa, sb) lane l of a GS-lane group owns positions p with     */
 /*    ((p - sa) / G) % GS == l, butterfly GS/2..1, and segment sums are      */
 /*    added to the row total in tile order (empty ones add 0).              */
+/*  3 stream: rows are dealt to chunks and 64-row waves (see bg_stream);    */
+/*    per tile a wave's segments (each padded to G) lie back to back and    */
+/*    are read in passes of 64 G-entry chunks; a pass's chunk sums are      */
+/*    combined by a segmented scan of fixed stages, and the value at each   */
+/*    segment's last chunk is added to its row total in tile, pass order.   */
 /* ------------------------------------------------------------------------ */
 double or_energy_reduce(const double *e, int64_t n) {
   if (n <= 0) return 0.0;
@@ -1045,11 +1050,13 @@ typedef struct {
   const int32_t *col;
   const void *coef;
   int kind;
-  int order;         /* 0 wave per row, 1 sequential, 2 LDS-tiled */
+  int order;         /* 0 wave per row, 1 sequential, 2 LDS-tiled, 3 stream */
   int tile_cols, group;
   const float *scale;
   const float *emit; /* 3n, current */
   float *add;        /* 3n */
+  const int32_t *rows_by_len; /* stream: rows longest first (ties by index) */
+  int64_t n_chunks;           /* stream: row chunks */
 } bg_ctx_t;
 
 static inline float coef_at(const bg_ctx_t *c, int64_t k) {
@@ -1118,6 +1125,129 @@ static void bg_range(void *vc, int64_t b, int64_t e) {
   }
 }
 
+/* Stream order, one chunk of rows per call (chunks are independent).
+ * Chunking: n_chunks = max(ceil(n / 1024), min(256, ceil(n / 256))); chunk c
+ * holds rows rows_by_len[c + r n_chunks]; wave w (0..15), slot j (0..63)
+ * takes rank r = 16 j + (j odd ? 15 - w : w).  In tile t a slot's segment
+ * is its row's entries with columns in [t C, (t + 1) C), padded with zero
+ * entries (column t C) to a multiple of G; the wave's segments follow each
+ * other in slot order from stream position 0.  Pass i covers positions
+ * [64 G i, 64 G (i + 1)); lane l's chunk starts at 64 G i + G l and its value
+ * is fmaf over the chunk's G entries from 0.  Lane l's segment start s(l) is
+ * the last lane <= l whose chunk starts a segment (lane 0 counts as one).
+ * Scan stages, each reading the values before the stage:
+ *   d = 1, 2, 4, 8: v[l] += v[l - d] if l - d >= max(s(l), l & ~15);
+ *   v[l] += v[(l & ~15) - 1] if (l & 16) and s(l) < (l & ~15);
+ *   v[l] += v[31]             if l >= 32 and s(l) < 32.
+ * The value at a segment's last lane in the pass is added to the row total. */
+static void bg_stream_range(void *vc, int64_t cb, int64_t ce) {
+  bg_ctx_t *c = (bg_ctx_t *)vc;
+  const int G = c->kind == 0 ? 4 : 8;
+  const int64_t PW = 64 * (int64_t)G;
+  const int64_t C = c->tile_cols;
+  const int64_t nt = (c->n + C - 1) / C;
+  for (int64_t ck = cb; ck < ce; ++ck) {
+    for (int w = 0; w < 16; ++w) {
+      int64_t row[64], cur[64], start[64], len[64], sa[65];
+      float acc[64][3];
+      for (int j = 0; j < 64; ++j) {
+        const int64_t rank = (int64_t)j * 16 + ((j & 1) ? 15 - w : w);
+        const int64_t oi = ck + rank * c->n_chunks;
+        row[j] = oi < c->n ? c->rows_by_len[oi] : -1;
+        cur[j] = 0;
+        acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
+      }
+      for (int64_t t = 0; t < nt; ++t) {
+        int64_t pos = 0;
+        for (int j = 0; j < 64; ++j) {
+          sa[j] = pos;
+          len[j] = 0;
+          start[j] = cur[j];
+          if (row[j] < 0) continue;
+          const int64_t s0 = c->rp[row[j]], L = c->rp[row[j] + 1] - s0;
+          int64_t q = cur[j];
+          while (q < L && (int64_t)c->col[s0 + q] < (t + 1) * C) ++q;
+          len[j] = q - cur[j];
+          cur[j] = q;
+          pos += (len[j] + G - 1) / G * G;
+        }
+        sa[64] = pos;
+        const float *pad = c->emit + 3 * t * C; /* LDS offset 0 of the tile */
+        for (int64_t p0 = 0; p0 < pos; p0 += PW) {
+          float v[64][3];
+          int s[64];
+          int js = 0, last = 0;
+          for (int l = 0; l < 64; ++l) {
+            const int64_t q = p0 + (int64_t)G * l;
+            v[l][0] = v[l][1] = v[l][2] = 0.0f;
+            if (q >= pos) { /* past the wave's last chunk: reaches no sum */
+              s[l] = last;
+              continue;
+            }
+            while (sa[js + 1] <= q) ++js;
+            const int64_t off = q - sa[js];
+            if (off == 0 || l == 0) last = l;
+            s[l] = last;
+            const int64_t s0 = c->rp[row[js]] + start[js];
+            for (int m = 0; m < G; ++m) {
+              const int64_t e = off + m;
+              float wgt;
+              const float *em;
+              if (e < len[js]) {
+                wgt = coef_at(c, s0 + e);
+                em = c->emit + 3 * (int64_t)c->col[s0 + e];
+              } else {
+                wgt = 0.0f;
+                em = pad;
+              }
+              v[l][0] = fmaf(wgt, em[0], v[l][0]);
+              v[l][1] = fmaf(wgt, em[1], v[l][1]);
+              v[l][2] = fmaf(wgt, em[2], v[l][2]);
+            }
+          }
+          float nv[64][3];
+          for (int d = 1; d <= 8; d <<= 1) {
+            for (int l = 0; l < 64; ++l) {
+              const int lo = s[l] > (l & ~15) ? s[l] : (l & ~15);
+              for (int q = 0; q < 3; ++q)
+                nv[l][q] = l - d >= lo ? v[l][q] + v[l - d][q] : v[l][q];
+            }
+            memcpy(v, nv, sizeof v);
+          }
+          for (int l = 0; l < 64; ++l)
+            for (int q = 0; q < 3; ++q)
+              nv[l][q] = ((l & 16) && s[l] < (l & ~15)) ? v[l][q] + v[(l & ~15) - 1][q] : v[l][q];
+          memcpy(v, nv, sizeof v);
+          for (int l = 0; l < 64; ++l)
+            for (int q = 0; q < 3; ++q)
+              nv[l][q] = (l >= 32 && s[l] < 32) ? v[l][q] + v[31][q] : v[l][q];
+          memcpy(v, nv, sizeof v);
+          for (int j = 0; j < 64; ++j) {
+            const int64_t a = sa[j], b = sa[j + 1];
+            if (!(a < b && a < p0 + PW && b > p0)) continue;
+            const int64_t tl = ((b < p0 + PW ? b : p0 + PW) - p0) / G - 1;
+            for (int q = 0; q < 3; ++q) acc[j][q] = acc[j][q] + v[tl][q];
+          }
+        }
+      }
+      for (int j = 0; j < 64; ++j) {
+        if (row[j] < 0) continue;
+        const float sc = c->kind == 0 ? 1.0f : c->scale[row[j]];
+        for (int q = 0; q < 3; ++q)
+          c->add[3 * row[j] + q] = c->kind == 0 ? acc[j][q] : acc[j][q] * sc;
+      }
+    }
+  }
+}
+
+static int cmp_len_desc(const void *a, const void *b, void *rp) {
+  const int64_t *r = (const int64_t *)rp;
+  const int32_t i = *(const int32_t *)a, j = *(const int32_t *)b;
+  const int64_t li = r[i + 1] - r[i], lj = r[j + 1] - r[j];
+  if (li != lj) return li > lj ? -1 : 1;
+  return i < j ? -1 : (i > j); /* ties by index: a stable order */
+}
+
 int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
                   const void *coef, int coef_kind, const float *row_scale,
                   const float *emit0, const float *refl, const float *area,
@@ -1127,11 +1257,22 @@ int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
   float *add = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
   double *e = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
   memcpy(emit, emit0, sizeof(float) * 3 * (size_t)n);
-  bg_ctx_t c = {n, row_ptr, col, coef, coef_kind, order, tile_cols, group, row_scale, emit, add};
+  bg_ctx_t c = {n, row_ptr, col, coef, coef_kind, order, tile_cols, group, row_scale, emit, add,
+                NULL, 0};
+  int32_t *by_len = NULL;
+  if (order == 3 && n > 0) {
+    by_len = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) by_len[i] = (int32_t)i;
+    qsort_r(by_len, (size_t)n, sizeof(int32_t), cmp_len_desc, (void *)row_ptr);
+    const int64_t a = (n + 1023) / 1024, b = (n + 255) / 256;
+    c.n_chunks = a > (b < 256 ? b : 256) ? a : (b < 256 ? b : 256);
+    c.rows_by_len = by_len;
+  }
   double cum = 0.0;
   int k = 0;
   while (k < max_bounces) {
-    or_parallel(n, nthreads, bg_range, &c);
+    if (order == 3) or_parallel(c.n_chunks, nthreads, bg_stream_range, &c);
+    else or_parallel(n, nthreads, bg_range, &c);
     for (int64_t i = 0; i < n; ++i) {
       float a0 = add[3 * i], a1 = add[3 * i + 1], a2 = add[3 * i + 2];
       total[3 * i + 0] = total[3 * i + 0] + a0;
@@ -1151,6 +1292,7 @@ int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
   free(emit);
   free(add);
   free(e);
+  free(by_len);
   return k;
 }
 

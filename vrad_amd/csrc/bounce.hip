@@ -213,10 +213,12 @@ struct BChunk<VR_COEF_F32> {
     o = make_uint2(a.x, a.y);
     w = make_float4(b.x, b.y, b.z, b.w);
   }
+  // bit 0 of an entry's offset marks a segment's first entry (not an address bit)
   __device__ uint32_t off(int m) const {
     const uint32_t d = m < 2 ? o.x : o.y;
-    return (m & 1) ? (d >> 16) : (d & 0xffffu);
+    return (m & 1) ? (d >> 16) : (d & 0xfffeu);
   }
+  __device__ bool head() const { return (o.x & 1u) != 0; }
   __device__ float coef(int m) const { return m == 0 ? w.x : m == 1 ? w.y : m == 2 ? w.z : w.w; }
 };
 template <int KIND>
@@ -230,8 +232,9 @@ struct BChunk {
   }
   __device__ uint32_t off(int m) const {
     const uint32_t d = (m >> 1) == 0 ? o.x : (m >> 1) == 1 ? o.y : (m >> 1) == 2 ? o.z : o.w;
-    return (m & 1) ? (d >> 16) : (d & 0xffffu);
+    return (m & 1) ? (d >> 16) : (d & 0xfffeu);
   }
+  __device__ bool head() const { return (o.x & 1u) != 0; }
   __device__ float coef(int m) const {
     const uint32_t wd = (m >> 1) == 0 ? h.x : (m >> 1) == 1 ? h.y : (m >> 1) == 2 ? h.z : h.w;
     const uint16_t v = (uint16_t)(wd >> (16 * (m & 1)));
@@ -330,6 +333,28 @@ __device__ __forceinline__ bool job_done(const BounceJob &jb) {
   return jb.state->converged || jb.state->k_done >= jb.max_bounces;
 }
 
+// A row's gathered sum (ar, ag, ab; times the row scale for 16-bit
+// coefficients): total += add, emit = add * refl, e_row = area * (r + g + b).
+template <int KIND>
+__device__ __forceinline__ void finish_row(const BounceJob &jb, int64_t row, float ar, float ag,
+                                           float ab, float *__restrict__ emit_out) {
+  if constexpr (KIND != VR_COEF_F32) {
+    const float sc = jb.x.scale[row];
+    ar = ar * sc;
+    ag = ag * sc;
+    ab = ab * sc;
+  }
+  float *__restrict__ total = jb.total;
+  const float *__restrict__ refl = jb.refl;
+  total[3 * row + 0] = total[3 * row + 0] + ar;
+  total[3 * row + 1] = total[3 * row + 1] + ag;
+  total[3 * row + 2] = total[3 * row + 2] + ab;
+  emit_out[3 * row + 0] = ar * refl[3 * row + 0];
+  emit_out[3 * row + 1] = ag * refl[3 * row + 1];
+  emit_out[3 * row + 2] = ab * refl[3 * row + 2];
+  jb.e_row[row] = (double)jb.area[row] * (((double)ar + (double)ag) + (double)ab);
+}
+
 // One workgroup of 16 waves per chunk of a job; emit tiles (r, g, b per
 // patch, as in global memory) are copied into two LDS buffers by DMA, tile
 // t + 1 while tile t is gathered.  The copy reads emit in whole 16-byte
@@ -346,10 +371,6 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(const BounceJob *__restri
   const DevTiles tl = jb.tl;
   const float *__restrict__ emit_in = jb.emit[kb & 1];
   float *__restrict__ emit_out = jb.emit[(kb + 1) & 1];
-  float *__restrict__ total = jb.total;
-  const float *__restrict__ refl = jb.refl;
-  const float *__restrict__ area = jb.area;
-  double *__restrict__ e_row = jb.e_row;
   constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
   constexpr int T = 1024;              // threads
   constexpr int RS = 1028;             // stride of a block's segment offsets
@@ -369,7 +390,7 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(const BounceJob *__restri
   for (int t = (int)threadIdx.x; t < nt; t += T) s_bs[t] = tl.bstart[c * nt + t];
   __syncthreads();
   const int j_own = gl * NG + gi;  // the slot whose row total this lane keeps
-  const int64_t oi = c + ((int64_t)j_own * 16 + wave) * tl.n_chunks;
+  const int64_t oi = c + slot_rank(wave, j_own) * tl.n_chunks;
   const int64_t row = oi < x.n ? (int64_t)x.order[oi] : -1;
   const char *bidx = reinterpret_cast<const char *>(tl.bidx);
   const char *bcoef = reinterpret_cast<const char *>(tl.bcoef);
@@ -473,21 +494,248 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(const BounceJob *__restri
     if (t + 1 < nt) tile(IntC<1>{}, t + 1);
   }
   __builtin_amdgcn_s_waitcnt(0);  // the last (unused) copies and loads
-  if (row >= 0) {
-    if constexpr (KIND != VR_COEF_F32) {
-      const float sc = x.scale[row];
-      ar = ar * sc;
-      ag = ag * sc;
-      ab = ab * sc;
+  if (row >= 0) finish_row<KIND>(jb, row, ar, ag, ab, emit_out);
+}
+
+// ---- stream gather ------------------------------------------------------------
+// Lane source through DPP for the segmented scan (0 where the source lane is
+// outside the row or the row is not in ROWS; never used there).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_src(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
+}
+
+// Per-stage conditions of the segmented scan for a lane whose segment starts
+// at lane s: the stage's source lane exists and lies in the segment.
+struct ScanMasks {
+  bool m1, m2, m4, m8, m15, m31;
+};
+__device__ __forceinline__ ScanMasks scan_masks(int lane, int s) {
+  const int row0 = lane & ~15;
+  const int sr = s > row0 ? s : row0;  // the segment's start within this 16-lane row
+  ScanMasks k;
+  k.m1 = lane - 1 >= sr;
+  k.m2 = lane - 2 >= sr;
+  k.m4 = lane - 4 >= sr;
+  k.m8 = lane - 8 >= sr;
+  k.m15 = (lane & 16) != 0 && s < row0;
+  k.m31 = lane >= 32 && s < 32;
+  return k;
+}
+
+// Segmented inclusive scan over the 64 lanes in a fixed stage order; a stage
+// whose condition holds sets v = v + (source lane's v):
+//   row shifts right by 1, 2, 4, 8 (source lane - d, same 16-lane row);
+//   rows 1 and 3 take lane 15 of rows 0 and 2; rows 2 and 3 take lane 31.
+// A segment's last lane ends with the sum of the segment's lanes.
+__device__ __forceinline__ float seg_scan(float v, const ScanMasks &k) {
+  float w;
+  w = v + dpp_src<0x111, 0xF>(v);  // row_shr:1
+  v = k.m1 ? w : v;
+  w = v + dpp_src<0x112, 0xF>(v);  // row_shr:2
+  v = k.m2 ? w : v;
+  w = v + dpp_src<0x114, 0xF>(v);  // row_shr:4
+  v = k.m4 ? w : v;
+  w = v + dpp_src<0x118, 0xF>(v);  // row_shr:8
+  v = k.m8 ? w : v;
+  w = v + dpp_src<0x142, 0xA>(v);  // row_bcast:15
+  v = k.m15 ? w : v;
+  w = v + dpp_src<0x143, 0xC>(v);  // row_bcast:31
+  v = k.m31 ? w : v;
+  return v;
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// One wave's share of one tile: its 64 owner segments back to back from entry
+// a to e of the block (padded to G), read in passes of 64 G entries.
+struct StreamMeta {
+  int a, e, np;
+  const char *ib, *cb;
+};
+
+// Stream gather: the same workgroups, tile-blocked copy and LDS emit tiles as
+// the LDS-tiled kernel, but each wave reads its 64 segments of a tile as one
+// stream, lane l taking the G-entry chunk at a + 64 G i + G l in pass i, so
+// every lane of a pass carries entries whatever the segment lengths.  A
+// segmented scan sums each segment's chunks inside the pass, and the owner
+// lane of the segment adds the sum at its last chunk's lane (ds_bpermute) to
+// its row total; row totals add these pass sums in tile and pass order.
+// Loads run D passes ahead; the first D passes of tile t + 1 are issued when
+// tile t starts, into the other half of a two-tile register ring.  The next
+// emit tile is loaded into registers when a tile starts and written to the
+// other LDS buffer when it ends (no LDS DMA: the compiler waits for every
+// pending load before an LDS read while a DMA to LDS is in flight).
+template <int KIND, int C, int D>
+__global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restrict__ jobs,
+                                                        int n_jobs) {
+  const int jid = find_job<&BounceJob::chunk_begin>(jobs, n_jobs, blockIdx.x);
+  const BounceJob &jb = jobs[jid];
+  if (job_done(jb)) return;
+  const int kb = jb.state->k_done;
+  const DevXfers x = jb.x;
+  const DevTiles tl = jb.tl;
+  const float *__restrict__ emit_in = jb.emit[kb & 1];
+  float *__restrict__ emit_out = jb.emit[(kb + 1) & 1];
+  constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
+  constexpr int LG = KIND == VR_COEF_F32 ? 2 : 3;
+  constexpr int PW = 64 * G;           // entries per pass of a wave
+  constexpr int T = 1024;              // threads
+  constexpr int RS = 1028;             // stride of a block's segment offsets
+  constexpr int EP = 3 * C / 4;        // 16-byte pieces of a full emit tile
+  constexpr int ER = (EP + T - 1) / T; // copy rounds per tile
+  constexpr int cbytes = KIND == VR_COEF_F32 ? 4 : 2;
+  __shared__ __attribute__((aligned(16))) float s_e[2][4 * T * ER];
+  __shared__ int64_t s_bs[kMaxTilesPerJob];  // this chunk's block starts
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t c = (int64_t)blockIdx.x - jb.chunk_begin;
+  const int nt = tl.nt;
+  for (int t = (int)threadIdx.x; t < nt; t += T) s_bs[t] = tl.bstart[c * nt + t];
+  __syncthreads();
+  const int64_t oi = c + slot_rank(wave, lane) * tl.n_chunks;  // this lane's owner slot
+  const int64_t row = oi < x.n ? (int64_t)x.order[oi] : -1;
+  const char *bidx = reinterpret_cast<const char *>(tl.bidx);
+  const char *bcoef = reinterpret_cast<const char *>(tl.bcoef);
+  const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+
+  // emit tile tn in 16-byte pieces, piece threadIdx.x + T m per thread
+  // (threads past the tile's end load its last piece again); the pieces are
+  // stored at the same places in an LDS buffer
+  f32x4 epc[ER];
+  auto load_emit = [&](int tn) {
+    const int64_t p1 = (int64_t)tn * C;
+    const int np = (int)min((int64_t)C, x.n - p1);
+    const int last = (12 * np + 15) / 16 - 1;
+    const char *src = reinterpret_cast<const char *>(emit_in + 3 * p1);
+#pragma unroll
+    for (int m = 0; m < ER; ++m) {
+      const int k = min((int)threadIdx.x + T * m, last);
+      epc[m] = load_global<f32x4>(src + 16 * k);
     }
-    total[3 * row + 0] = total[3 * row + 0] + ar;
-    total[3 * row + 1] = total[3 * row + 1] + ag;
-    total[3 * row + 2] = total[3 * row + 2] + ab;
-    emit_out[3 * row + 0] = ar * refl[3 * row + 0];
-    emit_out[3 * row + 1] = ag * refl[3 * row + 1];
-    emit_out[3 * row + 2] = ab * refl[3 * row + 2];
-    e_row[row] = (double)area[row] * (((double)ar + (double)ag) + (double)ab);
+  };
+  auto store_emit = [&](int buf) {
+#pragma unroll
+    for (int m = 0; m < ER; ++m)
+      *reinterpret_cast<f32x4 *>(&s_e[buf][4 * (T * m + (int)threadIdx.x)]) = epc[m];
+  };
+  // this lane's owner segment [sa, sb) in block (c, tn); global (not flat)
+  // loads: a flat load would count on the LDS counter too and force full
+  // waits on every pending load
+  auto slot_offsets = [&](int tn, int &sa, int &sb) {
+    const char *bo = reinterpret_cast<const char *>(tl.boff + (c * nt + tn) * (int64_t)RS +
+                                                    64 * wave + lane);
+    sa = load_global<int32_t>(bo);
+    sb = load_global<int32_t>(bo + 4);
+  };
+  auto meta = [&](int tn, int sa, int sb, bool live) {
+    StreamMeta m;
+    m.a = __builtin_amdgcn_readfirstlane(sa);  // lane 0's start: the wave's first entry
+    m.e = __builtin_amdgcn_readlane(sb, 63);   // lane 63's end: past the wave's last
+    m.np = live ? (m.e - m.a + PW - 1) / PW : 0;
+    const int64_t bs = uniform64(s_bs[tn]);
+    m.ib = bidx + 2 * bs;
+    m.cb = bcoef + cbytes * bs;
+    return m;
+  };
+  // loads of pass i (a pass past the tile's last loads one shared chunk)
+  auto issue = [&](int i, const StreamMeta &m, BChunk<KIND> &ch) {
+    const int q = i < m.np ? min(m.a + i * PW + G * lane, m.e - G) : m.a;
+    ch.load(m.ib, m.cb, (uint32_t)q);
+  };
+  auto consume = [&](int i, const StreamMeta &m, const BChunk<KIND> &ch, const char *te, int sa,
+                     int sb) {
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    gather_bchunk<KIND>(ch, te, r, g, b);
+    const uint64_t heads = __ballot(ch.head() || lane == 0);
+    const int s = 63 - __builtin_clzll(heads & upto);  // first lane of this lane's segment
+    const ScanMasks k = scan_masks(lane, s);
+    r = seg_scan(r, k);
+    g = seg_scan(g, k);
+    b = seg_scan(b, k);
+    const int p0 = m.a + i * PW, pe = p0 + PW;
+    const bool mine = sa < sb && sa < pe && sb > p0;  // the owner's segment has chunks here
+    const int src = mine ? ((min(sb, pe) - p0) >> LG) - 1 : lane;
+    const float pr = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(r)));
+    const float pg = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(g)));
+    const float pb = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(b)));
+    if (mine) {
+      ar = ar + pr;
+      ag = ag + pg;
+      ab = ab + pb;
+    }
+  };
+  // one pass: its prefetch, then its gather; the scheduling barriers keep
+  // every pass's loads in program order, so each load is consumed D passes
+  // (2 D loads) after it was issued and its wait leaves the later ones open
+  auto pass = [&](int i, const StreamMeta &m, BChunk<KIND> &ahead, bool has,
+                  const BChunk<KIND> &cur, const char *te, int sa, int sb) {
+    issue(i + D, m, ahead);
+    __builtin_amdgcn_sched_barrier(0);
+    if (has) consume(i, m, cur, te, sa, sb);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+
+  int sa0, sb0, sa1, sb1;
+  slot_offsets(0, sa0, sb0);
+  slot_offsets(min(1, nt - 1), sa1, sb1);
+  StreamMeta m0 = meta(0, sa0, sb0, true);
+  BChunk<KIND> ring[2][D + 1];  // [tile parity][pass % (D + 1)]
+  load_emit(0);
+#pragma unroll
+  for (int k = 0; k < D; ++k) issue(k, m0, ring[0][k]);
+  store_emit(0);
+
+  // Before a tile's barrier each wave needs its LDS writes of the tile's emit
+  // done (and its LDS reads of the previous tile): lgkmcnt 0, vmcnt left open.
+  constexpr int WAIT_LGKM0 = 15 | (3 << 14) | (7 << 4);
+  auto tile = [&](auto parc, int t) {
+    constexpr int p = decltype(parc)::value;
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();  // every wave's copy; tile t - 1 is done
+    __asm__ volatile("" ::: "memory");
+    load_emit(min(t + 1, nt - 1));  // on the last tile a harmless re-load
+    int sa2, sb2;
+    slot_offsets(min(t + 2, nt - 1), sa2, sb2);
+    const StreamMeta m1 = meta(min(t + 1, nt - 1), sa1, sb1, t + 1 < nt);
+#pragma unroll
+    for (int k = 0; k < D; ++k) issue(k, m1, ring[p ^ 1][k]);
+    const char *te = reinterpret_cast<const char *>(s_e[p]);
+    // every pass issues its two loads whether or not it has entries, so the
+    // compiler's waits on the ring are exact on every path
+#pragma unroll
+    for (int k = 0; k <= D; ++k)
+      pass(k, m0, ring[p][(k + D) % (D + 1)], k < m0.np, ring[p][k], te, sa0, sb0);
+    int i = D + 1;
+    for (; m0.np - i >= D + 1; i += D + 1) {
+#pragma unroll
+      for (int k = 0; k <= D; ++k)
+        pass(i + k, m0, ring[p][(k + D) % (D + 1)], true, ring[p][k], te, sa0, sb0);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      if (i + k < m0.np)
+        pass(i + k, m0, ring[p][(k + D) % (D + 1)], true, ring[p][k], te, sa0, sb0);
+    }
+    store_emit(p ^ 1);
+    m0 = m1;
+    sa0 = sa1;
+    sb0 = sb1;
+    sa1 = sa2;
+    sb1 = sb2;
+  };
+  for (int t = 0; t < nt; t += 2) {
+    tile(IntC<0>{}, t);
+    if (t + 1 < nt) tile(IntC<1>{}, t + 1);
   }
+  __builtin_amdgcn_s_waitcnt(0);  // the last (unused) loads
+  if (row >= 0) finish_row<KIND>(jb, row, ar, ag, ab, emit_out);
 }
 
 // Blocked layout, pass 1: one workgroup per block (c, t): padded segment
@@ -505,7 +753,7 @@ __global__ __launch_bounds__(1024) void k_block_offsets(DevXfers x, DevTiles tl,
   int64_t len = 0;
   if (u < R) {
     const int w = u / 64, j = u % 64;
-    const int64_t oi = c + ((int64_t)j * 16 + w) * tl.n_chunks;
+    const int64_t oi = c + slot_rank(w, j) * tl.n_chunks;
     if (oi < x.n) {
       const int64_t row = x.order[oi];
       const int32_t *tp = tl.tp + row * (int64_t)(tl.nt + 1);
@@ -530,8 +778,8 @@ __global__ __launch_bounds__(1024) void k_block_offsets(DevXfers x, DevTiles tl,
 }
 
 // Blocked layout, pass 2: copy each owner's segment with its columns turned
-// into LDS byte offsets 12 (col - t C); pad with (offset 0, coef 0).  One
-// wave per (block, owner slot).
+// into LDS byte offsets 12 (col - t C), bit 0 set on the segment's first
+// entry; pad with (offset 0, coef 0).  One wave per (block, owner slot).
 __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int G,
                                                     int coef_bytes, uint16_t *__restrict__ bidx,
                                                     void *__restrict__ bcoef) {
@@ -544,7 +792,7 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
   const int u = (int)(gw % R), w = u / 64, j = u % 64;
   const int64_t c = blk / tl.nt;
   const int t = (int)(blk % tl.nt);
-  const int64_t oi = c + ((int64_t)j * 16 + w) * tl.n_chunks;
+  const int64_t oi = c + slot_rank(w, j) * tl.n_chunks;
   if (oi >= x.n) return;
   const int64_t row = x.order[oi];
   const int32_t *tp = tl.tp + row * (int64_t)(tl.nt + 1);
@@ -556,7 +804,7 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
   const int p0 = t * tl.cols;
   for (int q = lane; q < plen; q += 64) {
     const bool real = q < len;
-    bidx[dst + q] = real ? (uint16_t)(12 * (x.col[src + q] - p0)) : (uint16_t)0;
+    bidx[dst + q] = real ? (uint16_t)(12 * (x.col[src + q] - p0) | (q == 0 ? 1 : 0)) : (uint16_t)0;
     if (coef_bytes == 4)
       reinterpret_cast<float *>(bcoef)[dst + q] =
           real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;
@@ -625,29 +873,54 @@ void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const in
   X(2048, 8, 2, 1)         \
   X(2048, 16, 2, 1)
 
-bool bounce_tiled_variant(int cols, int group, int passes, int prefetch) {
+// Compiled shapes (C, D) of the stream gather.
+#define VR_STREAM_SHAPES(X) \
+  X(4096, 2)                \
+  X(4096, 3)                \
+  X(4096, 4)                \
+  X(2048, 3)
+
+bool bounce_gather_variant(const GatherShape &sh) {
 #define VR_HAS(CC, GG, PP, DD) \
-  if (cols == CC && group == GG && passes == PP && prefetch == DD) return true;
-  VR_TILED_SHAPES(VR_HAS)
+  if (sh.cols == CC && sh.group == GG && sh.passes == PP && sh.prefetch == DD) return true;
+#define VR_HAS_S(CC, DD) \
+  if (sh.cols == CC && sh.prefetch == DD) return true;
+  if (sh.kernel == 0) {
+    VR_TILED_SHAPES(VR_HAS)
+  } else if (sh.kernel == 2) {
+    VR_STREAM_SHAPES(VR_HAS_S)
+  }
+#undef VR_HAS_S
 #undef VR_HAS
   return false;
 }
 
-void launch_gather_jobs(int kind, int cols, int group, int passes, int prefetch,
-                        const BounceJob *jobs, int n_jobs, int64_t total_chunks, hipStream_t st) {
+void launch_gather_jobs(int kind, const GatherShape &sh, const BounceJob *jobs, int n_jobs,
+                        int64_t total_chunks, hipStream_t st) {
   if (n_jobs <= 0 || total_chunks <= 0) return;
   const dim3 g((unsigned)total_chunks);
-#define VR_GO(K, CC, GG, PP, DD) \
-  hipLaunchKernelGGL((k_bounce_tiled<K, CC, GG, PP, DD>), g, dim3(1024), 0, st, jobs, n_jobs)
-#define VR_SHAPE(CC, GG, PP, DD)                                                  \
-  if (cols == CC && group == GG && passes == PP && prefetch == DD) {              \
-    if (kind == VR_COEF_F32) VR_GO(VR_COEF_F32, CC, GG, PP, DD);                  \
-    else if (kind == VR_COEF_F16) VR_GO(VR_COEF_F16, CC, GG, PP, DD);             \
-    else VR_GO(VR_COEF_BF16, CC, GG, PP, DD);                                     \
-    return;                                                                       \
+#define VR_GO(KER, K, ...) \
+  hipLaunchKernelGGL((KER<K, __VA_ARGS__>), g, dim3(1024), 0, st, jobs, n_jobs)
+#define VR_KINDS(KER, ...)                                   \
+  {                                                          \
+    if (kind == VR_COEF_F32) VR_GO(KER, VR_COEF_F32, __VA_ARGS__);  \
+    else if (kind == VR_COEF_F16) VR_GO(KER, VR_COEF_F16, __VA_ARGS__); \
+    else VR_GO(KER, VR_COEF_BF16, __VA_ARGS__);              \
+    return;                                                  \
   }
-  VR_TILED_SHAPES(VR_SHAPE)
+#define VR_SHAPE(CC, GG, PP, DD)                                                          \
+  if (sh.cols == CC && sh.group == GG && sh.passes == PP && sh.prefetch == DD)            \
+    VR_KINDS(k_bounce_tiled, CC, GG, PP, DD)
+#define VR_SHAPE_S(CC, DD) \
+  if (sh.cols == CC && sh.prefetch == DD) VR_KINDS(k_bounce_stream, CC, DD)
+  if (sh.kernel == 0) {
+    VR_TILED_SHAPES(VR_SHAPE)
+  } else if (sh.kernel == 2) {
+    VR_STREAM_SHAPES(VR_SHAPE_S)
+  }
+#undef VR_SHAPE_S
 #undef VR_SHAPE
+#undef VR_KINDS
 #undef VR_GO
 }
 
@@ -775,10 +1048,9 @@ void launch_energy_jobs(const BounceJob *jobs, int n_jobs, int64_t total_eblocks
                      jobs, n_jobs);
 }
 
-void launch_bounce_jobs(int kind, int cols, int group, int passes, int prefetch,
-                        const BounceJob *jobs, int n_jobs, int64_t total_chunks,
-                        int64_t total_eblocks, hipStream_t st) {
-  launch_gather_jobs(kind, cols, group, passes, prefetch, jobs, n_jobs, total_chunks, st);
+void launch_bounce_jobs(int kind, const GatherShape &sh, const BounceJob *jobs, int n_jobs,
+                        int64_t total_chunks, int64_t total_eblocks, hipStream_t st) {
+  launch_gather_jobs(kind, sh, jobs, n_jobs, total_chunks, st);
   launch_energy_jobs(jobs, n_jobs, total_eblocks, st);
 }
 

@@ -91,8 +91,6 @@ struct Prof {
       return fail(VR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
   } while (0)
 
-int g_cus = 256;
-
 int ensure_init() {
   if (g_init) return VR_OK;
   int n = 0;
@@ -100,10 +98,6 @@ int ensure_init() {
     return fail(VR_E_NODEVICE, "no HIP device available");
   VR_HIP(hipSetDevice(g_device));
   VR_HIP(hipStreamCreate(&g_stream));
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device) == hipSuccess &&
-      cus > 0)
-    g_cus = cus;
   g_init = true;
   return VR_OK;
 }
@@ -307,7 +301,7 @@ int vr_set_tuning(const char *name, int value) {
     if (value < 1 || value > (1 << 20)) return fail(VR_E_INVALID, "bounce_waves out of range");
     tuning().bounce_waves = value;
   } else if (k == "bounce_kernel") {
-    if (value != 0 && value != 1) return fail(VR_E_INVALID, "bounce_kernel must be 0 or 1");
+    if (value < 0 || value > 2) return fail(VR_E_INVALID, "bounce_kernel must be 0, 1 or 2");
     tuning().bounce_kernel = value;
   } else if (k == "bounce_tile_cols") {
     if (value != 2048 && value != 4096)
@@ -318,8 +312,8 @@ int vr_set_tuning(const char *name, int value) {
       return fail(VR_E_INVALID, "bounce_group must be 0 (by precision), 8 or 16");
     tuning().bounce_group = value;
   } else if (k == "bounce_prefetch") {
-    if (value != 1 && value != 3 && value != 7)
-      return fail(VR_E_INVALID, "bounce_prefetch must be 1, 3 or 7");
+    if (value < 1 || value > 7)
+      return fail(VR_E_INVALID, "bounce_prefetch must be 1 to 7");
     tuning().bounce_prefetch = value;
   } else if (k == "bounce_passes") {
     if (value < 0 || value > 2)
@@ -953,18 +947,22 @@ int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n, const 
 namespace vr {
 namespace {
 
-// Gather shape of the LDS-tiled kernel for x at the current tuning.
-struct TiledShape {
-  int cols, group, passes, prefetch;
-};
+// Gather shape of the LDS-tiled or stream kernel for x at the current tuning.
+using TiledShape = GatherShape;
 int tiled_shape(const vr_xfers *x, const Tuning &tu, TiledShape &sh) {
+  sh.kernel = tu.bounce_kernel;
   sh.cols = tu.bounce_tile_cols;
-  // defaults by precision: 16-lane groups for FP32, 8 for FP16/BF16 chunks
-  sh.group = tu.bounce_group ? tu.bounce_group : (x->precision == VR_COEF_F32 ? 16 : 8);
-  sh.passes = tu.bounce_passes ? tu.bounce_passes : 1;
+  if (sh.kernel == 2) {  // the stream gather has no lane groups or passes
+    sh.group = 0;
+    sh.passes = 0;
+  } else {
+    // defaults by precision: 16-lane groups for FP32, 8 for FP16/BF16 chunks
+    sh.group = tu.bounce_group ? tu.bounce_group : (x->precision == VR_COEF_F32 ? 16 : 8);
+    sh.passes = tu.bounce_passes ? tu.bounce_passes : 1;
+  }
   sh.prefetch = tu.bounce_prefetch;
-  if (!bounce_tiled_variant(sh.cols, sh.group, sh.passes, sh.prefetch))
-    return fail(VR_E_UNSUPPORTED, "no compiled LDS-tiled gather for this tuning shape");
+  if (!bounce_gather_variant(sh))
+    return fail(VR_E_UNSUPPORTED, "no compiled LDS gather for this tuning shape");
   return VR_OK;
 }
 
@@ -981,10 +979,7 @@ int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t s
   tl.nt = nt;
   tl.cols = cols;
   tl.group = sh.group;
-  // chunks of at most 1024 rows, and at least one per compute unit while a
-  // chunk keeps >= 256 rows (the chunk count changes the schedule only)
-  tl.n_chunks = n > 0 ? std::max<int64_t>((n + 1023) / 1024, std::min<int64_t>(g_cus, (n + 255) / 256))
-                      : 0;
+  tl.n_chunks = gather_chunks(n);
   if (n == 0) return VR_OK;
   const DevXfers dx = x->dev();
   if (x->tp_cols != cols || !x->tp.p) {
@@ -1076,7 +1071,7 @@ int run_jobs(const TiledShape &sh, int kind, const BounceJob *d_jobs, int n_jobs
   if (use_graph && K > 0) {
     VR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < 8; ++i)
-      launch_bounce_jobs(kind, sh.cols, sh.group, sh.passes, sh.prefetch, d_jobs, n_jobs,
+      launch_bounce_jobs(kind, sh, d_jobs, n_jobs,
                          total_chunks, total_eblocks, st);
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return fail(VR_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -1097,7 +1092,7 @@ int run_jobs(const TiledShape &sh, int kind, const BounceJob *d_jobs, int n_jobs
       for (int i = 0; i < std::min(8, K - done); ++i) {
         {
           Prof p("bounce_gather", st);
-          launch_gather_jobs(kind, sh.cols, sh.group, sh.passes, sh.prefetch, d_jobs, n_jobs,
+          launch_gather_jobs(kind, sh, d_jobs, n_jobs,
                              total_chunks, st);
         }
         {
@@ -1142,8 +1137,8 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl, const fl
   TRY(ensure_init());
   hipStream_t st = g_stream;
   const Tuning tu = tuning();
-  if (g_path == VR_PATH_OPT && tu.bounce_kernel == 0) {
-    // LDS-tiled gather: a batch of one job
+  if (g_path == VR_PATH_OPT && tu.bounce_kernel != 1) {
+    // LDS-tiled or stream gather: a batch of one job
     vr_bounce_job jb{x, emit0, refl, area, total, energy, 0};
     const int rc = vr_bounce_light_batch(&jb, 1, prm, 0);
     if (n_bounces) *n_bounces = jb.n_bounces;
@@ -1207,8 +1202,9 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs, const vr_bounce_p
   if (K < 0) return fail(VR_E_INVALID, "max_bounces < 0");
   if (n_jobs == 0) return VR_OK;
   const Tuning tu = tuning();
-  if (g_path != VR_PATH_OPT || tu.bounce_kernel != 0)
-    return fail(VR_E_UNSUPPORTED, "batched bounces run on the LDS-tiled gather (optimised path)");
+  if (g_path != VR_PATH_OPT || tu.bounce_kernel == 1)
+    return fail(VR_E_UNSUPPORTED,
+                "batched bounces run on the LDS-tiled or stream gather (optimised path)");
   const int kind = jobs[0].xfers ? jobs[0].xfers->precision : -1;
   for (int j = 0; j < n_jobs; ++j) {
     const vr_bounce_job &b = jobs[j];
@@ -1539,9 +1535,8 @@ void serve_loop(vr_server *sv) {
         hipError_t e = hipMemcpyAsync(sv->jobs_dev.p, sv->jobs_host, sizeof(BounceJob) * (size_t)na,
                                       hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
-          launch_bounce_jobs(sv->scenes[0]->xfers->precision, sv->shape.cols, sv->shape.group, sv->shape.passes,
-                             sv->shape.prefetch, sv->jobs_dev.as<BounceJob>(), na, chunks,
-                             eblocks, st);
+          launch_bounce_jobs(sv->scenes[0]->xfers->precision, sv->shape,
+                             sv->jobs_dev.as<BounceJob>(), na, chunks, eblocks, st);
           e = hipGetLastError();
         }
         if (e == hipSuccess)

@@ -17,7 +17,7 @@ struct Tuning {
   int bounce_waves = 8192;     // persistent waves of the wave-per-row gather
   // Bounce gather kernel and its shape.  Unlike the knobs above these fix the
   // FP32 summation order (the CPU restatement follows the same settings).
-  int bounce_kernel = 0;       // 0 = LDS-tiled, 1 = wave per row
+  int bounce_kernel = 0;       // 0 = LDS-tiled, 1 = wave per row, 2 = LDS stream
   int bounce_tile_cols = 4096; // 2048 / 4096 patches per LDS tile
   int bounce_group = 0;        // 8 / 16 lanes per row segment, 0 = by precision
   int bounce_passes = 0;       // passes of a segment loaded ahead of use, 0 = by precision
@@ -74,7 +74,8 @@ struct DevTiles {
   // tile-blocked copy: block (c, t) holds the segments of chunk c's rows for
   // column tile t back to back in owner order u = 64 wave + lane, each padded
   // to a multiple of G with zero coefficients; an entry's column is stored as
-  // its 16-bit byte offset in the LDS emit tile, 12 (col - t C)
+  // its 16-bit byte offset in the LDS emit tile, 12 (col - t C), whose bit 0
+  // (always 0 in an offset) is set on the first entry of each segment
   const uint16_t *bidx;
   const void *bcoef;
   const int64_t *bstart;   // n_chunks * nt block starts
@@ -104,17 +105,37 @@ struct BounceJob {
   double stop_frac;
 };
 constexpr int kMaxTilesPerJob = 1024;  // block starts of a chunk cached in LDS
+// Chunks of a transfer set: at most 1024 rows each, and at least
+// kChunkTarget of them while a chunk keeps >= 256 rows.  A constant (not the
+// device's CU count) because the stream gather's summation order depends on
+// the chunking.
+constexpr int64_t kChunkTarget = 256;
+__host__ __device__ inline int64_t gather_chunks(int64_t n) {
+  if (n <= 0) return 0;
+  const int64_t a = (n + 1023) / 1024, b = (n + 255) / 256;
+  return a > (b < kChunkTarget ? b : kChunkTarget) ? a : (b < kChunkTarget ? b : kChunkTarget);
+}
+// Rank (in the longest-first row order, within chunk c: row order[c + rank *
+// n_chunks]) of owner slot (wave w, lane j): serpentine over the 16 waves, so
+// each wave's 64 rows sum to about the same length.
+__host__ __device__ inline int64_t slot_rank(int w, int j) {
+  return (int64_t)j * 16 + ((j & 1) ? 15 - w : w);
+}
+// Gather kernel and shape: kernel 0 = LDS-tiled (group, passes, prefetch),
+// 2 = stream (prefetch only); cols = tile width.
+struct GatherShape {
+  int kernel, cols, group, passes, prefetch;
+};
 void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, float *total,
                    const float *refl, const float *area, double *e_row, uint32_t *row_counter,
                    const BounceState *state, int path, hipStream_t st);
-bool bounce_tiled_variant(int cols, int group, int passes, int prefetch);
+bool bounce_gather_variant(const GatherShape &sh);
 // one bounce of every job: gather (jobs share precision and shape), then the
 // energy tree and stop test of each job
-void launch_bounce_jobs(int kind, int cols, int group, int passes, int prefetch,
-                        const BounceJob *jobs, int n_jobs, int64_t total_chunks,
-                        int64_t total_eblocks, hipStream_t st);
-void launch_gather_jobs(int kind, int cols, int group, int passes, int prefetch,
-                        const BounceJob *jobs, int n_jobs, int64_t total_chunks, hipStream_t st);
+void launch_bounce_jobs(int kind, const GatherShape &sh, const BounceJob *jobs, int n_jobs,
+                        int64_t total_chunks, int64_t total_eblocks, hipStream_t st);
+void launch_gather_jobs(int kind, const GatherShape &sh, const BounceJob *jobs, int n_jobs,
+                        int64_t total_chunks, hipStream_t st);
 void launch_energy_jobs(const BounceJob *jobs, int n_jobs, int64_t total_eblocks, hipStream_t st);
 // total = direct; emit = direct * refl (3n floats, no aliasing)
 void launch_emit_init(int64_t n, const float *direct, const float *refl, float *emit,
