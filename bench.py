@@ -239,13 +239,16 @@ def run_bounce(cx, args):
     # (kernel, tile cols, group, passes loaded ahead of use, steps loads run ahead)
     shapes = [(0, 4096, 16, 1, 1), (0, 4096, 16, 1, 3), (0, 4096, 16, 1, 7), (0, 4096, 16, 2, 3),
               (0, 4096, 8, 2, 1), (0, 4096, 8, 2, 3), (0, 4096, 8, 1, 3), (0, 4096, 8, 1, 7),
-              (0, 2048, 8, 2, 1), (0, 2048, 16, 2, 1), (1, 4096, 8, 2, 1)]
+              (0, 2048, 8, 2, 1), (0, 2048, 16, 2, 1), (1, 4096, 8, 2, 1),
+              (2, 4096, 0, 0, 2), (2, 4096, 0, 0, 3), (2, 4096, 0, 0, 4), (2, 2048, 0, 0, 3)]
 
     def shape_name(shp):
+        if shp[0] == 2:
+            return "stream%d-d%d" % (shp[1], shp[4])
         return "tiled%dx%d-p%d-d%d" % shp[1:] if shp[0] == 0 else "wave"
 
     # best shapes of earlier sweeps, used when the sweep is skipped
-    default_shape = {"fp32": (0, 4096, 16, 1, 3), "fp16": (0, 4096, 8, 1, 3)}
+    default_shape = {"fp32": (2, 4096, 0, 0, 2), "fp16": (2, 4096, 0, 0, 2)}
 
     out = {}
     for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4)):

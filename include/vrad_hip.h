@@ -80,16 +80,19 @@ int vr_set_path(int path);
  *   "bounce_waves"       persistent waves of the wave-per-row gather
  * These select the FP32 summation order of BounceLight (the CPU
  * restatement takes the same values):
- *   "bounce_kernel"      0 = LDS-tiled gather (default), 1 = wave per row
- *   "bounce_tile_cols"   2048 / 4096 patches per LDS tile
- *   "bounce_group"       8 / 16 lanes per row segment; 0 (default) = 16 for
- *                        FP32 coefficients, 8 for FP16/BF16
- *   "bounce_passes"      1 / 2 passes of a row segment loaded ahead of use;
- *                        0 (default) = 1
- *   "bounce_prefetch"    1 / 3 (default) / 7 steps the gather's loads run ahead
- * The tile width and group fix the summation order; the other bounce knobs
- * change only the schedule.  vr_bounce_light fails with VR_E_UNSUPPORTED for a
- * combination that has no compiled kernel. */
+ *   "bounce_kernel"      2 = LDS stream gather (default), 0 = LDS-tiled
+ *                        gather, 1 = wave per row
+ *   "bounce_tile_cols"   2048 / 4096 (default) patches per LDS tile
+ *   "bounce_group"       tiled gather: 8 / 16 lanes per row segment;
+ *                        0 (default) = 16 for FP32 coefficients, 8 for 16-bit
+ *   "bounce_passes"      tiled gather: 1 / 2 passes of a row segment loaded
+ *                        ahead of use; 0 (default) = 1
+ *   "bounce_prefetch"    passes (stream: 2, 3, 4) or steps (tiled: 1, 3, 7)
+ *                        the gather's loads run ahead; 0 (default) = 2 for
+ *                        the stream gather, 3 for the tiled one
+ * The kernel, tile width and group fix the summation order; the other bounce
+ * knobs change only the schedule.  vr_bounce_light fails with
+ * VR_E_UNSUPPORTED for a combination that has no compiled kernel. */
 int vr_set_tuning(const char *name, int value);
 int vr_get_tuning(const char *name, int *value);
 /* Kernel timing with HIP events on the launching stream.  name = kernel

@@ -238,10 +238,11 @@ ORDERS = {"wave": 0, "sequential": 1, "tiled": 2, "stream": 3}
 
 
 def bounce_f32(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, stop_frac=1e-3,
-               kind=0, row_scale=None, threads=1, order="tiled", tile_cols=4096, group=None):
-    """FP32 bounce in a kernel's summation order (default: the LDS-tiled
-    gather with 4096-patch tiles, 16-lane groups for FP32 coefficients and
-    8-lane groups for FP16/BF16)."""
+               kind=0, row_scale=None, threads=1, order="stream", tile_cols=4096, group=None):
+    """FP32 bounce in a kernel's summation order: "stream" (the default
+    gather, 4096-patch tiles), "tiled" (16-lane groups for FP32 coefficients,
+    8-lane groups for FP16/BF16 unless `group` says otherwise), "wave" or
+    "sequential"."""
     if group is None:
         group = 16 if kind == 0 else 8
     n = len(rp) - 1

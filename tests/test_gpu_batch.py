@@ -53,7 +53,7 @@ def test_batch_equals_single(prec, use_graph):
         assert np.array_equal(en1.view(np.uint64), en2.view(np.uint64))
         ks.append(k1)
     assert len(set(ks[:-1])) > 1 or max(ks) == 21  # jobs stop at different bounces
-    # tie one job to the CPU restatement of the tiled order
+    # tie one job to the CPU restatement of the default (stream) order
     xf, e0, rf, ar = sets[1]
     if prec == _lib.VR_COEF_F32:
         rp, col, coef = xf.download()

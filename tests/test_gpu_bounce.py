@@ -64,7 +64,7 @@ KERNEL_IDS = ["tiled4096x8", "tiled4096x16-p1-d3", "tiled4096x16", "tiled4096x8-
               "stream4096-d2", "stream4096-d3", "stream4096-d4", "stream2048-d3"]
 KNOBS = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes",
          b"bounce_prefetch")
-DEFAULT_KNOBS = (0, 4096, 0, 0, 3)  # the library defaults (group/passes by precision)
+DEFAULT_KNOBS = (2, 4096, 0, 0, 0)  # the library defaults (stream gather, prefetch by kernel)
 
 
 def _set_kernel(*cfg):

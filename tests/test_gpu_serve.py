@@ -45,7 +45,7 @@ def _moved(lights, moves):
     return L
 
 
-def _expected(b, moves, precision, order="tiled"):
+def _expected(b, moves, precision, order="stream"):
     s = b.s
     L = _moved(s.lights, moves)
     direct = oracle.gather_light(b.kd, b.tri, s.p_origin, s.p_normal, L, threads=8)
@@ -55,7 +55,7 @@ def _expected(b, moves, precision, order="tiled"):
                                  max_bounces=BOUNCES, stop_frac=0.0, threads=8, order=order)
     return oracle.bounce_f32(b.rp, b.col, b.raw, emit0, s.p_refl, s.p_area, total=direct,
                              max_bounces=BOUNCES, stop_frac=0.0, kind=precision,
-                             row_scale=b.scale, threads=8)
+                             row_scale=b.scale, threads=8, order=order)
 
 
 def _server(mode, scs, **kw):

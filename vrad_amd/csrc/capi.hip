@@ -312,8 +312,8 @@ int vr_set_tuning(const char *name, int value) {
       return fail(VR_E_INVALID, "bounce_group must be 0 (by precision), 8 or 16");
     tuning().bounce_group = value;
   } else if (k == "bounce_prefetch") {
-    if (value < 1 || value > 7)
-      return fail(VR_E_INVALID, "bounce_prefetch must be 1 to 7");
+    if (value < 0 || value > 7)
+      return fail(VR_E_INVALID, "bounce_prefetch must be 0 (by kernel) to 7");
     tuning().bounce_prefetch = value;
   } else if (k == "bounce_passes") {
     if (value < 0 || value > 2)
@@ -960,7 +960,7 @@ int tiled_shape(const vr_xfers *x, const Tuning &tu, TiledShape &sh) {
     sh.group = tu.bounce_group ? tu.bounce_group : (x->precision == VR_COEF_F32 ? 16 : 8);
     sh.passes = tu.bounce_passes ? tu.bounce_passes : 1;
   }
-  sh.prefetch = tu.bounce_prefetch;
+  sh.prefetch = tu.bounce_prefetch ? tu.bounce_prefetch : (sh.kernel == 2 ? 2 : 3);
   if (!bounce_gather_variant(sh))
     return fail(VR_E_UNSUPPORTED, "no compiled LDS gather for this tuning shape");
   return VR_OK;

@@ -17,11 +17,11 @@ struct Tuning {
   int bounce_waves = 8192;     // persistent waves of the wave-per-row gather
   // Bounce gather kernel and its shape.  Unlike the knobs above these fix the
   // FP32 summation order (the CPU restatement follows the same settings).
-  int bounce_kernel = 0;       // 0 = LDS-tiled, 1 = wave per row, 2 = LDS stream
+  int bounce_kernel = 2;       // 0 = LDS-tiled, 1 = wave per row, 2 = LDS stream
   int bounce_tile_cols = 4096; // 2048 / 4096 patches per LDS tile
   int bounce_group = 0;        // 8 / 16 lanes per row segment, 0 = by precision
   int bounce_passes = 0;       // passes of a segment loaded ahead of use, 0 = by precision
-  int bounce_prefetch = 3;     // steps the gather's loads run ahead (1, 3 or 7)
+  int bounce_prefetch = 0;     // steps/passes the loads run ahead, 0 = by kernel (3 / 2)
 };
 Tuning &tuning();
 
