@@ -87,9 +87,9 @@ int vr_set_path(int path);
  *                        0 (default) = 16 for FP32 coefficients, 8 for 16-bit
  *   "bounce_passes"      tiled gather: 1 / 2 passes of a row segment loaded
  *                        ahead of use; 0 (default) = 1
- *   "bounce_prefetch"    passes (stream: 2, 3, 4) or steps (tiled: 1, 3, 7)
- *                        the gather's loads run ahead; 0 (default) = 2 for
- *                        the stream gather, 3 for the tiled one
+ *   "bounce_prefetch"    passes (stream: 1, 2) or steps (tiled: 1, 3, 7) the
+ *                        gather's loads run ahead; 0 (default) = 1 for the
+ *                        stream gather, 3 for the tiled one
  * The kernel, tile width and group fix the summation order; the other bounce
  * knobs change only the schedule.  vr_bounce_light fails with
  * VR_E_UNSUPPORTED for a combination that has no compiled kernel. */

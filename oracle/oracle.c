@@ -1130,20 +1130,28 @@ static void bg_range(void *vc, int64_t b, int64_t e) {
  * holds rows rows_by_len[c + r n_chunks]; wave w (0..15), slot j (0..63)
  * takes rank r = 16 j + (j odd ? 15 - w : w).  In tile t a slot's segment
  * is its row's entries with columns in [t C, (t + 1) C), padded with zero
- * entries (column t C) to a multiple of G; the wave's segments follow each
- * other in slot order from stream position 0.  Pass i covers positions
- * [64 G i, 64 G (i + 1)); lane l's chunk starts at 64 G i + G l and its value
- * is fmaf over the chunk's G entries from 0.  Lane l's segment start s(l) is
- * the last lane <= l whose chunk starts a segment (lane 0 counts as one).
- * Scan stages, each reading the values before the stage:
- *   d = 1, 2, 4, 8: v[l] += v[l - d] if l - d >= max(s(l), l & ~15);
- *   v[l] += v[(l & ~15) - 1] if (l & 16) and s(l) < (l & ~15);
- *   v[l] += v[31]             if l >= 32 and s(l) < 32.
- * The value at a segment's last lane in the pass is added to the row total. */
+ * entries (column t C) to a multiple of G = 4; the wave's segments follow
+ * each other in slot order from stream position 0, as G-entry chunks.  Pass i
+ * covers chunks [128 i, 128 (i + 1)); lane l takes chunks A = 2 l and
+ * B = 2 l + 1 (a chunk past the wave's last one is absent):
+ *   pA = fmaf over A's entries from 0;
+ *   v  = fmaf over B's entries from (B starts a segment ? 0 : pA), or pA
+ *        without B;
+ *   s(l) = the lane holding the first chunk, in this pass, of the segment of
+ *        the lane's last chunk (0 if that segment began in an earlier pass);
+ *   scan stages, each reading the values before the stage:
+ *     d = 1, 2, 4, 8: v[l] += v[l - d] if l - d >= max(s(l), l & ~15);
+ *     v[l] += v[(l & ~15) - 1] if (l & 16) and s(l) < (l & ~15);
+ *     v[l] += v[31]             if l >= 32 and s(l) < 32;
+ *   carry(l) = v[l - 1] after the scan (0 for lane 0);
+ *   if B starts a segment, A's segment adds (A starts a segment ? pA :
+ *   carry + pA) to its row total; if the lane's last chunk ends its segment
+ *   or l = 63, v[l] is added to that segment's row total.
+ * Row totals take these additions in tile and pass order. */
 static void bg_stream_range(void *vc, int64_t cb, int64_t ce) {
   bg_ctx_t *c = (bg_ctx_t *)vc;
-  const int G = c->kind == 0 ? 4 : 8;
-  const int64_t PW = 64 * (int64_t)G;
+  const int G = 4; /* entries per chunk, every precision */
+  const int64_t PW = 128 * (int64_t)G;
   const int64_t C = c->tile_cols;
   const int64_t nt = (c->n + C - 1) / C;
   for (int64_t ck = cb; ck < ce; ++ck) {
@@ -1174,36 +1182,55 @@ static void bg_stream_range(void *vc, int64_t cb, int64_t ce) {
         sa[64] = pos;
         const float *pad = c->emit + 3 * t * C; /* LDS offset 0 of the tile */
         for (int64_t p0 = 0; p0 < pos; p0 += PW) {
-          float v[64][3];
-          int s[64];
-          int js = 0, last = 0;
+          /* per chunk of the pass: value (from 0 or chained), segment, flags */
+          float pa[64][3], v[64][3];
+          int s[64], seg_a[64], seg_x[64], st_a[64], st_b[64], end_x[64], has_a[64], has_b[64];
+          int js = 0;
           for (int l = 0; l < 64; ++l) {
-            const int64_t q = p0 + (int64_t)G * l;
+            float acc_c[3] = {0.0f, 0.0f, 0.0f};
+            has_a[l] = has_b[l] = 0;
+            s[l] = 0;
             v[l][0] = v[l][1] = v[l][2] = 0.0f;
-            if (q >= pos) { /* past the wave's last chunk: reaches no sum */
-              s[l] = last;
-              continue;
-            }
-            while (sa[js + 1] <= q) ++js;
-            const int64_t off = q - sa[js];
-            if (off == 0 || l == 0) last = l;
-            s[l] = last;
-            const int64_t s0 = c->rp[row[js]] + start[js];
-            for (int m = 0; m < G; ++m) {
-              const int64_t e = off + m;
-              float wgt;
-              const float *em;
-              if (e < len[js]) {
-                wgt = coef_at(c, s0 + e);
-                em = c->emit + 3 * (int64_t)c->col[s0 + e];
+            pa[l][0] = pa[l][1] = pa[l][2] = 0.0f;
+            for (int h = 0; h < 2; ++h) {
+              const int64_t q = p0 + (int64_t)G * (2 * l + h);
+              if (q >= pos) break;
+              while (sa[js + 1] <= q) ++js;
+              const int64_t off = q - sa[js];
+              const int is_start = off == 0, is_last = q + G == sa[js + 1];
+              if (h == 0) {
+                has_a[l] = 1;
+                st_a[l] = is_start;
+                seg_a[l] = js;
               } else {
-                wgt = 0.0f;
-                em = pad;
+                has_b[l] = 1;
+                st_b[l] = is_start;
+                if (is_start) acc_c[0] = acc_c[1] = acc_c[2] = 0.0f;
               }
-              v[l][0] = fmaf(wgt, em[0], v[l][0]);
-              v[l][1] = fmaf(wgt, em[1], v[l][1]);
-              v[l][2] = fmaf(wgt, em[2], v[l][2]);
+              const int64_t s0 = c->rp[row[js]] + start[js];
+              for (int m = 0; m < G; ++m) {
+                const int64_t e = off + m;
+                float wgt;
+                const float *em;
+                if (e < len[js]) {
+                  wgt = coef_at(c, s0 + e);
+                  em = c->emit + 3 * (int64_t)c->col[s0 + e];
+                } else {
+                  wgt = 0.0f;
+                  em = pad;
+                }
+                acc_c[0] = fmaf(wgt, em[0], acc_c[0]);
+                acc_c[1] = fmaf(wgt, em[1], acc_c[1]);
+                acc_c[2] = fmaf(wgt, em[2], acc_c[2]);
+              }
+              if (h == 0) memcpy(pa[l], acc_c, sizeof acc_c);
+              memcpy(v[l], acc_c, sizeof acc_c);
+              /* the lane's last chunk so far: its segment, end flag, s */
+              seg_x[l] = js;
+              end_x[l] = is_last;
+              s[l] = sa[js] >= p0 ? (int)(((sa[js] - p0) / G) >> 1) : 0;
             }
+            if (!has_b[l]) st_b[l] = 0;
           }
           float nv[64][3];
           for (int d = 1; d <= 8; d <<= 1) {
@@ -1222,11 +1249,21 @@ static void bg_stream_range(void *vc, int64_t cb, int64_t ce) {
             for (int q = 0; q < 3; ++q)
               nv[l][q] = (l >= 32 && s[l] < 32) ? v[l][q] + v[31][q] : v[l][q];
           memcpy(v, nv, sizeof v);
-          for (int j = 0; j < 64; ++j) {
-            const int64_t a = sa[j], b = sa[j + 1];
-            if (!(a < b && a < p0 + PW && b > p0)) continue;
-            const int64_t tl = ((b < p0 + PW ? b : p0 + PW) - p0) / G - 1;
-            for (int q = 0; q < 3; ++q) acc[j][q] = acc[j][q] + v[tl][q];
+          for (int l = 0; l < 64; ++l) {
+            if (!has_a[l]) continue;
+            if (st_b[l]) {
+              const int j = seg_a[l];
+              for (int q = 0; q < 3; ++q) {
+                const float carry = l > 0 ? v[l - 1][q] : 0.0f;
+                const float val = st_a[l] ? pa[l][q] : carry + pa[l][q];
+                acc[j][q] = acc[j][q] + val;
+              }
+            }
+          }
+          for (int l = 0; l < 64; ++l) {
+            if (!has_a[l] || !(end_x[l] || l == 63)) continue;
+            const int j = seg_x[l];
+            for (int q = 0; q < 3; ++q) acc[j][q] = acc[j][q] + v[l][q];
           }
         }
       }

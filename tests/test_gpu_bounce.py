@@ -57,11 +57,10 @@ def _room_bake(room_scene):
 # (kernel, tile cols, group, passes loaded ahead of use, steps loads run ahead)
 KERNELS = [(0, 4096, 8, 2, 1), (0, 4096, 16, 1, 3), (0, 4096, 16, 2, 1), (0, 4096, 8, 2, 3),
            (0, 4096, 16, 1, 7), (0, 4096, 8, 1, 7), (0, 2048, 16, 2, 1), (0, 2048, 8, 2, 1),
-           (1, 4096, 8, 2, 1), (2, 4096, 0, 0, 2), (2, 4096, 0, 0, 3), (2, 4096, 0, 0, 4),
-           (2, 2048, 0, 0, 3)]
+           (1, 4096, 8, 2, 1), (2, 4096, 0, 0, 1), (2, 4096, 0, 0, 2), (2, 2048, 0, 0, 1)]
 KERNEL_IDS = ["tiled4096x8", "tiled4096x16-p1-d3", "tiled4096x16", "tiled4096x8-d3",
               "tiled4096x16-p1-d7", "tiled4096x8-p1-d7", "tiled2048x16", "tiled2048x8", "wave",
-              "stream4096-d2", "stream4096-d3", "stream4096-d4", "stream2048-d3"]
+              "stream4096-d1", "stream4096-d2", "stream2048-d1"]
 KNOBS = (b"bounce_kernel", b"bounce_tile_cols", b"bounce_group", b"bounce_passes",
          b"bounce_prefetch")
 DEFAULT_KNOBS = (2, 4096, 0, 0, 0)  # the library defaults (stream gather, prefetch by kernel)

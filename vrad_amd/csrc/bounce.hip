@@ -190,6 +190,7 @@ void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, flo
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x3 __attribute__((ext_vector_type(3)));  // 16-byte aligned
 // vector load through a global-address-space pointer, so it is issued as a
 // global (not flat) load: flat loads also count against lgkmcnt, and the LDS
 // gather's waits would then wait for them
@@ -197,6 +198,13 @@ template <class V>
 __device__ __forceinline__ V load_global(const char *p) {
   return *reinterpret_cast<const __attribute__((address_space(1))) V *>(
       (const __attribute__((address_space(1))) char *)p);
+}
+// the same with the non-temporal hint: data read once, not to displace the
+// emit tiles that every workgroup re-reads from L2
+template <class V>
+__device__ __forceinline__ V load_global_nt(const char *p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const __attribute__((address_space(1))) V *>(
+      (const __attribute__((address_space(1))) char *)p));
 }
 
 // One G-entry chunk of the tile-blocked copy: 16-bit LDS offsets and the
@@ -210,6 +218,12 @@ struct BChunk<VR_COEF_F32> {
   __device__ void load(const char *ib, const char *cb, uint32_t q) {
     const u32x2 a = load_global<u32x2>(ib + 2u * q);
     const f32x4 b = load_global<f32x4>(cb + 4u * q);
+    o = make_uint2(a.x, a.y);
+    w = make_float4(b.x, b.y, b.z, b.w);
+  }
+  __device__ void load_nt(const char *ib, const char *cb, uint32_t q) {
+    const u32x2 a = load_global_nt<u32x2>(ib + 2u * q);
+    const f32x4 b = load_global_nt<f32x4>(cb + 4u * q);
     o = make_uint2(a.x, a.y);
     w = make_float4(b.x, b.y, b.z, b.w);
   }
@@ -230,6 +244,12 @@ struct BChunk {
     o = make_uint4(a.x, a.y, a.z, a.w);
     h = make_uint4(b.x, b.y, b.z, b.w);
   }
+  __device__ void load_nt(const char *ib, const char *cb, uint32_t q) {
+    const u32x4 a = load_global_nt<u32x4>(ib + 2u * q);
+    const u32x4 b = load_global_nt<u32x4>(cb + 2u * q);
+    o = make_uint4(a.x, a.y, a.z, a.w);
+    h = make_uint4(b.x, b.y, b.z, b.w);
+  }
   __device__ uint32_t off(int m) const {
     const uint32_t d = (m >> 1) == 0 ? o.x : (m >> 1) == 1 ? o.y : (m >> 1) == 2 ? o.z : o.w;
     return (m & 1) ? (d >> 16) : (d & 0xfffeu);
@@ -241,6 +261,34 @@ struct BChunk {
     return KIND == VR_COEF_F16 ? f16_to_f32(v) : bf16_to_f32(v);
   }
 };
+
+// A 4-entry chunk of 16-bit coefficients (the stream layout's chunk for every
+// precision): 16-bit offsets and coefficients, 8 bytes each.
+template <int KIND>
+struct QChunk16 {
+  uint2 o, h;
+  __device__ void load_nt(const char *ib, const char *cb, uint32_t q) {
+    const u32x2 a = load_global_nt<u32x2>(ib + 2u * q);
+    const u32x2 b = load_global_nt<u32x2>(cb + 2u * q);
+    o = make_uint2(a.x, a.y);
+    h = make_uint2(b.x, b.y);
+  }
+  __device__ float coef(int m) const {
+    const uint32_t wd = m < 2 ? h.x : h.y;
+    const uint16_t v = (uint16_t)(wd >> (16 * (m & 1)));
+    return KIND == VR_COEF_F16 ? f16_to_f32(v) : bf16_to_f32(v);
+  }
+};
+template <int KIND>
+struct StreamChunkT {
+  using type = QChunk16<KIND>;
+};
+template <>
+struct StreamChunkT<VR_COEF_F32> {
+  using type = BChunk<VR_COEF_F32>;
+};
+template <int KIND>
+using StreamChunk = typename StreamChunkT<KIND>::type;
 
 // Exact xor-lane exchange inside 16-lane rows with DPP (no LDS traffic):
 // quad_perm for 1 and 2, bank-masked row shifts for 4, row rotate for 8.
@@ -301,6 +349,22 @@ __device__ __forceinline__ void gather_bchunk(const BChunk<KIND> &ch, const char
     r = __builtin_fmaf(w, e[0], r);
     g = __builtin_fmaf(w, e[1], g);
     b = __builtin_fmaf(w, e[2], b);
+  }
+}
+
+// The same over 16-byte LDS records (offset = 16 patch): one 12-byte LDS
+// read per entry.
+template <int KIND>
+__device__ __forceinline__ void gather_bchunk16(const BChunk<KIND> &ch, const char *te, float &r,
+                                                float &g, float &b) {
+  constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
+#pragma unroll
+  for (int m = 0; m < G; ++m) {
+    const f32x3 e = *reinterpret_cast<const f32x3 *>(te + ch.off(m));
+    const float w = ch.coef(m);
+    r = __builtin_fmaf(w, e.x, r);
+    g = __builtin_fmaf(w, e.y, g);
+    b = __builtin_fmaf(w, e.z, b);
   }
 }
 
@@ -499,7 +563,7 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(const BounceJob *__restri
 
 // ---- stream gather ------------------------------------------------------------
 // Lane source through DPP for the segmented scan (0 where the source lane is
-// outside the row or the row is not in ROWS; never used there).
+// outside the row; the scan's conditions never use such a lane).
 template <int CTRL, int ROWS>
 __device__ __forceinline__ float dpp_src(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
@@ -538,9 +602,9 @@ __device__ __forceinline__ float seg_scan(float v, const ScanMasks &k) {
   v = k.m4 ? w : v;
   w = v + dpp_src<0x118, 0xF>(v);  // row_shr:8
   v = k.m8 ? w : v;
-  w = v + dpp_src<0x142, 0xA>(v);  // row_bcast:15
+  w = v + dpp_src<0x142, 0xF>(v);  // row_bcast:15 (used in rows 1 and 3)
   v = k.m15 ? w : v;
-  w = v + dpp_src<0x143, 0xC>(v);  // row_bcast:31
+  w = v + dpp_src<0x143, 0xF>(v);  // row_bcast:31 (used in rows 2 and 3)
   v = k.m31 ? w : v;
   return v;
 }
@@ -552,24 +616,97 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
 }
 
 // One wave's share of one tile: its 64 owner segments back to back from entry
-// a to e of the block (padded to G), read in passes of 64 G entries.
+// a to e of the block (padded to G), read in passes of 128 G entries.
 struct StreamMeta {
   int a, e, np;
   const char *ib, *cb;
 };
 
+// Stream-layout metadata of a G-entry chunk, kept in the low 4 bits of its
+// entries' LDS offsets (offsets are multiples of 16):
+//   entry 0: bit 0 start (first chunk of its segment), bit 1 last (last
+//            chunk of its segment), bits 2-3 = s bits 0-1
+//   entry 2: s bits 2-5, s = the lane of the segment's first chunk in this
+//            pass (0 when the segment began in an earlier pass)
+//   entry 1: slot bits 0-3, entry 3: slot bits 4-5 (the segment's owner
+//            slot, 0..63)
+__host__ __device__ inline uint32_t stream_chunk_meta(int m, bool start, bool last, int s,
+                                                      int slot) {
+  if (m == 0) return (start ? 1u : 0u) | (last ? 2u : 0u) | (uint32_t)((s & 3) << 2);
+  if (m == 2) return (uint32_t)((s >> 2) & 15);
+  if (m == 1) return (uint32_t)(slot & 15);
+  if (m == 3) return (uint32_t)((slot >> 4) & 3);
+  return 0u;
+}
+
+struct ChunkMeta {
+  bool start, last;
+  int s, slot;
+};
+template <class CH>
+__device__ __forceinline__ ChunkMeta chunk_meta(const CH &ch) {
+  ChunkMeta r;
+  const uint32_t x = ch.o.x, y = ch.o.y;
+  r.start = (x & 1u) != 0;
+  r.last = (x & 2u) != 0;
+  r.s = (int)(((x >> 2) & 3u) | ((y & 15u) << 2));
+  r.slot = (int)(((x >> 16) & 15u) | (((y >> 16) & 3u) << 4));
+  return r;
+}
+
+// Gather over 16-byte LDS records of a 4-entry stream-layout chunk
+// (metadata bits masked off the offsets), continuing the sums r, g, b.
+template <class CH>
+__device__ __forceinline__ void gather_stream(const CH &ch, const char *te, float &r, float &g,
+                                              float &b) {
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const uint32_t d = m < 2 ? ch.o.x : ch.o.y;
+    const uint32_t off = (m & 1) ? (d >> 16) & 0xfff0u : d & 0xfff0u;
+    const f32x3 e = *reinterpret_cast<const f32x3 *>(te + off);
+    const float w = ch.coef(m);
+    r = __builtin_fmaf(w, e.x, r);
+    g = __builtin_fmaf(w, e.y, g);
+    b = __builtin_fmaf(w, e.z, b);
+  }
+}
+
+// The two chunks a lane reads in one pass (A = stream chunk 2 lane, B =
+// chunk 2 lane + 1).
+template <int KIND>
+struct PassLoad {
+  StreamChunk<KIND> a, b;
+};
+
+// Memory slot (in chunks from the pass start) of stream chunk c of a pass of
+// nc chunks: the A chunks (even c) first, then the B chunks (odd c).
+__host__ __device__ inline int stream_pass_slot(int c, int nc) {
+  return (c & 1) ? ((nc + 1) >> 1) + (c >> 1) : (c >> 1);
+}
+
 // Stream gather: the same workgroups, tile-blocked copy and LDS emit tiles as
 // the LDS-tiled kernel, but each wave reads its 64 segments of a tile as one
-// stream, lane l taking the G-entry chunk at a + 64 G i + G l in pass i, so
-// every lane of a pass carries entries whatever the segment lengths.  A
-// segmented scan sums each segment's chunks inside the pass, and the owner
-// lane of the segment adds the sum at its last chunk's lane (ds_bpermute) to
-// its row total; row totals add these pass sums in tile and pass order.
-// Loads run D passes ahead; the first D passes of tile t + 1 are issued when
-// tile t starts, into the other half of a two-tile register ring.  The next
-// emit tile is loaded into registers when a tile starts and written to the
-// other LDS buffer when it ends (no LDS DMA: the compiler waits for every
-// pending load before an LDS read while a DMA to LDS is in flight).
+// stream of G-entry chunks, in passes of 128 chunks: lane l takes chunks
+// 2 l (A) and 2 l + 1 (B), so every lane of a pass carries entries whatever
+// the segment lengths.  Per pass and lane:
+//   pA = fma chain over A's entries from 0;
+//   v  = fma chain over B's entries from (B starts a segment ? 0 : pA), or pA
+//        when B lies past the wave's last chunk;
+//   S  = segmented inclusive scan of v over the lanes (the lane's last
+//        chunk's segment, from lane s), carry = S of the lane before (0 for
+//        lane 0);
+//   when B starts a segment, A's segment ends here: it adds
+//        (A starts a segment ? pA : carry + pA) to its slot;
+//   when the lane's last chunk ends its segment, or it is lane 63, S is added
+//        to that chunk's slot.
+// Slot sums live in LDS (per wave, 64 x 16 bytes) and are added to in pass
+// order; each slot gets at most one addition per pass.  Loads run D passes
+// ahead; the first D passes of tile t + 1 are issued when tile t starts, into
+// the other half of a two-tile register ring.  Emit tiles sit in LDS as
+// 16-byte records (offsets 16 (col - t C)); the next emit tile is loaded into
+// registers when a tile starts and written to the other LDS buffer when it
+// ends (no LDS DMA: the compiler waits for every pending load before an LDS
+// read while a DMA to LDS is in flight).
 template <int KIND, int C, int D>
 __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restrict__ jobs,
                                                         int n_jobs) {
@@ -581,111 +718,127 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
   const DevTiles tl = jb.tl;
   const float *__restrict__ emit_in = jb.emit[kb & 1];
   float *__restrict__ emit_out = jb.emit[(kb + 1) & 1];
-  constexpr int G = KIND == VR_COEF_F32 ? 4 : 8;
-  constexpr int LG = KIND == VR_COEF_F32 ? 2 : 3;
-  constexpr int PW = 64 * G;           // entries per pass of a wave
+  constexpr int G = 4;                 // entries per chunk, every precision
+  constexpr int PW = 128 * G;          // entries per pass of a wave
   constexpr int T = 1024;              // threads
   constexpr int RS = 1028;             // stride of a block's segment offsets
-  constexpr int EP = 3 * C / 4;        // 16-byte pieces of a full emit tile
-  constexpr int ER = (EP + T - 1) / T; // copy rounds per tile
+  constexpr int EPT = C / T;           // emit records copied per thread and tile
   constexpr int cbytes = KIND == VR_COEF_F32 ? 4 : 2;
-  __shared__ __attribute__((aligned(16))) float s_e[2][4 * T * ER];
+  static_assert(C % T == 0 && 16 * C <= 65536, "16-byte LDS records with 16-bit offsets");
+  __shared__ __attribute__((aligned(16))) float s_e[2][4 * C];  // (r, g, b, -) per patch
+  __shared__ __attribute__((aligned(16))) float s_acc[16 * 64 * 4];  // slot sums per wave
   __shared__ int64_t s_bs[kMaxTilesPerJob];  // this chunk's block starts
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t c = (int64_t)blockIdx.x - jb.chunk_begin;
   const int nt = tl.nt;
   for (int t = (int)threadIdx.x; t < nt; t += T) s_bs[t] = tl.bstart[c * nt + t];
+  f32x3 *const acc = reinterpret_cast<f32x3 *>(s_acc) + 64 * wave;
+  acc[lane] = f32x3{0.0f, 0.0f, 0.0f};
   __syncthreads();
   const int64_t oi = c + slot_rank(wave, lane) * tl.n_chunks;  // this lane's owner slot
   const int64_t row = oi < x.n ? (int64_t)x.order[oi] : -1;
   const char *bidx = reinterpret_cast<const char *>(tl.bidx);
   const char *bcoef = reinterpret_cast<const char *>(tl.bcoef);
-  const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
-  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
 
-  // emit tile tn in 16-byte pieces, piece threadIdx.x + T m per thread
-  // (threads past the tile's end load its last piece again); the pieces are
-  // stored at the same places in an LDS buffer
-  f32x4 epc[ER];
+  // emit tile tn, patch threadIdx.x + T m per thread (threads past the
+  // tile's end load its last patch again), stored as 16-byte LDS records
+  f32x3 epc[EPT];
   auto load_emit = [&](int tn) {
     const int64_t p1 = (int64_t)tn * C;
-    const int np = (int)min((int64_t)C, x.n - p1);
-    const int last = (12 * np + 15) / 16 - 1;
+    const int last = (int)min((int64_t)C, x.n - p1) - 1;
     const char *src = reinterpret_cast<const char *>(emit_in + 3 * p1);
 #pragma unroll
-    for (int m = 0; m < ER; ++m) {
+    for (int m = 0; m < EPT; ++m) {
       const int k = min((int)threadIdx.x + T * m, last);
-      epc[m] = load_global<f32x4>(src + 16 * k);
+      const char *q = src + 12 * k;
+      epc[m] = f32x3{load_global<float>(q), load_global<float>(q + 4), load_global<float>(q + 8)};
     }
   };
   auto store_emit = [&](int buf) {
 #pragma unroll
-    for (int m = 0; m < ER; ++m)
-      *reinterpret_cast<f32x4 *>(&s_e[buf][4 * (T * m + (int)threadIdx.x)]) = epc[m];
+    for (int m = 0; m < EPT; ++m)
+      *reinterpret_cast<f32x3 *>(&s_e[buf][4 * (T * m + (int)threadIdx.x)]) = epc[m];
   };
-  // this lane's owner segment [sa, sb) in block (c, tn); global (not flat)
-  // loads: a flat load would count on the LDS counter too and force full
-  // waits on every pending load
-  auto slot_offsets = [&](int tn, int &sa, int &sb) {
-    const char *bo = reinterpret_cast<const char *>(tl.boff + (c * nt + tn) * (int64_t)RS +
-                                                    64 * wave + lane);
-    sa = load_global<int32_t>(bo);
-    sb = load_global<int32_t>(bo + 4);
+  // the wave's stream [a, e) of block (c, tn): its first and past-the-last
+  // entries, read through the scalar cache (wave-uniform, read-only here), so
+  // they do not count on the vector memory counter
+  auto stream_bounds = [&](int tn, int &sa, int &sb) {
+    const __attribute__((address_space(4))) int32_t *bo =
+        (const __attribute__((address_space(4))) int32_t *)(tl.boff +
+                                                            (c * nt + tn) * (int64_t)RS + 64 * wave);
+    sa = bo[0];
+    sb = bo[64];
   };
   auto meta = [&](int tn, int sa, int sb, bool live) {
     StreamMeta m;
-    m.a = __builtin_amdgcn_readfirstlane(sa);  // lane 0's start: the wave's first entry
-    m.e = __builtin_amdgcn_readlane(sb, 63);   // lane 63's end: past the wave's last
+    m.a = sa;
+    m.e = sb;
     m.np = live ? (m.e - m.a + PW - 1) / PW : 0;
     const int64_t bs = uniform64(s_bs[tn]);
     m.ib = bidx + 2 * bs;
     m.cb = bcoef + cbytes * bs;
     return m;
   };
-  // loads of pass i (a pass past the tile's last loads one shared chunk)
-  auto issue = [&](int i, const StreamMeta &m, BChunk<KIND> &ch) {
-    const int q = i < m.np ? min(m.a + i * PW + G * lane, m.e - G) : m.a;
-    ch.load(m.ib, m.cb, (uint32_t)q);
+  // loads of pass i: in memory a pass holds its A chunks, then its B chunks
+  // (stream_pass_slot), so each load reads whole lines; lanes without a
+  // chunk, and passes past the tile's last, load the pass's first chunk
+  auto issue = [&](int i, const StreamMeta &m, PassLoad<KIND> &L) {
+    const int p0 = m.a + i * PW;
+    const int nc = i < m.np ? min(128, (m.e - p0) / G) : 0;
+    const int ha = (nc + 1) >> 1;
+    const int qa = lane < ha ? p0 + G * lane : m.a;
+    const int qb = lane < nc - ha ? p0 + G * (ha + lane) : m.a;
+    L.a.load_nt(m.ib, m.cb, (uint32_t)qa);
+    L.b.load_nt(m.ib, m.cb, (uint32_t)qb);
   };
-  auto consume = [&](int i, const StreamMeta &m, const BChunk<KIND> &ch, const char *te, int sa,
-                     int sb) {
-    float r = 0.0f, g = 0.0f, b = 0.0f;
-    gather_bchunk<KIND>(ch, te, r, g, b);
-    const uint64_t heads = __ballot(ch.head() || lane == 0);
-    const int s = 63 - __builtin_clzll(heads & upto);  // first lane of this lane's segment
+  auto deliver = [&](int slot, float r, float g, float b) {
+    f32x3 v = acc[slot];
+    v.x = v.x + r;
+    v.y = v.y + g;
+    v.z = v.z + b;
+    acc[slot] = v;
+  };
+  auto consume = [&](int i, const StreamMeta &m, const PassLoad<KIND> &L, const char *te) {
+    const int qa = m.a + i * PW + 2 * G * lane;  // A's stream position
+    const bool va = qa < m.e, vb = qa + G < m.e;
+    const ChunkMeta ma = chunk_meta(L.a), mb = chunk_meta(L.b);
+    const bool startb = vb && mb.start;
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    gather_stream(L.a, te, ar, ag, ab);
+    float r = startb ? 0.0f : ar, g = startb ? 0.0f : ag, b = startb ? 0.0f : ab;
+    if (vb) gather_stream(L.b, te, r, g, b);
+    const int s = vb ? mb.s : ma.s;
     const ScanMasks k = scan_masks(lane, s);
     r = seg_scan(r, k);
     g = seg_scan(g, k);
     b = seg_scan(b, k);
-    const int p0 = m.a + i * PW, pe = p0 + PW;
-    const bool mine = sa < sb && sa < pe && sb > p0;  // the owner's segment has chunks here
-    const int src = mine ? ((min(sb, pe) - p0) >> LG) - 1 : lane;
-    const float pr = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(r)));
-    const float pg = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(g)));
-    const float pb = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(b)));
-    if (mine) {
-      ar = ar + pr;
-      ag = ag + pg;
-      ab = ab + pb;
+    // S of the lane before (whole-wave shift right by one, 0 into lane 0)
+    const float cr = dpp_src<0x138, 0xF>(r), cg = dpp_src<0x138, 0xF>(g),
+                cb = dpp_src<0x138, 0xF>(b);
+    if (va && startb) {
+      const bool own = ma.start;  // A's segment began at A: no carry
+      deliver(ma.slot, own ? ar : cr + ar, own ? ag : cg + ag, own ? ab : cb + ab);
     }
+    const bool endl = vb ? (mb.last || lane == 63) : (ma.last || lane == 63);
+    if (va && endl) deliver(vb ? mb.slot : ma.slot, r, g, b);
   };
   // one pass: its prefetch, then its gather; the scheduling barriers keep
   // every pass's loads in program order, so each load is consumed D passes
-  // (2 D loads) after it was issued and its wait leaves the later ones open
-  auto pass = [&](int i, const StreamMeta &m, BChunk<KIND> &ahead, bool has,
-                  const BChunk<KIND> &cur, const char *te, int sa, int sb) {
+  // after it was issued and its wait leaves the later ones open
+  auto pass = [&](int i, const StreamMeta &m, PassLoad<KIND> &ahead, bool has,
+                  const PassLoad<KIND> &cur, const char *te) {
     issue(i + D, m, ahead);
     __builtin_amdgcn_sched_barrier(0);
-    if (has) consume(i, m, cur, te, sa, sb);
+    if (has) consume(i, m, cur, te);
     __builtin_amdgcn_sched_barrier(0);
   };
 
   int sa0, sb0, sa1, sb1;
-  slot_offsets(0, sa0, sb0);
-  slot_offsets(min(1, nt - 1), sa1, sb1);
+  stream_bounds(0, sa0, sb0);
+  stream_bounds(min(1, nt - 1), sa1, sb1);
   StreamMeta m0 = meta(0, sa0, sb0, true);
-  BChunk<KIND> ring[2][D + 1];  // [tile parity][pass % (D + 1)]
+  PassLoad<KIND> ring[2][D + 1];  // [tile parity][pass % (D + 1)]
   load_emit(0);
 #pragma unroll
   for (int k = 0; k < D; ++k) issue(k, m0, ring[0][k]);
@@ -702,31 +855,28 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
     __asm__ volatile("" ::: "memory");
     load_emit(min(t + 1, nt - 1));  // on the last tile a harmless re-load
     int sa2, sb2;
-    slot_offsets(min(t + 2, nt - 1), sa2, sb2);
+    stream_bounds(min(t + 2, nt - 1), sa2, sb2);
     const StreamMeta m1 = meta(min(t + 1, nt - 1), sa1, sb1, t + 1 < nt);
 #pragma unroll
     for (int k = 0; k < D; ++k) issue(k, m1, ring[p ^ 1][k]);
     const char *te = reinterpret_cast<const char *>(s_e[p]);
-    // every pass issues its two loads whether or not it has entries, so the
+    // every pass issues its loads whether or not it has entries, so the
     // compiler's waits on the ring are exact on every path
 #pragma unroll
     for (int k = 0; k <= D; ++k)
-      pass(k, m0, ring[p][(k + D) % (D + 1)], k < m0.np, ring[p][k], te, sa0, sb0);
+      pass(k, m0, ring[p][(k + D) % (D + 1)], k < m0.np, ring[p][k], te);
     int i = D + 1;
     for (; m0.np - i >= D + 1; i += D + 1) {
 #pragma unroll
       for (int k = 0; k <= D; ++k)
-        pass(i + k, m0, ring[p][(k + D) % (D + 1)], true, ring[p][k], te, sa0, sb0);
+        pass(i + k, m0, ring[p][(k + D) % (D + 1)], true, ring[p][k], te);
     }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      if (i + k < m0.np)
-        pass(i + k, m0, ring[p][(k + D) % (D + 1)], true, ring[p][k], te, sa0, sb0);
+      if (i + k < m0.np) pass(i + k, m0, ring[p][(k + D) % (D + 1)], true, ring[p][k], te);
     }
     store_emit(p ^ 1);
     m0 = m1;
-    sa0 = sa1;
-    sb0 = sb1;
     sa1 = sa2;
     sb1 = sb2;
   };
@@ -735,7 +885,10 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
     if (t + 1 < nt) tile(IntC<1>{}, t + 1);
   }
   __builtin_amdgcn_s_waitcnt(0);  // the last (unused) loads
-  if (row >= 0) finish_row<KIND>(jb, row, ar, ag, ab, emit_out);
+  if (row >= 0) {
+    const f32x3 v = acc[lane];
+    finish_row<KIND>(jb, row, v.x, v.y, v.z, emit_out);
+  }
 }
 
 // Blocked layout, pass 1: one workgroup per block (c, t): padded segment
@@ -778,7 +931,7 @@ __global__ __launch_bounds__(1024) void k_block_offsets(DevXfers x, DevTiles tl,
 }
 
 // Blocked layout, pass 2: copy each owner's segment with its columns turned
-// into LDS byte offsets 12 (col - t C), bit 0 set on the segment's first
+// into LDS byte offsets ostride (col - t C), bit 0 set on the segment's first
 // entry; pad with (offset 0, coef 0).  One wave per (block, owner slot).
 __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int G,
                                                     int coef_bytes, uint16_t *__restrict__ bidx,
@@ -802,14 +955,28 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
   const int64_t dst = tl.bstart[blk] + bo[u];
   const int plen = bo[u + 1] - bo[u];
   const int p0 = t * tl.cols;
+  const int ss = bo[u] - bo[64 * w];  // the segment's start in the wave's stream
+  const int wlen = bo[64 * w + 64] - bo[64 * w];
+  const int nch = plen / G;
   for (int q = lane; q < plen; q += 64) {
     const bool real = q < len;
-    bidx[dst + q] = real ? (uint16_t)(12 * (x.col[src + q] - p0) | (q == 0 ? 1 : 0)) : (uint16_t)0;
+    uint32_t bits;
+    int64_t at = dst + q;
+    if (tl.ostride == 16) {  // stream layout: chunk metadata, A/B chunk order per pass
+      const int pw = 128 * G, pos = ss + q, ps = pos / pw * pw, k = q / G;
+      const int sl = ss >= ps ? ((ss - ps) / G) >> 1 : 0;
+      bits = stream_chunk_meta(q % G, k == 0, k == nch - 1, sl, j);
+      const int nc = min(128, (wlen - ps) / G);
+      at = tl.bstart[blk] + bo[64 * w] + ps + G * stream_pass_slot((pos - ps) / G, nc) + q % G;
+    } else {  // tiled layout: bit 0 marks the segment's first entry
+      bits = q == 0 ? 1u : 0u;
+    }
+    bidx[at] = (uint16_t)((real ? tl.ostride * (x.col[src + q] - p0) : 0) | (int)bits);
     if (coef_bytes == 4)
-      reinterpret_cast<float *>(bcoef)[dst + q] =
+      reinterpret_cast<float *>(bcoef)[at] =
           real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;
     else
-      reinterpret_cast<uint16_t *>(bcoef)[dst + q] =
+      reinterpret_cast<uint16_t *>(bcoef)[at] =
           real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
   }
 }
@@ -875,10 +1042,9 @@ void launch_tile_ptrs(int64_t n, const int64_t *rp, const int32_t *cnt, const in
 
 // Compiled shapes (C, D) of the stream gather.
 #define VR_STREAM_SHAPES(X) \
+  X(4096, 1)                \
   X(4096, 2)                \
-  X(4096, 3)                \
-  X(4096, 4)                \
-  X(2048, 3)
+  X(2048, 1)
 
 bool bounce_gather_variant(const GatherShape &sh) {
 #define VR_HAS(CC, GG, PP, DD) \

@@ -183,7 +183,7 @@ struct vr_xfers {
   int tp_cols = 0;
   // tile-blocked copy of the coefficients for the LDS-tiled gather
   DBuf bidx, bcoef, bstart, boff;
-  int blk_cols = 0, blk_prec = -1;
+  int blk_cols = 0, blk_prec = -1, blk_stride = 0;
   int64_t blk_chunks = 0;
   DevXfers dev() const {
     DevXfers x;
@@ -960,7 +960,7 @@ int tiled_shape(const vr_xfers *x, const Tuning &tu, TiledShape &sh) {
     sh.group = tu.bounce_group ? tu.bounce_group : (x->precision == VR_COEF_F32 ? 16 : 8);
     sh.passes = tu.bounce_passes ? tu.bounce_passes : 1;
   }
-  sh.prefetch = tu.bounce_prefetch ? tu.bounce_prefetch : (sh.kernel == 2 ? 2 : 3);
+  sh.prefetch = tu.bounce_prefetch ? tu.bounce_prefetch : (sh.kernel == 2 ? 1 : 3);
   if (!bounce_gather_variant(sh))
     return fail(VR_E_UNSUPPORTED, "no compiled LDS gather for this tuning shape");
   return VR_OK;
@@ -991,8 +991,11 @@ int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t s
     x->blk_cols = 0;
   }
   tl.tp = x->tp.as<int32_t>();
-  if (x->blk_cols != cols || x->blk_prec != x->precision || x->blk_chunks != tl.n_chunks) {
-    const int G = x->precision == VR_COEF_F32 ? 4 : 8;
+  tl.ostride = sh.kernel == 2 ? 16 : 12;
+  if (x->blk_cols != cols || x->blk_prec != x->precision || x->blk_chunks != tl.n_chunks ||
+      x->blk_stride != tl.ostride) {
+    // entries per chunk: 4 in the stream layout, else 4 (FP32) / 8 (16-bit)
+    const int G = (tl.ostride == 16 || x->precision == VR_COEF_F32) ? 4 : 8;
     const int cb = x->precision == VR_COEF_F32 ? 4 : 2;
     const int64_t nblk = tl.n_chunks * nt;
     DBuf btot;
@@ -1019,6 +1022,7 @@ int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t s
     x->blk_cols = cols;
     x->blk_prec = x->precision;
     x->blk_chunks = tl.n_chunks;
+    x->blk_stride = tl.ostride;
   }
   tl.bidx = x->bidx.as<uint16_t>();
   tl.bcoef = x->bcoef.p;

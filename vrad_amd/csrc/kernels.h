@@ -21,7 +21,7 @@ struct Tuning {
   int bounce_tile_cols = 4096; // 2048 / 4096 patches per LDS tile
   int bounce_group = 0;        // 8 / 16 lanes per row segment, 0 = by precision
   int bounce_passes = 0;       // passes of a segment loaded ahead of use, 0 = by precision
-  int bounce_prefetch = 0;     // steps/passes the loads run ahead, 0 = by kernel (3 / 2)
+  int bounce_prefetch = 0;     // steps/passes the loads run ahead, 0 = by kernel (3 / 1)
 };
 Tuning &tuning();
 
@@ -70,12 +70,13 @@ struct DevXfers {
 struct DevTiles {
   const int32_t *tp;       // n x (nt + 1) row positions of the tile boundaries
   int nt, cols, group;
+  int ostride;             // bytes per patch in the LDS emit tile: 12 (tiled), 16 (stream)
   int64_t n_chunks;        // workgroups of 1024 rows
   // tile-blocked copy: block (c, t) holds the segments of chunk c's rows for
   // column tile t back to back in owner order u = 64 wave + lane, each padded
   // to a multiple of G with zero coefficients; an entry's column is stored as
-  // its 16-bit byte offset in the LDS emit tile, 12 (col - t C), whose bit 0
-  // (always 0 in an offset) is set on the first entry of each segment
+  // its 16-bit byte offset in the LDS emit tile, ostride (col - t C), whose
+  // bit 0 (always 0 in an offset) is set on the first entry of each segment
   const uint16_t *bidx;
   const void *bcoef;
   const int64_t *bstart;   // n_chunks * nt block starts
