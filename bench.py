@@ -8,12 +8,14 @@ resident in HBM.  One step = one 4M-segment batch through the visibility
 kernel.
 
 Also reported (same run):
-  visibility   the traversal-stack autotune sweep, the plain path and, with
-               --random-pairs, an incoherent batch;
+  visibility   the autotuner's sweep (traversal-stack entries in LDS x rays
+               per wavefront), the plain path and, with --random-pairs, an
+               incoherent batch;
   cpu_baseline the CPU restatement on the same segments (host cores);
   bounce       BounceLight gather bandwidth on the precision-form transfer
                shape (250k rows, lognormal lengths, mean 2000; FP32 and FP16)
-               after a kernel-shape sweep;
+               after the autotuner's sweep of the gather shapes (kernel, tile
+               width, prefetch), beside a plain streaming-read probe;
   room_bake    the CPU-runnable closed room baked end to end on the GPU and
                by the CPU restatement, compared bit for bit;
   batch_bake   64 procedural scenes of uneven size: bounces of all of them in
@@ -157,21 +159,19 @@ def run_visibility(cx, args):
             return el, cx.kernel_ms(b"test_line")
 
     batch = Batch(st, en)
-    # autotune the LDS share of the traversal stack on this batch
-    sweep = {}
-    for S in (4, 8, 12, 16, 24):
-        check(lib.vr_set_tuning(b"trace_lds_entries", S))
-        _, km = batch.run(3, 1)
-        sweep[S] = round(n / (km * 1e-3) / 1e6, 1)
-    best_s = max(sweep, key=sweep.get)
-    check(lib.vr_set_tuning(b"trace_lds_entries", best_s))
+    # autotune traversal-stack entries in LDS x rays per wavefront on this batch
+    from vrad_amd import autotune
+    rec = autotune.Autotuner(reps=3).tune_visibility(batch.step, key=f"level:{n}")
+    sweep = {"S%d-R%d" % (r["knobs"]["trace_lds_entries"], r["knobs"]["trace_rays_per_lane"]):
+             round(n / (r["ms"] * 1e-3) / 1e6, 1) for r in rec["table"]}
+    best_s = rec["best"]
     elapsed, kern_ms = batch.run(args.steps, args.warmup)
     vis = batch.vis.cpu().numpy()
     # the plain path (scratch stacks, no XCD remap) and an incoherent batch
     check(lib.vr_set_path(L.VR_PATH_PLAIN))
     _, plain_ms = batch.run(3, 1)
     check(lib.vr_set_path(L.VR_PATH_OPT))
-    extra = {"lds_entries_sweep_Mrays": sweep, "lds_entries": best_s,
+    extra = {"autotune_Mrays": sweep, "tuned": best_s,
              "plain_path_Mrays": round(n / (plain_ms * 1e-3) / 1e6, 1)}
     if args.random_pairs:
         rb = Batch(*random_batch(s, args.segments, seed=20))
@@ -243,10 +243,8 @@ def run_bounce(cx, args):
         return float(np.median(out))
 
     # (kernel, tile cols, group, passes loaded ahead of use, steps loads run ahead)
-    shapes = [(0, 4096, 16, 1, 1), (0, 4096, 16, 1, 3), (0, 4096, 16, 1, 7), (0, 4096, 16, 2, 3),
-              (0, 4096, 8, 2, 1), (0, 4096, 8, 2, 3), (0, 4096, 8, 1, 3), (0, 4096, 8, 1, 7),
-              (0, 2048, 8, 2, 1), (0, 2048, 16, 2, 1), (1, 4096, 8, 2, 1),
-              (2, 4096, 0, 0, 1), (2, 4096, 0, 0, 2), (2, 2048, 0, 0, 1)]
+    from vrad_amd import autotune
+    shapes = list(autotune.BOUNCE_SHAPES)
 
     def shape_name(shp):
         if shp[0] == 2:
@@ -264,13 +262,15 @@ def run_bounce(cx, args):
         # reflectivity 12, emit write 12, total read+write 24, scale
         B = T * (2 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
         B_i32 = T * (4 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
-        sweep = {}
-        for shp in (shapes if args.bounce_sweep else [default_shape[name]]):
-            set_bounce(*shp)
-            key = shape_name(shp)
-            sweep[key] = round(bounce_ms(3), 4)
-        best = min(sweep, key=sweep.get)
-        set_bounce(*{shape_name(x): x for x in shapes}[best])
+        # autotune every compiled gather shape (each is its own restated order)
+        tu = autotune.Autotuner(reps=1)
+        rec = tu.tune_bounce(
+            lambda: lightmap.BounceLight(xf, e0, refl, area, max_bounces=3, stop_frac=0.0),
+            key=f"{name}:{N}:{T}", keep_order=False,
+            shapes=shapes if args.bounce_sweep else [default_shape[name]])
+        sweep = {shape_name(tuple(r["knobs"][k] for k in autotune.BOUNCE_KNOBS)): round(r["ms"], 4)
+                 for r in rec["table"]}
+        best = shape_name(tuple(rec["best"][k] for k in autotune.BOUNCE_KNOBS))
         per = bounce_ms(8, reps=5)
         gbps = B / (per * 1e-3) / 1e9
         out[name] = {"ms_per_bounce": round(per, 4), "GBps": round(gbps, 1),
@@ -279,6 +279,11 @@ def run_bounce(cx, args):
                      "transfers": T, "rows": N, "kernel": best, "sweep_ms": sweep,
                      "traffic": measured_traffic("bounce_gather_" + name)[0]}
     set_bounce(2, 4096, 0, 0, 0)  # the library defaults, which the later sections use
+    # what a plain streaming read reaches on this device (16-byte loads over
+    # 4 GiB, best of 5): the practical ceiling beside the 8 TB/s peak
+    g = L.C.c_double()
+    check(lib.vr_measure_read_bandwidth(4 << 30, 5, L.C.byref(g)))
+    out["read_probe_GBps"] = round(g.value, 1)
     xf.close()
     return out
 

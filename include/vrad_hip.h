@@ -77,6 +77,8 @@ int vr_set_path(int path);
  * Results do not depend on:
  *   "trace_lds_entries"  traversal-stack entries per lane held in LDS
  *                        (4, 8, 12, 16, 24; the rest spill to scratch)
+ *   "trace_rays_per_lane" rays each lane of the visibility kernels takes
+ *                        (1, 2, 4): rays per wavefront = 64 x this
  *   "bounce_waves"       persistent waves of the wave-per-row gather
  * These select the FP32 summation order of BounceLight (the CPU
  * restatement takes the same values):
@@ -101,6 +103,10 @@ int vr_get_tuning(const char *name, int *value);
 int vr_profile_enable(int on);
 int vr_profile_reset(void);
 int vr_profile_get(const char *name, double *total_ms, int64_t *launches);
+/* Read bandwidth (GB/s) a plain streaming kernel reaches on this device: the
+ * best of `reps` timed passes of 16-byte loads over `bytes` of device memory
+ * (no reference counterpart; the practical ceiling beside the HBM peak). */
+int vr_measure_read_bandwidth(int64_t bytes, int32_t reps, double *gbps);
 
 /* ---- scene: raytracer.Environment ------------------------------------ */
 typedef struct vr_scene vr_scene;

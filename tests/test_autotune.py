@@ -1,0 +1,60 @@
+"""The autotuner's selection and cache (no GPU: injected knob setter and
+timings)."""
+import json
+
+from vrad_amd import autotune
+
+
+class Knobs:
+    def __init__(self):
+        self.v = {}
+
+    def set(self, k, val):
+        self.v[k] = val
+
+
+def test_visibility_sweep_picks_fastest_and_applies_it(tmp_path):
+    kn = Knobs()
+    tu = autotune.Autotuner(device="dev", cache_path=str(tmp_path / "t.json"), setter=kn.set)
+    seen = []
+
+    def measure(k):  # fastest at 12 entries, 2 rays per lane
+        seen.append(dict(k))
+        return abs(k["trace_lds_entries"] - 12) + abs(k["trace_rays_per_lane"] - 2) + 1.0
+
+    rec = tu.tune_visibility(run=None, key="level:4M", measure=measure)
+    assert len(seen) == len(autotune.VIS_STACKS) * len(autotune.VIS_RAYS)
+    assert rec["best"] == {"trace_lds_entries": 12, "trace_rays_per_lane": 2}
+    assert rec["best_ms"] == 1.0 and not rec["cached"]
+    assert kn.v == rec["best"]                       # the winner is left applied
+    saved = json.load(open(tmp_path / "t.json"))
+    assert saved["dev|visibility|level:4M"]["best"] == rec["best"]
+    # a second tuner with the same cache applies the choice without timing
+    kn2 = Knobs()
+    tu2 = autotune.Autotuner(device="dev", cache_path=str(tmp_path / "t.json"), setter=kn2.set)
+    rec2 = tu2.tune_visibility(run=None, key="level:4M",
+                               measure=lambda k: (_ for _ in ()).throw(AssertionError("timed")))
+    assert rec2["cached"] and kn2.v == rec["best"]
+    # another device or workload key is tuned afresh
+    rec3 = autotune.Autotuner(device="other", setter=kn2.set).tune_visibility(
+        run=None, key="level:4M", measure=measure)
+    assert not rec3["cached"]
+
+
+def test_bounce_sweep_keeps_the_summation_order():
+    kn = Knobs()
+    tu = autotune.Autotuner(setter=kn.set)
+    tried = []
+
+    def measure(k):
+        tried.append((k["bounce_kernel"], k["bounce_tile_cols"], k["bounce_prefetch"]))
+        return 1.0 / k["bounce_prefetch"]
+
+    rec = tu.tune_bounce(run=None, key="250k", measure=measure, current=(2, 4096))
+    # only shapes with the current kernel and tile width (same FP32 order)
+    assert tried and all(t[:2] == (2, 4096) for t in tried)
+    assert rec["best"]["bounce_prefetch"] == max(t[2] for t in tried)
+    tried.clear()
+    rec = tu.tune_bounce(run=None, key="250k", keep_order=False, measure=measure)
+    assert len(tried) == len(autotune.BOUNCE_SHAPES)
+    assert {t[0] for t in tried} == {0, 1, 2}

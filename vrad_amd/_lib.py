@@ -111,6 +111,7 @@ _SIGS = {
     "vr_profile_enable": (C.c_int, [C.c_int]),
     "vr_profile_reset": (C.c_int, []),
     "vr_profile_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "vr_measure_read_bandwidth": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "vr_scene_create": (C.c_int, [C.POINTER(_P), _P, _P, _P, C.c_int64]),
     "vr_scene_build": (C.c_int, [_P, C.c_int32]),
     "vr_scene_build_host": (C.c_int, [_P, C.c_int32]),

@@ -1347,4 +1347,19 @@ void launch_decode16(int64_t total, const uint16_t *coef16, float *out, int kind
                      coef16, out, kind);
 }
 
+// ---- streaming-read probe: the read bandwidth a plain kernel reaches ----
+__global__ __launch_bounds__(256) void k_read_probe(const f32x4 *__restrict__ in, int64_t n4,
+                                                    float *__restrict__ out) {
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t nth = (int64_t)gridDim.x * 256;
+  f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t i = tid; i < n4; i += nth) a += in[i];
+  out[tid] = (a.x + a.y) + (a.z + a.w);
+}
+
+void launch_read_probe(const void *in, int64_t bytes, float *out, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_read_probe, dim3((unsigned)blocks), dim3(256), 0, st,
+                     reinterpret_cast<const f32x4 *>(in), bytes / 16, out);
+}
+
 }  // namespace vr

@@ -297,6 +297,10 @@ int vr_set_tuning(const char *name, int value) {
     if (value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
       return fail(VR_E_INVALID, "trace_lds_entries must be 4, 8, 12, 16 or 24");
     tuning().trace_lds_entries = value;
+  } else if (k == "trace_rays_per_lane") {
+    if (value != 1 && value != 2 && value != 4)
+      return fail(VR_E_INVALID, "trace_rays_per_lane must be 1, 2 or 4");
+    tuning().trace_rays_per_lane = value;
   } else if (k == "bounce_waves") {
     if (value < 1 || value > (1 << 20)) return fail(VR_E_INVALID, "bounce_waves out of range");
     tuning().bounce_waves = value;
@@ -331,6 +335,7 @@ int vr_get_tuning(const char *name, int *value) {
   if (!name || !value) return fail(VR_E_INVALID, "NULL argument");
   const std::string k(name);
   if (k == "trace_lds_entries") *value = tuning().trace_lds_entries;
+  else if (k == "trace_rays_per_lane") *value = tuning().trace_rays_per_lane;
   else if (k == "bounce_waves") *value = tuning().bounce_waves;
   else if (k == "bounce_kernel") *value = tuning().bounce_kernel;
   else if (k == "bounce_tile_cols") *value = tuning().bounce_tile_cols;
@@ -362,6 +367,39 @@ int vr_profile_get(const char *name, double *total_ms, int64_t *launches) {
   if (total_ms) *total_ms = it == g_prof_tot.end() ? 0.0 : it->second.first;
   if (launches) *launches = it == g_prof_tot.end() ? 0 : it->second.second;
   return VR_OK;
+}
+
+int vr_measure_read_bandwidth(int64_t bytes, int32_t reps, double *gbps) {
+  LOCK;
+  if (!gbps || bytes < (1 << 20) || reps < 1) return fail(VR_E_INVALID, "bad read-probe arguments");
+  TRY(ensure_init());
+  bytes &= ~(int64_t)15;
+  const int blocks = 2048;
+  DBuf in, out;
+  TRY(in.alloc((size_t)bytes));
+  TRY(out.alloc(4 * (size_t)blocks * 256));
+  hipStream_t st = g_stream;
+  VR_HIP(hipMemsetAsync(in.p, 0, (size_t)bytes, st));
+  hipEvent_t e0, e1;
+  VR_HIP(hipEventCreate(&e0));
+  VR_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int rc = VR_OK;
+  for (int r = 0; r <= reps && rc == VR_OK; ++r) {  // the first launch is untimed
+    hipError_t e = hipEventRecord(e0, st);
+    launch_read_probe(in.p, bytes, out.as<float>(), blocks, st);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = fail(VR_E_HIP, std::string("read probe: ") + hipGetErrorString(e));
+    if (r > 0 && ms < best) best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == VR_OK) *gbps = (double)bytes / (best * 1e-3) / 1e9;
+  return rc;
 }
 
 // ---- scene ----------------------------------------------------------------

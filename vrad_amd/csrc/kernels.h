@@ -14,6 +14,8 @@ constexpr int kStackDepth = 24;  // > max kd depth (21 + 1 leaf level)
 // Launch-shape knobs (vr_set_tuning); every value gives identical results.
 struct Tuning {
   int trace_lds_entries = 8;   // traversal-stack entries per lane kept in LDS: 4/8/12/16/24
+  int trace_rays_per_lane = 1; // rays per lane of the visibility kernels (1, 2, 4): a wave
+                               // covers 64 x this many rays
   int bounce_waves = 8192;     // persistent waves of the wave-per-row gather
   // Bounce gather kernel and its shape.  Unlike the knobs above these fix the
   // FP32 summation order (the CPU restatement follows the same settings).
@@ -157,6 +159,9 @@ void launch_encode16(int64_t n, const int64_t *rp, const float *coef32, uint16_t
                      float *scale, int kind, hipStream_t st);
 void launch_decode16(int64_t total, const uint16_t *coef16, float *out, int kind,
                      hipStream_t st);
+// grid-stride 16-byte loads over `bytes` (a multiple of 16); one float per
+// thread into out (blocks x 256)
+void launch_read_probe(const void *in, int64_t bytes, float *out, int blocks, hipStream_t st);
 
 // ---- direct light (light.hip) ----
 struct DevSamples {

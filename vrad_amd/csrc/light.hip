@@ -105,10 +105,8 @@ __global__ __launch_bounds__(64) void k_gather_lds(DevScene sc, DevSamples s, in
   __shared__ float2 s_t[S * 64];
   const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (i >= n) return;
-  HybridStack<S, D> st;
-  st.node = s_node;
-  st.t = s_t;
-  st.lane = (int)threadIdx.x;
+  StackSpill<S, D> spill;
+  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
   float r, g, b;
   gather_one<D>(sc, st, s.px[i], s.py[i], s.pz[i], s.nx[i], s.ny[i], s.nz[i], lights, n_lights,
                 dirs, eps, skip, r, g, b);

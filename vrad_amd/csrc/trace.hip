@@ -9,30 +9,33 @@
 
 namespace vr {
 
-template <int S, int D>
+// R rays per lane: a wave takes 64 R consecutive rays in R rounds of 64
+// (round q: rays base + 64 q + lane), reusing its LDS stack.
+template <int S, int D, int R>
 __global__ __launch_bounds__(64) void k_trace_closest_lds(DevScene sc, DevRays r, int64_t n,
                                                           int32_t skip, int32_t *__restrict__ hit,
                                                           float *__restrict__ hit_t,
                                                           float *__restrict__ hit_n) {
   __shared__ uint32_t s_node[S * 64];
   __shared__ float2 s_t[S * 64];
-  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  if (i >= n) return;
-  HybridStack<S, D> st;
-  st.node = s_node;
-  st.t = s_t;
-  st.lane = (int)threadIdx.x;
-  float t;
-  int32_t best, id;
-  trace_ray<D, false>(sc, st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.tmin[i],
-                      r.tmax[i], skip, t, best, id);
-  hit[i] = best;
-  hit_t[i] = t;
-  if (hit_n) {
-    float4 pl = best >= 0 ? sc.planes[best] : make_float4(0.f, 0.f, 0.f, 0.f);
-    hit_n[3 * i + 0] = pl.x;
-    hit_n[3 * i + 1] = pl.y;
-    hit_n[3 * i + 2] = pl.z;
+  const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R + threadIdx.x;
+  StackSpill<S, D> spill;
+  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
+  for (int q = 0; q < R; ++q) {
+    const int64_t i = base + 64 * q;
+    if (i >= n) return;
+    float t;
+    int32_t best, id;
+    trace_ray<D, false>(sc, st, r.ox[i], r.oy[i], r.oz[i], r.dx[i], r.dy[i], r.dz[i], r.tmin[i],
+                        r.tmax[i], skip, t, best, id);
+    hit[i] = best;
+    hit_t[i] = t;
+    if (hit_n) {
+      float4 pl = best >= 0 ? sc.planes[best] : make_float4(0.f, 0.f, 0.f, 0.f);
+      hit_n[3 * i + 0] = pl.x;
+      hit_n[3 * i + 1] = pl.y;
+      hit_n[3 * i + 2] = pl.z;
+    }
   }
 }
 
@@ -57,22 +60,23 @@ __global__ __launch_bounds__(256) void k_trace_closest_plain(DevScene sc, DevRay
   }
 }
 
-template <int S, int D>
+template <int S, int D, int R>
 __global__ __launch_bounds__(64) void k_test_line_lds(DevScene sc, DevSegs s, int64_t n,
                                                       int32_t skip, uint8_t *__restrict__ vis,
                                                       uint8_t *__restrict__ sky) {
   __shared__ uint32_t s_node[S * 64];
   __shared__ float2 s_t[S * 64];
-  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  if (i >= n) return;
-  HybridStack<S, D> st;
-  st.node = s_node;
-  st.t = s_t;
-  st.lane = (int)threadIdx.x;
-  bool hs;
-  bool v = test_line<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i], s.ez[i], skip, hs);
-  vis[i] = v ? 1 : 0;
-  if (sky) sky[i] = hs ? 1 : 0;
+  const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R + threadIdx.x;
+  StackSpill<S, D> spill;
+  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
+  for (int q = 0; q < R; ++q) {
+    const int64_t i = base + 64 * q;
+    if (i >= n) return;
+    bool hs;
+    bool v = test_line<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i], s.ez[i], skip, hs);
+    vis[i] = v ? 1 : 0;
+    if (sky) sky[i] = hs ? 1 : 0;
+  }
 }
 
 template <int D>
@@ -88,7 +92,10 @@ __global__ __launch_bounds__(256) void k_test_line_plain(DevScene sc, DevSegs s,
   if (sky) sky[i] = hs ? 1 : 0;
 }
 
-#define VR_LDS_VARIANTS(X) X(4) X(8) X(12) X(16) X(24)
+// (traversal-stack entries in LDS, rays per lane)
+#define VR_LDS_VARIANTS(X)                                                                     \
+  X(4, 1) X(8, 1) X(12, 1) X(16, 1) X(24, 1) X(4, 2) X(8, 2) X(12, 2) X(16, 2) X(24, 2) X(4, 4) \
+  X(8, 4) X(12, 4) X(16, 4) X(24, 4)
 
 void launch_trace_closest(const DevScene &sc, const DevRays &r, int64_t n, int32_t skip,
                           int32_t *hit, float *hit_t, float *hit_n, int path, hipStream_t st) {
@@ -98,12 +105,13 @@ void launch_trace_closest(const DevScene &sc, const DevRays &r, int64_t n, int32
                        dim3(256), 0, st, sc, r, n, skip, hit, hit_t, hit_n);
     return;
   }
-  const dim3 g((unsigned)((n + 63) / 64));
-  switch (tuning().trace_lds_entries) {
-#define X(S)                                                                                  \
-  case S:                                                                                     \
-    hipLaunchKernelGGL((k_trace_closest_lds<S, kStackDepth>), g, dim3(64), 0, st, sc, r, n, skip, \
-                       hit, hit_t, hit_n);                                                    \
+  const int R = tuning().trace_rays_per_lane;
+  const dim3 g((unsigned)((n + 64 * R - 1) / (64 * R)));
+  switch (tuning().trace_lds_entries * 8 + R) {
+#define X(S, RR)                                                                              \
+  case S * 8 + RR:                                                                            \
+    hipLaunchKernelGGL((k_trace_closest_lds<S, kStackDepth, RR>), g, dim3(64), 0, st, sc, r, n, \
+                       skip, hit, hit_t, hit_n);                                              \
     break;
     VR_LDS_VARIANTS(X)
 #undef X
@@ -118,12 +126,13 @@ void launch_test_line(const DevScene &sc, const DevSegs &s, int64_t n, int32_t s
                        dim3(256), 0, st, sc, s, n, skip, vis, sky);
     return;
   }
-  const dim3 g((unsigned)((n + 63) / 64));
-  switch (tuning().trace_lds_entries) {
-#define X(S)                                                                                   \
-  case S:                                                                                      \
-    hipLaunchKernelGGL((k_test_line_lds<S, kStackDepth>), g, dim3(64), 0, st, sc, s, n, skip, vis, \
-                       sky);                                                                   \
+  const int R = tuning().trace_rays_per_lane;
+  const dim3 g((unsigned)((n + 64 * R - 1) / (64 * R)));
+  switch (tuning().trace_lds_entries * 8 + R) {
+#define X(S, RR)                                                                               \
+  case S * 8 + RR:                                                                             \
+    hipLaunchKernelGGL((k_test_line_lds<S, kStackDepth, RR>), g, dim3(64), 0, st, sc, s, n, skip, \
+                       vis, sky);                                                              \
     break;
     VR_LDS_VARIANTS(X)
 #undef X

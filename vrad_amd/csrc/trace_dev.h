@@ -79,13 +79,29 @@ __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32
 // Per-lane traversal stack: the first S entries in LDS (entry e of lane l at
 // [e * 64 + l], conflict-free), deeper entries in private scratch.  S = D
 // keeps the whole stack in LDS; S = 0 is the plain path's scratch stack.
+// The deeper entries live in a separate object (StackSpill) the stack points
+// to: an array indexed at run time sits in private memory, and keeping it out
+// of this struct lets the LDS pointers and the lane stay in registers.
+template <int S, int D>
+struct StackSpill {
+  uint32_t n[D - S > 0 ? D - S : 1];
+  float a[D - S > 0 ? D - S : 1], b[D - S > 0 ? D - S : 1];
+};
+#define VR_PRIVATE __attribute__((address_space(5)))
 template <int S, int D>
 struct HybridStack {
   uint32_t *node;
   float2 *t;
   int lane;
-  uint32_t pn[D - S > 0 ? D - S : 1];
-  float pa[D - S > 0 ? D - S : 1], pb[D - S > 0 ? D - S : 1];
+  VR_PRIVATE uint32_t *pn;  // private-memory pointers: scratch (not flat) accesses
+  VR_PRIVATE float *pa, *pb;
+  __device__ HybridStack(uint32_t *node_, float2 *t_, int lane_, StackSpill<S, D> &sp)
+      : node(node_),
+        t(t_),
+        lane(lane_),
+        pn((VR_PRIVATE uint32_t *)sp.n),
+        pa((VR_PRIVATE float *)sp.a),
+        pb((VR_PRIVATE float *)sp.b) {}
   __device__ void push(int sp, uint32_t n, float a, float b) {
     if (S == D || sp < S) {
       node[sp * 64 + lane] = n;

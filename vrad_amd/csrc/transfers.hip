@@ -95,10 +95,8 @@ __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p
   if (k >= n) return;
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
-  HybridStack<S, D> st;
-  st.node = s_node;
-  st.t = s_t;
-  st.lane = (int)threadIdx.x;
+  StackSpill<S, D> spill;
+  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
   bool sky;
   const bool vis = test_line<D>(sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
                                 p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
