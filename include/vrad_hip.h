@@ -78,7 +78,10 @@ int vr_set_path(int path);
  *   "trace_lds_entries"  traversal-stack entries per lane held in LDS
  *                        (4, 8, 12, 16, 24; the rest spill to scratch)
  *   "trace_rays_per_lane" rays each lane of the visibility kernels takes
- *                        (1, 2, 4): rays per wavefront = 64 x this
+ *                        (1, 2, 4, 8): rays per wavefront = 64 x this; for
+ *                        TestLine above 1 a lane's consecutive segments are
+ *                        first tested against the triangle that blocked the
+ *                        one before (results unchanged)
  *   "bounce_waves"       persistent waves of the wave-per-row gather
  * These select the FP32 summation order of BounceLight (the CPU
  * restatement takes the same values):

@@ -25,16 +25,21 @@ def _env(v9, ids):
     return env
 
 
-@pytest.fixture(params=[(_lib.VR_PATH_OPT, 8), (_lib.VR_PATH_OPT, 4), (_lib.VR_PATH_OPT, 24),
-                        (_lib.VR_PATH_PLAIN, 8)], ids=["opt-lds8", "opt-lds4", "opt-lds24", "plain"])
+@pytest.fixture(params=[(_lib.VR_PATH_OPT, 8, 1), (_lib.VR_PATH_OPT, 4, 1), (_lib.VR_PATH_OPT, 24, 1),
+                        (_lib.VR_PATH_OPT, 8, 2), (_lib.VR_PATH_OPT, 4, 4), (_lib.VR_PATH_OPT, 8, 8),
+                        (_lib.VR_PATH_PLAIN, 8, 1)],
+                ids=["opt-lds8", "opt-lds4", "opt-lds24", "opt-lds8-r2", "opt-lds4-r4", "opt-lds8-r8",
+                     "plain"])
 def path(request):
-    p, lds = request.param
-    print(f"[gpu] path={p} trace_lds_entries={lds}", flush=True)
+    p, lds, rays = request.param
+    print(f"[gpu] path={p} trace_lds_entries={lds} rays_per_lane={rays}", flush=True)
     _lib.check(_lib.lib.vr_set_path(p))
     _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", lds))
+    _lib.check(_lib.lib.vr_set_tuning(b"trace_rays_per_lane", rays))
     yield p
     _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_OPT))
     _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", 8))
+    _lib.check(_lib.lib.vr_set_tuning(b"trace_rays_per_lane", 1))
 
 
 @pytest.fixture(scope="module")
@@ -154,3 +159,30 @@ def test_empty_scene(path):
     assert np.all(hit == -1)
     vis, sky = env.TestLines(np.zeros((2, 3)), np.ones((2, 3)))
     assert np.all(vis == 1) and np.all(sky == 0)
+
+
+def test_test_line_rows_with_sky_vs_oracle(path):
+    # MakeTransfers-order segments (one origin patch per run of rows) in a
+    # level with skylights: consecutive segments are blocked by the same
+    # triangles, so with several rays per lane the occluder of a lane's
+    # previous segment is tested first; results must not change
+    from vrad_amd import scenes
+    s = scenes.level(seed=5, rooms_x=3, rooms_y=3, chop=64.0, tess=64.0, n_lights=4, sun=True)
+    assert np.any(s.ids & 0x01000000)
+    env = _env(s.verts9, s.ids)
+    o, nrm = s.p_origin, s.p_normal
+    i = np.repeat(np.arange(0, len(o), 7), 200)[:120000]
+    j = (i + 1 + np.arange(len(i)) % 200 * 3) % len(o)
+    a = (o[i] + np.float32(0.1) * nrm[i]).astype(np.float32)
+    b = (o[j] + np.float32(0.1) * nrm[j]).astype(np.float32)
+    # every 4th run of 64 goes straight up out of the level (skylights: sky hits)
+    up = (np.arange(len(a)) // 64) % 4 == 3
+    b[up] = a[up] + np.array([0.0, 0.0, 5000.0], np.float32)
+    vg, sg = env.TestLines(a, b)
+    tri = oracle.precompute(s.verts9, s.ids)
+    kd = oracle.KdTree(s.verts9)
+    vo, so = oracle.testline(kd, tri, a, b, threads=8)
+    print(f"[gpu] rows-with-sky: {len(a)} segments, visible {vo.mean():.3f}, sky {so.mean():.3f}",
+          flush=True)
+    assert 0.0 < vo.mean() < 1.0 and so.any()
+    assert np.array_equal(vg, vo) and np.array_equal(sg, so)

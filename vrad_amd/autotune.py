@@ -6,7 +6,7 @@ on the caller's own workload with the library's HIP-event kernel timers, and
 keeps the fastest:
 
   visibility  traversal-stack entries in LDS ("trace_lds_entries") x rays per
-              wavefront ("trace_rays_per_lane", 64 x 1/2/4 rays per wave)
+              wavefront ("trace_rays_per_lane", 64 x 1/2/4/8 rays per wave)
   bounce      gather tile width, kernel and prefetch depth ("bounce_kernel",
               "bounce_tile_cols", "bounce_group", "bounce_passes",
               "bounce_prefetch")
@@ -27,7 +27,7 @@ import time
 from . import _lib
 
 VIS_STACKS = (4, 8, 12, 16, 24)
-VIS_RAYS = (1, 2, 4)
+VIS_RAYS = (1, 2, 4, 8)
 BOUNCE_KNOBS = ("bounce_kernel", "bounce_tile_cols", "bounce_group", "bounce_passes",
                 "bounce_prefetch")
 # every compiled gather shape: (kernel, tile cols, group, passes, prefetch)

@@ -298,8 +298,8 @@ int vr_set_tuning(const char *name, int value) {
       return fail(VR_E_INVALID, "trace_lds_entries must be 4, 8, 12, 16 or 24");
     tuning().trace_lds_entries = value;
   } else if (k == "trace_rays_per_lane") {
-    if (value != 1 && value != 2 && value != 4)
-      return fail(VR_E_INVALID, "trace_rays_per_lane must be 1, 2 or 4");
+    if (value != 1 && value != 2 && value != 4 && value != 8)
+      return fail(VR_E_INVALID, "trace_rays_per_lane must be 1, 2, 4 or 8");
     tuning().trace_rays_per_lane = value;
   } else if (k == "bounce_waves") {
     if (value < 1 || value > (1 << 20)) return fail(VR_E_INVALID, "bounce_waves out of range");

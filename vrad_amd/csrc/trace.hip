@@ -60,22 +60,69 @@ __global__ __launch_bounds__(256) void k_trace_closest_plain(DevScene sc, DevRay
   }
 }
 
+// TestLine batches, R segments per lane.  R = 1: one segment per lane.
+// R > 1: lane l owns segments base + R l + q (q < R), consecutive ones (in a
+// MakeTransfers batch: one origin patch and neighbouring targets, mostly
+// blocked by the same triangle).  A wave walks each lane's first segment,
+// then tests each lane's following segments only against the triangle that
+// blocked the segment before it (occluded_by: exact, the walk would accept
+// that hit); the segments it does not decide are queued in LDS and walked
+// 64 at a time, so lanes stay busy whatever the hit pattern.
 template <int S, int D, int R>
 __global__ __launch_bounds__(64) void k_test_line_lds(DevScene sc, DevSegs s, int64_t n,
                                                       int32_t skip, uint8_t *__restrict__ vis,
                                                       uint8_t *__restrict__ sky) {
   __shared__ uint32_t s_node[S * 64];
   __shared__ float2 s_t[S * 64];
-  const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R + threadIdx.x;
+  __shared__ int32_t s_q[R > 1 ? 64 * (R - 1) : 1];  // queued segments (offsets from base)
+  const int lane = (int)threadIdx.x;
+  const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R;
   StackSpill<S, D> spill;
-  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
-  for (int q = 0; q < R; ++q) {
-    const int64_t i = base + 64 * q;
-    if (i >= n) return;
+  HybridStack<S, D> st(s_node, s_t, lane, spill);
+  auto walk = [&](int64_t i, uint32_t &occ) {
     bool hs;
-    bool v = test_line<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i], s.ez[i], skip, hs);
+    const bool v = test_line_cached<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i],
+                                       s.ez[i], skip, hs, occ);
     vis[i] = v ? 1 : 0;
     if (sky) sky[i] = hs ? 1 : 0;
+  };
+  if constexpr (R == 1) {
+    const int64_t i = base + lane;
+    uint32_t none = kNoRef;
+    if (i < n) walk(i, none);
+  } else {
+    const int64_t first = base + (int64_t)lane * R;
+    uint32_t occ = kNoRef;
+    if (first < n) walk(first, occ);
+    uint32_t qmask = 0;
+    for (int q = 1; q < R; ++q) {
+      const int64_t i = first + q;
+      if (i >= n) break;
+      if (occ != kNoRef &&
+          occluded_by<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i], s.ez[i], skip, occ)) {
+        vis[i] = 0;
+        if (sky) sky[i] = 0;
+      } else {
+        qmask |= 1u << q;
+      }
+    }
+    // exclusive prefix of the lanes' queue counts
+    const int cnt = __builtin_popcount(qmask);
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    int at = incl - cnt;
+    const int total = __shfl(incl, 63);
+    for (uint32_t m = qmask; m; m &= m - 1) s_q[at++] = lane * R + __builtin_ctz(m);
+    __syncthreads();  // one wave: the queue is written before it is read
+    for (int r = 0; r < total; r += 64) {
+      const int k = r + lane;
+      uint32_t none = kNoRef;
+      if (k < total) walk(base + s_q[k], none);
+    }
   }
 }
 
@@ -95,7 +142,7 @@ __global__ __launch_bounds__(256) void k_test_line_plain(DevScene sc, DevSegs s,
 // (traversal-stack entries in LDS, rays per lane)
 #define VR_LDS_VARIANTS(X)                                                                     \
   X(4, 1) X(8, 1) X(12, 1) X(16, 1) X(24, 1) X(4, 2) X(8, 2) X(12, 2) X(16, 2) X(24, 2) X(4, 4) \
-  X(8, 4) X(12, 4) X(16, 4) X(24, 4)
+  X(8, 4) X(12, 4) X(16, 4) X(24, 4) X(4, 8) X(8, 8) X(12, 8) X(16, 8) X(24, 8)
 
 void launch_trace_closest(const DevScene &sc, const DevRays &r, int64_t n, int32_t skip,
                           int32_t *hit, float *hit_t, float *hit_n, int path, hipStream_t st) {

@@ -50,7 +50,7 @@ template <bool SKIPSKY = false>
 __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32_t k, float ox,
                                          float oy, float oz, float dx, float dy, float dz,
                                          float tmin, int32_t skip, float &tbest, int32_t &best,
-                                         int32_t &best_id) {
+                                         int32_t &best_id, uint32_t &best_ref) {
   const float4 q0 = refs[3 * (size_t)k + 0];
   const float4 q1 = refs[3 * (size_t)k + 1];
   const float4 q2 = refs[3 * (size_t)k + 2];
@@ -73,8 +73,10 @@ __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32
     tbest = t;
     best = tri;
     best_id = id;
+    best_ref = k;
   }
 }
+constexpr uint32_t kNoRef = 0xffffffffu;
 
 // Per-lane traversal stack: the first S entries in LDS (entry e of lane l at
 // [e * 64 + l], conflict-free), deeper entries in private scratch.  S = D
@@ -146,14 +148,15 @@ struct ScratchStack {
 // come before (tbest, best) in (t, triangle index) order, updating them; the
 // result is the first of the hits in that order, bit for bit the brute-force
 // result.  ANY: return at the first accepted hit.  SKIPSKY: ignore
-// TRACE_ID_SKY triangles.  Returns whether a hit was accepted.  D = stack
-// capacity; the host guarantees tree depth < D.
+// TRACE_ID_SKY triangles.  Returns whether a hit was accepted (best_ref =
+// its leaf-reference index).  D = stack capacity; the host guarantees tree
+// depth < D.
 template <int D, bool ANY, bool SKIPSKY, class Stack>
 __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restrict__ refs,
                           const float *bmin, const float *bmax, float M, Stack &st, float ox,
                           float oy, float oz, float dx, float dy, float dz, float tmin,
                           float tmax, int32_t skip, float &tbest, int32_t &best,
-                          int32_t &best_id) {
+                          int32_t &best_id, uint32_t &best_ref) {
   const int32_t best0 = best;
   const float om = fmaxf(fabsf(ox), fmaxf(fabsf(oy), fabsf(oz)));
   const float dl = (M + om) * 0x1p-15f;
@@ -212,7 +215,8 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
       const uint32_t start = nd.x >> 2;
       const uint32_t cnt = (uint32_t)__uint_as_float(nd.y);
       for (uint32_t k = start; k < start + cnt; ++k)
-        tri_test<SKIPSKY>(refs, k, ox, oy, oz, dx, dy, dz, tmin, skip, tbest, best, best_id);
+        tri_test<SKIPSKY>(refs, k, ox, oy, oz, dx, dy, dz, tmin, skip, tbest, best, best_id,
+                          best_ref);
       if (ANY && best != best0) return true;
     }
     bool found = false;
@@ -244,8 +248,9 @@ __device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, flo
   best_out = -1;
   id_out = 0;
   if (sc.empty) return;
+  uint32_t ref = kNoRef;
   if (walk_tree<D, ANY, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, sc.M, st, ox, oy, oz, dx, dy,
-                               dz, tmin, tmax, skip, tbest, best, best_id)) {
+                               dz, tmin, tmax, skip, tbest, best, best_id, ref)) {
     t_out = tbest;
     best_out = best;
     id_out = best_id;
@@ -259,10 +264,17 @@ __device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, flo
 // result: the closest sky hit (ts, is) on the sky-only tree, then the full
 // tree stops at the first non-sky hit before (ts, is) in (t, index) order;
 // the segment is occluded iff there is one.
+//
+// occluder (in/out): a leaf reference of the full tree to test first (kNoRef
+// for none), typically the triangle that blocked the caller's previous,
+// nearby segment.  It is tested with the same arithmetic and bound as the
+// walk, so a hit there is exactly a hit the walk would accept and the result
+// does not depend on it.  On return it holds the blocking triangle found, if
+// any, else it is unchanged.
 template <int D, class Stack>
-__device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float sx, float sy,
-                                          float sz, float ex, float ey, float ez, int32_t skip,
-                                          bool &sky) {
+__device__ __forceinline__ bool test_line_cached(const DevScene &sc, Stack &st, float sx,
+                                                 float sy, float sz, float ex, float ey, float ez,
+                                                 int32_t skip, bool &sky, uint32_t &occluder) {
   const float vx = ex - sx, vy = ey - sy, vz = ez - sz;
   const float len2 = (vx * vx + vy * vy) + vz * vz;
   const float len = __builtin_sqrtf(len2);
@@ -272,14 +284,56 @@ __device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float s
   if (sc.empty) return true;
   float t = len;
   int32_t best = -1, id = 0;
+  uint32_t ref = kNoRef;
   if (sc.has_sky)
     walk_tree<D, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, sc.M, st, sx,
-                               sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
+                               sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
   const int32_t sky_best = best;
-  const bool occluded = walk_tree<D, true, true>(sc.nodes, sc.refs, sc.bmin, sc.bmax, sc.M, st, sx,
-                                                 sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id);
+  bool occluded = false;
+  if (occluder != kNoRef) {
+    tri_test<true>(sc.refs, occluder, sx, sy, sz, dx, dy, dz, 0.0f, skip, t, best, id, ref);
+    occluded = best != sky_best;
+  }
+  if (!occluded) {
+    ref = kNoRef;
+    occluded = walk_tree<D, true, true>(sc.nodes, sc.refs, sc.bmin, sc.bmax, sc.M, st, sx, sy, sz,
+                                        dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
+    if (occluded) occluder = ref;
+  }
   sky = !occluded && sky_best >= 0;
   return !occluded;
+}
+template <int D, class Stack>
+__device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float sx, float sy,
+                                          float sz, float ex, float ey, float ez, int32_t skip,
+                                          bool &sky) {
+  uint32_t none = kNoRef;
+  return test_line_cached<D>(sc, st, sx, sy, sz, ex, ey, ez, skip, sky, none);
+}
+
+// The first half of test_line_cached alone: whether the segment is occluded
+// by `occluder` (a full-tree leaf reference) as test_line would decide it
+// (the closest sky hit first, then the same hit test and bound).  False
+// means only that this triangle does not decide it.
+template <int D, class Stack>
+__device__ __forceinline__ bool occluded_by(const DevScene &sc, Stack &st, float sx, float sy,
+                                            float sz, float ex, float ey, float ez, int32_t skip,
+                                            uint32_t occluder) {
+  const float vx = ex - sx, vy = ey - sy, vz = ez - sz;
+  const float len2 = (vx * vx + vy * vy) + vz * vz;
+  const float len = __builtin_sqrtf(len2);
+  const float inv = 1.0f / len;
+  const float dx = vx * inv, dy = vy * inv, dz = vz * inv;
+  if (sc.empty) return false;
+  float t = len;
+  int32_t best = -1, id = 0;
+  uint32_t ref = kNoRef;
+  if (sc.has_sky)
+    walk_tree<D, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, sc.M, st, sx,
+                               sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
+  const int32_t sky_best = best;
+  tri_test<true>(sc.refs, occluder, sx, sy, sz, dx, dy, dz, 0.0f, skip, t, best, id, ref);
+  return best != sky_best;
 }
 
 }  // namespace vr
