@@ -180,10 +180,28 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
   if (!(ta <= tb)) return false;
   int sp = 0;
   uint32_t node = 0;
+  // next stacked node whose interval can still hold a hit before tbest
+  auto pop = [&]() -> bool {
+    while (sp > 0) {
+      --sp;
+      float a, b;
+      uint32_t n;
+      st.get(sp, n, a, b);
+      if (a > tbest) continue;
+      node = n;
+      ta = a;
+      tb = b;
+      return true;
+    }
+    return false;
+  };
+  // "while-while": a lane descends through inner nodes to its next leaf
+  // before the wave tests leaves, so lanes test triangles together.  Each
+  // lane visits the same nodes in the same order as one step per iteration.
   for (;;) {
-    const uint2 nd = nodes[node];
-    const uint32_t type = nd.x & 3u;
-    if (type != 3u) {
+    uint2 nd = nodes[node];
+    while ((nd.x & 3u) != 3u) {
+      const uint32_t type = nd.x & 3u;
       const float sv = __uint_as_float(nd.y);
       const float o_a = sel3((int)type, ox, oy, oz);
       const float i_a = sel3((int)type, ix, iy, iz);
@@ -204,35 +222,21 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
         }
         node = near_n;
         tb = fminf(tb, near_end);
-        continue;
-      }
-      if (far_ok) {
+      } else if (far_ok) {
         node = far_n;
         ta = fs;
-        continue;
+      } else if (!pop()) {
+        return best != best0;
       }
-    } else {
-      const uint32_t start = nd.x >> 2;
-      const uint32_t cnt = (uint32_t)__uint_as_float(nd.y);
-      for (uint32_t k = start; k < start + cnt; ++k)
-        tri_test<SKIPSKY>(refs, k, ox, oy, oz, dx, dy, dz, tmin, skip, tbest, best, best_id,
-                          best_ref);
-      if (ANY && best != best0) return true;
+      nd = nodes[node];
     }
-    bool found = false;
-    while (sp > 0) {
-      --sp;
-      float a, b;
-      uint32_t n;
-      st.get(sp, n, a, b);
-      if (a > tbest) continue;
-      node = n;
-      ta = a;
-      tb = b;
-      found = true;
-      break;
-    }
-    if (!found) break;
+    const uint32_t start = nd.x >> 2;
+    const uint32_t cnt = (uint32_t)__uint_as_float(nd.y);
+    for (uint32_t k = start; k < start + cnt; ++k)
+      tri_test<SKIPSKY>(refs, k, ox, oy, oz, dx, dy, dz, tmin, skip, tbest, best, best_id,
+                        best_ref);
+    if (ANY && best != best0) return true;
+    if (!pop()) break;
   }
   return best != best0;
 }
