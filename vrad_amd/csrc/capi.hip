@@ -323,7 +323,13 @@ int vr_set_tuning(const char *name, int value) {
     if (value < 0 || value > 2)
       return fail(VR_E_INVALID, "bounce_passes must be 0 (by precision), 1 or 2");
     tuning().bounce_passes = value;
-
+  } else if (k == "kd_build") {
+    if (value < 0 || value > 2)
+      return fail(VR_E_INVALID, "kd_build must be 0 (host), 1 (device) or 2 (by size)");
+    tuning().kd_build = value;
+  } else if (k == "kd_build_device_min") {
+    if (value < 0) return fail(VR_E_INVALID, "kd_build_device_min must be >= 0");
+    tuning().kd_build_device_min = value;
   } else {
     return fail(VR_E_INVALID, "unknown tuning knob " + k);
   }
@@ -342,6 +348,8 @@ int vr_get_tuning(const char *name, int *value) {
   else if (k == "bounce_group") *value = tuning().bounce_group;
   else if (k == "bounce_passes") *value = tuning().bounce_passes;
   else if (k == "bounce_prefetch") *value = tuning().bounce_prefetch;
+  else if (k == "kd_build") *value = tuning().kd_build;
+  else if (k == "kd_build_device_min") *value = tuning().kd_build_device_min;
   else return fail(VR_E_INVALID, "unknown tuning knob " + k);
   return VR_OK;
 }
@@ -421,15 +429,19 @@ int vr_scene_create(vr_scene **out, const float *verts9, const int32_t *ids, con
   return VR_OK;
 }
 
-int vr_scene_build_host(vr_scene *s, int32_t n_threads) {
-  LOCK;
-  if (!s) return fail(VR_E_INVALID, "scene is NULL");
+// Triangle intersection records and both kd-trees of a scene; the trees on
+// the host threads or, with `device`, on the GPU (the same trees).
+static int build_scene_trees(vr_scene *s, int32_t n_threads, bool device) {
   if (s->host_built) return VR_OK;
   const int nt = host_threads(n_threads);
   s->tris.resize((size_t)s->n_tri);
   precompute_triangles(s->v9.data(), s->ids.data(), s->flags.data(), s->n_tri, s->tris.data(), nt);
+  auto build = [&](const float *v9, int64_t n, KdTree *out) {
+    if (device) return build_kdtree_device(v9, n, g_stream, out);
+    return build_kdtree(v9, n, nt, out);
+  };
   s->kd = KdTree();
-  TRY(build_kdtree(s->v9.data(), s->n_tri, nt, &s->kd));
+  TRY(build(s->v9.data(), s->n_tri, &s->kd));
   if (s->kd.max_depth >= kStackDepth)
     return fail(VR_E_CAPACITY, "kd-tree deeper than the traversal stack");
   s->has_sky = 0;
@@ -446,7 +458,7 @@ int vr_scene_build_host(vr_scene *s, int32_t n_threads) {
         s->sky_index.push_back((int32_t)i);
         v9s.insert(v9s.end(), s->v9.begin() + 9 * i, s->v9.begin() + 9 * i + 9);
       }
-    TRY(build_kdtree(v9s.data(), (int64_t)s->sky_index.size(), nt, &s->kd_sky));
+    TRY(build(v9s.data(), (int64_t)s->sky_index.size(), &s->kd_sky));
     if (s->kd_sky.max_depth >= kStackDepth)
       return fail(VR_E_CAPACITY, "sky kd-tree deeper than the traversal stack");
   }
@@ -454,11 +466,19 @@ int vr_scene_build_host(vr_scene *s, int32_t n_threads) {
   return VR_OK;
 }
 
+int vr_scene_build_host(vr_scene *s, int32_t n_threads) {
+  LOCK;
+  if (!s) return fail(VR_E_INVALID, "scene is NULL");
+  return build_scene_trees(s, n_threads, false);
+}
+
 int vr_scene_build(vr_scene *s, int32_t n_threads) {
   LOCK;
   if (!s) return fail(VR_E_INVALID, "scene is NULL");
-  TRY(vr_scene_build_host(s, n_threads));
   TRY(ensure_init());
+  const int kb = tuning().kd_build;
+  const bool device = kb == 1 || (kb == 2 && s->n_tri >= tuning().kd_build_device_min);
+  TRY(build_scene_trees(s, n_threads, device));
   const KdTree &kd = s->kd;
   // device records of a tree: nodes, and 3 float4 per leaf reference whose
   // meta holds the triangle's index in OptimizedTriangleList (`map` turns a

@@ -24,8 +24,17 @@ struct Tuning {
   int bounce_group = 0;        // 8 / 16 lanes per row segment, 0 = by precision
   int bounce_passes = 0;       // passes of a segment loaded ahead of use, 0 = by precision
   int bounce_prefetch = 0;     // steps/passes the loads run ahead, 0 = by kernel (3 / 1)
+  // kd-tree construction in vr_scene_build (the same tree either way)
+  int kd_build = 2;                 // 0 = host threads, 1 = GPU, 2 = GPU from a size on
+  int kd_build_device_min = 2048;   // triangles at which kd_build = 2 uses the GPU
 };
 Tuning &tuning();
+
+// ---- kd-tree construction on the device (kdbuild.hip) ----
+struct KdTree;
+// The host builder's tree (build_kdtree) built on the GPU on stream st and
+// returned in host memory.
+int build_kdtree_device(const float *v9, int64_t n, hipStream_t st, KdTree *out);
 
 struct DevRays {
   const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
