@@ -114,6 +114,34 @@ class Ctx:
         return ms.value / max(n.value, 1)
 
 
+def kd_build_compare(s, env):
+    """Scene build (triangle records, both kd-trees, upload) with the tree built
+    on the GPU and on the host threads; both must give env's tree."""
+    from vrad_amd import autotune
+    from vrad_amd.raytracer import Environment
+    out = {}
+    old = autotune.get_knob("kd_build")
+    try:
+        for name, mode in (("device", 1), ("host", 0)):
+            autotune.set_knob("kd_build", mode)
+            e = Environment()
+            e.AddTriangles(s.verts9, s.ids)
+            t0 = time.perf_counter()
+            e.SetupAccelerationStructure(n_threads=16)
+            out[f"{name}_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+            same = (np.array_equal(e.OptimizedKDTree[0], env.OptimizedKDTree[0])
+                    and np.array_equal(e.OptimizedKDTree[1].view(np.uint32),
+                                       env.OptimizedKDTree[1].view(np.uint32))
+                    and np.array_equal(e.TriangleIndexList, env.TriangleIndexList))
+            out[f"{name}_same_tree"] = bool(same)
+            e.close()
+    finally:
+        autotune.set_knob("kd_build", old)
+    out["host_threads"] = 16
+    out["note"] = "whole vr_scene_build: triangle records (host), kd-trees, upload"
+    return out
+
+
 def run_visibility(cx, args):
     progress("visibility")
     from vrad_amd import scenes
@@ -126,6 +154,7 @@ def run_visibility(cx, args):
     env.SetupAccelerationStructure()
     build_s = time.time() - tb
     info = env.info()
+    kd_build = kd_build_compare(s, env)
     st, en = row_batch(s, args.segments, i0=len(s.p_area) // 2)
     n = len(st)
 
@@ -172,7 +201,7 @@ def run_visibility(cx, args):
     _, plain_ms = batch.run(3, 1)
     check(lib.vr_set_path(L.VR_PATH_OPT))
     extra = {"autotune_Mrays": sweep, "tuned": best_s,
-             "plain_path_Mrays": round(n / (plain_ms * 1e-3) / 1e6, 1)}
+             "plain_path_Mrays": round(n / (plain_ms * 1e-3) / 1e6, 1), "kd_build": kd_build}
     if args.random_pairs:
         rb = Batch(*random_batch(s, args.segments, seed=20))
         _, rand_ms = rb.run(3, 1)

@@ -37,7 +37,8 @@ constexpr float kKdCostTrav = 75.0f;    // :25
 constexpr float kKdCostIsect = 167.0f;  // :26
 constexpr int kAxisCands = 32;          // candidates per axis: at most 1 + 3 * 9
 constexpr int kCands = 3 * kAxisCands;
-constexpr int kChunk = 2048;            // references per evaluation / partition chunk
+constexpr int kChunk = 2048;            // references per partition chunk
+constexpr int kEvalChunk = 512;         // references per candidate-evaluation chunk
 constexpr uint32_t kLeafType = 3u;
 
 struct KNode {
@@ -94,20 +95,21 @@ __device__ __forceinline__ float box_sa(const float mn[3], const float mx[3]) {
   return 2.0f * ((d0 * d2 + d0 * d1) + d1 * d2);
 }
 
-// Exclusive scan of one value per thread over a 1024-thread block; *total
+// Exclusive scan of one value per thread over a block of B threads; *total
 // receives the sum.  Every thread of the block must call it.
+template <int B = 1024>
 __device__ int64_t block_scan(int64_t v, int64_t *sh, int64_t *total) {
   const int t = (int)threadIdx.x;
   sh[t] = v;
   __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
+  for (int d = 1; d < B; d <<= 1) {
     const int64_t x = t >= d ? sh[t - d] : 0;
     __syncthreads();
     sh[t] += x;
     __syncthreads();
   }
   const int64_t incl = sh[t];
-  if (total) *total = sh[1023];
+  if (total) *total = sh[B - 1];
   __syncthreads();
   return incl - v;
 }
@@ -176,35 +178,65 @@ __device__ __forceinline__ bool refinable(const KNode &nd) {
   return nd.n >= 3 && nd.depth <= kKdMaxDepth;
 }
 
-// chunk_start[i] = chunks of the nodes before i (ceil(n / kChunk) each);
-// chunk_start[N] = total.  One block.
+__device__ __forceinline__ int64_t part_chunks(const KNode &nd) {
+  return ((int64_t)nd.n + kChunk - 1) / kChunk;
+}
+__device__ __forceinline__ int64_t eval_chunks(const KNode &nd) {
+  return refinable(nd) ? ((int64_t)nd.n + kEvalChunk - 1) / kEvalChunk : 0;
+}
+
+// chunk_start[i] = partition chunks of the nodes before i (ceil(n / kChunk)
+// each), eval_start[i] = evaluation chunks (ceil(n / kEvalChunk) per node to
+// refine); entry N = totals.  One block.
 __global__ __launch_bounds__(1024) void k_kd_chunks(const KNode *__restrict__ nodes, int32_t N,
-                                                    int64_t *__restrict__ chunk_start) {
+                                                    int64_t *__restrict__ chunk_start,
+                                                    int64_t *__restrict__ eval_start) {
   __shared__ int64_t sh[1024];
   const int t = (int)threadIdx.x;
   const int32_t per = (N + 1023) / 1024;
   const int32_t a = min(N, t * per), b = min(N, a + per);
-  int64_t s = 0;
-  for (int32_t i = a; i < b; ++i) s += ((int64_t)nodes[i].n + kChunk - 1) / kChunk;
-  int64_t total;
-  int64_t run = block_scan(s, sh, &total);
+  int64_t s = 0, se = 0;
   for (int32_t i = a; i < b; ++i) {
-    chunk_start[i] = run;
-    run += ((int64_t)nodes[i].n + kChunk - 1) / kChunk;
+    const KNode nd = nodes[i];
+    s += part_chunks(nd);
+    se += eval_chunks(nd);
   }
-  if (t == 0) chunk_start[N] = total;
+  int64_t total, etotal;
+  int64_t run = block_scan(s, sh, &total);
+  int64_t erun = block_scan(se, sh, &etotal);
+  for (int32_t i = a; i < b; ++i) {
+    const KNode nd = nodes[i];
+    chunk_start[i] = run;
+    eval_start[i] = erun;
+    run += part_chunks(nd);
+    erun += eval_chunks(nd);
+  }
+  if (t == 0) {
+    chunk_start[N] = total;
+    eval_start[N] = etotal;
+  }
 }
 
 // Candidate splits of node i on axis a, in the host builder's order.
+// Also clears the node's counters for the evaluation: cl, cr of the axis's
+// candidates and the axis's two extent keys.
 __global__ void k_kd_cands(const KNode *__restrict__ nodes, int32_t N,
                            const int32_t *__restrict__ refs, const float *__restrict__ v9,
-                           float *__restrict__ cv, int32_t *__restrict__ cc) {
+                           float *__restrict__ cv, int32_t *__restrict__ cc,
+                           int32_t *__restrict__ cl, int32_t *__restrict__ cr,
+                           unsigned long long *__restrict__ ext) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= 3 * (int64_t)N) return;
   const int32_t i = (int32_t)(g / 3);
   const int a = (int)(g % 3);
   const KNode nd = nodes[i];
   int cnt = 0;
+  for (int j = 0; j < kAxisCands; ++j) {
+    cl[(int64_t)i * kCands + a * kAxisCands + j] = 0;
+    cr[(int64_t)i * kCands + a * kAxisCands + j] = 0;
+  }
+  ext[6 * (int64_t)i + a] = kNone;
+  ext[6 * (int64_t)i + 3 + a] = kNone;
   if (refinable(nd)) {
     float *o = cv + (int64_t)i * kCands + a * kAxisCands;
     const float mn = nd.mn[a], mx = nd.mx[a];
@@ -222,23 +254,24 @@ __global__ void k_kd_cands(const KNode *__restrict__ nodes, int32_t N,
   cc[g] = cnt;
 }
 
-// Left/right counts of every candidate of a node over one chunk of its list
-// (atomically added), and the chunk's per-axis extents (first of the least
-// lo / greatest hi values, as list positions).
-__global__ __launch_bounds__(128) void k_kd_eval(
-    const KNode *__restrict__ nodes, int32_t N, const int64_t *__restrict__ chunk_start,
+// Left/right counts of every candidate of a node over one evaluation chunk
+// of its list (atomically added; two threads per candidate, one per half of
+// the chunk), and the chunk's per-axis extents (first of the least lo /
+// greatest hi values, as list positions).
+__global__ __launch_bounds__(256) void k_kd_eval(
+    const KNode *__restrict__ nodes, int32_t N, const int64_t *__restrict__ eval_start,
     const int32_t *__restrict__ refs, const float *__restrict__ lo, const float *__restrict__ hi,
     const float *__restrict__ cv, const int32_t *__restrict__ cc, int32_t *__restrict__ cl,
     int32_t *__restrict__ cr, unsigned long long *__restrict__ ext) {
-  __shared__ float s_lo[3][kChunk], s_hi[3][kChunk];
+  __shared__ float s_lo[3][kEvalChunk], s_hi[3][kEvalChunk];
   const int64_t ch = blockIdx.x;
-  if (ch >= chunk_start[N]) return;
-  const int32_t i = last_le(chunk_start, N, ch);
+  if (ch >= eval_start[N]) return;
+  const int32_t i = last_le(eval_start, N, ch);
   const KNode nd = nodes[i];
-  if (!refinable(nd)) return;
-  const int64_t k0 = (ch - chunk_start[i]) * kChunk;
-  const int32_t m = (int32_t)min((int64_t)kChunk, (int64_t)nd.n - k0);
-  for (int32_t j = (int)threadIdx.x; j < m; j += 128) {
+  if (!refinable(nd)) return;  // (such nodes have no evaluation chunks)
+  const int64_t k0 = (ch - eval_start[i]) * kEvalChunk;
+  const int32_t m = (int32_t)min((int64_t)kEvalChunk, (int64_t)nd.n - k0);
+  for (int32_t j = (int)threadIdx.x; j < m; j += 256) {
     const int64_t t = refs[nd.off + k0 + j];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -247,27 +280,30 @@ __global__ __launch_bounds__(128) void k_kd_eval(
     }
   }
   __syncthreads();
-  const int c = (int)threadIdx.x;
-  if (c < kCands) {
+  const int tid = (int)threadIdx.x;
+  if (tid < 2 * kCands) {
+    const int c = tid % kCands, half = tid / kCands;
     const int a = c / kAxisCands, j = c % kAxisCands;
     if (j < cc[3 * (int64_t)i + a]) {
       const float sv = cv[(int64_t)i * kCands + c];
+      const int32_t mh = (m + 1) >> 1;
+      const int32_t q0 = half ? mh : 0, q1 = half ? m : mh;
       int32_t l = 0, r = 0;
-      for (int32_t q = 0; q < m; ++q) {
+      for (int32_t q = q0; q < q1; ++q) {
         const int k = kd_classify(s_lo[a][q], s_hi[a][q], sv);
         r += k > 0;
         l += k < 0;
       }
-      atomicAdd(&cl[(int64_t)i * kCands + c], l);
-      atomicAdd(&cr[(int64_t)i * kCands + c], r);
+      if (l) atomicAdd(&cl[(int64_t)i * kCands + c], l);
+      if (r) atomicAdd(&cr[(int64_t)i * kCands + c], r);
     }
   } else {
-    // 32 threads: per axis, the first least lo and first greatest hi (NaN
+    // 64 threads: per axis, the first least lo and first greatest hi (NaN
     // never taken, as `v < min` / `v > max` in the host code)
-    const int u = c - kCands;
+    const int u = tid - 2 * kCands;
     for (int a = 0; a < 3; ++a) {
       unsigned long long bmin = kNone, bmax = kNone;
-      for (int32_t q = u; q < m; q += 32) {
+      for (int32_t q = u; q < m; q += 64) {
         const uint32_t pos = (uint32_t)(k0 + q);
         const float l = s_lo[a][q], h = s_hi[a][q];
         if (l == l) bmin = kmin(bmin, min_key(l, pos));
@@ -430,8 +466,10 @@ __global__ void k_kd_children(const KNode *__restrict__ nodes, const KDec *__res
   next[d.child + 1] = R;
 }
 
-// Per chunk of a split node's list: counts of negatives, straddlers and
-// positives (packed 16 bits each; a chunk holds at most 2048 references).
+// Per chunk of a split node whose list spans several chunks: counts of
+// negatives, straddlers and positives (packed 16 bits each; a chunk holds at
+// most 2048 references).  A single-chunk list needs none: its ranks start
+// at 0.
 __global__ __launch_bounds__(256) void k_kd_partcount(
     const KNode *__restrict__ nodes, const KDec *__restrict__ dec, int32_t N,
     const int64_t *__restrict__ chunk_start, const int32_t *__restrict__ refs,
@@ -441,8 +479,9 @@ __global__ __launch_bounds__(256) void k_kd_partcount(
   if (ch >= chunk_start[N]) return;
   const int32_t i = last_le(chunk_start, N, ch);
   const KDec d = dec[i];
+  if (d.kind == (int32_t)kLeafType || nodes[i].n <= kChunk) return;
   int64_t v = 0;
-  if (d.kind != (int32_t)kLeafType) {
+  {
     const KNode nd = nodes[i];
     const int64_t k0 = (ch - chunk_start[i]) * kChunk;
     const int32_t m = (int32_t)min((int64_t)kChunk, (int64_t)nd.n - k0);
@@ -461,36 +500,63 @@ __global__ __launch_bounds__(256) void k_kd_partcount(
   if (threadIdx.x == 0) ccnt[ch] = sh[0];
 }
 
-// Exclusive prefix over all chunks of the three packed counts, unpacked into
-// pfx[3 ch + class] (int64 each; one block).
-__global__ __launch_bounds__(1024) void k_kd_chunkscan(const int64_t *__restrict__ ccnt,
-                                                       const int64_t *__restrict__ chunk_start,
-                                                       int32_t N, int64_t *__restrict__ pfx) {
-  __shared__ int64_t sh[1024];
-  const int64_t C = chunk_start[N];
-  const int t = (int)threadIdx.x;
-  const int64_t per = (C + 1023) / 1024;
-  const int64_t a = min(C, t * per), b = min(C, a + per);
-  for (int cls = 0; cls < 3; ++cls) {
-    int64_t s = 0;
-    for (int64_t c = a; c < b; ++c) s += (ccnt[c] >> (16 * cls)) & 0xffff;
-    int64_t run = block_scan(s, sh, nullptr);
-    for (int64_t c = a; c < b; ++c) {
-      pfx[3 * c + cls] = run;
-      run += (ccnt[c] >> (16 * cls)) & 0xffff;
+// Per split node spanning several chunks (one thread each): exclusive prefix
+// of its chunks' class counts, pfx[3 ch + class] = references of that class
+// in the node's earlier chunks.
+__global__ void k_kd_chunkbase(const KNode *__restrict__ nodes, const KDec *__restrict__ dec,
+                               int32_t N, const int64_t *__restrict__ chunk_start,
+                               const int64_t *__restrict__ ccnt, int64_t *__restrict__ pfx) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= N) return;
+  if (dec[i].kind == (int32_t)kLeafType || nodes[i].n <= kChunk) return;
+  int64_t run[3] = {0, 0, 0};
+  for (int64_t c = chunk_start[i]; c < chunk_start[i + 1]; ++c) {
+    const int64_t v = ccnt[c];
+    for (int cls = 0; cls < 3; ++cls) {
+      pfx[3 * c + cls] = run[cls];
+      run[cls] += (v >> (16 * cls)) & 0xffff;
     }
   }
 }
 
-// Stable partition of one chunk: thread t takes references 8t .. 8t + 7.
-__global__ __launch_bounds__(1024) void k_kd_scatter(
+// Writes reference tri of class cls (0 negative, 1 straddling, 2 positive)
+// with rank k within its class into the children's lists.
+__device__ __forceinline__ void kd_place_ref(const KDec &d, int cls, int64_t k, int32_t tri,
+                                             int32_t *__restrict__ next_refs, int64_t next_total,
+                                             int32_t *__restrict__ err) {
+  const int64_t nl = d.nl, nb = d.nb, nr = d.nr;
+  if (cls == 0) {  // negative: left, in order
+    if (k >= nl || d.loff + k >= next_total) {
+      atomicOr(err, 1);
+      return;
+    }
+    next_refs[d.loff + k] = tri;
+  } else if (cls == 1) {  // straddling: both, in order
+    if (k >= nb || d.roff + k >= next_total) {
+      atomicOr(err, 1);
+      return;
+    }
+    next_refs[d.loff + nl + k] = tri;
+    next_refs[d.roff + k] = tri;
+  } else {  // positive: right, reversed
+    if (k >= nr || d.roff + nb + nr - 1 - k >= next_total) {
+      atomicOr(err, 1);
+      return;
+    }
+    next_refs[d.roff + nb + (nr - 1 - k)] = tri;
+  }
+}
+
+// Stable partition of one chunk of a split node's list.  Up to 64
+// references: one wave, ranks from ballots.  Otherwise thread t takes
+// references 8t .. 8t + 7 and ranks come from a block scan, offset by the
+// node's earlier chunks (pfx).
+__global__ __launch_bounds__(256) void k_kd_scatter(
     const KNode *__restrict__ nodes, const KDec *__restrict__ dec, int32_t N,
     const int64_t *__restrict__ chunk_start, const int32_t *__restrict__ refs,
     const float *__restrict__ lo, const float *__restrict__ hi, const int64_t *__restrict__ pfx,
     int32_t *__restrict__ next_refs, int64_t next_total, int32_t *__restrict__ err) {
-  // 256 threads would do; 1024 keeps block_scan's shape (threads past the
-  // chunk's references take none)
-  __shared__ int64_t sh[1024];
+  __shared__ int64_t sh[256];
   const int64_t ch = blockIdx.x;
   if (ch >= chunk_start[N]) return;
   const int32_t i = last_le(chunk_start, N, ch);
@@ -500,45 +566,47 @@ __global__ __launch_bounds__(1024) void k_kd_scatter(
   const int64_t c0 = chunk_start[i];
   const int64_t k0 = (ch - c0) * kChunk;
   const int32_t m = (int32_t)min((int64_t)kChunk, (int64_t)nd.n - k0);
-  constexpr int PER = 2;  // 1024 threads x 2 = one chunk
   const int t = (int)threadIdx.x;
+  auto classify_ref = [&](int32_t tri) {
+    const int k = kd_classify(lo[3 * (int64_t)tri + d.kind], hi[3 * (int64_t)tri + d.kind], d.trial);
+    return k < 0 ? 0 : (k == 0 ? 1 : 2);
+  };
+  if (m <= 64) {  // block-uniform: one wave, no barrier
+    if (t >= 64) return;
+    const bool live = t < m;
+    const int32_t tri = live ? refs[nd.off + k0 + t] : 0;
+    const int cls = live ? classify_ref(tri) : 3;
+    const uint64_t below = (1ull << t) - 1ull;
+    const uint64_t b0 = __ballot(cls == 0), b1 = __ballot(cls == 1), b2 = __ballot(cls == 2);
+    if (!live) return;
+    const uint64_t mine = cls == 0 ? b0 : (cls == 1 ? b1 : b2);
+    const int64_t base = nd.n > kChunk ? pfx[3 * ch + cls] : 0;
+    kd_place_ref(d, cls, base + __popcll(mine & below), tri, next_refs, next_total, err);
+    return;
+  }
+  constexpr int PER = 8;  // 256 threads x 8 = one chunk
   int cls[PER];
   int32_t tri[PER];
   int64_t v = 0;
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int32_t j = t * PER + u;
-    cls[u] = 2;  // none
+    cls[u] = 3;  // none
     tri[u] = 0;
     if (j < m) {
       tri[u] = refs[nd.off + k0 + j];
-      const int k = kd_classify(lo[3 * (int64_t)tri[u] + d.kind], hi[3 * (int64_t)tri[u] + d.kind],
-                                d.trial);
-      cls[u] = k < 0 ? 0 : (k == 0 ? 1 : 3);
-      v += cls[u] == 0 ? 1 : (cls[u] == 1 ? (1ll << 16) : (1ll << 32));
+      cls[u] = classify_ref(tri[u]);
+      v += 1ll << (16 * cls[u]);
     }
   }
-  int64_t ex = block_scan(v, sh, nullptr);
+  const int64_t ex = block_scan<256>(v, sh, nullptr);
   int64_t rk[3];
-  for (int c = 0; c < 3; ++c) rk[c] = (pfx[3 * ch + c] - pfx[3 * c0 + c]) + ((ex >> (16 * c)) & 0xffff);
-  const int64_t nl = d.nl, nb = d.nb, nr = d.nr;
+  for (int c = 0; c < 3; ++c)
+    rk[c] = (nd.n > kChunk ? pfx[3 * ch + c] : 0) + ((ex >> (16 * c)) & 0xffff);
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
-    if (cls[u] == 2) continue;
-    if (cls[u] == 0) {  // negative: left, in order
-      const int64_t k = rk[0]++;
-      if (k >= nl || d.loff + k >= next_total) { atomicOr(err, 1); continue; }
-      next_refs[d.loff + k] = tri[u];
-    } else if (cls[u] == 1) {  // straddling: both, in order
-      const int64_t k = rk[1]++;
-      if (k >= nb || d.roff + k >= next_total) { atomicOr(err, 1); continue; }
-      next_refs[d.loff + nl + k] = tri[u];
-      next_refs[d.roff + k] = tri[u];
-    } else {  // positive: right, reversed
-      const int64_t k = rk[2]++;
-      if (k >= nr || d.roff + nb + nr - 1 - k >= next_total) { atomicOr(err, 1); continue; }
-      next_refs[d.roff + nb + (nr - 1 - k)] = tri[u];
-    }
+    if (cls[u] == 3) continue;
+    kd_place_ref(d, cls[u], rk[cls[u]]++, tri[u], next_refs, next_total, err);
   }
 }
 
@@ -714,7 +782,7 @@ int build(const float *v9_host, int64_t n, hipStream_t st, KdTree *out) {
     float *cv;
     int32_t *cc, *cl, *cr;
     unsigned long long *ext;
-    int64_t *cs;
+    int64_t *cs, *es;
     KD_TRY(keep.get(&c.dec, N));
     KD_TRY(tmp.get(&cv, (int64_t)N * kCands));
     KD_TRY(tmp.get(&cc, 3 * (int64_t)N));
@@ -722,14 +790,13 @@ int build(const float *v9_host, int64_t n, hipStream_t st, KdTree *out) {
     KD_TRY(tmp.get(&cr, (int64_t)N * kCands));
     KD_TRY(tmp.get(&ext, 6 * (int64_t)N));
     KD_TRY(tmp.get(&cs, (int64_t)N + 1));
+    KD_TRY(tmp.get(&es, (int64_t)N + 1));
     const int64_t max_chunks = c.R / kChunk + N + 1;
-    KD_HIP(hipMemsetAsync(cl, 0, sizeof(int32_t) * (size_t)N * kCands, st));
-    KD_HIP(hipMemsetAsync(cr, 0, sizeof(int32_t) * (size_t)N * kCands, st));
-    KD_HIP(hipMemsetAsync(ext, 0xff, sizeof(unsigned long long) * 6 * (size_t)N, st));  // kNone
-    hipLaunchKernelGGL(k_kd_chunks, dim3(1), dim3(1024), 0, st, c.nodes, N, cs);
+    const int64_t max_echunks = c.R / kEvalChunk + N + 1;
+    hipLaunchKernelGGL(k_kd_chunks, dim3(1), dim3(1024), 0, st, c.nodes, N, cs, es);
     hipLaunchKernelGGL(k_kd_cands, dim3(grid(3 * (int64_t)N, 256)), dim3(256), 0, st, c.nodes, N,
-                       c.refs, v9, cv, cc);
-    hipLaunchKernelGGL(k_kd_eval, dim3(grid(max_chunks, 1)), dim3(128), 0, st, c.nodes, N, cs,
+                       c.refs, v9, cv, cc, cl, cr, ext);
+    hipLaunchKernelGGL(k_kd_eval, dim3(grid(max_echunks, 1)), dim3(256), 0, st, c.nodes, N, es,
                        c.refs, lo, hi, cv, cc, cl, cr, ext);
     hipLaunchKernelGGL(k_kd_select, dim3(grid(N, 4)), dim3(256), 0, st, c.nodes, N, c.refs, lo,
                        hi, cv, cc, cl, cr, ext, c.dec);
@@ -764,8 +831,9 @@ int build(const float *v9_host, int64_t n, hipStream_t st, KdTree *out) {
                        nx.nodes);
     hipLaunchKernelGGL(k_kd_partcount, dim3(grid(max_chunks, 1)), dim3(256), 0, st, c2.nodes,
                        c2.dec, N, cs, c2.refs, lo, hi, ccnt);
-    hipLaunchKernelGGL(k_kd_chunkscan, dim3(1), dim3(1024), 0, st, ccnt, cs, N, pfx);
-    hipLaunchKernelGGL(k_kd_scatter, dim3(grid(max_chunks, 1)), dim3(1024), 0, st, c2.nodes,
+    hipLaunchKernelGGL(k_kd_chunkbase, dim3(grid(N, 256)), dim3(256), 0, st, c2.nodes, c2.dec, N,
+                       cs, ccnt, pfx);
+    hipLaunchKernelGGL(k_kd_scatter, dim3(grid(max_chunks, 1)), dim3(256), 0, st, c2.nodes,
                        c2.dec, N, cs, c2.refs, lo, hi, pfx, nx.refs, nx.R, err);
     KD_HIP(hipGetLastError());
   }
