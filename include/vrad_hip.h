@@ -83,6 +83,10 @@ int vr_set_path(int path);
  *                        first tested against the triangle that blocked the
  *                        one before (results unchanged)
  *   "bounce_waves"       persistent waves of the wave-per-row gather
+ *   "kd_build"           kd-tree construction in vr_scene_build: 0 = host
+ *                        threads, 1 = GPU, 2 (default) = GPU from
+ *                        "kd_build_device_min" (default 2048) triangles; the
+ *                        tree is the same bit for bit
  * These select the FP32 summation order of BounceLight (the CPU
  * restatement takes the same values):
  *   "bounce_kernel"      2 = LDS stream gather (default), 0 = LDS-tiled
@@ -125,12 +129,13 @@ int vr_scene_create(vr_scene **out, const float *verts9, const int32_t *ids,
  * SAH kd-tree after RefineNode/CalculateCostsOfSplit (:181-387) with the
  * costs of raytracer/kdtree/constants.go:25-28, then
  * ChangeIntoIntersectionFormat per triangle
- * (raytracer/cache/optimisedtriangle.go:30-78), upload to HBM.
- * n_threads <= 0 uses every host core. */
+ * (raytracer/cache/optimisedtriangle.go:30-78), upload to HBM.  The tree is
+ * built on the GPU or on n_threads host threads (tuning knob "kd_build"),
+ * the same tree either way.  n_threads <= 0 uses every host core. */
 int vr_scene_build(vr_scene *s, int32_t n_threads);
-/* The host half of vr_scene_build (kd-tree + intersection format, no GPU):
- * lets vr_scene_get_kdtree / vr_scene_get_triangles run on a host without a
- * device.  vr_scene_build calls it when it has not run. */
+/* vr_scene_build's kd-tree and intersection format on the host alone (no
+ * GPU): lets vr_scene_get_kdtree / vr_scene_get_triangles run on a host
+ * without a device.  A later vr_scene_build keeps its result. */
 int vr_scene_build_host(vr_scene *s, int32_t n_threads);
 
 typedef struct {
