@@ -198,6 +198,11 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
   // "while-while": a lane descends through inner nodes to its next leaf
   // before the wave tests leaves, so lanes test triangles together.  Each
   // lane visits the same nodes in the same order as one step per iteration.
+  // A step assigns each loop value once, by selects, and the push is a
+  // predicated store.  A step where neither child can hold a hit leaves the
+  // descent as an empty leaf: the pop that follows every leaf continues the
+  // walk, as a pop right there would.
+  constexpr uint2 kEmptyLeaf = {3u, 0u};  // leaf, no triangles
   for (;;) {
     uint2 nd = nodes[node];
     while ((nd.x & 3u) != 3u) {
@@ -215,18 +220,17 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
       const bool near_ok = ta <= fminf(near_end, tbe);
       const float fs = fmaxf(far_start, ta);
       const bool far_ok = fs <= tbe;
-      if (near_ok) {
-        if (far_ok && sp < D) {  // depth < D is checked on the host
-          st.push(sp, far_n, fs, tb);
-          ++sp;
-        }
-        node = near_n;
-        tb = fminf(tb, near_end);
-      } else if (far_ok) {
-        node = far_n;
-        ta = fs;
-      } else if (!pop()) {
-        return best != best0;
+      if (near_ok && far_ok && sp < D) {  // depth < D is checked on the host
+        st.push(sp, far_n, fs, tb);
+        ++sp;
+      }
+      node = near_ok ? near_n : far_n;
+      const float tb_near = fminf(tb, near_end);
+      ta = near_ok ? ta : fs;
+      tb = near_ok ? tb_near : tb;
+      if (!near_ok && !far_ok) {
+        nd = kEmptyLeaf;
+        break;
       }
       nd = nodes[node];
     }
