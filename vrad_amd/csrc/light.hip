@@ -101,12 +101,11 @@ __global__ __launch_bounds__(64) void k_gather_lds(DevScene sc, DevSamples s, in
                                                    int32_t n_lights, const float *__restrict__ dirs,
                                                    float eps, int32_t skip,
                                                    float *__restrict__ rgb) {
-  __shared__ uint32_t s_node[S * 64];
-  __shared__ float2 s_t[S * 64];
+  __shared__ uint4 s_stk[S * 64];
   const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (i >= n) return;
   StackSpill<S, D> spill;
-  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
+  HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
   float r, g, b;
   gather_one<D>(sc, st, s.px[i], s.py[i], s.pz[i], s.nx[i], s.ny[i], s.nz[i], lights, n_lights,
                 dirs, eps, skip, r, g, b);

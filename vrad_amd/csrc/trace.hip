@@ -2,7 +2,7 @@
 // (segment visibility) over the resident kd-tree.
 //
 // Optimised path: one wave per block, XCD-aware ray ranges, per-lane
-// traversal stacks with the top S entries in LDS (S x 12 B x 64 lanes per
+// traversal stacks with the top S entries in LDS (S x 16 B x 64 lanes per
 // wave) and the rest in scratch.  Plain path: one thread per ray, 256-thread
 // blocks, the whole stack in private scratch memory.
 #include "kernels.h"
@@ -16,11 +16,10 @@ __global__ __launch_bounds__(64) void k_trace_closest_lds(DevScene sc, DevRays r
                                                           int32_t skip, int32_t *__restrict__ hit,
                                                           float *__restrict__ hit_t,
                                                           float *__restrict__ hit_n) {
-  __shared__ uint32_t s_node[S * 64];
-  __shared__ float2 s_t[S * 64];
+  __shared__ uint4 s_stk[S * 64];
   const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R + threadIdx.x;
   StackSpill<S, D> spill;
-  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
+  HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
   for (int q = 0; q < R; ++q) {
     const int64_t i = base + 64 * q;
     if (i >= n) return;
@@ -72,13 +71,12 @@ template <int S, int D, int R>
 __global__ __launch_bounds__(64) void k_test_line_lds(DevScene sc, DevSegs s, int64_t n,
                                                       int32_t skip, uint8_t *__restrict__ vis,
                                                       uint8_t *__restrict__ sky) {
-  __shared__ uint32_t s_node[S * 64];
-  __shared__ float2 s_t[S * 64];
+  __shared__ uint4 s_stk[S * 64];
   __shared__ int32_t s_q[R > 1 ? 64 * (R - 1) : 1];  // queued segments (offsets from base)
   const int lane = (int)threadIdx.x;
   const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R;
   StackSpill<S, D> spill;
-  HybridStack<S, D> st(s_node, s_t, lane, spill);
+  HybridStack<S, D> st(s_stk, lane, spill);
   auto walk = [&](int64_t i, uint32_t &occ) {
     bool hs;
     const bool v = test_line_cached<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i],

@@ -78,12 +78,13 @@ __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32
 }
 constexpr uint32_t kNoRef = 0xffffffffu;
 
-// Per-lane traversal stack: the first S entries in LDS (entry e of lane l at
-// [e * 64 + l], conflict-free), deeper entries in private scratch.  S = D
+// Per-lane traversal stack: the first S entries in LDS as 16-byte records
+// (entry e of lane l at [e * 64 + l]: one ds_write_b128 / ds_read_b128 per
+// push / pop, conflict-free), deeper entries in private scratch.  S = D
 // keeps the whole stack in LDS; S = 0 is the plain path's scratch stack.
 // The deeper entries live in a separate object (StackSpill) the stack points
 // to: an array indexed at run time sits in private memory, and keeping it out
-// of this struct lets the LDS pointers and the lane stay in registers.
+// of this struct lets the LDS pointer stay in a register.
 template <int S, int D>
 struct StackSpill {
   uint32_t n[D - S > 0 ? D - S : 1];
@@ -92,22 +93,17 @@ struct StackSpill {
 #define VR_PRIVATE __attribute__((address_space(5)))
 template <int S, int D>
 struct HybridStack {
-  uint32_t *node;
-  float2 *t;
-  int lane;
+  uint4 *e;  // this lane's entry 0 in LDS; entry k at e + 64 k
   VR_PRIVATE uint32_t *pn;  // private-memory pointers: scratch (not flat) accesses
   VR_PRIVATE float *pa, *pb;
-  __device__ HybridStack(uint32_t *node_, float2 *t_, int lane_, StackSpill<S, D> &sp)
-      : node(node_),
-        t(t_),
-        lane(lane_),
+  __device__ HybridStack(uint4 *lds, int lane, StackSpill<S, D> &sp)
+      : e(lds + lane),
         pn((VR_PRIVATE uint32_t *)sp.n),
         pa((VR_PRIVATE float *)sp.a),
         pb((VR_PRIVATE float *)sp.b) {}
   __device__ void push(int sp, uint32_t n, float a, float b) {
     if (S == D || sp < S) {
-      node[sp * 64 + lane] = n;
-      t[sp * 64 + lane] = make_float2(a, b);
+      e[64 * sp] = make_uint4(n, __float_as_uint(a), __float_as_uint(b), 0u);
     } else {
       pn[sp - S] = n;
       pa[sp - S] = a;
@@ -116,10 +112,10 @@ struct HybridStack {
   }
   __device__ void get(int sp, uint32_t &n, float &a, float &b) const {
     if (S == D || sp < S) {
-      n = node[sp * 64 + lane];
-      const float2 v = t[sp * 64 + lane];
-      a = v.x;
-      b = v.y;
+      const uint4 v = e[64 * sp];
+      n = v.x;
+      a = __uint_as_float(v.y);
+      b = __uint_as_float(v.z);
     } else {
       n = pn[sp - S];
       a = pa[sp - S];
@@ -220,7 +216,9 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
       const bool near_ok = ta <= fminf(near_end, tbe);
       const float fs = fmaxf(far_start, ta);
       const bool far_ok = fs <= tbe;
-      if (near_ok && far_ok && sp < D) {  // depth < D is checked on the host
+      // the stack holds at most one entry per level of the path to the
+      // current node, and the host checks tree depth < D
+      if (near_ok && far_ok) {
         st.push(sp, far_n, fs, tb);
         ++sp;
       }

@@ -88,15 +88,14 @@ __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p
                                                        uint32_t cap, float eps, int32_t skip,
                                                        uint32_t *__restrict__ bits,
                                                        int64_t words) {
-  __shared__ uint32_t s_node[S * 64];
-  __shared__ float2 s_t[S * 64];
+  __shared__ uint4 s_stk[S * 64];
   const uint32_t n = min(*count, cap);
   const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (k >= n) return;
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
   StackSpill<S, D> spill;
-  HybridStack<S, D> st(s_node, s_t, (int)threadIdx.x, spill);
+  HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
   bool sky;
   const bool vis = test_line<D>(sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
                                 p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
