@@ -347,6 +347,43 @@ int vr_server_get_stats(vr_server *sv, vr_server_stats *out);
 /* Finishes the queued requests, then stops the scheduler. */
 int vr_server_destroy(vr_server *sv);
 
+/* ---- patch generation ---------------------------------------------------- */
+#define VR_FACE_MAX_POINTS 16
+#define VR_FACE_SKY 1u    /* SURF_SKY: the face's patch is never subdivided */
+#define VR_FACE_NOCHOP 2u /* PreventSubdivision (SURF_NOCHOP, or NOLIGHT without LIGHT) */
+/* One face: its winding, plane normal and lightmap vectors (texinfo
+ * lightmapVecsLuxelsPerWorldUnits, xyz parts). */
+typedef struct {
+  int32_t n_points;
+  float points[VR_FACE_MAX_POINTS][3];
+  float normal[3];
+  float lux_s[3], lux_t[3];
+  float base_light[3]; /* BaseLightForFace: emitted light of a light surface */
+  uint32_t flags;      /* VR_FACE_* */
+} vr_face;
+/* One patch (common/types/patch.go): face, parent and children indices (-1
+ * none), centre, normal, area, chop (luxel widths), luxels per world unit,
+ * bounds and winding. */
+typedef struct {
+  int32_t face, parent, child1, child2;
+  float origin[3], normal[3];
+  float area, chop, lux_scale;
+  float mins[3], maxs[3];
+  int32_t n_points;
+  float points[VR_FACE_MAX_POINTS][3];
+} vr_patch;
+/* MakePatches / MakePatchForFace (rad/patches/build.go:21-68,
+ * rad/patches/face.go:30-197) then SubdividePatches (rad/patches/
+ * subdivide.go:22-128): a base patch per face of positive area (in face
+ * order), then each base patch's subdivision (SubdividePatch, ClipWinding-
+ * Epsilon, CreateChildPatch, WindingAreaAndBalancePoint, :166-438) appended
+ * in the order it creates children; leaves have child1 = -1.  max_chop /
+ * min_chop are maxChop / minChop (face.go:24-25, 4 and 4).  Runs on the GPU.
+ * Returns the patch count in *n_out; with out == NULL or cap too small only
+ * counts (VR_E_CAPACITY when out != NULL). */
+int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
+                    vr_patch *out, int64_t cap, int64_t *n_out);
+
 /* ---- procedural scenes (inputs for tests and benchmarks) --------------- */
 typedef struct vr_gen vr_gen;
 /* Closed room 1024x1024x256 with n_pillars seeded axis-aligned pillars,
@@ -366,6 +403,11 @@ int vr_gen_counts(const vr_gen *g, int64_t *n_tri, int64_t *n_patches,
 int vr_gen_copy(const vr_gen *g, float *verts9, int32_t *ids, uint8_t *flags,
                 float *patch_origin, float *patch_normal, float *patch_area,
                 float *patch_refl, vr_light *lights);
+/* The faces the generator's patches were chopped from (rectangles, in
+ * emission order), as vr_make_patches input with lightmap vectors of
+ * 1/luxel_size luxels per unit along the rectangle's edges; out may be NULL
+ * (count only). */
+int vr_gen_faces(const vr_gen *g, float luxel_size, vr_face *out, int64_t *n_faces);
 int vr_gen_destroy(vr_gen *g);
 
 #ifdef __cplusplus

@@ -21,6 +21,16 @@ LIGHT_DTYPE = np.dtype([("type", "<i4"), ("origin", "<f4", 3), ("intensity", "<f
                         ("exponent", "<f4"), ("start_fade", "<f4"), ("end_fade", "<f4"),
                         ("cap_distance", "<f4")])
 
+# patch generation records (layout of or_face_t / or_patch_t)
+FACE_DTYPE = np.dtype([("n_points", "<i4"), ("points", "<f4", (16, 3)), ("normal", "<f4", 3),
+                       ("lux_s", "<f4", 3), ("lux_t", "<f4", 3), ("base_light", "<f4", 3),
+                       ("flags", "<u4")])
+PATCH_DTYPE = np.dtype([("face", "<i4"), ("parent", "<i4"), ("child1", "<i4"), ("child2", "<i4"),
+                        ("origin", "<f4", 3), ("normal", "<f4", 3), ("area", "<f4"),
+                        ("chop", "<f4"), ("lux_scale", "<f4"), ("mins", "<f4", 3),
+                        ("maxs", "<f4", 3), ("n_points", "<i4"), ("points", "<f4", (16, 3))])
+assert FACE_DTYPE.itemsize == 248 and PATCH_DTYPE.itemsize == 272
+
 FP = C.POINTER(C.c_float)
 P = C.c_void_p
 
@@ -85,6 +95,8 @@ def _load():
         "or_f16_to_f32": (C.c_float, [C.c_uint16]),
         "or_f32_to_bf16": (C.c_uint16, [C.c_float]),
         "or_bf16_to_f32": (C.c_float, [C.c_uint16]),
+        "or_make_patches": (C.c_int, [P, C.c_int64, C.c_float, C.c_float, P, C.c_int64,
+                                      C.POINTER(C.c_int64)]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -294,3 +306,17 @@ def f32_to_f16(x):
 
 def f32_to_bf16(x):
     return lib.or_f32_to_bf16(float(x))
+
+
+def make_patches(faces, max_chop=4.0, min_chop=4.0):
+    """MakePatchForFace + SubdividePatches over faces (FACE_DTYPE): patches
+    (PATCH_DTYPE) in the reference's array order."""
+    f = np.ascontiguousarray(faces, FACE_DTYPE)
+    n = C.c_int64()
+    if lib.or_make_patches(_p(f), len(f), max_chop, min_chop, None, 0, C.byref(n)) != 0:
+        raise ValueError("a winding exceeds 16 points")
+    out = np.zeros(n.value, PATCH_DTYPE)
+    if lib.or_make_patches(_p(f), len(f), max_chop, min_chop, _p(out), len(out),
+                           C.byref(n)) != 0:
+        raise ValueError("a winding exceeds 16 points")
+    return out

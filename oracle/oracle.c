@@ -1368,3 +1368,290 @@ int or_bounce_f64(int64_t n, const int64_t *row_ptr, const int32_t *col,
   free(add);
   return k;
 }
+
+/* ---------------------------------------------------------------------------
+ * Patch generation.  MakePatchForFace (rad/patches/face.go:30-197), then
+ * SubdividePatches (rad/patches/subdivide.go:22-128): SubdividePatch
+ * (:166-267), ClipWindingEpsilon (:269-347), CreateChildPatch (:353-410),
+ * WindingAreaAndBalancePoint (:412-438), WindingArea / WindingBounds
+ * (vmath/polygon/winding.go:217-260), with mathgl's float32 vector ops
+ * (Sub, Cross, Len, Dot; vector.MA / Scale, vmath/vector) rounding once per
+ * operation.  Where the Go code does not do what it intends, the intent:
+ *   - WindingCenter (winding.go:231-239) writes a local pointer and divides by
+ *     an integer: the centre is the points' average, scaled by 1 / n;
+ *   - WindingAreaAndBalancePoint writes its centre to a local pointer: the
+ *     area-weighted centre is returned.
+ * A child's normal is its face's normal: GetPhongNormal
+ * (rad/lightmap/normallist.go:51-144) with no neighbouring face within the
+ * smoothing threshold.  "Zero base light" is mathgl's ApproxEqual to 0
+ * (every component of magnitude below 1e-20).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int n;
+  float p[OR_FACE_MAX_POINTS][3];
+} or_wind_t;
+
+static float pw_len3(const float v[3]) { return sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
+static void pw_cross(const float a[3], const float b[3], float o[3]) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static float pw_area(const or_wind_t *w) {
+  float total = 0.0f;
+  for (int i = 2; i < w->n; ++i) {
+    float d1[3], d2[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+      d1[k] = w->p[i - 1][k] - w->p[0][k];
+      d2[k] = w->p[i][k] - w->p[0][k];
+    }
+    pw_cross(d1, d2, c);
+    total += pw_len3(c);
+  }
+  return total * 0.5f;
+}
+static void pw_bounds(const or_wind_t *w, float mn[3], float mx[3]) {
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = 99999.0f;
+    mx[k] = -99999.0f;
+  }
+  for (int i = 0; i < w->n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      const float v = w->p[i][k];
+      if (v < mn[k]) mn[k] = v;
+      if (v > mx[k]) mx[k] = v;
+    }
+}
+static float pw_area_balance(const or_wind_t *w, float c[3]) {
+  float total = 0.0f;
+  c[0] = c[1] = c[2] = 0.0f;
+  if (!w) return 0.0f;
+  for (int i = 2; i < w->n; ++i) {
+    float d1[3], d2[3], x[3];
+    for (int k = 0; k < 3; ++k) {
+      d1[k] = w->p[i - 1][k] - w->p[0][k];
+      d2[k] = w->p[i][k] - w->p[0][k];
+    }
+    pw_cross(d1, d2, x);
+    const float area = pw_len3(x);
+    total += area;
+    const float s = area / 3.0f;
+    for (int k = 0; k < 3; ++k) c[k] = c[k] + s * w->p[i - 1][k];
+    for (int k = 0; k < 3; ++k) c[k] = c[k] + s * w->p[i][k];
+    for (int k = 0; k < 3; ++k) c[k] = c[k] + s * w->p[0][k];
+  }
+  if (total != 0.0f) {
+    const float inv = 1.0f / total;
+    for (int k = 0; k < 3; ++k) c[k] = c[k] * inv;
+  }
+  return total * 0.5f;
+}
+/* ClipWindingEpsilon against the axis plane x[axis] = dist: has_f / has_b
+   say whether front / back windings exist.  Returns -1 past the point cap. */
+static int pw_clip(const or_wind_t *in, int axis, float dist, float eps, or_wind_t *f, int *has_f,
+                   or_wind_t *b, int *has_b) {
+  float dists[OR_FACE_MAX_POINTS + 4];
+  int sides[OR_FACE_MAX_POINTS + 4], counts[3] = {0, 0, 0};
+  const float nrm[3] = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+  int i;
+  for (i = 0; i < in->n; ++i) {
+    float dot = (in->p[i][0] * nrm[0] + in->p[i][1] * nrm[1]) + in->p[i][2] * nrm[2];
+    dot = dot - dist;
+    dists[i] = dot;
+    sides[i] = dot > eps ? 0 : (dot < -eps ? 1 : 2);
+    counts[sides[i]]++;
+  }
+  sides[i] = sides[0];
+  dists[i] = dists[0];
+  *has_f = *has_b = 0;
+  if (counts[0] == 0) {
+    *b = *in;
+    *has_b = 1;
+    return 0;
+  }
+  if (counts[1] == 0) {
+    *f = *in;
+    *has_f = 1;
+    return 0;
+  }
+  f->n = b->n = 0;
+  *has_f = *has_b = 1;
+  for (i = 0; i < in->n; ++i) {
+    const float *p1 = in->p[i];
+    if (f->n + 2 > OR_FACE_MAX_POINTS || b->n + 2 > OR_FACE_MAX_POINTS) return -1;
+    if (sides[i] == 2) {
+      memcpy(f->p[f->n++], p1, 12);
+      memcpy(b->p[b->n++], p1, 12);
+      continue;
+    }
+    if (sides[i] == 0) memcpy(f->p[f->n++], p1, 12);
+    if (sides[i] == 1) memcpy(b->p[b->n++], p1, 12);
+    if (sides[i + 1] == 2 || sides[i + 1] == sides[i]) continue;
+    const float *p2 = in->p[(i + 1) % in->n];
+    const float dot = dists[i] / (dists[i] - dists[i + 1]);
+    float mid[3];
+    for (int j = 0; j < 3; ++j) {
+      if (nrm[j] == 1.0f) mid[j] = dist;
+      else if (nrm[j] == -1.0f) mid[j] = -dist;
+      else mid[j] = p1[j] + dot * (p2[j] - p1[j]);
+    }
+    memcpy(f->p[f->n++], mid, 12);
+    memcpy(b->p[b->n++], mid, 12);
+  }
+  return 0;
+}
+
+typedef struct {
+  or_patch_t *out;
+  int64_t cap, n;
+  const or_face_t *faces;
+  float face_mn[3], face_mx[3]; /* of the face being subdivided */
+  float min_chop;
+  int err;
+} pw_ctx_t;
+
+static int pw_zero_light(const float v[3]) {
+  return fabsf(v[0]) < 1e-20f && fabsf(v[1]) < 1e-20f && fabsf(v[2]) < 1e-20f;
+}
+
+/* the patch record at index k is kept in *P (out may be too small) */
+static void pw_put(pw_ctx_t *c, int64_t k, const or_patch_t *P) {
+  if (k < c->cap) c->out[k] = *P;
+}
+
+static void pw_subdivide(pw_ctx_t *c, int64_t k, or_patch_t *P) {
+  const or_face_t *fc = &c->faces[P->face];
+  if (fc->flags & 1u) return; /* never subdivide sky patches */
+  float total[3];
+  float widest = -1.0f;
+  int wa = -1, should = 0;
+  for (int i = 0; i < 3; ++i) total[i] = (P->maxs[i] - P->mins[i]) * P->lux_scale;
+  for (int i = 0; i < 3; ++i) {
+    if (total[i] > widest) {
+      wa = i;
+      widest = total[i];
+    }
+    if (total[i] >= P->chop && total[i] >= c->min_chop) should = 1;
+  }
+  if (!should && wa != -1) {
+    if (total[wa] > total[(wa + 1) % 3] * 2.0f && total[wa] > total[(wa + 2) % 3] * 2.0f) {
+      if (P->chop > c->min_chop) {
+        should = 1;
+        const float h = P->chop / 2.0f;
+        P->chop = c->min_chop > h ? c->min_chop : h;
+        pw_put(c, k, P);
+      }
+    }
+  }
+  if (!should) return;
+  const float dist = (P->mins[wa] + P->maxs[wa]) * 0.5f;
+  or_wind_t w, o1, o2;
+  w.n = P->n_points;
+  memcpy(w.p, P->points, sizeof w.p);
+  int h1, h2;
+  if (pw_clip(&w, wa, dist, 0.1f, &o1, &h1, &o2, &h2)) {
+    c->err = 1;
+    return;
+  }
+  float c1[3], c2[3];
+  const float a1 = pw_area_balance(h1 ? &o1 : NULL, c1);
+  const float a2 = pw_area_balance(h2 ? &o2 : NULL, c2);
+  if (a1 == 0.0f || a2 == 0.0f) return;
+  or_patch_t ch[2];
+  const or_wind_t *ow[2] = {&o1, &o2};
+  const float *ca[2] = {c1, c2};
+  const float ar[2] = {a1, a2};
+  int64_t idx[2];
+  for (int q = 0; q < 2; ++q) {
+    or_patch_t *C = &ch[q];
+    *C = *P;
+    idx[q] = c->n++;
+    C->child1 = C->child2 = -1;
+    C->parent = (int32_t)k;
+    C->n_points = ow[q]->n;
+    memset(C->points, 0, sizeof C->points);
+    memcpy(C->points, ow[q]->p, 12 * (size_t)ow[q]->n);
+    C->area = ar[q];
+    memcpy(C->origin, ca[q], 12);
+    memcpy(C->normal, fc->normal, 12);
+    pw_bounds(ow[q], C->mins, C->maxs);
+    if (pw_zero_light(fc->base_light)) {
+      float t[3];
+      for (int i = 0; i < 3; ++i) t[i] = (C->maxs[i] - C->mins[i]) * C->lux_scale;
+      if (C->chop > c->min_chop && t[0] < C->chop && t[1] < C->chop && t[2] < C->chop) {
+        for (int i = 0; i < 3; ++i) {
+          if ((c->face_mx[i] == C->maxs[i] || c->face_mn[i] == C->mins[i]) && t[i] > c->min_chop) {
+            const float h = C->chop / 2.0f;
+            C->chop = c->min_chop > h ? c->min_chop : h;
+            break;
+          }
+        }
+      }
+    }
+    pw_put(c, idx[q], C);
+  }
+  P->child1 = (int32_t)idx[0];
+  P->child2 = (int32_t)idx[1];
+  pw_put(c, k, P);
+  pw_subdivide(c, idx[0], &ch[0]);
+  pw_subdivide(c, idx[1], &ch[1]);
+}
+
+int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, float min_chop,
+                    or_patch_t *out, int64_t cap, int64_t *n_out) {
+  pw_ctx_t c;
+  memset(&c, 0, sizeof c);
+  c.out = out;
+  c.cap = out ? cap : 0;
+  c.faces = faces;
+  c.min_chop = min_chop;
+  /* base patches, one per face of positive area */
+  or_patch_t *base = (or_patch_t *)malloc(sizeof(or_patch_t) * (size_t)(n_faces > 0 ? n_faces : 1));
+  int64_t nb = 0;
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const or_face_t *fc = &faces[f];
+    if (fc->n_points > OR_FACE_MAX_POINTS || fc->n_points < 0) {
+      free(base);
+      return -1;
+    }
+    or_wind_t w;
+    w.n = fc->n_points;
+    memcpy(w.p, fc->points, sizeof w.p);
+    const float area = pw_area(&w);
+    if (area <= 0.0f) continue;
+    or_patch_t *P = &base[nb];
+    memset(P, 0, sizeof *P);
+    P->face = (int32_t)f;
+    P->parent = P->child1 = P->child2 = -1;
+    float cs[2] = {0.0f, 0.0f};
+    for (int j = 0; j < 3; ++j) cs[0] += fc->lux_s[j] * fc->lux_s[j];
+    for (int j = 0; j < 3; ++j) cs[1] += fc->lux_t[j] * fc->lux_t[j];
+    cs[0] = sqrtf(cs[0]);
+    cs[1] = sqrtf(cs[1]);
+    P->lux_scale = (cs[0] + cs[1]) / 2.0f;
+    P->chop = max_chop;
+    P->area = area;
+    float ctr[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < w.n; ++i)
+      for (int k = 0; k < 3; ++k) ctr[k] = w.p[i][k] + ctr[k];
+    const float sc = 1.0f / (float)w.n;
+    for (int k = 0; k < 3; ++k) P->origin[k] = ctr[k] * sc;
+    memcpy(P->normal, fc->normal, 12);
+    pw_bounds(&w, P->mins, P->maxs);
+    P->n_points = w.n;
+    memcpy(P->points, w.p, 12 * (size_t)w.n);
+    ++nb;
+  }
+  c.n = nb;
+  for (int64_t i = 0; i < nb; ++i) pw_put(&c, i, &base[i]);
+  for (int64_t i = 0; i < nb; ++i) {
+    if (faces[base[i].face].flags & 2u) continue; /* PreventSubdivision */
+    memcpy(c.face_mn, base[i].mins, 12); /* the face's bounds (FaceMins / FaceMaxs) */
+    memcpy(c.face_mx, base[i].maxs, 12);
+    pw_subdivide(&c, i, &base[i]);
+    if (c.err) break;
+  }
+  free(base);
+  *n_out = c.n;
+  return c.err ? -1 : 0;
+}

@@ -147,6 +147,31 @@ int or_bounce_f64(int64_t n, const int64_t *row_ptr, const int32_t *col,
 /* fixed-tree FP64 reduction used by the energy kernels */
 double or_energy_reduce(const double *e, int64_t n);
 
+/* Patch generation: MakePatchForFace (rad/patches/face.go:30-197) and
+ * SubdividePatches (rad/patches/subdivide.go:22-438).  Layouts equal the
+ * product's vr_face / vr_patch. */
+#define OR_FACE_MAX_POINTS 16
+typedef struct {
+  int32_t n_points;
+  float points[OR_FACE_MAX_POINTS][3];
+  float normal[3];
+  float lux_s[3], lux_t[3];
+  float base_light[3];
+  uint32_t flags; /* 1 sky, 2 no subdivision */
+} or_face_t;
+typedef struct {
+  int32_t face, parent, child1, child2;
+  float origin[3], normal[3];
+  float area, chop, lux_scale;
+  float mins[3], maxs[3];
+  int32_t n_points;
+  float points[OR_FACE_MAX_POINTS][3];
+} or_patch_t;
+/* Writes up to cap patches, returns the count in *n_out (0 ok, -1 a winding
+ * past OR_FACE_MAX_POINTS). */
+int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, float min_chop,
+                    or_patch_t *out, int64_t cap, int64_t *n_out);
+
 /* FP16 / BF16 encode-decode (round to nearest even), used for the precision
  * form's coefficient storage. */
 uint16_t or_f32_to_f16(float f);

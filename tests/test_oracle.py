@@ -492,3 +492,96 @@ def test_half_conversions():
     wb = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
     gb = np.array([oracle.f32_to_bf16(v) for v in x], np.uint16)
     assert np.array_equal(gb, wb)
+
+
+# ---- patch generation (MakePatchForFace + SubdividePatches) ----------------
+def _quad_face(x0, y0, x1, y1, z=0.0, luxel=16.0, flags=0):
+    f = np.zeros(1, oracle.FACE_DTYPE)
+    f["n_points"] = 4
+    f["points"][0, :4] = [[x0, y0, z], [x1, y0, z], [x1, y1, z], [x0, y1, z]]
+    f["normal"] = [0, 0, 1]
+    f["lux_s"] = [1.0 / luxel, 0, 0]
+    f["lux_t"] = [0, 1.0 / luxel, 0]
+    f["flags"] = flags
+    return f
+
+
+def _check_patch_tree(p):
+    """Structure of the reference's patch array: children in pairs created in
+    depth-first order after every base patch; leaves tile their parent."""
+    for i, q in enumerate(p):
+        if q["child1"] >= 0:
+            c1, c2 = q["child1"], q["child2"]
+            assert c2 == c1 + 1 and c1 > i
+            assert p[c1]["parent"] == i and p[c2]["parent"] == i
+            assert np.isclose(p[c1]["area"] + p[c2]["area"], q["area"], rtol=1e-5)
+        else:
+            assert q["child2"] == -1
+
+
+def test_patches_closed_form_square():
+    # 256 x 256 at 16 units per luxel: 16 luxels wide; a patch is split while
+    # it spans >= 4 luxels (maxChop = minChop = 4), so leaves are 32 x 32
+    p = oracle.make_patches(_quad_face(0, 0, 256, 256))
+    leaves = p[p["child1"] < 0]
+    assert len(p) == 127 and len(leaves) == 64
+    assert np.all(leaves["area"] == 1024.0)
+    assert np.all(p["lux_scale"] == np.float32(1.0 / 16.0)) and np.all(p["chop"] == 4.0)
+    c = np.sort(np.round(leaves["origin"][:, :2], 3), axis=0)
+    assert np.allclose(np.unique(c[:, 0]), 16 + 32 * np.arange(8), atol=1e-3)
+    assert np.all(leaves["normal"] == [0, 0, 1])
+    _check_patch_tree(p)
+    # the first split is along x at 128 (the widest axis, x before y on a
+    # tie); the front side (x > 128) is child 1 and is subdivided first
+    assert p[0]["child1"] == 1 and p[1]["mins"][0] == 128 and p[2]["maxs"][0] == 128
+    assert p[3]["parent"] == 1
+
+
+def test_patches_special_faces():
+    f = np.concatenate([_quad_face(0, 0, 256, 256, flags=1),      # sky: never subdivided
+                        _quad_face(0, 0, 0, 256),                  # zero area: no patch
+                        _quad_face(0, 0, 256, 256, flags=2),       # PreventSubdivision
+                        _quad_face(0, 0, 60, 60)])                 # under 4 luxels: a leaf
+    p = oracle.make_patches(f)
+    assert len(p) == 3 and list(p["face"]) == [0, 2, 3]
+    assert np.all(p["child1"] == -1) and np.all(p["parent"] == -1)
+    # WindingCenter: the points' average
+    assert np.array_equal(p[2]["origin"], np.float32([30, 30, 0]))
+
+
+def test_patches_triangle_and_ramp():
+    f = np.zeros(2, oracle.FACE_DTYPE)
+    f["n_points"] = [3, 4]
+    f["points"][0, :3] = [[0, 0, 0], [300, 0, 0], [0, 200, 0]]
+    f["normal"][0] = [0, 0, 1]
+    s = np.float32(1 / np.sqrt(2))
+    f["points"][1, :4] = [[0, 0, 0], [200, 0, 200], [200, 100, 200], [0, 100, 0]]  # 45 degree ramp
+    f["normal"][1] = [-s, 0, s]
+    f["lux_s"] = [1 / 16, 0, 0]
+    f["lux_t"] = [0, 1 / 16, 0]
+    p = oracle.make_patches(f)
+    _check_patch_tree(p)
+    for face, area in ((0, 30000.0), (1, 200 * np.sqrt(2) * 100)):
+        leaves = p[(p["face"] == face) & (p["child1"] < 0)]
+        assert len(leaves) > 4
+        assert np.isclose(leaves["area"].astype(np.float64).sum(), area, rtol=1e-5)
+        # children lie inside the face's bounds
+        base = p[(p["face"] == face) & (p["parent"] < 0)][0]
+        assert np.all(leaves["mins"] >= base["mins"] - 1e-3)
+        assert np.all(leaves["maxs"] <= base["maxs"] + 1e-3)
+
+
+def test_patches_generated_level_invariants():
+    from vrad_amd import scenes
+    f = scenes.faces_of("level", rooms_x=3, rooms_y=3, chop=64.0, tess=64.0, n_lights=2)
+    p = oracle.make_patches(f)
+    _check_patch_tree(p)
+    base = p[p["parent"] < 0]
+    leaves = p[p["child1"] < 0]
+    assert len(base) == len(f)
+    per_face = np.bincount(leaves["face"], weights=leaves["area"].astype(np.float64),
+                           minlength=len(f))
+    assert np.allclose(per_face, base["area"], rtol=1e-5)
+    # leaves are at most 4 luxels (64 units at 16 units per luxel) on a side
+    ext = (leaves["maxs"] - leaves["mins"]).max(1) * leaves["lux_scale"]
+    assert ext.max() < 4.0 + 1e-4

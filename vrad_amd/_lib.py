@@ -73,6 +73,17 @@ class Light(C.Structure):
                 ("cap_distance", C.c_float)]
 
 
+# vr_face / vr_patch records
+FACE_DTYPE = np.dtype([("n_points", "<i4"), ("points", "<f4", (16, 3)), ("normal", "<f4", 3),
+                       ("lux_s", "<f4", 3), ("lux_t", "<f4", 3), ("base_light", "<f4", 3),
+                       ("flags", "<u4")])
+PATCH_DTYPE = np.dtype([("face", "<i4"), ("parent", "<i4"), ("child1", "<i4"), ("child2", "<i4"),
+                        ("origin", "<f4", 3), ("normal", "<f4", 3), ("area", "<f4"),
+                        ("chop", "<f4"), ("lux_scale", "<f4"), ("mins", "<f4", 3),
+                        ("maxs", "<f4", 3), ("n_points", "<i4"), ("points", "<f4", (16, 3))])
+assert FACE_DTYPE.itemsize == 248 and PATCH_DTYPE.itemsize == 272
+VR_FACE_SKY, VR_FACE_NOCHOP = 1, 2
+
 LIGHT_DTYPE = np.dtype([("type", "<i4"), ("origin", "<f4", 3), ("intensity", "<f4", 3),
                         ("normal", "<f4", 3), ("constant_attn", "<f4"), ("linear_attn", "<f4"),
                         ("quadratic_attn", "<f4"), ("stopdot", "<f4"), ("stopdot2", "<f4"),
@@ -144,6 +155,9 @@ _SIGS = {
     "vr_gen_counts": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_int32)]),
     "vr_gen_copy": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vr_gen_faces": (C.c_int, [_P, C.c_float, _P, C.POINTER(C.c_int64)]),
+    "vr_make_patches": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, _P, C.c_int64,
+                                  C.POINTER(C.c_int64)]),
     "vr_gen_destroy": (C.c_int, [_P]),
     "vr_server_create": (C.c_int, [C.POINTER(_P), C.POINTER(ServerParams)]),
     "vr_server_add_scene": (C.c_int, [_P, _P, _P, C.POINTER(Samples), _P, _P, _P, C.c_int32,

@@ -1850,6 +1850,55 @@ int vr_gen_copy(const vr_gen *g, float *verts9, int32_t *ids, uint8_t *flags, fl
   return VR_OK;
 }
 
+int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
+                    vr_patch *out, int64_t cap, int64_t *n_out) {
+  LOCK;
+  if (!n_out) return fail(VR_E_INVALID, "n_out is NULL");
+  *n_out = 0;
+  if (n_faces < 0) return fail(VR_E_INVALID, "n_faces < 0");
+  if (n_faces > 0 && !faces) return fail(VR_E_INVALID, "faces is NULL");
+  if (n_faces >= ((int64_t)1 << 31)) return fail(VR_E_CAPACITY, "more than 2^31 faces");
+  if (!(max_chop > 0.0f) || !(min_chop > 0.0f))
+    return fail(VR_E_INVALID, "max_chop and min_chop must be > 0");
+  for (int64_t f = 0; f < n_faces; ++f)
+    if (faces[f].n_points < 0 || faces[f].n_points > VR_FACE_MAX_POINTS)
+      return fail(VR_E_INVALID, "face " + std::to_string(f) + ": n_points outside 0.." +
+                                    std::to_string(VR_FACE_MAX_POINTS));
+  TRY(ensure_init());
+  Prof p("make_patches", g_stream);
+  return make_patches_device(faces, n_faces, max_chop, min_chop, out, cap, n_out, g_stream);
+}
+
+int vr_gen_faces(const vr_gen *g, float luxel_size, vr_face *out, int64_t *n_faces) {
+  if (!g || !n_faces) return fail(VR_E_INVALID, "NULL argument");
+  if (!(luxel_size > 0.0f)) return fail(VR_E_INVALID, "luxel_size must be > 0");
+  const std::vector<double> &F = g->g.faces;
+  const int64_t n = (int64_t)(F.size() / 12);
+  *n_faces = n;
+  if (!out) return VR_OK;
+  for (int64_t k = 0; k < n; ++k) {
+    const double *f = &F[(size_t)(12 * k)];
+    vr_face &o = out[k];
+    std::memset(&o, 0, sizeof o);
+    o.n_points = 4;
+    for (int c = 0; c < 3; ++c) {
+      const double P = f[c], U = f[3 + c], V = f[6 + c];
+      o.points[0][c] = (float)P;
+      o.points[1][c] = (float)(P + U);
+      o.points[2][c] = (float)(P + U + V);
+      o.points[3][c] = (float)(P + V);
+      o.normal[c] = (float)f[9 + c];
+    }
+    const double lu = std::sqrt(f[3] * f[3] + f[4] * f[4] + f[5] * f[5]);
+    const double lv = std::sqrt(f[6] * f[6] + f[7] * f[7] + f[8] * f[8]);
+    for (int c = 0; c < 3; ++c) {
+      o.lux_s[c] = (float)(f[3 + c] / (lu * luxel_size));
+      o.lux_t[c] = (float)(f[6 + c] / (lv * luxel_size));
+    }
+  }
+  return VR_OK;
+}
+
 int vr_gen_destroy(vr_gen *g) {
   delete g;
   return VR_OK;

@@ -30,6 +30,12 @@ struct Tuning {
 };
 Tuning &tuning();
 
+// ---- patch generation (patches.hip) ----
+// MakePatchForFace + SubdividePatches on the GPU; *n_out = patch count,
+// written to out when cap allows (out == NULL: count only).
+int make_patches_device(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
+                        vr_patch *out, int64_t cap, int64_t *n_out, hipStream_t st);
+
 // ---- kd-tree construction on the device (kdbuild.hip) ----
 struct KdTree;
 // The host builder's tree (build_kdtree) built on the GPU on stream st and

@@ -1,0 +1,443 @@
+// Patch generation on the GPU: MakePatchForFace (rad/patches/face.go:30-197)
+// and SubdividePatches (rad/patches/subdivide.go:22-438).
+//
+// One thread per face runs the reference's depth-first subdivision of the
+// face's base patch (SubdividePatch recursing into child 1, then child 2),
+// holding the pending second children on a per-face stack in device memory.
+// A first pass counts each face's children; an exclusive scan of the counts
+// places every face's children after all base patches in face order, and a
+// second pass writes them: the reference's patch array order (base patches,
+// then each base patch's children in the order SubdividePatch creates
+// them).  Subdivision is control flow over a few points per patch, so a
+// thread per face (not a wave) is the unit of work.
+//
+// Float operations follow mathgl's float32 vector ops, one rounding each
+// (built with -ffp-contract=off), as the CPU restatement does.  Where the Go
+// code does not do what it intends, the intent: WindingCenter
+// (vmath/polygon/winding.go:231-239) averages the points (scale 1 / n), and
+// WindingAreaAndBalancePoint (subdivide.go:412-438) returns its area-weighted
+// centre.  A child's normal is its face's (GetPhongNormal,
+// rad/lightmap/normallist.go:51-144, with no neighbour within the smoothing
+// threshold).
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+#include "vr_internal.h"
+
+namespace vr {
+namespace {
+
+constexpr int kMaxPts = VR_FACE_MAX_POINTS;
+constexpr float kOnEpsilon = 0.1f;  // vmath.ON_EPSILON
+constexpr int kMaxDepth = 40;       // pending second children per face (tree depth)
+
+struct Wind {
+  int n;
+  float p[kMaxPts][3];
+};
+
+__device__ __forceinline__ float len3(const float v[3]) {
+  return __builtin_sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+}
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+// WindingArea (winding.go:217-229)
+__device__ float wind_area(const Wind &w) {
+  float total = 0.0f;
+  for (int i = 2; i < w.n; ++i) {
+    float d1[3], d2[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+      d1[k] = w.p[i - 1][k] - w.p[0][k];
+      d2[k] = w.p[i][k] - w.p[0][k];
+    }
+    cross3(d1, d2, c);
+    total += len3(c);
+  }
+  return total * 0.5f;
+}
+// WindingBounds (winding.go:241-260)
+__device__ void wind_bounds(const Wind &w, float mn[3], float mx[3]) {
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = 99999.0f;
+    mx[k] = -99999.0f;
+  }
+  for (int i = 0; i < w.n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      const float v = w.p[i][k];
+      if (v < mn[k]) mn[k] = v;
+      if (v > mx[k]) mx[k] = v;
+    }
+}
+// WindingAreaAndBalancePoint (subdivide.go:412-438); has = false: no winding
+__device__ float wind_area_balance(const Wind &w, bool has, float c[3]) {
+  float total = 0.0f;
+  c[0] = c[1] = c[2] = 0.0f;
+  if (!has) return 0.0f;
+  for (int i = 2; i < w.n; ++i) {
+    float d1[3], d2[3], x[3];
+    for (int k = 0; k < 3; ++k) {
+      d1[k] = w.p[i - 1][k] - w.p[0][k];
+      d2[k] = w.p[i][k] - w.p[0][k];
+    }
+    cross3(d1, d2, x);
+    const float area = len3(x);
+    total += area;
+    const float s = area / 3.0f;
+    for (int k = 0; k < 3; ++k) c[k] = c[k] + s * w.p[i - 1][k];
+    for (int k = 0; k < 3; ++k) c[k] = c[k] + s * w.p[i][k];
+    for (int k = 0; k < 3; ++k) c[k] = c[k] + s * w.p[0][k];
+  }
+  if (total != 0.0f) {
+    const float inv = 1.0f / total;
+    for (int k = 0; k < 3; ++k) c[k] = c[k] * inv;
+  }
+  return total * 0.5f;
+}
+// ClipWindingEpsilon (subdivide.go:269-347) against x[axis] = dist.  Returns
+// false past the point capacity.
+__device__ bool clip_axis(const Wind &in, int axis, float dist, Wind &f, bool &has_f, Wind &b,
+                          bool &has_b) {
+  float dists[kMaxPts + 1];
+  int sides[kMaxPts + 1], counts[3] = {0, 0, 0};
+  const float nrm[3] = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+  int i;
+  for (i = 0; i < in.n; ++i) {
+    float dot = (in.p[i][0] * nrm[0] + in.p[i][1] * nrm[1]) + in.p[i][2] * nrm[2];
+    dot = dot - dist;
+    dists[i] = dot;
+    sides[i] = dot > kOnEpsilon ? 0 : (dot < -kOnEpsilon ? 1 : 2);
+    counts[sides[i]]++;
+  }
+  sides[i] = sides[0];
+  dists[i] = dists[0];
+  has_f = has_b = false;
+  if (counts[0] == 0) {
+    b = in;
+    has_b = true;
+    return true;
+  }
+  if (counts[1] == 0) {
+    f = in;
+    has_f = true;
+    return true;
+  }
+  f.n = b.n = 0;
+  has_f = has_b = true;
+  for (i = 0; i < in.n; ++i) {
+    if (f.n + 2 > kMaxPts || b.n + 2 > kMaxPts) return false;
+    const float *p1 = in.p[i];
+    if (sides[i] == 2) {
+      for (int k = 0; k < 3; ++k) f.p[f.n][k] = b.p[b.n][k] = p1[k];
+      ++f.n;
+      ++b.n;
+      continue;
+    }
+    if (sides[i] == 0) {
+      for (int k = 0; k < 3; ++k) f.p[f.n][k] = p1[k];
+      ++f.n;
+    }
+    if (sides[i] == 1) {
+      for (int k = 0; k < 3; ++k) b.p[b.n][k] = p1[k];
+      ++b.n;
+    }
+    if (sides[i + 1] == 2 || sides[i + 1] == sides[i]) continue;
+    const float *p2 = in.p[(i + 1) % in.n];
+    const float dot = dists[i] / (dists[i] - dists[i + 1]);
+    for (int j = 0; j < 3; ++j) {
+      float m;
+      if (nrm[j] == 1.0f) m = dist;
+      else if (nrm[j] == -1.0f) m = -dist;
+      else m = p1[j] + dot * (p2[j] - p1[j]);
+      f.p[f.n][j] = m;
+      b.p[b.n][j] = m;
+    }
+    ++f.n;
+    ++b.n;
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool zero_light(const float v[3]) {
+  return fabsf(v[0]) < 1e-20f && fabsf(v[1]) < 1e-20f && fabsf(v[2]) < 1e-20f;
+}
+
+// MakePatchForFace (face.go:30-197): false when the face has no patch
+// (area <= 0).
+__device__ bool base_patch(const vr_face &fc, int32_t face, float max_chop, vr_patch &P) {
+  Wind w;
+  w.n = fc.n_points;
+  for (int i = 0; i < w.n; ++i)
+    for (int k = 0; k < 3; ++k) w.p[i][k] = fc.points[i][k];
+  const float area = wind_area(w);
+  if (area <= 0.0f) return false;
+  P.face = face;
+  P.parent = P.child1 = P.child2 = -1;
+  float cs0 = 0.0f, cs1 = 0.0f;
+  for (int j = 0; j < 3; ++j) cs0 += fc.lux_s[j] * fc.lux_s[j];
+  for (int j = 0; j < 3; ++j) cs1 += fc.lux_t[j] * fc.lux_t[j];
+  cs0 = __builtin_sqrtf(cs0);
+  cs1 = __builtin_sqrtf(cs1);
+  P.lux_scale = (cs0 + cs1) / 2.0f;
+  P.chop = max_chop;
+  P.area = area;
+  float c[3] = {0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < w.n; ++i)
+    for (int k = 0; k < 3; ++k) c[k] = w.p[i][k] + c[k];
+  const float sc = 1.0f / (float)w.n;
+  for (int k = 0; k < 3; ++k) P.origin[k] = c[k] * sc;
+  for (int k = 0; k < 3; ++k) P.normal[k] = fc.normal[k];
+  wind_bounds(w, P.mins, P.maxs);
+  P.n_points = w.n;
+  for (int i = 0; i < kMaxPts; ++i)
+    for (int k = 0; k < 3; ++k) P.points[i][k] = i < w.n ? w.p[i][k] : 0.0f;
+  return true;
+}
+
+struct Pending {  // a second child waiting for its subdivision
+  int64_t idx;
+  vr_patch p;
+};
+
+// The subdivision of one base patch (index bidx), children written (WRITE)
+// from out[cbase] on; returns the number of children, or -1 when a winding
+// or the stack overflows.
+template <bool WRITE>
+__device__ int64_t subdivide_face(const vr_face &fc, vr_patch P, int64_t bidx, int64_t cbase,
+                                  float min_chop, vr_patch *out, Pending *stk) {
+  const float face_mn[3] = {P.mins[0], P.mins[1], P.mins[2]};
+  const float face_mx[3] = {P.maxs[0], P.maxs[1], P.maxs[2]};
+  int64_t made = 0, k = bidx;
+  int sp = 0;
+  for (;;) {
+    // SubdividePatch(k) with the record P
+    bool split = false;
+    Wind o1, o2;
+    float c1[3], c2[3], a1 = 0.0f, a2 = 0.0f;
+    if (!(fc.flags & VR_FACE_SKY)) {
+      float total[3];
+      float widest = -1.0f;
+      int wa = -1;
+      bool should = false;
+      for (int i = 0; i < 3; ++i) total[i] = (P.maxs[i] - P.mins[i]) * P.lux_scale;
+      for (int i = 0; i < 3; ++i) {
+        if (total[i] > widest) {
+          wa = i;
+          widest = total[i];
+        }
+        if (total[i] >= P.chop && total[i] >= min_chop) should = true;
+      }
+      if (!should && wa != -1) {
+        if (total[wa] > total[(wa + 1) % 3] * 2.0f && total[wa] > total[(wa + 2) % 3] * 2.0f) {
+          if (P.chop > min_chop) {
+            should = true;
+            const float h = P.chop / 2.0f;
+            P.chop = min_chop > h ? min_chop : h;
+          }
+        }
+      }
+      if (should) {
+        const float dist = (P.mins[wa] + P.maxs[wa]) * 0.5f;
+        Wind w;
+        w.n = P.n_points;
+        for (int i = 0; i < w.n; ++i)
+          for (int q = 0; q < 3; ++q) w.p[i][q] = P.points[i][q];
+        bool h1, h2;
+        if (!clip_axis(w, wa, dist, o1, h1, o2, h2)) return -1;
+        a1 = wind_area_balance(o1, h1, c1);
+        a2 = wind_area_balance(o2, h2, c2);
+        split = !(a1 == 0.0f || a2 == 0.0f);
+      }
+    }
+    if (split) {
+      // CreateChildPatch (subdivide.go:353-410) twice
+      vr_patch ch[2];
+      const Wind *ow[2] = {&o1, &o2};
+      const float *ca[2] = {c1, c2};
+      const float ar[2] = {a1, a2};
+      for (int q = 0; q < 2; ++q) {
+        vr_patch &C = ch[q];
+        C = P;
+        C.child1 = C.child2 = -1;
+        C.parent = (int32_t)k;
+        C.n_points = ow[q]->n;
+        for (int i = 0; i < kMaxPts; ++i)
+          for (int z = 0; z < 3; ++z) C.points[i][z] = i < ow[q]->n ? ow[q]->p[i][z] : 0.0f;
+        C.area = ar[q];
+        for (int z = 0; z < 3; ++z) C.origin[z] = ca[q][z];
+        for (int z = 0; z < 3; ++z) C.normal[z] = fc.normal[z];
+        wind_bounds(*ow[q], C.mins, C.maxs);
+        if (zero_light(fc.base_light)) {
+          float t[3];
+          for (int i = 0; i < 3; ++i) t[i] = (C.maxs[i] - C.mins[i]) * C.lux_scale;
+          if (C.chop > min_chop && t[0] < C.chop && t[1] < C.chop && t[2] < C.chop) {
+            for (int i = 0; i < 3; ++i) {
+              if ((face_mx[i] == C.maxs[i] || face_mn[i] == C.mins[i]) && t[i] > min_chop) {
+                const float h = C.chop / 2.0f;
+                C.chop = min_chop > h ? min_chop : h;
+                break;
+              }
+            }
+          }
+        }
+      }
+      const int64_t i1 = cbase + made, i2 = cbase + made + 1;
+      made += 2;
+      P.child1 = (int32_t)i1;
+      P.child2 = (int32_t)i2;
+      if (WRITE) {
+        out[k] = P;
+        out[i1] = ch[0];
+        out[i2] = ch[1];
+      }
+      if (sp >= kMaxDepth) return -1;
+      stk[sp].idx = i2;
+      stk[sp].p = ch[1];
+      ++sp;
+      k = i1;  // child 1 next, child 2 after child 1's subtree
+      P = ch[0];
+      continue;
+    }
+    if (WRITE) out[k] = P;  // final state (its chop may have changed)
+    if (sp == 0) break;
+    --sp;
+    k = stk[sp].idx;
+    P = stk[sp].p;
+  }
+  return made;
+}
+
+// Pass 1: base[f] = 1 if the face has a patch; cnt[f] = its children
+// (-1: overflow).  Pass 2 (WRITE): the base patch at bofs[f], children from
+// cofs[f].
+template <bool WRITE>
+__global__ __launch_bounds__(64) void k_make_patches(const vr_face *__restrict__ faces,
+                                                     int64_t n_faces, float max_chop,
+                                                     float min_chop, int32_t *__restrict__ base,
+                                                     int64_t *__restrict__ cnt,
+                                                     const int64_t *__restrict__ bofs,
+                                                     const int64_t *__restrict__ cofs,
+                                                     vr_patch *__restrict__ out,
+                                                     Pending *__restrict__ stacks) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  const vr_face fc = faces[f];
+  if (fc.n_points < 0 || fc.n_points > kMaxPts) {
+    if (!WRITE) {
+      base[f] = 0;
+      cnt[f] = -1;
+    }
+    return;
+  }
+  vr_patch P;
+  const bool has = base_patch(fc, (int32_t)f, max_chop, P);
+  if (!WRITE) base[f] = has ? 1 : 0;
+  if (!has) {
+    if (!WRITE) cnt[f] = 0;
+    return;
+  }
+  Pending *stk = stacks + f * kMaxDepth;
+  if (fc.flags & VR_FACE_NOCHOP) {  // PreventSubdivision
+    if (WRITE) out[bofs[f]] = P;
+    else cnt[f] = 0;
+    return;
+  }
+  const int64_t m = subdivide_face<WRITE>(fc, P, WRITE ? bofs[f] : 0, WRITE ? cofs[f] : 0,
+                                          min_chop, out, stk);
+  if (!WRITE) cnt[f] = m;
+}
+
+}  // namespace
+
+int make_patches_device(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
+                        vr_patch *out, int64_t cap, int64_t *n_out, hipStream_t st) {
+  *n_out = 0;
+  if (n_faces == 0) return VR_OK;
+  void *d_faces = nullptr, *d_base = nullptr, *d_cnt = nullptr, *d_ofs = nullptr,
+       *d_out = nullptr, *d_stk = nullptr;
+  int rc = VR_OK;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (void *p : {d_faces, d_base, d_cnt, d_ofs, d_out, d_stk})
+      if (p) (void)hipFree(p);
+  };
+  auto H = [&](hipError_t e, const char *what) {
+    if (e == hipSuccess) return VR_OK;
+    (void)hipGetLastError();
+    return fail(VR_E_HIP, std::string("make patches: ") + what + ": " + hipGetErrorString(e));
+  };
+  const size_t fb = sizeof(vr_face) * (size_t)n_faces;
+  const size_t sb = sizeof(Pending) * kMaxDepth * (size_t)n_faces;
+  if ((rc = H(hipMalloc(&d_faces, fb), "hipMalloc")) || (rc = H(hipMalloc(&d_base, 4 * n_faces), "hipMalloc")) ||
+      (rc = H(hipMalloc(&d_cnt, 8 * n_faces), "hipMalloc")) ||
+      (rc = H(hipMalloc(&d_ofs, 16 * n_faces), "hipMalloc")) ||
+      (rc = H(hipMalloc(&d_stk, sb), "hipMalloc")) ||
+      (rc = H(hipMemcpyAsync(d_faces, faces, fb, hipMemcpyHostToDevice, st), "upload"))) {
+    release();
+    return rc == VR_E_HIP ? VR_E_NOMEM : rc;
+  }
+  const dim3 g((unsigned)((n_faces + 63) / 64));
+  hipLaunchKernelGGL(k_make_patches<false>, g, dim3(64), 0, st, (const vr_face *)d_faces, n_faces,
+                     max_chop, min_chop, (int32_t *)d_base, (int64_t *)d_cnt,
+                     (const int64_t *)nullptr, (const int64_t *)nullptr, (vr_patch *)nullptr,
+                     (Pending *)d_stk);
+  std::vector<int32_t> hb((size_t)n_faces);
+  std::vector<int64_t> hc((size_t)n_faces);
+  if ((rc = H(hipGetLastError(), "count launch")) ||
+      (rc = H(hipMemcpyAsync(hb.data(), d_base, 4 * n_faces, hipMemcpyDeviceToHost, st), "download")) ||
+      (rc = H(hipMemcpyAsync(hc.data(), d_cnt, 8 * n_faces, hipMemcpyDeviceToHost, st), "download")) ||
+      (rc = H(hipStreamSynchronize(st), "sync"))) {
+    release();
+    return rc;
+  }
+  // offsets: base patches in face order, then children in face order
+  std::vector<int64_t> ofs(2 * (size_t)n_faces);
+  int64_t nb = 0, nc = 0;
+  for (int64_t f = 0; f < n_faces; ++f) {
+    if (hc[(size_t)f] < 0) {
+      release();
+      return fail(VR_E_CAPACITY, "make patches: a winding or the subdivision stack overflows");
+    }
+    ofs[(size_t)f] = nb;
+    nb += hb[(size_t)f];
+  }
+  for (int64_t f = 0; f < n_faces; ++f) {
+    ofs[(size_t)(n_faces + f)] = nb + nc;
+    nc += hc[(size_t)f];
+  }
+  const int64_t total = nb + nc;
+  *n_out = total;
+  if (!out) {
+    release();
+    return VR_OK;
+  }
+  if (cap < total) {
+    release();
+    return fail(VR_E_CAPACITY, "make patches: output holds " + std::to_string(cap) + " of " +
+                                   std::to_string(total) + " patches");
+  }
+  const size_t ob = sizeof(vr_patch) * (size_t)std::max<int64_t>(total, 1);
+  if ((rc = H(hipMalloc(&d_out, ob), "hipMalloc")) ||
+      (rc = H(hipMemcpyAsync(d_ofs, ofs.data(), 16 * n_faces, hipMemcpyHostToDevice, st), "upload"))) {
+    release();
+    return rc;
+  }
+  hipLaunchKernelGGL(k_make_patches<true>, g, dim3(64), 0, st, (const vr_face *)d_faces, n_faces,
+                     max_chop, min_chop, (int32_t *)nullptr, (int64_t *)nullptr,
+                     (const int64_t *)d_ofs, (const int64_t *)d_ofs + n_faces, (vr_patch *)d_out,
+                     (Pending *)d_stk);
+  if ((rc = H(hipGetLastError(), "fill launch")) ||
+      (rc = H(hipMemcpyAsync(out, d_out, sizeof(vr_patch) * (size_t)total, hipMemcpyDeviceToHost, st),
+              "download")) ||
+      (rc = H(hipStreamSynchronize(st), "sync"))) {
+    release();
+    return rc;
+  }
+  release();
+  return VR_OK;
+}
+
+}  // namespace vr

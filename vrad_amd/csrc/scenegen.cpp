@@ -69,6 +69,10 @@ class Builder {
     quad(id, {mn.x, mx.y, mx.z}, {mx.x, mx.y, mx.z}, {mx.x, mx.y, mn.z}, {mn.x, mx.y, mn.z});
     quad(id, {mn.x, mn.y, mx.z}, {mx.x, mn.y, mx.z}, {mx.x, mn.y, mn.z}, {mn.x, mn.y, mn.z});
   }
+  void face(V3 P, V3 U, V3 V, V3 n) {
+    const double f[12] = {P.x, P.y, P.z, U.x, U.y, U.z, V.x, V.y, V.z, n.x, n.y, n.z};
+    g.faces.insert(g.faces.end(), f, f + 12);
+  }
   void patch(V3 o, V3 n, double area) {
     g.po.push_back((float)o.x);
     g.po.push_back((float)o.y);
@@ -97,6 +101,7 @@ class Builder {
         quad(id, c00, c10, c11, c01);
       }
     if (!patches) return;
+    face(P, U, V, n);
     int mu = std::max(1, (int)std::ceil(lu / chop - 1e-9)), mv = std::max(1, (int)std::ceil(lv / chop - 1e-9));
     double area = (lu / mu) * (lv / mv);
     for (int b = 0; b < mv; ++b)
@@ -114,6 +119,7 @@ class Builder {
 struct PatchOnly {
   Builder &b;
   void rect(V3 P, V3 U, V3 V, V3 n, double chop) {
+    b.face(P, U, V, n);
     double lu = len(U), lv = len(V);
     int mu = std::max(1, (int)std::ceil(lu / chop - 1e-9)), mv = std::max(1, (int)std::ceil(lv / chop - 1e-9));
     double area = (lu / mu) * (lv / mv);

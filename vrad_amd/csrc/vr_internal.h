@@ -47,6 +47,7 @@ struct GenScene {
   std::vector<uint8_t> flags;
   std::vector<float> po, pn, parea, prefl;  // patches
   std::vector<vr_light> lights;
+  std::vector<double> faces;  // per patch rectangle P + a U + b V: P, U, V, normal (12 each)
 };
 
 // ---- device-side records ---------------------------------------------------

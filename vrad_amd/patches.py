@@ -95,3 +95,24 @@ def MakeTransfers(env, origin, normal, area, normal_offset=ON_EPSILON, skip_id=-
     h = C.c_void_p()
     check(lib.vr_make_transfers(env.handle, C.byref(p), len(keep[-1]), C.byref(prm), C.byref(h)))
     return Transfers(h)
+
+
+MAX_CHOP = 4.0  # maxChop, rad/patches/face.go:24 (luxel widths)
+MIN_CHOP = 4.0  # minChop, rad/patches/face.go:25
+
+
+def MakePatches(faces, max_chop=MAX_CHOP, min_chop=MIN_CHOP):
+    """MakePatches / MakePatchForFace (rad/patches/build.go:21-68,
+    rad/patches/face.go:30-197) and SubdividePatches (rad/patches/subdivide.go
+    :22-438) on the GPU: faces is an array of _lib.FACE_DTYPE; returns the
+    patches (_lib.PATCH_DTYPE) in the reference's array order: a base patch
+    per face of positive area, then each base patch's children in creation
+    order.  Leaves (child1 == -1) are the patches the bake uses."""
+    f = np.ascontiguousarray(faces, _lib.FACE_DTYPE)
+    n = C.c_int64()
+    check(lib.vr_make_patches(ptr(f), len(f), float(max_chop), float(min_chop), None, 0,
+                              C.byref(n)))
+    out = np.zeros(n.value, _lib.PATCH_DTYPE)
+    check(lib.vr_make_patches(ptr(f), len(f), float(max_chop), float(min_chop), ptr(out),
+                              len(out), C.byref(n)))
+    return out

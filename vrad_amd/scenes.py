@@ -47,3 +47,26 @@ def level(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=T
     check(lib.vr_gen_level(C.byref(h), seed, rooms_x, rooms_y, chop, tess, n_lights,
                            1 if sun else 0))
     return _take(h)
+
+
+def faces_of(kind="level", luxel_size=16.0, **kw):
+    """The rectangles a generated scene's patches are chopped from, as
+    vr_make_patches faces (lightmap vectors of 1/luxel_size along the edges)."""
+    h = C.c_void_p()
+    if kind == "room":
+        a = dict(seed=1, n_pillars=40, chop=64.0, tess=32.0)
+        a.update(kw)
+        check(lib.vr_gen_room(C.byref(h), a["seed"], a["n_pillars"], a["chop"], a["tess"]))
+    else:
+        a = dict(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
+        a.update(kw)
+        check(lib.vr_gen_level(C.byref(h), a["seed"], a["rooms_x"], a["rooms_y"], a["chop"],
+                               a["tess"], a["n_lights"], 1 if a["sun"] else 0))
+    try:
+        n = C.c_int64()
+        check(lib.vr_gen_faces(h, float(luxel_size), None, C.byref(n)))
+        out = np.zeros(n.value, _lib.FACE_DTYPE)
+        check(lib.vr_gen_faces(h, float(luxel_size), ptr(out), C.byref(n)))
+    finally:
+        lib.vr_gen_destroy(h)
+    return out
