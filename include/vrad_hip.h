@@ -384,6 +384,57 @@ typedef struct {
 int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
                     vr_patch *out, int64_t cap, int64_t *n_out);
 
+/* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- */
+/* The BSP lumps PointInLeaf reads (cache.GetLumpCache().Planes / Nodes /
+ * Leafs, rad/clustertable/point.go:14-36): a plane's normal and distance, a
+ * node's plane and children (child < 0: leaf -1-child), a leaf's cluster.
+ * Node 0 is the root; the nodes reachable from it must form a tree of depth
+ * <= VR_BSP_MAX_DEPTH. */
+#define VR_BSP_MAX_DEPTH 64
+typedef struct {
+  float normal[3];
+  float dist;
+} vr_bsp_plane;
+typedef struct {
+  int32_t planenum;
+  int32_t children[2];
+} vr_bsp_node;
+typedef struct {
+  const vr_bsp_plane *planes;
+  int64_t n_planes;
+  const vr_bsp_node *nodes;
+  int64_t n_nodes;
+  const int16_t *leaf_cluster;
+  int64_t n_leafs;
+} vr_bsp;
+/* ClusterFromPoint (rad/clustertable/point.go:10-36) per point: descend by
+ * dist = p.normal - plane.dist against TEST_EPSILON (vmath/constants.go:7);
+ * a point within the epsilon tries children[0] first and takes children[1]
+ * when that gives cluster -1.  xyz is 3n floats.  Runs on the GPU. */
+int vr_cluster_from_points(const vr_bsp *bsp, const float *xyz, int64_t n, int16_t *cluster);
+/* Patch.ClusterNumber as SubdividePatches assigns it (rad/patches/
+ * subdivide.go:100-115): the cluster of the patch origin, or when that is -1
+ * of the first winding point whose cluster is not -1.  Runs on the GPU. */
+int vr_patch_clusters(const vr_bsp *bsp, const vr_patch *patches, int64_t n, int32_t *cluster);
+/* GetVisCache / DecompressVis (rad/lightmap/vis.go:9-94) for every cluster:
+ * row c (row = (n_clusters + 7) >> 3 bytes at rows + c * row) is the
+ * run-length-decoded PVS at visdata[pvs_ofs[c]] (a nonzero byte is copied,
+ * a zero byte is followed by a count of zero bytes).  A run past the row is
+ * clipped (counted in *n_overrun, may be NULL); a zero count, an offset of -1
+ * or a read past vis_len is VR_E_INVALID.  Runs on the GPU. */
+int vr_decompress_vis(const uint8_t *visdata, int64_t vis_len, const int32_t *pvs_ofs,
+                      int32_t n_clusters, uint8_t *rows, int64_t *n_overrun);
+/* MakeTransfers with PVS culling of candidate pairs: a pair (i, j) is
+ * considered only when PVSCheck (rad/lightmap/lightmap.go:413-422) passes
+ * for it in either direction, patch_cluster[i] < 0 passing everything (the
+ * -1 rule of PVSCheck and GetVisCache).  pvs_rows is n_clusters rows as
+ * vr_decompress_vis writes them; n_clusters == 0 considers every pair.
+ * Otherwise as vr_make_transfers, whose result it equals whenever the PVS
+ * holds every visible pair. */
+int vr_make_transfers_pvs(vr_scene *s, const vr_patches *p, int64_t n_patches,
+                          const vr_xfer_params *params, const int32_t *patch_cluster,
+                          const uint8_t *pvs_rows, int32_t n_clusters, vr_xfers **out);
+
 /* ---- procedural scenes (inputs for tests and benchmarks) --------------- */
 typedef struct vr_gen vr_gen;
 /* Closed room 1024x1024x256 with n_pillars seeded axis-aligned pillars,
@@ -408,6 +459,11 @@ int vr_gen_copy(const vr_gen *g, float *verts9, int32_t *ids, uint8_t *flags,
  * 1/luxel_size luxels per unit along the rectangle's edges; out may be NULL
  * (count only). */
 int vr_gen_faces(const vr_gen *g, float luxel_size, vr_face *out, int64_t *n_faces);
+/* The generator's room grid: rooms span [xs[c], xs[c+1]] x [ys[r], ys[r+1]]
+ * (wall centre lines; a room scene is one cell 0..1024); room r * nx + c.
+ * xs / ys may be NULL (counts only: *nx and *ny are the cell counts, the
+ * arrays hold one more). */
+int vr_gen_grid(const vr_gen *g, double *xs, int32_t *nx, double *ys, int32_t *ny);
 int vr_gen_destroy(vr_gen *g);
 
 #ifdef __cplusplus

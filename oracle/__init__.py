@@ -97,6 +97,12 @@ def _load():
         "or_bf16_to_f32": (C.c_float, [C.c_uint16]),
         "or_make_patches": (C.c_int, [P, C.c_int64, C.c_float, C.c_float, P, C.c_int64,
                                       C.POINTER(C.c_int64)]),
+        "or_cluster_from_point": (C.c_int16, [P, P, P, P]),
+        "or_patch_clusters": (None, [P, P, P, P, C.c_int64, P]),
+        "or_decompress_vis": (C.c_int, [P, C.c_int64, P, C.c_int32, P, C.POINTER(C.c_int64)]),
+        "or_make_transfers_pvs": (C.c_int, [C.POINTER(_Kd), P, C.c_int64, C.POINTER(_Patches),
+                                            C.c_int64, C.c_float, C.c_int32, C.c_int, C.c_int,
+                                            P, P, C.c_int32, C.POINTER(_Csr)]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -209,14 +215,18 @@ def testline_stats(kd, tris, s, e, skip=-1, threads=1):
     return vis, int(cnt[0]), int(cnt[1])
 
 
-def make_transfers(kd, tris, origin, normal, area, eps=0.1, skip=-1, normalize=True, threads=1):
+def make_transfers(kd, tris, origin, normal, area, eps=0.1, skip=-1, normalize=True, threads=1,
+                   cluster=None, pvs_rows=None):
     cols = _cols(origin, 3) + _cols(normal, 3) + [np.ascontiguousarray(area, np.float32)]
     pt = _Patches(*[_f(c) for c in cols])
     n = len(cols[0])
     csr = _Csr()
-    rc = lib.or_make_transfers(C.byref(kd.kd) if kd is not None else None, _p(tris), len(tris),
-                               C.byref(pt), n, eps, skip, 1 if normalize else 0, threads,
-                               C.byref(csr))
+    cl = None if cluster is None else np.ascontiguousarray(cluster, np.int32)
+    rows = None if pvs_rows is None else np.ascontiguousarray(pvs_rows, np.uint8)
+    rc = lib.or_make_transfers_pvs(C.byref(kd.kd) if kd is not None else None, _p(tris),
+                                   len(tris), C.byref(pt), n, eps, skip, 1 if normalize else 0,
+                                   threads, _p(cl), _p(rows), 0 if rows is None else len(rows),
+                                   C.byref(csr))
     if rc:
         raise MemoryError("oracle transfers failed")
     nnz = csr.nnz
@@ -320,3 +330,41 @@ def make_patches(faces, max_chop=4.0, min_chop=4.0):
                            C.byref(n)) != 0:
         raise ValueError("a winding exceeds 16 points")
     return out
+
+
+PLANE_DTYPE = np.dtype([("normal", "<f4", 3), ("dist", "<f4")])
+NODE_DTYPE = np.dtype([("planenum", "<i4"), ("children", "<i4", 2)])
+
+
+def cluster_from_points(planes, nodes, leaf_cluster, points):
+    """ClusterFromPoint per point (recursive PointInLeaf, as written)."""
+    pl = np.ascontiguousarray(planes, PLANE_DTYPE)
+    nd = np.ascontiguousarray(nodes, NODE_DTYPE)
+    lc = np.ascontiguousarray(leaf_cluster, np.int16)
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.zeros(len(pts), np.int16)
+    for k in range(len(pts)):
+        out[k] = lib.or_cluster_from_point(_p(pl), _p(nd), _p(lc), _p(pts[k]))
+    return out
+
+
+def patch_clusters(planes, nodes, leaf_cluster, patches):
+    pl = np.ascontiguousarray(planes, PLANE_DTYPE)
+    nd = np.ascontiguousarray(nodes, NODE_DTYPE)
+    lc = np.ascontiguousarray(leaf_cluster, np.int16)
+    p = np.ascontiguousarray(patches, PATCH_DTYPE)
+    out = np.zeros(len(p), np.int32)
+    lib.or_patch_clusters(_p(pl), _p(nd), _p(lc), _p(p), len(p), _p(out))
+    return out
+
+
+def decompress_vis(data, ofs, n_clusters):
+    """(rows, overruns); raises ValueError on a zero count or a short buffer."""
+    d = np.ascontiguousarray(data, np.uint8)
+    o = np.ascontiguousarray(ofs, np.int32)
+    rows = np.zeros((n_clusters, (n_clusters + 7) >> 3), np.uint8)
+    over = C.c_int64()
+    rc = lib.or_decompress_vis(_p(d), len(d), _p(o), n_clusters, _p(rows), C.byref(over))
+    if rc:
+        raise ValueError({-1: "zero repeat", -2: "data ends inside a row"}[rc])
+    return rows, over.value

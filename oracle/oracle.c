@@ -704,13 +704,26 @@ typedef struct {
   uint32_t *bits; /* np x W words, symmetric */
   int64_t W;
   float M;
+  const int32_t *cl;   /* PVS: cluster per patch, NULL = every pair */
+  const uint8_t *pvs;  /* n_clusters rows of (n_clusters + 7) >> 3 bytes */
+  int64_t pvs_row;
 } mt_ctx_t;
+
+/* PVSCheck (rad/lightmap/lightmap.go:413-422) on the row of cluster a; the
+ * row of a negative cluster is all ones (GetVisCache, rad/lightmap/vis.go:
+ * 11-32) and a negative target passes. */
+static int pvs_check(const mt_ctx_t *c, int32_t a, int32_t b) {
+  if (a < 0 || b < 0) return 1;
+  return (c->pvs[(int64_t)a * c->pvs_row + (b >> 3)] >> (b & 7)) & 1;
+}
 
 static void mt_range(void *vc, int64_t b, int64_t e) {
   mt_ctx_t *c = (mt_ctx_t *)vc;
   const or_patches_t *p = c->p;
   for (int64_t i = b; i < e; ++i) {
     for (int64_t j = i + 1; j < c->np; ++j) {
+      if (c->cl && !pvs_check(c, c->cl[i], c->cl[j]) && !pvs_check(c, c->cl[j], c->cl[i]))
+        continue;
       float dx = p->ox[j] - p->ox[i], dy = p->oy[j] - p->oy[i], dz = p->oz[j] - p->oz[i];
       float di = (p->nx[i] * dx + p->ny[i] * dy) + p->nz[i] * dz;
       float dj = -((p->nx[j] * dx + p->ny[j] * dy) + p->nz[j] * dz);
@@ -742,12 +755,21 @@ static inline float or_ff(const or_patches_t *p, int64_t i, int64_t j) {
 int or_make_transfers(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
                       const or_patches_t *p, int64_t np, float normal_offset,
                       int32_t skip_id, int normalize, int nthreads, or_csr_t *out) {
+  return or_make_transfers_pvs(kd, tris, ntri, p, np, normal_offset, skip_id, normalize, nthreads,
+                               NULL, NULL, 0, out);
+}
+
+int or_make_transfers_pvs(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                          const or_patches_t *p, int64_t np, float normal_offset,
+                          int32_t skip_id, int normalize, int nthreads, const int32_t *cluster,
+                          const uint8_t *pvs_rows, int32_t n_clusters, or_csr_t *out) {
   memset(out, 0, sizeof *out);
   int64_t W = (np + 31) / 32;
   uint32_t *bits = (uint32_t *)calloc((size_t)(np * W > 0 ? np * W : 1), sizeof(uint32_t));
   if (!bits) return -1;
   mt_ctx_t c = {kd, tris, ntri, p, np, normal_offset, skip_id, bits, W,
-                kd ? kd_scene_maxabs(kd) : 1.0f};
+                kd ? kd_scene_maxabs(kd) : 1.0f, n_clusters > 0 ? cluster : NULL, pvs_rows,
+                ((int64_t)n_clusters + 7) >> 3};
   /* rows have unequal work (j > i); interleave by splitting into many ranges */
   int64_t chunks = np;
   (void)chunks;
@@ -1654,4 +1676,88 @@ int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, flo
   free(base);
   *n_out = c.n;
   return c.err ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BSP clusters and PVS.                                                     */
+/* ------------------------------------------------------------------------ */
+/* PointInLeaf (rad/clustertable/point.go:14-36): dist = point.Dot(normal) -
+ * distance (mgl32 Vec3.Dot, left to right); in front of the plane by more
+ * than TEST_EPSILON (vmath/constants.go:7) -> children[0], behind by more
+ * -> children[1], otherwise children[0]'s leaf unless its cluster is -1. */
+#define OR_TEST_EPSILON 0.03125f
+static int16_t point_in_leaf(const or_plane_t *planes, const or_bspnode_t *nodes,
+                             const int16_t *leaf_cluster, int node, const float p[3]) {
+  if (node < 0) return leaf_cluster[-1 - node];
+  const or_bspnode_t *n = &nodes[node];
+  const or_plane_t *pl = &planes[n->planenum];
+  float dot = p[0] * pl->normal[0] + p[1] * pl->normal[1];
+  dot = dot + p[2] * pl->normal[2];
+  const float dist = dot - pl->dist;
+  if (dist > OR_TEST_EPSILON) return point_in_leaf(planes, nodes, leaf_cluster, n->children[0], p);
+  if (dist < -OR_TEST_EPSILON) return point_in_leaf(planes, nodes, leaf_cluster, n->children[1], p);
+  int16_t t = point_in_leaf(planes, nodes, leaf_cluster, n->children[0], p);
+  if (t != -1) return t;
+  return point_in_leaf(planes, nodes, leaf_cluster, n->children[1], p);
+}
+
+int16_t or_cluster_from_point(const or_plane_t *planes, const or_bspnode_t *nodes,
+                              const int16_t *leaf_cluster, const float p[3]) {
+  return point_in_leaf(planes, nodes, leaf_cluster, 0, p);
+}
+
+void or_patch_clusters(const or_plane_t *planes, const or_bspnode_t *nodes,
+                       const int16_t *leaf_cluster, const or_patch_t *patches, int64_t n,
+                       int32_t *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const or_patch_t *q = &patches[i];
+    int32_t c = or_cluster_from_point(planes, nodes, leaf_cluster, q->origin);
+    if (c == -1) {
+      for (int j = 0; j < q->n_points && j < OR_FACE_MAX_POINTS; ++j) {
+        const int32_t cj = or_cluster_from_point(planes, nodes, leaf_cluster, q->points[j]);
+        if (cj != -1) {
+          c = cj;
+          break;
+        }
+      }
+    }
+    out[i] = c;
+  }
+}
+
+/* DecompressVis (rad/lightmap/vis.go:53-94) as the Source run-length format
+ * defines it: the Go loop copies its input byte for byte, and its bound
+ * compares a byte value against the row length; the restatement takes the
+ * format its comments name. */
+int or_decompress_vis(const uint8_t *vis, int64_t vis_len, const int32_t *ofs, int32_t n_clusters,
+                      uint8_t *rows, int64_t *n_overrun) {
+  const int64_t row = ((int64_t)n_clusters + 7) >> 3;
+  int64_t over = 0;
+  int rc = 0;
+  memset(rows, 0, (size_t)(row * n_clusters));
+  for (int32_t cl = 0; cl < n_clusters && rc == 0; ++cl) {
+    uint8_t *out = rows + (int64_t)cl * row;
+    int64_t in = ofs[cl], o = 0;
+    do {
+      if (in >= vis_len) { rc = -2; break; }
+      if (vis[in]) {
+        out[o++] = vis[in++];
+        continue;
+      }
+      if (in + 1 >= vis_len) { rc = -2; break; }
+      int64_t c = vis[in + 1];
+      if (c == 0) { rc = -1; break; }
+      in += 2;
+      if (o + c > row) {
+        c = row - o;
+        ++over;
+      }
+      while (c) {
+        out[o++] = 0;
+        --c;
+      }
+    } while (o < row);
+  }
+  if (n_overrun) *n_overrun = over;
+  return rc;
 }

@@ -172,6 +172,34 @@ typedef struct {
 int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, float min_chop,
                     or_patch_t *out, int64_t cap, int64_t *n_out);
 
+/* ---- BSP clusters and PVS (rad/clustertable/point.go, rad/lightmap/vis.go) */
+typedef struct {
+  float normal[3];
+  float dist;
+} or_plane_t;
+typedef struct {
+  int32_t planenum;
+  int32_t children[2]; /* < 0: leaf -1-child */
+} or_bspnode_t;
+/* ClusterFromPoint / PointInLeaf from node 0 (recursive, as written). */
+int16_t or_cluster_from_point(const or_plane_t *planes, const or_bspnode_t *nodes,
+                              const int16_t *leaf_cluster, const float p[3]);
+/* Patch.ClusterNumber (rad/patches/subdivide.go:100-115). */
+void or_patch_clusters(const or_plane_t *planes, const or_bspnode_t *nodes,
+                       const int16_t *leaf_cluster, const or_patch_t *patches, int64_t n,
+                       int32_t *out);
+/* DecompressVis into n_clusters rows of (n_clusters + 7) >> 3 bytes; returns
+ * 0, -1 on a zero count, -2 on a read past vis_len; *n_overrun counts
+ * clipped runs. */
+int or_decompress_vis(const uint8_t *vis, int64_t vis_len, const int32_t *ofs, int32_t n_clusters,
+                      uint8_t *rows, int64_t *n_overrun);
+/* MakeTransfers restricted to pairs whose clusters pass PVSCheck in either
+ * direction (cluster < 0 passes); n_clusters == 0: every pair. */
+int or_make_transfers_pvs(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                          const or_patches_t *p, int64_t np, float normal_offset,
+                          int32_t skip_id, int normalize, int nthreads, const int32_t *cluster,
+                          const uint8_t *pvs_rows, int32_t n_clusters, or_csr_t *out);
+
 /* FP16 / BF16 encode-decode (round to nearest even), used for the precision
  * form's coefficient storage. */
 uint16_t or_f32_to_f16(float f);

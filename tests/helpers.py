@@ -71,3 +71,60 @@ def random_rays(rng, n, lo, hi):
     d = rng.normal(size=(n, 3))
     d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
     return o, d
+
+
+def random_bsp(rng, n_clusters=20, max_depth=10, p_leaf=0.15):
+    """A random BSP as (planes, nodes, leaf_cluster) structured arrays: half
+    the planes axis-aligned at integer distances, half arbitrary; leaves take
+    clusters in [-2, n_clusters) (-1 the most common negative)."""
+    import oracle
+    planes, nodes, leaves = [], [], []
+
+    def leaf():
+        u = rng.uniform()
+        leaves.append(-1 if u < 0.25 else (-2 if u < 0.3 else int(rng.integers(0, n_clusters))))
+        return -1 - (len(leaves) - 1)
+
+    def build(depth):
+        if depth > 1 and (depth >= max_depth or rng.uniform() < p_leaf):
+            return leaf()
+        if rng.uniform() < 0.5:
+            n = np.zeros(3, np.float32)
+            n[int(rng.integers(0, 3))] = rng.choice([-1.0, 1.0])
+            d = float(rng.integers(-40, 41))
+        else:
+            n = rng.normal(size=3)
+            n = (n / np.linalg.norm(n)).astype(np.float32)
+            d = float(np.float32(rng.uniform(-40, 40)))
+        planes.append((n, d))
+        k = len(nodes)
+        nodes.append([len(planes) - 1, 0, 0])
+        nodes[k][1] = build(depth + 1)
+        nodes[k][2] = build(depth + 1)
+        return k
+
+    build(1)
+    pl = np.zeros(len(planes), oracle.PLANE_DTYPE)
+    for k, (n, d) in enumerate(planes):
+        pl[k] = (n, d)
+    nd = np.zeros(len(nodes), oracle.NODE_DTYPE)
+    for k, (p, c0, c1) in enumerate(nodes):
+        nd[k] = (p, (c0, c1))
+    return pl, nd, np.array(leaves, np.int16)
+
+
+def bsp_points(rng, planes, n):
+    """Points in [-50, 50]^3, a third of them moved onto a random plane or to
+    within / just past TEST_EPSILON of it."""
+    p = rng.uniform(-50, 50, (n, 3)).astype(np.float32)
+    for k in range(0, n, 3):
+        pl = planes[int(rng.integers(0, len(planes)))]
+        nrm = pl["normal"].astype(np.float64)
+        off = rng.choice([0.0, 0.03125, -0.03125, 0.02, -0.03, 0.0313, -0.0313])
+        q = p[k].astype(np.float64)
+        q = q - (q @ nrm - pl["dist"] - off) * nrm
+        if np.count_nonzero(nrm) == 1:  # axis plane: exact coordinate
+            ax = int(np.flatnonzero(nrm)[0])
+            q[ax] = (pl["dist"] + off) * nrm[ax]
+        p[k] = q.astype(np.float32)
+    return p

@@ -84,6 +84,16 @@ PATCH_DTYPE = np.dtype([("face", "<i4"), ("parent", "<i4"), ("child1", "<i4"), (
 assert FACE_DTYPE.itemsize == 248 and PATCH_DTYPE.itemsize == 272
 VR_FACE_SKY, VR_FACE_NOCHOP = 1, 2
 
+# BSP lumps for ClusterFromPoint: vr_bsp_plane, vr_bsp_node
+PLANE_DTYPE = np.dtype([("normal", "<f4", 3), ("dist", "<f4")])
+NODE_DTYPE = np.dtype([("planenum", "<i4"), ("children", "<i4", 2)])
+VR_BSP_MAX_DEPTH = 64
+
+
+class Bsp(C.Structure):
+    _fields_ = [("planes", C.c_void_p), ("n_planes", C.c_int64), ("nodes", C.c_void_p),
+                ("n_nodes", C.c_int64), ("leaf_cluster", C.c_void_p), ("n_leafs", C.c_int64)]
+
 LIGHT_DTYPE = np.dtype([("type", "<i4"), ("origin", "<f4", 3), ("intensity", "<f4", 3),
                         ("normal", "<f4", 3), ("constant_attn", "<f4"), ("linear_attn", "<f4"),
                         ("quadratic_attn", "<f4"), ("stopdot", "<f4"), ("stopdot2", "<f4"),
@@ -158,7 +168,14 @@ _SIGS = {
     "vr_gen_faces": (C.c_int, [_P, C.c_float, _P, C.POINTER(C.c_int64)]),
     "vr_make_patches": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, _P, C.c_int64,
                                   C.POINTER(C.c_int64)]),
+    "vr_gen_grid": (C.c_int, [_P, _P, C.POINTER(C.c_int32), _P, C.POINTER(C.c_int32)]),
     "vr_gen_destroy": (C.c_int, [_P]),
+    "vr_cluster_from_points": (C.c_int, [C.POINTER(Bsp), _P, C.c_int64, _P]),
+    "vr_patch_clusters": (C.c_int, [C.POINTER(Bsp), _P, C.c_int64, _P]),
+    "vr_decompress_vis": (C.c_int, [_P, C.c_int64, _P, C.c_int32, _P, C.POINTER(C.c_int64)]),
+    "vr_make_transfers_pvs": (C.c_int, [_P, C.POINTER(Patches), C.c_int64,
+                                        C.POINTER(XferParams), _P, _P, C.c_int32,
+                                        C.POINTER(_P)]),
     "vr_server_create": (C.c_int, [C.POINTER(_P), C.POINTER(ServerParams)]),
     "vr_server_add_scene": (C.c_int, [_P, _P, _P, C.POINTER(Samples), _P, _P, _P, C.c_int32,
                                       C.POINTER(C.c_int32)]),

@@ -683,9 +683,59 @@ __global__ void k_add_count(const uint32_t *c, uint32_t cap, unsigned long long 
   *tot += (unsigned long long)min(*c, cap);
 }
 
-int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfer_params *prm,
-                      vr_xfers **out) {
-  LOCK;
+// The candidate-pair plan of a PVS: patches grouped by cluster, the group
+// pairs that see each other, and their block prefix (k_pair_cull_pvs).
+struct PvsPlan {
+  std::vector<int32_t> mem;
+  std::vector<vr::PvsEntry> tab;
+  std::vector<int64_t> blk_pre;
+  int64_t pairs = 0;
+};
+
+static int pvs_plan(const int32_t *cl, int64_t np, const uint8_t *rows, int32_t nc, PvsPlan *P) {
+  const int32_t ng = nc + 1;  // group nc: patches with a negative cluster
+  std::vector<int64_t> goff((size_t)ng + 1, 0);
+  for (int64_t i = 0; i < np; ++i) {
+    if (cl[i] >= nc) return fail(VR_E_INVALID, "patch " + std::to_string(i) + ": cluster >= n_clusters");
+    ++goff[(size_t)(cl[i] < 0 ? nc : cl[i]) + 1];
+  }
+  for (int32_t g = 0; g < ng; ++g) goff[(size_t)g + 1] += goff[(size_t)g];
+  P->mem.assign((size_t)np, 0);
+  {
+    std::vector<int64_t> at(goff.begin(), goff.end() - 1);
+    for (int64_t i = 0; i < np; ++i) P->mem[(size_t)at[(size_t)(cl[i] < 0 ? nc : cl[i])]++] = (int32_t)i;
+  }
+  const int64_t row = ((int64_t)nc + 7) >> 3;
+  auto bit = [&](int32_t a, int32_t b) { return (rows[(int64_t)a * row + (b >> 3)] >> (b & 7)) & 1; };
+  const int64_t bc = vr::pvs_block_cols();
+  P->tab.clear();
+  P->blk_pre.assign(1, 0);
+  P->pairs = 0;
+  for (int32_t a = 0; a < ng; ++a) {
+    const int64_t na = goff[(size_t)a + 1] - goff[(size_t)a];
+    if (na == 0) continue;
+    for (int32_t b = a; b < ng; ++b) {
+      const int64_t nb = goff[(size_t)b + 1] - goff[(size_t)b];
+      if (nb == 0) continue;
+      if (a < nc && b < nc && !bit(a, b) && !bit(b, a)) continue;
+      vr::PvsEntry e;
+      e.a_off = (int32_t)goff[(size_t)a];
+      e.a_n = (int32_t)na;
+      e.b_off = (int32_t)goff[(size_t)b];
+      e.b_n = (int32_t)nb;
+      e.cpr = (int32_t)((nb + bc - 1) / bc);
+      e.diag = a == b;
+      P->tab.push_back(e);
+      P->blk_pre.push_back(P->blk_pre.back() + na * e.cpr);
+      P->pairs += a == b ? na * (na - 1) / 2 : na * nb;
+    }
+  }
+  return VR_OK;
+}
+
+static int make_transfers_impl(vr_scene *s, const vr_patches *p, int64_t np,
+                               const vr_xfer_params *prm, const int32_t *cl,
+                               const uint8_t *pvs, int32_t nc, vr_xfers **out) {
   if (!s || !p || !out) return fail(VR_E_INVALID, "NULL argument");
   *out = nullptr;
   if (!s->built) return fail(VR_E_STATE, "scene not built");
@@ -695,8 +745,11 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfe
   TRY(ensure_init());
   vr_xfer_params P{0.1f, -1, 1, 0};
   if (prm) P = *prm;
-  const int64_t cap = std::max<int64_t>(P.batch_pairs > 0 ? P.batch_pairs : (int64_t)8 << 20,
-                                        std::max<int64_t>(np - 1, 1));
+  // a batch holds one row of the unculled enumeration, or one block of the
+  // PVS enumeration
+  const int64_t cap = std::max<int64_t>(
+      P.batch_pairs > 0 ? P.batch_pairs : (int64_t)8 << 20,
+      std::max<int64_t>(np - 1, nc > 0 ? (int64_t)vr::pvs_block_cols() : 1));
   if (cap >= (1ll << 31)) return fail(VR_E_INVALID, "batch_pairs too large");
   hipStream_t st = g_stream;
   std::unique_ptr<vr_xfers> x(new vr_xfers);
@@ -718,21 +771,47 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfe
   TRY(segtot.alloc(8));
   VR_HIP(hipMemsetAsync(segtot.p, 0, 8, st));
   {
-    Prof pr("make_transfers", st);
-    int64_t i0 = 0;
-    while (i0 + 1 < np) {
-      const int64_t rowlen = np - 1 - i0;
-      int64_t rows = std::max<int64_t>(1, cap / rowlen);
-      const int64_t i1 = std::min<int64_t>(np - 1, i0 + rows);
-      x->pairs += (2 * np - i0 - i1 - 1) * (i1 - i0) / 2;
-      VR_HIP(hipMemsetAsync(count.p, 0, 4, st));
-      launch_pair_cull(dp, np, i0, i1, pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap, st);
+    Prof pr(nc > 0 ? "make_transfers_pvs" : "make_transfers", st);
+    auto trace_batch = [&]() {
       launch_pair_trace(s->dev, dp, pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap,
                         P.normal_offset, P.skip_id, bits.as<uint32_t>(), words, g_path, st);
       hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, st, count.as<uint32_t>(),
                          (uint32_t)cap, segtot.as<unsigned long long>());
-      TRY(check_launch());
-      i0 = i1;
+      return check_launch();
+    };
+    if (nc > 0) {
+      PvsPlan plan;
+      TRY(pvs_plan(cl, np, pvs, nc, &plan));
+      x->pairs = plan.pairs;
+      const int64_t n_blk = plan.blk_pre.back();
+      if (n_blk > 0) {
+        DBuf mem, tab, pre;
+        TRY(mem.upload(plan.mem.data(), 4 * plan.mem.size(), st));
+        TRY(tab.upload(plan.tab.data(), sizeof(vr::PvsEntry) * plan.tab.size(), st));
+        TRY(pre.upload(plan.blk_pre.data(), 8 * plan.blk_pre.size(), st));
+        // a block keeps at most pvs_block_cols() pairs
+        const int64_t per = std::max<int64_t>(1, cap / vr::pvs_block_cols());
+        for (int64_t b0 = 0; b0 < n_blk; b0 += per) {
+          VR_HIP(hipMemsetAsync(count.p, 0, 4, st));
+          launch_pair_cull_pvs(dp, mem.as<int32_t>(), tab.as<vr::PvsEntry>(), pre.as<int64_t>(),
+                               (int32_t)plan.tab.size(), b0, std::min(per, n_blk - b0),
+                               pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap, st);
+          TRY(trace_batch());
+        }
+        VR_HIP(hipStreamSynchronize(st));  // mem / tab / pre are freed on return
+      }
+    } else {
+      int64_t i0 = 0;
+      while (i0 + 1 < np) {
+        const int64_t rowlen = np - 1 - i0;
+        int64_t rows = std::max<int64_t>(1, cap / rowlen);
+        const int64_t i1 = std::min<int64_t>(np - 1, i0 + rows);
+        x->pairs += (2 * np - i0 - i1 - 1) * (i1 - i0) / 2;
+        VR_HIP(hipMemsetAsync(count.p, 0, 4, st));
+        launch_pair_cull(dp, np, i0, i1, pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap, st);
+        TRY(trace_batch());
+        i0 = i1;
+      }
     }
     TRY(x->cnt.alloc(fb));
     launch_row_count(bits.as<uint32_t>(), np, words, x->cnt.as<int32_t>(), st);
@@ -760,6 +839,22 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfe
   }
   *out = x.release();
   return VR_OK;
+}
+
+int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfer_params *prm,
+                      vr_xfers **out) {
+  LOCK;
+  return make_transfers_impl(s, p, np, prm, nullptr, nullptr, 0, out);
+}
+
+int vr_make_transfers_pvs(vr_scene *s, const vr_patches *p, int64_t np,
+                          const vr_xfer_params *prm, const int32_t *patch_cluster,
+                          const uint8_t *pvs_rows, int32_t n_clusters, vr_xfers **out) {
+  LOCK;
+  if (n_clusters < 0) return fail(VR_E_INVALID, "n_clusters < 0");
+  if (n_clusters > 0 && np > 0 && (!patch_cluster || !pvs_rows))
+    return fail(VR_E_INVALID, "patch_cluster / pvs_rows is NULL");
+  return make_transfers_impl(s, p, np, prm, patch_cluster, pvs_rows, np > 0 ? n_clusters : 0, out);
 }
 
 int vr_xfers_from_csr(vr_xfers **out, int64_t n, const int64_t *row_ptr, const int32_t *col,
@@ -1869,6 +1964,141 @@ int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float
   return make_patches_device(faces, n_faces, max_chop, min_chop, out, cap, n_out, g_stream);
 }
 
+// Checks the BSP the kernels walk: every plane and child index reachable
+// from node 0 is in range, no node is reached twice (a tree, so every walk
+// ends), and no root-to-leaf path holds more than VR_BSP_MAX_DEPTH nodes (the
+// bound of the walk's stack).  Then uploads the three lumps.
+struct BspDev {
+  DBuf planes, nodes, leaves;
+  DevBsp dev{};
+};
+static int bsp_upload(const vr_bsp *b, BspDev *d, hipStream_t st) {
+  if (!b) return fail(VR_E_INVALID, "bsp is NULL");
+  if (b->n_nodes < 1 || b->n_planes < 1 || b->n_leafs < 0)
+    return fail(VR_E_INVALID, "BSP needs at least one node and one plane");
+  if (b->n_nodes >= (1ll << 31) || b->n_planes >= (1ll << 31) || b->n_leafs > (1ll << 31))
+    return fail(VR_E_INVALID, "BSP lump too large");
+  if (!b->planes || !b->nodes || (b->n_leafs > 0 && !b->leaf_cluster))
+    return fail(VR_E_INVALID, "BSP lump is NULL");
+  std::vector<uint8_t> seen((size_t)b->n_nodes, 0);
+  std::vector<std::pair<int32_t, int32_t>> stk{{0, 1}};
+  seen[0] = 1;
+  while (!stk.empty()) {
+    const auto [n, depth] = stk.back();
+    stk.pop_back();
+    const vr_bsp_node &nd = b->nodes[n];
+    if (nd.planenum < 0 || nd.planenum >= b->n_planes)
+      return fail(VR_E_INVALID, "node " + std::to_string(n) + ": plane index out of range");
+    for (int k = 0; k < 2; ++k) {
+      const int32_t c = nd.children[k];
+      if (c < 0) {
+        if ((int64_t)(-1 - (int64_t)c) >= b->n_leafs)
+          return fail(VR_E_INVALID, "node " + std::to_string(n) + ": leaf index out of range");
+        continue;
+      }
+      if (c >= b->n_nodes)
+        return fail(VR_E_INVALID, "node " + std::to_string(n) + ": child index out of range");
+      if (seen[(size_t)c]) return fail(VR_E_INVALID, "BSP nodes do not form a tree");
+      if (depth + 1 > VR_BSP_MAX_DEPTH)
+        return fail(VR_E_INVALID, "BSP deeper than " + std::to_string(VR_BSP_MAX_DEPTH));
+      seen[(size_t)c] = 1;
+      stk.push_back({c, depth + 1});
+    }
+  }
+  static_assert(sizeof(vr_bsp_plane) == 16 && sizeof(vr_bsp_node) == 12, "lump records");
+  TRY(d->planes.upload(b->planes, 16 * (size_t)b->n_planes, st));
+  TRY(d->nodes.upload(b->nodes, 12 * (size_t)b->n_nodes, st));
+  TRY(d->leaves.upload(b->leaf_cluster, 2 * (size_t)b->n_leafs, st));
+  d->dev.planes = d->planes.as<float4>();
+  d->dev.nodes = d->nodes.as<int>();
+  d->dev.leaf_cluster = d->leaves.as<int16_t>();
+  return VR_OK;
+}
+
+int vr_cluster_from_points(const vr_bsp *bsp, const float *xyz, int64_t n, int16_t *cluster) {
+  LOCK;
+  if (n < 0) return fail(VR_E_INVALID, "n < 0");
+  if (n > 0 && (!xyz || !cluster)) return fail(VR_E_INVALID, "xyz / cluster is NULL");
+  TRY(ensure_init());
+  hipStream_t st = g_stream;
+  BspDev d;
+  TRY(bsp_upload(bsp, &d, st));
+  if (n == 0) return VR_OK;
+  DBuf pts, out;
+  TRY(pts.upload(xyz, 12 * (size_t)n, st));
+  TRY(out.alloc(2 * (size_t)n));
+  {
+    Prof pr("cluster_from_points", st);
+    launch_point_cluster(d.dev, pts.as<float>(), n, out.as<int16_t>(), st);
+    TRY(check_launch());
+  }
+  VR_HIP(hipMemcpyAsync(cluster, out.p, 2 * (size_t)n, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
+}
+
+int vr_patch_clusters(const vr_bsp *bsp, const vr_patch *patches, int64_t n, int32_t *cluster) {
+  LOCK;
+  if (n < 0) return fail(VR_E_INVALID, "n < 0");
+  if (n > 0 && (!patches || !cluster)) return fail(VR_E_INVALID, "patches / cluster is NULL");
+  TRY(ensure_init());
+  hipStream_t st = g_stream;
+  BspDev d;
+  TRY(bsp_upload(bsp, &d, st));
+  if (n == 0) return VR_OK;
+  DBuf pp, out;
+  TRY(pp.upload(patches, sizeof(vr_patch) * (size_t)n, st));
+  TRY(out.alloc(4 * (size_t)n));
+  {
+    Prof pr("patch_clusters", st);
+    launch_patch_cluster(d.dev, pp.as<vr_patch>(), n, out.as<int32_t>(), st);
+    TRY(check_launch());
+  }
+  VR_HIP(hipMemcpyAsync(cluster, out.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
+}
+
+int vr_decompress_vis(const uint8_t *visdata, int64_t vis_len, const int32_t *pvs_ofs,
+                      int32_t n_clusters, uint8_t *rows, int64_t *n_overrun) {
+  LOCK;
+  if (n_overrun) *n_overrun = 0;
+  if (n_clusters < 0 || vis_len < 0) return fail(VR_E_INVALID, "negative size");
+  if (n_clusters == 0) return VR_OK;
+  if (!pvs_ofs || !rows || (vis_len > 0 && !visdata))
+    return fail(VR_E_INVALID, "visdata / pvs_ofs / rows is NULL");
+  for (int32_t c = 0; c < n_clusters; ++c) {
+    if (pvs_ofs[c] == -1) return fail(VR_E_INVALID, "cluster " + std::to_string(c) + ": visofs == -1");
+    if (pvs_ofs[c] < 0 || pvs_ofs[c] >= vis_len)
+      return fail(VR_E_INVALID, "cluster " + std::to_string(c) + ": PVS offset outside the vis data");
+  }
+  TRY(ensure_init());
+  hipStream_t st = g_stream;
+  const size_t row = ((size_t)n_clusters + 7) >> 3;
+  DBuf vis, ofs, out, flags;
+  TRY(vis.upload(visdata, (size_t)vis_len, st));
+  TRY(ofs.upload(pvs_ofs, 4 * (size_t)n_clusters, st));
+  TRY(out.alloc(row * (size_t)n_clusters));
+  TRY(flags.alloc(16));
+  VR_HIP(hipMemsetAsync(flags.p, 0, 16, st));
+  {
+    Prof pr("decompress_vis", st);
+    launch_decompress_vis(vis.as<uint8_t>(), vis_len, ofs.as<int32_t>(), n_clusters,
+                          out.as<uint8_t>(), flags.as<unsigned long long>(),
+                          flags.as<uint32_t>() + 2, st);
+    TRY(check_launch());
+  }
+  unsigned long long f[2] = {0, 0};
+  VR_HIP(hipMemcpyAsync(f, flags.p, 16, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipMemcpyAsync(rows, out.p, row * (size_t)n_clusters, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  const uint32_t err = (uint32_t)f[1];
+  if (err & 1u) return fail(VR_E_INVALID, "DecompressVis: 0 repeat");
+  if (err & 2u) return fail(VR_E_INVALID, "DecompressVis: run-length data ends inside a row");
+  if (n_overrun) *n_overrun = (int64_t)f[0];
+  return VR_OK;
+}
+
 int vr_gen_faces(const vr_gen *g, float luxel_size, vr_face *out, int64_t *n_faces) {
   if (!g || !n_faces) return fail(VR_E_INVALID, "NULL argument");
   if (!(luxel_size > 0.0f)) return fail(VR_E_INVALID, "luxel_size must be > 0");
@@ -1896,6 +2126,16 @@ int vr_gen_faces(const vr_gen *g, float luxel_size, vr_face *out, int64_t *n_fac
       o.lux_t[c] = (float)(f[6 + c] / (lv * luxel_size));
     }
   }
+  return VR_OK;
+}
+
+int vr_gen_grid(const vr_gen *g, double *xs, int32_t *nx, double *ys, int32_t *ny) {
+  if (!g || !nx || !ny) return fail(VR_E_INVALID, "NULL argument");
+  const std::vector<double> &X = g->g.grid_x, &Y = g->g.grid_y;
+  *nx = X.empty() ? 0 : (int32_t)X.size() - 1;
+  *ny = Y.empty() ? 0 : (int32_t)Y.size() - 1;
+  if (xs) std::copy(X.begin(), X.end(), xs);
+  if (ys) std::copy(Y.begin(), Y.end(), ys);
   return VR_OK;
 }
 

@@ -74,6 +74,29 @@ void launch_row_fill(const uint32_t *bits, int64_t np, int64_t words, const DevP
 void launch_row_normalize(int64_t n, const int64_t *rp, float *coef, double *row_sum,
                           int normalize, hipStream_t st);
 
+// ---- BSP clusters and PVS (pvs.hip) ----
+struct DevBsp {
+  const float4 *planes;         // normal xyz, dist
+  const int *nodes;             // planenum, children[0], children[1]
+  const int16_t *leaf_cluster;
+};
+// A pair of patch groups that see each other: |a| virtual rows of |b|
+// columns, cpr blocks per row; diag = the same group (columns > row only).
+struct PvsEntry {
+  int32_t a_off, a_n, b_off, b_n, cpr, diag;
+};
+void launch_point_cluster(const DevBsp &b, const float *xyz, int64_t n, int16_t *out,
+                          hipStream_t st);
+void launch_patch_cluster(const DevBsp &b, const vr_patch *p, int64_t n, int32_t *out,
+                          hipStream_t st);
+void launch_decompress_vis(const uint8_t *vis, int64_t vis_len, const int32_t *ofs, int32_t nc,
+                           uint8_t *rows, unsigned long long *overrun, uint32_t *err,
+                           hipStream_t st);
+int pvs_block_cols();  // columns one block of the PVS pair cull covers
+void launch_pair_cull_pvs(const DevPatches &p, const int32_t *mem, const PvsEntry *tab,
+                          const int64_t *blk_pre, int32_t n_tab, int64_t blk0, int64_t n_blk,
+                          int2 *pairs, uint32_t *count, uint32_t cap, hipStream_t st);
+
 // ---- bounce (bounce.hip) ----
 struct DevXfers {
   int64_t n;

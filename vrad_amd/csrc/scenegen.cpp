@@ -202,6 +202,8 @@ int gen_room(uint64_t seed, int n_pillars, double chop, double tess, GenScene *o
     if (!inside_xy(pil, lx, ly, 8)) break;
   }
   B.g.lights.push_back(point_light(lx, ly, lz, 10000.0f));
+  B.g.grid_x = {0.0, X};
+  B.g.grid_y = {0.0, Y};
   *out = std::move(B.g);
   return VR_OK;
 }
@@ -389,6 +391,8 @@ int gen_level(uint64_t seed, int rx, int ry, double chop, double tess, int n_lig
     L.cap_distance = 1.0e22f;
     B.g.lights.push_back(L);
   }
+  B.g.grid_x = Xs;
+  B.g.grid_y = Ys;
   *out = std::move(B.g);
   return VR_OK;
 }

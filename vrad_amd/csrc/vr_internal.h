@@ -48,6 +48,7 @@ struct GenScene {
   std::vector<float> po, pn, parea, prefl;  // patches
   std::vector<vr_light> lights;
   std::vector<double> faces;  // per patch rectangle P + a U + b V: P, U, V, normal (12 each)
+  std::vector<double> grid_x, grid_y;  // room cell boundaries (wall centre lines)
 };
 
 // ---- device-side records ---------------------------------------------------

@@ -87,13 +87,29 @@ def patch_struct(origin, normal, area):
 
 
 def MakeTransfers(env, origin, normal, area, normal_offset=ON_EPSILON, skip_id=-1,
-                  normalize=True, batch_pairs=0):
-    """Leaf patches (origin (n,3), normal (n,3), area (n,)) -> Transfers."""
+                  normalize=True, batch_pairs=0, cluster=None, pvs_rows=None):
+    """Leaf patches (origin (n,3), normal (n,3), area (n,)) -> Transfers.
+
+    With cluster (int32 per patch) and pvs_rows (the (n_clusters, row) PVS
+    rows of pvs.DecompressVis), only pairs whose clusters pass PVSCheck in
+    either direction are considered (a negative cluster passes)."""
     p, keep = patch_struct(origin, normal, area)
     prm = _lib.XferParams(float(normal_offset), int(skip_id), 1 if normalize else 0,
                           int(batch_pairs))
     h = C.c_void_p()
-    check(lib.vr_make_transfers(env.handle, C.byref(p), len(keep[-1]), C.byref(prm), C.byref(h)))
+    n = len(keep[-1])
+    if pvs_rows is None:
+        check(lib.vr_make_transfers(env.handle, C.byref(p), n, C.byref(prm), C.byref(h)))
+    else:
+        cl = np.ascontiguousarray(cluster, np.int32)
+        rows = np.ascontiguousarray(pvs_rows, np.uint8)
+        if cl.shape != (n,):
+            raise ValueError("cluster must hold one int per patch")
+        nc = rows.shape[0]
+        if rows.ndim != 2 or rows.shape[1] != (nc + 7) >> 3:
+            raise ValueError("pvs_rows must be (n_clusters, (n_clusters + 7) >> 3) bytes")
+        check(lib.vr_make_transfers_pvs(env.handle, C.byref(p), n, C.byref(prm), ptr(cl),
+                                        ptr(rows), nc, C.byref(h)))
     return Transfers(h)
 
 
