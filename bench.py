@@ -24,7 +24,10 @@ Also reported (same run):
   serving      re-bake requests (Poisson arrivals) against 32 resident scenes:
                p50/p99 latency and throughput of continuous batching vs the
                plain sequential re-bake, alone-vs-mix equality, FP16 luxel
-               error vs the FP64 recurrence.
+               error vs the FP64 recurrence;
+  pvs_culling  MakeTransfers with PVS culling of candidate pairs (room-grid
+               BSP, adjacency PVS) on the 4x4-room level against the unculled
+               run, and on the 8x8-room level.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -380,6 +383,67 @@ def run_room_bake(cx, args):
                                                                 otot.view(np.uint32)))}
 
 
+def run_pvs(cx, args):
+    """PVS culling of MakeTransfers candidate pairs on generated levels (room
+    grid BSP, each room's PVS = itself and the rooms one doorway away): the
+    4x4-room level unculled and culled, and the 8x8-room level culled."""
+    from vrad_amd import pvs, scenes
+    from vrad_amd.patches import MakeTransfers
+    from vrad_amd.raytracer import Environment
+
+    def level_case(rx, ry, unculled):
+        progress(f"PVS: {rx}x{ry}-room level")
+        s = scenes.level(seed=2, rooms_x=rx, rooms_y=ry, chop=32.0, tess=64.0)
+        env = Environment()
+        env.AddTriangles(s.verts9, s.ids)
+        env.SetupAccelerationStructure()
+        bsp, vis = scenes.grid_bsp(s.grid_x, s.grid_y, pvs_hops=1)
+        pvs.ClusterFromPoint(bsp, s.p_origin[:1])  # set-up outside the timing
+        t0 = time.perf_counter()
+        cl = pvs.ClusterFromPoint(bsp, s.p_origin).astype(np.int32)
+        t1 = time.perf_counter()
+        rows, _ = pvs.DecompressVis(vis)
+        t2 = time.perf_counter()
+        r = {"patches": int(len(s.p_area)), "clusters": int(vis.n_clusters),
+             "cluster_from_point_ms": round((t1 - t0) * 1e3, 2),
+             "decompress_vis_ms": round((t2 - t1) * 1e3, 2),
+             "patches_without_cluster": int((cl < 0).sum())}
+        t3 = time.perf_counter()
+        xp = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, cluster=cl, pvs_rows=rows,
+                           normalize=False)
+        ip = xp.info()
+        r["pvs"] = {"make_transfers_s": round(time.perf_counter() - t3, 3),
+                    "pairs_considered": int(ip.pairs_considered),
+                    "segments_traced": int(ip.segments_traced), "transfers": int(ip.nnz)}
+        if unculled:
+            t4 = time.perf_counter()
+            xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, normalize=False)
+            inf = xf.info()
+            r["unculled"] = {"make_transfers_s": round(time.perf_counter() - t4, 3),
+                             "pairs_considered": int(inf.pairs_considered),
+                             "segments_traced": int(inf.segments_traced),
+                             "transfers": int(inf.nnz)}
+            rp, col, coef = xf.download()
+            prp, pcol, pcoef = xp.download()
+            row = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+            a, b = cl[row], cl[col]
+            rv = np.unpackbits(rows, axis=1, bitorder="little")[:, :vis.n_clusters].astype(bool)
+            keep = rv[a, b] | rv[b, a]
+            r["pvs_result_equals_filtered_unculled"] = bool(
+                np.array_equal(pcol, col[keep]) and
+                np.array_equal(pcoef.view(np.uint32), coef[keep].view(np.uint32)))
+            r["transfers_outside_pvs"] = int((~keep).sum())
+            r["speedup"] = round(r["unculled"]["make_transfers_s"] / r["pvs"]["make_transfers_s"], 2)
+            xf.close()
+        xp.close()
+        env.close()
+        return r
+
+    return {"note": "PVS = each room and its grid neighbours (one doorway); normalize off so "
+                    "culled rows are the unculled rows minus the culled pairs",
+            "level_4x4": level_case(4, 4, True), "level_8x8": level_case(8, 8, False)}
+
+
 def run_batch_bake(cx, args):
     """The batching form: 64 procedural scenes of uneven size baked together.
     Per scene MakeTransfers and direct light, then every scene's FP32 bounces
@@ -587,6 +651,7 @@ def main():
     ap.add_argument("--no-room", action="store_true", help="skip the closed-room full bake")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched bake")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
+    ap.add_argument("--no-pvs", action="store_true", help="skip the PVS-culled MakeTransfers")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--serve-scenes", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -610,6 +675,7 @@ def main():
     room = None if args.no_room else run_room_bake(cx, args)
     batch = None if args.no_batch else run_batch_bake(cx, args)
     serving = None if args.no_serve else run_serving(cx, args)
+    pvs_out = None if args.no_pvs else run_pvs(cx, args)
 
     if args.calib_copy_mb > 0:
         torch = cx.torch
@@ -627,7 +693,7 @@ def main():
         "config": {"workload": "plain GPU path at scale: 8x8-room procedural level, "
                                "4M MakeTransfers TestLine segments per batch (row order)"},
         "roofline": None, "cpu_baseline": cpu, "visibility": None, "bounce": bounce,
-        "room_bake": room, "batch_bake": batch, "serving": serving,
+        "room_bake": room, "batch_bake": batch, "serving": serving, "pvs_culling": pvs_out,
     }
     if v is not None:
         n, info = v["n"], v["info"]
