@@ -384,6 +384,49 @@ typedef struct {
 int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
                     vr_patch *out, int64_t cap, int64_t *n_out);
 
+/* ---- luxel stage: lightmap extents and luxels ------------------------- */
+#define VR_FACE_NOLIGHT 4u /* SURF_NOLIGHT: no lightmap, like SURF_SKY */
+/* MAX_LIGHTMAP_DIM_WITHOUT_BORDER = MAX_DISP_LIGHTMAP_DIM_WITHOUT_BORDER
+ * (common/constants/constants.go:22,27): one limit for faces and
+ * displacements. */
+#define VR_LIGHTMAP_MAX_DIM 125
+#define VR_LM_UNLIT 1u      /* VR_FACE_SKY or VR_FACE_NOLIGHT: skipped, no luxels */
+#define VR_LM_TOO_BIG 2u    /* a side above VR_LIGHTMAP_MAX_DIM + 1 luxels: no luxels */
+#define VR_LM_NO_POINTS 4u  /* n_points < 1: no luxels */
+#define VR_LM_DEGENERATE 8u /* lightmap vectors and normal dependent: luxel positions 0 */
+/* A face's lightmap: LightmapTextureMinsInLuxels / SizeInLuxels and where
+ * its (size[0] + 1) x (size[1] + 1) luxels start (luxel (s, t) at index
+ * luxel_ofs + t * (size[0] + 1) + s). */
+typedef struct {
+  int32_t mins[2], size[2];
+  int64_t luxel_ofs;
+  int32_t n_luxels;
+  uint32_t flags; /* VR_LM_* */
+} vr_face_lightmap;
+/* CalcFaceExtents (rad/world/face.go:14-90) for every face, skipping sky and
+ * nolight faces as UpdateAllFaceLightmapExtents does (rad/start.go:102-
+ * 112): per point, val = p . lightmapVecs[j].xyz + lightmapVecs[j][3] in
+ * FP32; mins = floor, size = ceil(max) - floor(min).  lux_ofs holds the two
+ * offsets lightmapVecs[0][3], [1][3] per face (NULL: 0).  Luxel offsets are
+ * assigned in face order; *n_luxels is the total.  Runs on the GPU. */
+int vr_calc_face_extents(const vr_face *faces, int64_t n_faces, const float *lux_ofs,
+                         vr_face_lightmap *out, int64_t *n_luxels);
+/* The luxels of every face (patch-to-luxel interpolation, FinalLightFace,
+ * is absent in the reference): luxel (s, t) lies on the face plane at
+ * lightmap coordinates (mins[0] + s, mins[1] + t), solved in FP64; it takes
+ * the value of the first leaf patch of its face (patch order) whose
+ * winding, mapped to lightmap coordinates, contains it, else of the leaf
+ * patch whose centre is nearest in lightmap coordinates.  lm is
+ * vr_calc_face_extents's output for the same faces; patches as
+ * vr_make_patches writes them (face indices into faces).  Outputs per
+ * luxel, each may be NULL: pos 3, normal 3 (the face normal), patch 1 (-1:
+ * the face has no leaf patch), value 3 (patch_value[patch], 3 per patch;
+ * 0 without a patch).  Runs on the GPU. */
+int vr_face_luxels(const vr_face *faces, int64_t n_faces, const float *lux_ofs,
+                   const vr_face_lightmap *lm, const vr_patch *patches, int64_t n_patches,
+                   const float *patch_value, float *pos, float *normal, int32_t *patch,
+                   float *value);
+
 /* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- */
 /* The BSP lumps PointInLeaf reads (cache.GetLumpCache().Planes / Nodes /
  * Leafs, rad/clustertable/point.go:14-36): a plane's normal and distance, a

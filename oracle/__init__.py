@@ -97,6 +97,8 @@ def _load():
         "or_bf16_to_f32": (C.c_float, [C.c_uint16]),
         "or_make_patches": (C.c_int, [P, C.c_int64, C.c_float, C.c_float, P, C.c_int64,
                                       C.POINTER(C.c_int64)]),
+        "or_calc_face_extents": (C.c_int64, [P, C.c_int64, P, P]),
+        "or_face_luxels": (None, [P, C.c_int64, P, P, P, C.c_int64, P, P]),
         "or_cluster_from_point": (C.c_int16, [P, P, P, P]),
         "or_patch_clusters": (None, [P, P, P, P, C.c_int64, P]),
         "or_decompress_vis": (C.c_int, [P, C.c_int64, P, C.c_int32, P, C.POINTER(C.c_int64)]),
@@ -368,3 +370,29 @@ def decompress_vis(data, ofs, n_clusters):
     if rc:
         raise ValueError({-1: "zero repeat", -2: "data ends inside a row"}[rc])
     return rows, over.value
+
+
+FACE_LIGHTMAP_DTYPE = np.dtype([("mins", "<i4", 2), ("size", "<i4", 2), ("luxel_ofs", "<i8"),
+                                ("n_luxels", "<i4"), ("flags", "<u4")])
+
+
+def calc_face_extents(faces, lux_ofs=None):
+    """CalcFaceExtents per face: (FACE_LIGHTMAP_DTYPE records, luxel total)."""
+    f = np.ascontiguousarray(faces, FACE_DTYPE)
+    o = None if lux_ofs is None else np.ascontiguousarray(lux_ofs, np.float32).reshape(-1)
+    out = np.zeros(len(f), FACE_LIGHTMAP_DTYPE)
+    n = lib.or_calc_face_extents(_p(f), len(f), _p(o), _p(out))
+    return out, int(n)
+
+
+def face_luxels(faces, lm, patches, lux_ofs=None):
+    """(positions (n, 3), chosen patch per luxel)."""
+    f = np.ascontiguousarray(faces, FACE_DTYPE)
+    o = None if lux_ofs is None else np.ascontiguousarray(lux_ofs, np.float32).reshape(-1)
+    L = np.ascontiguousarray(lm, FACE_LIGHTMAP_DTYPE)
+    p = np.ascontiguousarray(patches, PATCH_DTYPE)
+    n = int(L["n_luxels"].sum())
+    pos = np.zeros((n, 3), np.float32)
+    pat = np.zeros(n, np.int32)
+    lib.or_face_luxels(_p(f), len(f), _p(o), _p(L), _p(p), len(p), _p(pos), _p(pat))
+    return pos, pat

@@ -1761,3 +1761,150 @@ int or_decompress_vis(const uint8_t *vis, int64_t vis_len, const int32_t *ofs, i
   if (n_overrun) *n_overrun = over;
   return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Luxel stage: CalcFaceExtents (rad/world/face.go:14-90; the Go function    */
+/* takes its two output arrays by value, so its results never reach the      */
+/* face: the restatement returns them) and the luxels of each face           */
+/* (FinalLightFace is absent in the reference; the rule is the build's).     */
+/* ------------------------------------------------------------------------ */
+static float lm_coord(const float *p, const float *l, float o) {
+  float v = p[0] * l[0];
+  v = v + p[1] * l[1];
+  v = v + p[2] * l[2];
+  return v + o;
+}
+
+int64_t or_calc_face_extents(const or_face_t *faces, int64_t n, const float *lux_ofs,
+                             or_face_lightmap_t *out) {
+  int64_t at = 0;
+  for (int64_t f = 0; f < n; ++f) {
+    const or_face_t *fc = &faces[f];
+    or_face_lightmap_t *r = &out[f];
+    memset(r, 0, sizeof *r);
+    r->luxel_ofs = at;
+    if (fc->flags & (1u | 4u)) {
+      r->flags = 1;
+      continue;
+    }
+    if (fc->n_points < 1 || fc->n_points > OR_FACE_MAX_POINTS) {
+      r->flags = 4;
+      continue;
+    }
+    const float o[2] = {lux_ofs ? lux_ofs[2 * f] : 0.0f, lux_ofs ? lux_ofs[2 * f + 1] : 0.0f};
+    float mn[2] = {1e24f, 1e24f}, mx[2] = {-1e24f, -1e24f};
+    for (int i = 0; i < fc->n_points; ++i)
+      for (int j = 0; j < 2; ++j) {
+        float v = lm_coord(fc->points[i], j ? fc->lux_t : fc->lux_s, o[j]);
+        if (v < mn[j]) mn[j] = v;
+        if (v > mx[j]) mx[j] = v;
+      }
+    int big = 0;
+    for (int i = 0; i < 2; ++i) {
+      float lo = (float)floor((double)mn[i]), hi = (float)ceil((double)mx[i]);
+      if (!(lo >= -2147483648.0f && hi <= 2147483520.0f)) {
+        big = 1;
+        continue;
+      }
+      r->mins[i] = (int32_t)lo;
+      float sz = hi - lo;
+      if (sz > 126.0f) big = 1; /* MAX_LIGHTMAP_DIM_WITHOUT_BORDER + 1 */
+      r->size[i] = sz < 2147483520.0f ? (int32_t)sz : 0;
+    }
+    if (big) {
+      r->flags = 2;
+      continue;
+    }
+    r->n_luxels = (r->size[0] + 1) * (r->size[1] + 1);
+    at += r->n_luxels;
+  }
+  return at;
+}
+
+/* LuxelSpaceToWorld by Cramer's rule in double, in this order. */
+static int lm_to_world(const or_face_t *fc, const float *o, float us, float ut, float *P) {
+  double a0 = fc->lux_s[0], a1 = fc->lux_s[1], a2 = fc->lux_s[2];
+  double b0 = fc->lux_t[0], b1 = fc->lux_t[1], b2 = fc->lux_t[2];
+  double n0 = fc->normal[0], n1 = fc->normal[1], n2 = fc->normal[2];
+  double bn0 = b1 * n2 - b2 * n1, bn1 = b2 * n0 - b0 * n2, bn2 = b0 * n1 - b1 * n0;
+  double na0 = n1 * a2 - n2 * a1, na1 = n2 * a0 - n0 * a2, na2 = n0 * a1 - n1 * a0;
+  double ab0 = a1 * b2 - a2 * b1, ab1 = a2 * b0 - a0 * b2, ab2 = a0 * b1 - a1 * b0;
+  double det = (a0 * bn0 + a1 * bn1) + a2 * bn2;
+  if (det == 0.0) {
+    P[0] = P[1] = P[2] = 0.0f;
+    return 0;
+  }
+  double d = (n0 * (double)fc->points[0][0] + n1 * (double)fc->points[0][1]) +
+             n2 * (double)fc->points[0][2];
+  double r0 = (double)us - (double)o[0], r1 = (double)ut - (double)o[1];
+  P[0] = (float)(((r0 * bn0 + r1 * na0) + d * ab0) / det);
+  P[1] = (float)(((r0 * bn1 + r1 * na1) + d * ab1) / det);
+  P[2] = (float)(((r0 * bn2 + r1 * na2) + d * ab2) / det);
+  return 1;
+}
+
+void or_face_luxels(const or_face_t *faces, int64_t n_faces, const float *lux_ofs,
+                    const or_face_lightmap_t *lm, const or_patch_t *patches, int64_t n_patches,
+                    float *pos, int32_t *patch) {
+  /* each face's leaf patches in patch order */
+  int64_t *lo = (int64_t *)calloc((size_t)n_faces + 1, sizeof(int64_t));
+  int64_t *li = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n_patches > 0 ? n_patches : 1));
+  int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n_faces > 0 ? n_faces : 1));
+  for (int64_t p = 0; p < n_patches; ++p)
+    if (patches[p].child1 == -1 && patches[p].face >= 0 && patches[p].face < n_faces)
+      ++lo[patches[p].face + 1];
+  for (int64_t f = 0; f < n_faces; ++f) lo[f + 1] += lo[f];
+  for (int64_t f = 0; f < n_faces; ++f) fill[f] = lo[f];
+  for (int64_t p = 0; p < n_patches; ++p)
+    if (patches[p].child1 == -1 && patches[p].face >= 0 && patches[p].face < n_faces)
+      li[fill[patches[p].face]++] = p;
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const or_face_t *fc = &faces[f];
+    const or_face_lightmap_t *L = &lm[f];
+    const float o[2] = {lux_ofs ? lux_ofs[2 * f] : 0.0f, lux_ofs ? lux_ofs[2 * f + 1] : 0.0f};
+    const int w = L->size[0] + 1;
+    for (int l = 0; l < L->n_luxels; ++l) {
+      const float us = (float)(L->mins[0] + l % w), ut = (float)(L->mins[1] + l / w);
+      const int64_t at = L->luxel_ofs + l;
+      if (pos) lm_to_world(fc, o, us, ut, pos + 3 * at);
+      int64_t found = -1, best = -1;
+      float bestd = 0.0f;
+      for (int64_t k = lo[f]; k < lo[f + 1] && found < 0; ++k) {
+        const int64_t p = li[k];
+        const or_patch_t *q = &patches[p];
+        int np = q->n_points < 0 ? 0 : (q->n_points > OR_FACE_MAX_POINTS ? OR_FACE_MAX_POINTS
+                                                                           : q->n_points);
+        if (np >= 3) {
+          int ps = 0, ns = 0;
+          for (int e = 0; e < np; ++e) {
+            int e1 = e + 1 == np ? 0 : e + 1;
+            float u0 = lm_coord(q->points[e], fc->lux_s, o[0]);
+            float v0 = lm_coord(q->points[e], fc->lux_t, o[1]);
+            float u1 = lm_coord(q->points[e1], fc->lux_s, o[0]);
+            float v1 = lm_coord(q->points[e1], fc->lux_t, o[1]);
+            float c1 = (u1 - u0) * (ut - v0);
+            float c2 = (v1 - v0) * (us - u0);
+            float cr = c1 - c2;
+            ps |= cr > 0.0f;
+            ns |= cr < 0.0f;
+          }
+          if ((ps || ns) && !(ps && ns)) {
+            found = p;
+            break;
+          }
+        }
+        float du = us - lm_coord(q->origin, fc->lux_s, o[0]);
+        float dv = ut - lm_coord(q->origin, fc->lux_t, o[1]);
+        float d2 = du * du + dv * dv;
+        if (best < 0 || d2 < bestd) {
+          best = p;
+          bestd = d2;
+        }
+      }
+      if (patch) patch[at] = (int32_t)(found >= 0 ? found : best);
+    }
+  }
+  free(lo);
+  free(li);
+  free(fill);
+}

@@ -172,6 +172,25 @@ typedef struct {
 int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, float min_chop,
                     or_patch_t *out, int64_t cap, int64_t *n_out);
 
+/* ---- luxel stage ---- */
+typedef struct {
+  int32_t mins[2], size[2];
+  int64_t luxel_ofs;
+  int32_t n_luxels;
+  uint32_t flags; /* 1 unlit, 2 too big, 4 no points, 8 degenerate lightmap vectors */
+} or_face_lightmap_t;
+/* CalcFaceExtents (rad/world/face.go:14-90) per face; face flags 1 (sky)
+ * and 4 (nolight) are skipped as UpdateAllFaceLightmapExtents does
+ * (rad/start.go:102-112).  Returns the luxel total. */
+int64_t or_calc_face_extents(const or_face_t *faces, int64_t n, const float *lux_ofs,
+                             or_face_lightmap_t *out);
+/* Luxel positions and the leaf patch each luxel takes (first containing
+ * winding in lightmap coordinates, else nearest centre); outputs may be
+ * NULL. */
+void or_face_luxels(const or_face_t *faces, int64_t n_faces, const float *lux_ofs,
+                    const or_face_lightmap_t *lm, const or_patch_t *patches, int64_t n_patches,
+                    float *pos, int32_t *patch);
+
 /* ---- BSP clusters and PVS (rad/clustertable/point.go, rad/lightmap/vis.go) */
 typedef struct {
   float normal[3];

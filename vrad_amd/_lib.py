@@ -84,6 +84,13 @@ PATCH_DTYPE = np.dtype([("face", "<i4"), ("parent", "<i4"), ("child1", "<i4"), (
 assert FACE_DTYPE.itemsize == 248 and PATCH_DTYPE.itemsize == 272
 VR_FACE_SKY, VR_FACE_NOCHOP = 1, 2
 
+VR_FACE_NOLIGHT = 4
+VR_LIGHTMAP_MAX_DIM = 125
+VR_LM_UNLIT, VR_LM_TOO_BIG, VR_LM_NO_POINTS, VR_LM_DEGENERATE = 1, 2, 4, 8
+FACE_LIGHTMAP_DTYPE = np.dtype([("mins", "<i4", 2), ("size", "<i4", 2), ("luxel_ofs", "<i8"),
+                                ("n_luxels", "<i4"), ("flags", "<u4")])
+assert FACE_LIGHTMAP_DTYPE.itemsize == 32
+
 # BSP lumps for ClusterFromPoint: vr_bsp_plane, vr_bsp_node
 PLANE_DTYPE = np.dtype([("normal", "<f4", 3), ("dist", "<f4")])
 NODE_DTYPE = np.dtype([("planenum", "<i4"), ("children", "<i4", 2)])
@@ -170,6 +177,8 @@ _SIGS = {
                                   C.POINTER(C.c_int64)]),
     "vr_gen_grid": (C.c_int, [_P, _P, C.POINTER(C.c_int32), _P, C.POINTER(C.c_int32)]),
     "vr_gen_destroy": (C.c_int, [_P]),
+    "vr_calc_face_extents": (C.c_int, [_P, C.c_int64, _P, _P, C.POINTER(C.c_int64)]),
+    "vr_face_luxels": (C.c_int, [_P, C.c_int64, _P, _P, _P, C.c_int64, _P, _P, _P, _P, _P]),
     "vr_cluster_from_points": (C.c_int, [C.POINTER(Bsp), _P, C.c_int64, _P]),
     "vr_patch_clusters": (C.c_int, [C.POINTER(Bsp), _P, C.c_int64, _P]),
     "vr_decompress_vis": (C.c_int, [_P, C.c_int64, _P, C.c_int32, _P, C.POINTER(C.c_int64)]),

@@ -2015,6 +2015,114 @@ static int bsp_upload(const vr_bsp *b, BspDev *d, hipStream_t st) {
   return VR_OK;
 }
 
+int vr_calc_face_extents(const vr_face *faces, int64_t n, const float *lux_ofs,
+                         vr_face_lightmap *out, int64_t *n_luxels) {
+  LOCK;
+  if (!n_luxels) return fail(VR_E_INVALID, "n_luxels is NULL");
+  *n_luxels = 0;
+  if (n < 0 || n >= (1ll << 31)) return fail(VR_E_INVALID, "bad face count");
+  if (n == 0) return VR_OK;
+  if (!faces || !out) return fail(VR_E_INVALID, "faces / out is NULL");
+  TRY(ensure_init());
+  static_assert(sizeof(vr_face_lightmap) == 32, "vr_face_lightmap layout");
+  hipStream_t st = g_stream;
+  DBuf df, dofs, dout;
+  TRY(df.upload(faces, sizeof(vr_face) * (size_t)n, st));
+  if (lux_ofs) TRY(dofs.upload(lux_ofs, 8 * (size_t)n, st));
+  TRY(dout.alloc(sizeof(vr_face_lightmap) * (size_t)n));
+  {
+    Prof pr("face_extents", st);
+    launch_face_extents(df.as<vr_face>(), n, lux_ofs ? dofs.as<float>() : nullptr,
+                        dout.as<vr_face_lightmap>(), st);
+    TRY(check_launch());
+  }
+  VR_HIP(hipMemcpyAsync(out, dout.p, sizeof(vr_face_lightmap) * (size_t)n, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  int64_t at = 0;
+  for (int64_t f = 0; f < n; ++f) {
+    out[f].luxel_ofs = at;
+    at += out[f].n_luxels;
+  }
+  *n_luxels = at;
+  return VR_OK;
+}
+
+int vr_face_luxels(const vr_face *faces, int64_t n_faces, const float *lux_ofs,
+                   const vr_face_lightmap *lm, const vr_patch *patches, int64_t n_patches,
+                   const float *patch_value, float *pos, float *normal, int32_t *patch,
+                   float *value) {
+  LOCK;
+  if (n_faces < 0 || n_faces >= (1ll << 31) || n_patches < 0 || n_patches >= (1ll << 31))
+    return fail(VR_E_INVALID, "bad face or patch count");
+  if (n_faces == 0) return VR_OK;
+  if (!faces || !lm) return fail(VR_E_INVALID, "faces / lm is NULL");
+  if (n_patches > 0 && !patches) return fail(VR_E_INVALID, "patches is NULL");
+  // the luxel layout must be the one vr_calc_face_extents assigns
+  int64_t total = 0;
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const vr_face_lightmap &L = lm[f];
+    const bool ok = L.n_luxels == 0 ||
+                    (L.mins[0] > -(1 << 30) && L.mins[0] < (1 << 30) && L.mins[1] > -(1 << 30) &&
+                     L.mins[1] < (1 << 30) && L.size[0] >= 0 && L.size[1] >= 0 &&
+                     L.size[0] <= VR_LIGHTMAP_MAX_DIM + 1 &&
+                     L.size[1] <= VR_LIGHTMAP_MAX_DIM + 1 &&
+                     L.n_luxels == (L.size[0] + 1) * (L.size[1] + 1));
+    if (!ok || L.n_luxels < 0 || L.luxel_ofs != total)
+      return fail(VR_E_INVALID, "face " + std::to_string(f) + ": lightmap record is not "
+                                "vr_calc_face_extents's layout");
+    total += L.n_luxels;
+  }
+  // each face's leaf patches, in patch order
+  std::vector<int32_t> leaf_ofs((size_t)n_faces + 1, 0), leaf_idx;
+  for (int64_t p = 0; p < n_patches; ++p) {
+    if (patches[p].child1 != -1) continue;
+    if (patches[p].face < 0 || patches[p].face >= n_faces)
+      return fail(VR_E_INVALID, "patch " + std::to_string(p) + ": face index out of range");
+    ++leaf_ofs[(size_t)patches[p].face + 1];
+  }
+  for (int64_t f = 0; f < n_faces; ++f) leaf_ofs[(size_t)f + 1] += leaf_ofs[(size_t)f];
+  leaf_idx.assign((size_t)leaf_ofs.back(), 0);
+  {
+    std::vector<int32_t> at(leaf_ofs.begin(), leaf_ofs.end() - 1);
+    for (int64_t p = 0; p < n_patches; ++p)
+      if (patches[p].child1 == -1) leaf_idx[(size_t)at[(size_t)patches[p].face]++] = (int32_t)p;
+  }
+  std::vector<int2> blocks;
+  for (int64_t f = 0; f < n_faces; ++f)
+    for (int32_t l = 0; l < lm[f].n_luxels; l += 256) blocks.push_back(make_int2((int)f, l));
+  if (total == 0) return VR_OK;
+  TRY(ensure_init());
+  hipStream_t st = g_stream;
+  DBuf df, dofs, dlm, dblk, dlo, dli, dp, dv, opos, onrm, opat, oval;
+  TRY(df.upload(faces, sizeof(vr_face) * (size_t)n_faces, st));
+  if (lux_ofs) TRY(dofs.upload(lux_ofs, 8 * (size_t)n_faces, st));
+  TRY(dlm.upload(lm, sizeof(vr_face_lightmap) * (size_t)n_faces, st));
+  TRY(dblk.upload(blocks.data(), sizeof(int2) * blocks.size(), st));
+  TRY(dlo.upload(leaf_ofs.data(), 4 * leaf_ofs.size(), st));
+  TRY(dli.upload(leaf_idx.data(), 4 * leaf_idx.size(), st));
+  TRY(dp.upload(patches, sizeof(vr_patch) * (size_t)n_patches, st));
+  if (patch_value && value) TRY(dv.upload(patch_value, 12 * (size_t)n_patches, st));
+  if (pos) TRY(opos.alloc(12 * (size_t)total));
+  if (normal) TRY(onrm.alloc(12 * (size_t)total));
+  if (patch) TRY(opat.alloc(4 * (size_t)total));
+  if (value) TRY(oval.alloc(12 * (size_t)total));
+  {
+    Prof pr("face_luxels", st);
+    launch_face_luxels(df.as<vr_face>(), lux_ofs ? dofs.as<float>() : nullptr,
+                       dlm.as<vr_face_lightmap>(), dblk.as<int2>(), (int64_t)blocks.size(),
+                       dlo.as<int32_t>(), dli.as<int32_t>(), dp.as<vr_patch>(),
+                       (patch_value && value) ? dv.as<float>() : nullptr, opos.as<float>(),
+                       onrm.as<float>(), opat.as<int32_t>(), oval.as<float>(), st);
+    TRY(check_launch());
+  }
+  if (pos) VR_HIP(hipMemcpyAsync(pos, opos.p, 12 * (size_t)total, hipMemcpyDeviceToHost, st));
+  if (normal) VR_HIP(hipMemcpyAsync(normal, onrm.p, 12 * (size_t)total, hipMemcpyDeviceToHost, st));
+  if (patch) VR_HIP(hipMemcpyAsync(patch, opat.p, 4 * (size_t)total, hipMemcpyDeviceToHost, st));
+  if (value) VR_HIP(hipMemcpyAsync(value, oval.p, 12 * (size_t)total, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
+}
+
 int vr_cluster_from_points(const vr_bsp *bsp, const float *xyz, int64_t n, int16_t *cluster) {
   LOCK;
   if (n < 0) return fail(VR_E_INVALID, "n < 0");

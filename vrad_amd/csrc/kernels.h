@@ -74,6 +74,17 @@ void launch_row_fill(const uint32_t *bits, int64_t np, int64_t words, const DevP
 void launch_row_normalize(int64_t n, const int64_t *rp, float *coef, double *row_sum,
                           int normalize, hipStream_t st);
 
+// ---- luxel stage (luxels.hip) ----
+void launch_face_extents(const vr_face *faces, int64_t n, const float *lux_ofs,
+                         vr_face_lightmap *out, hipStream_t st);
+// blocks: (face, first luxel) per 256-thread workgroup; leaf_ofs / leaf_idx:
+// each face's leaf patches in patch order
+void launch_face_luxels(const vr_face *faces, const float *lux_ofs, const vr_face_lightmap *lm,
+                        const int2 *blocks, int64_t n_blocks, const int32_t *leaf_ofs,
+                        const int32_t *leaf_idx, const vr_patch *patches,
+                        const float *patch_value, float *pos, float *nrm, int32_t *lpatch,
+                        float *value, hipStream_t st);
+
 // ---- BSP clusters and PVS (pvs.hip) ----
 struct DevBsp {
   const float4 *planes;         // normal xyz, dist
