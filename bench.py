@@ -27,7 +27,10 @@ Also reported (same run):
                error vs the FP64 recurrence;
   pvs_culling  MakeTransfers with PVS culling of candidate pairs (room-grid
                BSP, adjacency PVS) on the 4x4-room level against the unculled
-               run, and on the 8x8-room level.
+               run, and on the 8x8-room level;
+  luxel_stage  patch generation, lightmap extents and luxels of the 8x8-room
+               level's faces against the CPU restatement, and a whole bake
+               from faces to luxels on a 3x3-room level.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -444,6 +447,90 @@ def run_pvs(cx, args):
             "level_4x4": level_case(4, 4, True), "level_8x8": level_case(8, 8, False)}
 
 
+def run_luxel_stage(cx, args):
+    """The luxel stage on the 8x8-room level's faces (lightmap vectors of 16
+    units per luxel): patch generation, lightmap extents and the luxels of
+    every face, against the CPU restatement (one thread); then a whole bake
+    from faces to luxels on a 3x3-room level: patches, clusters, PVS,
+    MakeTransfers, direct light, bounces, FinalLightFace."""
+    import oracle
+    from vrad_amd import lightmap, luxels, pvs, scenes
+    from vrad_amd.patches import MakePatches, MakeTransfers
+    from vrad_amd.raytracer import Environment
+    progress("luxel stage: 8x8-room level faces")
+    faces = scenes.faces_of("level", **LEVEL)
+    MakePatches(faces[:1])  # set-up outside the timing
+    t0 = time.perf_counter()
+    patches = MakePatches(faces)
+    t1 = time.perf_counter()
+    lm, n_lux = luxels.CalcFaceExtents(faces)
+    t2 = time.perf_counter()
+    val = np.random.default_rng(0).uniform(0, 1, (len(patches), 3)).astype(np.float32)
+    lx = luxels.FaceLuxels(faces, lm, patches, val)
+    t3 = time.perf_counter()
+    c0 = time.perf_counter()
+    olm, _ = oracle.calc_face_extents(faces)
+    opos, opat = oracle.face_luxels(faces, olm, patches)
+    c1 = time.perf_counter()
+    out = {"faces": int(len(faces)), "patches": int(len(patches)),
+           "leaf_patches": int((patches["child1"] == -1).sum()), "luxels": int(n_lux),
+           "gpu_ms": {"make_patches": round((t1 - t0) * 1e3, 2),
+                      "face_extents": round((t2 - t1) * 1e3, 2),
+                      "face_luxels": round((t3 - t2) * 1e3, 2)},
+           "cpu_restatement_ms": round((c1 - c0) * 1e3, 1), "cpu_threads": 1,
+           "bitexact_vs_cpu": bool(lm.tobytes() == olm.tobytes()
+                                   and np.array_equal(lx["patch"], opat)
+                                   and np.array_equal(lx["pos"].view(np.uint32),
+                                                      opos.view(np.uint32))),
+           "note": "GPU times are whole calls from host arrays (upload, kernels, download)"}
+
+    progress("bake from faces to luxels: 3x3-room level")
+    kw = dict(seed=5, rooms_x=3, rooms_y=3, chop=32.0, tess=64.0, n_lights=4, sun=True)
+    s = scenes.level(**kw)
+    faces = scenes.faces_of("level", **kw)
+    T = {}
+    tb = time.perf_counter()
+    env = Environment()
+    env.AddTriangles(s.verts9, s.ids)
+    env.SetupAccelerationStructure()
+    T["scene_build"] = time.perf_counter() - tb
+    t = time.perf_counter()
+    patches = MakePatches(faces)
+    leaf = np.flatnonzero(patches["child1"] == -1)
+    T["make_patches"] = time.perf_counter() - t
+    t = time.perf_counter()
+    bsp, vis = scenes.grid_bsp(s.grid_x, s.grid_y, pvs_hops=1)
+    cl = pvs.PatchClusters(bsp, patches[leaf])
+    rows, _ = pvs.DecompressVis(vis)
+    T["clusters_and_pvs"] = time.perf_counter() - t
+    o, nrm, area = patches["origin"][leaf], patches["normal"][leaf], patches["area"][leaf]
+    t = time.perf_counter()
+    xf = MakeTransfers(env, o, nrm, area, cluster=cl, pvs_rows=rows)
+    T["make_transfers"] = time.perf_counter() - t
+    refl = np.full((len(leaf), 3), 0.5, np.float32)
+    t = time.perf_counter()
+    direct = lightmap.GatherSampleLight(env, o, nrm, s.lights)
+    T["direct"] = time.perf_counter() - t
+    t = time.perf_counter()
+    tot, _, k = lightmap.BounceLight(xf, (direct * refl).astype(np.float32), refl, area,
+                                     total=direct)
+    T["bounce"] = time.perf_counter() - t
+    pt = np.zeros((len(patches), 3), np.float32)
+    pd = np.zeros((len(patches), 3), np.float32)
+    pt[leaf], pd[leaf] = tot, direct
+    t = time.perf_counter()
+    flm, light, _ = luxels.FinalLightFace(env, faces, patches, pt, pd, s.lights)
+    T["final_light_face"] = time.perf_counter() - t
+    T["total"] = time.perf_counter() - tb
+    out["bake_3x3"] = {"faces": int(len(faces)), "leaf_patches": int(len(leaf)),
+                       "transfers": int(xf.info().nnz), "bounces": int(k),
+                       "luxels": int(len(light)), "ms": {a: round(b * 1e3, 2) for a, b in T.items()},
+                       "luxel_light_finite": bool(np.isfinite(light).all())}
+    xf.close()
+    env.close()
+    return out
+
+
 def run_batch_bake(cx, args):
     """The batching form: 64 procedural scenes of uneven size baked together.
     Per scene MakeTransfers and direct light, then every scene's FP32 bounces
@@ -652,6 +739,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched bake")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
     ap.add_argument("--no-pvs", action="store_true", help="skip the PVS-culled MakeTransfers")
+    ap.add_argument("--no-luxels", action="store_true", help="skip the luxel stage")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--serve-scenes", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -676,6 +764,7 @@ def main():
     batch = None if args.no_batch else run_batch_bake(cx, args)
     serving = None if args.no_serve else run_serving(cx, args)
     pvs_out = None if args.no_pvs else run_pvs(cx, args)
+    lux = None if args.no_luxels else run_luxel_stage(cx, args)
 
     if args.calib_copy_mb > 0:
         torch = cx.torch
@@ -694,6 +783,7 @@ def main():
                                "4M MakeTransfers TestLine segments per batch (row order)"},
         "roofline": None, "cpu_baseline": cpu, "visibility": None, "bounce": bounce,
         "room_bake": room, "batch_bake": batch, "serving": serving, "pvs_culling": pvs_out,
+        "luxel_stage": lux,
     }
     if v is not None:
         n, info = v["n"], v["info"]
