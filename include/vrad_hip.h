@@ -76,7 +76,8 @@ int vr_set_path(int path);
 /* Kernel-shape knobs (the autotuner's interface; no reference counterpart).
  * Results do not depend on:
  *   "trace_lds_entries"  traversal-stack entries per lane held in LDS
- *                        (4, 8, 12, 16, 24; the rest spill to scratch)
+ *                        (2, 4, 8, 12, 16, 24, default 4; the rest spill
+ *                        to scratch)
  *   "trace_rays_per_lane" rays each lane of the visibility kernels takes
  *                        (1, 2, 4, 8): rays per wavefront = 64 x this; for
  *                        TestLine above 1 a lane's consecutive segments are

@@ -26,9 +26,10 @@ def _env(v9, ids):
 
 
 @pytest.fixture(params=[(_lib.VR_PATH_OPT, 8, 1), (_lib.VR_PATH_OPT, 4, 1), (_lib.VR_PATH_OPT, 24, 1),
+                        (_lib.VR_PATH_OPT, 2, 1), (_lib.VR_PATH_OPT, 2, 8),
                         (_lib.VR_PATH_OPT, 8, 2), (_lib.VR_PATH_OPT, 4, 4), (_lib.VR_PATH_OPT, 8, 8),
                         (_lib.VR_PATH_PLAIN, 8, 1)],
-                ids=["opt-lds8", "opt-lds4", "opt-lds24", "opt-lds8-r2", "opt-lds4-r4", "opt-lds8-r8",
+                ids=["opt-lds8", "opt-lds4", "opt-lds24", "opt-lds2", "opt-lds2-r8", "opt-lds8-r2", "opt-lds4-r4", "opt-lds8-r8",
                      "plain"])
 def path(request):
     p, lds, rays = request.param
@@ -38,7 +39,7 @@ def path(request):
     _lib.check(_lib.lib.vr_set_tuning(b"trace_rays_per_lane", rays))
     yield p
     _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_OPT))
-    _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", 8))
+    _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", 4))
     _lib.check(_lib.lib.vr_set_tuning(b"trace_rays_per_lane", 1))
 
 

@@ -294,8 +294,8 @@ int vr_set_tuning(const char *name, int value) {
   if (!name) return fail(VR_E_INVALID, "name is NULL");
   const std::string k(name);
   if (k == "trace_lds_entries") {
-    if (value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
-      return fail(VR_E_INVALID, "trace_lds_entries must be 4, 8, 12, 16 or 24");
+    if (value != 2 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
+      return fail(VR_E_INVALID, "trace_lds_entries must be 2, 4, 8, 12, 16 or 24");
     tuning().trace_lds_entries = value;
   } else if (k == "trace_rays_per_lane") {
     if (value != 1 && value != 2 && value != 4 && value != 8)

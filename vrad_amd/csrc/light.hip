@@ -138,9 +138,21 @@ void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
   if (path == VR_PATH_PLAIN)
     hipLaunchKernelGGL(k_gather_plain<kStackDepth>, dim3((unsigned)((n + 255) / 256)), dim3(256),
                        0, st, sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb);
-  else
-    hipLaunchKernelGGL((k_gather_lds<8, kStackDepth>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
-                       sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb);
+  else {
+    // the traversal-stack entries kept in LDS follow the autotuned knob
+#define VR_GATHER(S)                                                                           \
+  hipLaunchKernelGGL((k_gather_lds<S, kStackDepth>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, \
+                     st, sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb)
+    switch (tuning().trace_lds_entries) {
+      case 2: VR_GATHER(2); break;
+      case 4: VR_GATHER(4); break;
+      case 12: VR_GATHER(12); break;
+      case 16: VR_GATHER(16); break;
+      case 24: VR_GATHER(24); break;
+      default: VR_GATHER(8); break;
+    }
+#undef VR_GATHER
+  }
 }
 
 }  // namespace vr

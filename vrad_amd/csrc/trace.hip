@@ -139,6 +139,7 @@ __global__ __launch_bounds__(256) void k_test_line_plain(DevScene sc, DevSegs s,
 
 // (traversal-stack entries in LDS, rays per lane)
 #define VR_LDS_VARIANTS(X)                                                                     \
+  X(2, 1) X(2, 2) X(2, 4) X(2, 8)                                                              \
   X(4, 1) X(8, 1) X(12, 1) X(16, 1) X(24, 1) X(4, 2) X(8, 2) X(12, 2) X(16, 2) X(24, 2) X(4, 4) \
   X(8, 4) X(12, 4) X(16, 4) X(24, 4) X(4, 8) X(8, 8) X(12, 8) X(16, 8) X(24, 8)
 

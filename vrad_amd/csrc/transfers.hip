@@ -130,12 +130,21 @@ void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pair
   if (path == VR_PATH_PLAIN)
     hipLaunchKernelGGL(k_pair_trace_plain<kStackDepth>, dim3((cap + 255) / 256), dim3(256), 0, st,
                        sc, p, pairs, count, cap, eps, skip, bits, words);
-  else if (tuning().trace_lds_entries == 24)
-    hipLaunchKernelGGL((k_pair_trace_lds<24, kStackDepth>), dim3((cap + 63) / 64), dim3(64), 0, st,
-                       sc, p, pairs, count, cap, eps, skip, bits, words);
-  else
-    hipLaunchKernelGGL((k_pair_trace_lds<8, kStackDepth>), dim3((cap + 63) / 64), dim3(64), 0, st,
-                       sc, p, pairs, count, cap, eps, skip, bits, words);
+  else {
+    // the traversal-stack entries kept in LDS follow the autotuned knob
+#define VR_PAIR_TRACE(S)                                                                    \
+  hipLaunchKernelGGL((k_pair_trace_lds<S, kStackDepth>), dim3((cap + 63) / 64), dim3(64), 0, st, \
+                     sc, p, pairs, count, cap, eps, skip, bits, words)
+    switch (tuning().trace_lds_entries) {
+      case 2: VR_PAIR_TRACE(2); break;
+      case 4: VR_PAIR_TRACE(4); break;
+      case 12: VR_PAIR_TRACE(12); break;
+      case 16: VR_PAIR_TRACE(16); break;
+      case 24: VR_PAIR_TRACE(24); break;
+      default: VR_PAIR_TRACE(8); break;
+    }
+#undef VR_PAIR_TRACE
+  }
 }
 
 // one wave per row
