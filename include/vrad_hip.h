@@ -88,6 +88,10 @@ int vr_set_path(int path);
  *                        threads, 1 = GPU, 2 (default) = GPU from
  *                        "kd_build_device_min" (default 2048) triangles; the
  *                        tree is the same bit for bit
+ *   "patch_build"        vr_make_patches: 1 (default) = level-parallel (every
+ *                        patch of a subdivision level split by its own
+ *                        thread, then renumbered), 0 = a thread per face;
+ *                        the same patches in the same order
  * These select the FP32 summation order of BounceLight (the CPU
  * restatement takes the same values):
  *   "bounce_kernel"      2 = LDS stream gather (default), 0 = LDS-tiled

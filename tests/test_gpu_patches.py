@@ -13,6 +13,15 @@ from vrad_amd.patches import MakePatches
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=[1, 0], ids=["level-parallel", "thread-per-face"])
+def build(request):
+    """Both patch builders (knob patch_build) must give the same patches."""
+    print(f"[patches] patch_build={request.param}", flush=True)
+    _lib.check(_lib.lib.vr_set_tuning(b"patch_build", request.param))
+    yield request.param
+    _lib.check(_lib.lib.vr_set_tuning(b"patch_build", 1))
+
+
 def _same(faces, label, **kw):
     print(f"[patches] {label}: {len(faces)} faces", flush=True)
     t0 = time.perf_counter()
@@ -58,13 +67,13 @@ def _random_convex(rng, n):
     return f
 
 
-def test_gpu_patches_generated_scenes():
+def test_gpu_patches_generated_scenes(build):
     _same(scenes.faces_of("room"), "room")
     _same(scenes.faces_of("level"), "level 8x8")
     _same(scenes.faces_of("level", luxel_size=8.0), "level 8x8, 8 units per luxel")
 
 
-def test_gpu_patches_random_polygons_and_chops():
+def test_gpu_patches_random_polygons_and_chops(build):
     rng = np.random.default_rng(17)
     f = _random_convex(rng, 3000)
     f["flags"][::11] = _lib.VR_FACE_SKY
@@ -75,7 +84,7 @@ def test_gpu_patches_random_polygons_and_chops():
     _same(f, "random convex polygons, maxChop 16 / minChop 4", max_chop=16.0, min_chop=4.0)
 
 
-def test_gpu_patches_count_only_and_capacity():
+def test_gpu_patches_count_only_and_capacity(build):
     f = scenes.faces_of("room")
     import ctypes as C
     n = C.c_int64()

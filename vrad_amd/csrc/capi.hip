@@ -330,6 +330,10 @@ int vr_set_tuning(const char *name, int value) {
   } else if (k == "kd_build_device_min") {
     if (value < 0) return fail(VR_E_INVALID, "kd_build_device_min must be >= 0");
     tuning().kd_build_device_min = value;
+  } else if (k == "patch_build") {
+    if (value < 0 || value > 1)
+      return fail(VR_E_INVALID, "patch_build must be 0 (a thread per face) or 1 (level-parallel)");
+    tuning().patch_build = value;
   } else {
     return fail(VR_E_INVALID, "unknown tuning knob " + k);
   }
@@ -350,6 +354,7 @@ int vr_get_tuning(const char *name, int *value) {
   else if (k == "bounce_prefetch") *value = tuning().bounce_prefetch;
   else if (k == "kd_build") *value = tuning().kd_build;
   else if (k == "kd_build_device_min") *value = tuning().kd_build_device_min;
+  else if (k == "patch_build") *value = tuning().patch_build;
   else return fail(VR_E_INVALID, "unknown tuning knob " + k);
   return VR_OK;
 }

@@ -27,6 +27,8 @@ struct Tuning {
   // kd-tree construction in vr_scene_build (the same tree either way)
   int kd_build = 2;                 // 0 = host threads, 1 = GPU, 2 = GPU from a size on
   int kd_build_device_min = 2048;   // triangles at which kd_build = 2 uses the GPU
+  // patch generation in vr_make_patches (the same patches either way)
+  int patch_build = 1;              // 0 = a thread per face, 1 = level-parallel
 };
 Tuning &tuning();
 
