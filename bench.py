@@ -454,7 +454,7 @@ def run_luxel_stage(cx, args):
     from faces to luxels on a 3x3-room level: patches, clusters, PVS,
     MakeTransfers, direct light, bounces, FinalLightFace."""
     import oracle
-    from vrad_amd import lightmap, luxels, pvs, scenes
+    from vrad_amd import _lib, lightmap, luxels, pvs, scenes
     from vrad_amd.patches import MakePatches, MakeTransfers
     from vrad_amd.raytracer import Environment
     progress("luxel stage: 8x8-room level faces")
@@ -489,7 +489,7 @@ def run_luxel_stage(cx, args):
     s = scenes.level(**kw)
     faces = scenes.faces_of("level", **kw)
 
-    def bake():
+    def bake(errors=False):
         T = {}
         tb = time.perf_counter()
         env = Environment()
@@ -527,12 +527,37 @@ def run_luxel_stage(cx, args):
         r = {"faces": int(len(faces)), "leaf_patches": int(len(leaf)),
              "transfers": int(xf.info().nnz), "bounces": int(k), "luxels": int(len(light)),
              "ms": {a: round(b * 1e3, 2) for a, b in T.items()}}
+        if errors:
+            # luxel error against the FP64 recurrence of the same k bounces
+            # (same transfers, same direct light), FP32 and FP16 coefficients
+            emit0 = (direct * refl).astype(np.float32)
+            rp_, col_, coef_ = xf.download()
+            t64, _, _ = oracle.bounce_f64(rp_, col_, coef_, emit0, refl, area,
+                                          total=direct.astype(np.float64), max_bounces=k,
+                                          stop_frac=0.0)
+            dlux = lightmap.GatherSampleLight(env, lx["pos"], lx["normal"], s.lights)
+            b64 = np.zeros((len(patches), 3))
+            b64[leaf] = t64 - direct
+            l64 = dlux.astype(np.float64) + b64[lx["patch"]]
+            nz = l64 != 0
+
+            def rel(l):
+                return float(np.max(np.abs(l.astype(np.float64)[nz] - l64[nz]) / np.abs(l64[nz])))
+            xf.set_precision(_lib.VR_COEF_F16)
+            t16, _, _ = lightmap.BounceLight(xf, emit0, refl, area, total=direct, max_bounces=k,
+                                             stop_frac=0.0)
+            b16 = np.zeros((len(patches), 3), np.float32)
+            b16[leaf] = t16 - direct
+            r["luxel_rel_err_vs_fp64"] = {"fp32": rel(light),
+                                          "fp16": rel((dlux + b16[lx["patch"]]).astype(np.float32)),
+                                          "luxels_compared": int(nz.any(axis=1).sum()),
+                                          "zero_where_fp64_zero": bool(np.all(light[~nz] == 0))}
         xf.close()
         env.close()
         return r, patches, leaf, cl, rows, light, lx, k
 
     first, *_ = bake()
-    r, patches, leaf, cl, rows, light, lx, k = bake()
+    r, patches, leaf, cl, rows, light, lx, k = bake(errors=True)
     r["first_call_total_ms"] = first["ms"]["total"]
     # the same bake by the CPU restatement, from the same patches
     threads = min(16, os.cpu_count() or 1)
