@@ -172,3 +172,22 @@ def test_gpu_transfers_pvs_conservative_equals_unculled(pvs_level):
     keep = ~(((cl[row] == a) & (cl[col] == b)) | ((cl[row] == b) & (cl[col] == a)))
     assert np.array_equal(g[1], col[keep])
     assert np.array_equal(g[2].view(np.uint32), coef[keep].view(np.uint32))
+
+
+def test_gpu_transfers_pvs_argument_errors(pvs_level):
+    s, cl, rows = pvs_level
+    env = _env(s.verts9, s.ids)
+    bad = cl.copy()
+    bad[3] = len(rows)  # a cluster without a PVS row
+    with pytest.raises(_lib.VrError):
+        MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, cluster=bad, pvs_rows=rows)
+    with pytest.raises(ValueError):
+        MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, cluster=cl[:-1], pvs_rows=rows)
+    with pytest.raises(ValueError):
+        MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, cluster=cl, pvs_rows=rows[:, :0])
+    # zero clusters: every pair is considered, as without a PVS
+    a = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, cluster=cl,
+                      pvs_rows=np.zeros((0, 0), np.uint8))
+    b = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area)
+    assert all(np.array_equal(x, y) for x, y in zip(a.download(), b.download()))
+    assert a.info().pairs_considered == b.info().pairs_considered
