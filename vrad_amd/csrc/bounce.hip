@@ -263,15 +263,16 @@ struct BChunk {
 };
 
 // A 4-entry chunk of 16-bit coefficients (the stream layout's chunk for every
-// precision): 16-bit offsets and coefficients, 8 bytes each.
+// precision): 16-bit offsets and coefficients interleaved as one 16-byte
+// record per chunk (4 offsets, then 4 coefficients) at ib + 4 q, q the
+// chunk's first entry: one 128-bit load per chunk.
 template <int KIND>
 struct QChunk16 {
   uint2 o, h;
-  __device__ void load_nt(const char *ib, const char *cb, uint32_t q) {
-    const u32x2 a = load_global_nt<u32x2>(ib + 2u * q);
-    const u32x2 b = load_global_nt<u32x2>(cb + 2u * q);
-    o = make_uint2(a.x, a.y);
-    h = make_uint2(b.x, b.y);
+  __device__ void load_nt(const char *ib, const char *, uint32_t q) {
+    const u32x4 v = load_global_nt<u32x4>(ib + 4u * q);
+    o = make_uint2(v.x, v.y);
+    h = make_uint2(v.z, v.w);
   }
   __device__ float coef(int m) const {
     const uint32_t wd = m < 2 ? h.x : h.y;
@@ -776,7 +777,8 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
     m.e = sb;
     m.np = live ? (m.e - m.a + PW - 1) / PW : 0;
     const int64_t bs = uniform64(s_bs[tn]);
-    m.ib = bidx + 2 * bs;
+    // 16-bit kinds: offsets and coefficients interleaved, 4 bytes per entry
+    m.ib = bidx + (KIND == VR_COEF_F32 ? 2 : 4) * bs;
     m.cb = bcoef + cbytes * bs;
     return m;
   };
@@ -971,7 +973,16 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
     } else {  // tiled layout: bit 0 marks the segment's first entry
       bits = q == 0 ? 1u : 0u;
     }
-    bidx[at] = (uint16_t)((real ? tl.ostride * (x.col[src + q] - p0) : 0) | (int)bits);
+    const uint16_t off = (uint16_t)((real ? tl.ostride * (x.col[src + q] - p0) : 0) | (int)bits);
+    if (tl.ostride == 16 && coef_bytes == 2) {
+      // stream layout, 16-bit: chunk record of 4 offsets then 4 coefficients
+      // (chunks start at multiples of G = 4 entries)
+      const int64_t rec = (at >> 2) * 8, m4 = at & 3;
+      bidx[rec + m4] = off;
+      bidx[rec + 4 + m4] = real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
+      continue;
+    }
+    bidx[at] = off;
     if (coef_bytes == 4)
       reinterpret_cast<float *>(bcoef)[at] =
           real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;

@@ -1171,8 +1171,10 @@ int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t s
       run += tot[(size_t)b];
     }
     TRY(upload_vec(x->bstart, start, st));
-    TRY(x->bidx.alloc(2 * (size_t)std::max<int64_t>(run, 1) + 64));
-    TRY(x->bcoef.alloc((size_t)cb * (size_t)std::max<int64_t>(run, 1) + 64));
+    // the stream layout interleaves 16-bit offsets and coefficients in bidx
+    const bool inter = tl.ostride == 16 && cb == 2;
+    TRY(x->bidx.alloc((inter ? 4 : 2) * (size_t)std::max<int64_t>(run, 1) + 64));
+    TRY(x->bcoef.alloc(inter ? 64 : (size_t)cb * (size_t)std::max<int64_t>(run, 1) + 64));
     tl.bstart = x->bstart.as<int64_t>();
     launch_block_fill(dx, tl, G, cb, x->bidx.as<uint16_t>(), x->bcoef.p, st);
     TRY(check_launch());

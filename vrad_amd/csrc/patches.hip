@@ -504,8 +504,12 @@ __global__ __launch_bounds__(256) void k_lv_scatter(const vr_patch *__restrict__
 static int make_patches_levels(const vr_face *faces, int64_t n_faces, float max_chop,
                                float min_chop, vr_patch *out, int64_t cap, int64_t *n_out,
                                hipStream_t st) {
-  struct Buf {
+  struct Buf {  // owns one device allocation; moves, never copies
     void *p = nullptr;
+    Buf() = default;
+    Buf(const Buf &) = delete;
+    Buf &operator=(const Buf &) = delete;
+    Buf(Buf &&o) noexcept : p(o.p) { o.p = nullptr; }
     ~Buf() {
       if (p) (void)hipFree(p);
     }
