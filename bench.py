@@ -297,8 +297,9 @@ def run_bounce(cx, args):
         # reflectivity 12, emit write 12, total read+write 24, scale
         B = T * (2 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
         B_i32 = T * (4 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
-        # autotune every compiled gather shape (each is its own restated order)
-        tu = autotune.Autotuner(reps=1)
+        # autotune every compiled gather shape (each is its own restated order;
+        # the median of 3 timings per shape)
+        tu = autotune.Autotuner(reps=3)
         rec = tu.tune_bounce(
             lambda: lightmap.BounceLight(xf, e0, refl, area, max_bounces=3, stop_frac=0.0),
             key=f"{name}:{N}:{T}", keep_order=False,
