@@ -137,6 +137,57 @@ def test_bad_arguments_return_status():
     assert lib.vr_server_destroy(None) == _lib.VR_OK
 
 
+def test_bsp_vis_luxel_arguments_checked_before_device_work():
+    from vrad_amd import luxels, pvs
+    pts = np.zeros((2, 3), np.float32)
+    pl = np.zeros(1, _lib.PLANE_DTYPE)
+    pl[0] = ((1, 0, 0), 0.0)
+    lc = np.array([-1, 3], np.int16)
+
+    def bsp(nodes):
+        return pvs.Bsp(pl, np.array(nodes, _lib.NODE_DTYPE), lc)
+    for nodes, what in (([(0, (-1, 0))], b"tree"),            # a cycle back to the root
+                        ([(1, (-1, -2))], b"plane"),          # plane index out of range
+                        ([(0, (-1, -3))], b"leaf"),           # leaf index out of range
+                        ([(0, (-1, 4))], b"child"),           # node index out of range
+                        ([(0, (1, -1)), (0, (-1, 1))], b"tree")):  # a node reached twice
+        with pytest.raises(_lib.VrError) as e:
+            pvs.ClusterFromPoint(bsp(nodes), pts)
+        assert e.value.code == _lib.VR_E_INVALID and what in lib.vr_last_error(), what
+    chain = [(0, (-1, k + 1)) for k in range(_lib.VR_BSP_MAX_DEPTH)] + [(0, (-1, -2))]
+    with pytest.raises(_lib.VrError):
+        pvs.ClusterFromPoint(bsp(chain), pts)
+    assert b"deeper" in lib.vr_last_error()
+    # vis data: offsets of -1 and past the data, before any device work
+    rows = np.zeros((1, 1), np.uint8)
+    ofs = np.array([-1], np.int32)
+    data = np.array([1], np.uint8)
+    assert lib.vr_decompress_vis(ptr(data), 1, ptr(ofs), 1, ptr(rows), None) == _lib.VR_E_INVALID
+    assert b"visofs == -1" in lib.vr_last_error()
+    ofs[0] = 1
+    assert lib.vr_decompress_vis(ptr(data), 1, ptr(ofs), 1, ptr(rows), None) == _lib.VR_E_INVALID
+    assert lib.vr_decompress_vis(ptr(data), 1, ptr(ofs), -1, ptr(rows), None) == _lib.VR_E_INVALID
+    assert lib.vr_decompress_vis(None, 0, None, 0, None, None) == _lib.VR_OK  # no clusters
+    # luxel stage: counts and the lightmap layout
+    n = C.c_int64()
+    assert lib.vr_calc_face_extents(None, -1, None, None, C.byref(n)) == _lib.VR_E_INVALID
+    assert lib.vr_calc_face_extents(None, 0, None, None, C.byref(n)) == _lib.VR_OK
+    faces = np.zeros(2, _lib.FACE_DTYPE)
+    lm = np.zeros(2, _lib.FACE_LIGHTMAP_DTYPE)
+    lm["size"] = [[1, 1], [0, 0]]
+    lm["n_luxels"] = [4, 1]
+    lm["luxel_ofs"] = [0, 5]  # the second face must start at 4
+    with pytest.raises(_lib.VrError):
+        luxels.FaceLuxels(faces, lm, np.zeros(0, _lib.PATCH_DTYPE))
+    lm["luxel_ofs"][1] = 4
+    p = np.zeros(1, _lib.PATCH_DTYPE)
+    p["child1"] = -1
+    p["face"] = 7  # no such face
+    with pytest.raises(_lib.VrError):
+        luxels.FaceLuxels(faces, lm, p)
+    assert b"face index" in lib.vr_last_error()
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device error path")
 def test_no_device_is_an_error():
     v9, ids = H.to_arrays(H.box((0, 0, 0), (1, 1, 1)))

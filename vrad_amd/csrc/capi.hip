@@ -1979,7 +1979,7 @@ struct BspDev {
   DBuf planes, nodes, leaves;
   DevBsp dev{};
 };
-static int bsp_upload(const vr_bsp *b, BspDev *d, hipStream_t st) {
+static int bsp_check(const vr_bsp *b) {
   if (!b) return fail(VR_E_INVALID, "bsp is NULL");
   if (b->n_nodes < 1 || b->n_planes < 1 || b->n_leafs < 0)
     return fail(VR_E_INVALID, "BSP needs at least one node and one plane");
@@ -2012,6 +2012,9 @@ static int bsp_upload(const vr_bsp *b, BspDev *d, hipStream_t st) {
       stk.push_back({c, depth + 1});
     }
   }
+  return VR_OK;
+}
+static int bsp_upload(const vr_bsp *b, BspDev *d, hipStream_t st) {
   static_assert(sizeof(vr_bsp_plane) == 16 && sizeof(vr_bsp_node) == 12, "lump records");
   TRY(d->planes.upload(b->planes, 16 * (size_t)b->n_planes, st));
   TRY(d->nodes.upload(b->nodes, 12 * (size_t)b->n_nodes, st));
@@ -2134,6 +2137,7 @@ int vr_cluster_from_points(const vr_bsp *bsp, const float *xyz, int64_t n, int16
   LOCK;
   if (n < 0) return fail(VR_E_INVALID, "n < 0");
   if (n > 0 && (!xyz || !cluster)) return fail(VR_E_INVALID, "xyz / cluster is NULL");
+  TRY(bsp_check(bsp));
   TRY(ensure_init());
   hipStream_t st = g_stream;
   BspDev d;
@@ -2156,6 +2160,7 @@ int vr_patch_clusters(const vr_bsp *bsp, const vr_patch *patches, int64_t n, int
   LOCK;
   if (n < 0) return fail(VR_E_INVALID, "n < 0");
   if (n > 0 && (!patches || !cluster)) return fail(VR_E_INVALID, "patches / cluster is NULL");
+  TRY(bsp_check(bsp));
   TRY(ensure_init());
   hipStream_t st = g_stream;
   BspDev d;
