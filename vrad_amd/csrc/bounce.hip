@@ -791,8 +791,22 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
     const int ha = (nc + 1) >> 1;
     const int qa = lane < ha ? p0 + G * lane : m.a;
     const int qb = lane < nc - ha ? p0 + G * (ha + lane) : m.a;
-    L.a.load_nt(m.ib, m.cb, (uint32_t)qa);
-    L.b.load_nt(m.ib, m.cb, (uint32_t)qb);
+    if constexpr (KIND == VR_COEF_F32) {
+      // FP32 offsets are stored in stream order: the lane's A and B chunks
+      // (2 lane, 2 lane + 1) are adjacent, one 16-byte load; coefficients
+      // keep the A-then-B slots
+      const int qn = lane < ha ? p0 + 2 * G * lane : m.a;
+      const u32x4 o = load_global_nt<u32x4>(m.ib + 2u * (uint32_t)qn);
+      const f32x4 wa = load_global_nt<f32x4>(m.cb + 4u * (uint32_t)qa);
+      const f32x4 wb = load_global_nt<f32x4>(m.cb + 4u * (uint32_t)qb);
+      L.a.o = make_uint2(o.x, o.y);
+      L.b.o = make_uint2(o.z, o.w);
+      L.a.w = make_float4(wa.x, wa.y, wa.z, wa.w);
+      L.b.w = make_float4(wb.x, wb.y, wb.z, wb.w);
+    } else {
+      L.a.load_nt(m.ib, m.cb, (uint32_t)qa);
+      L.b.load_nt(m.ib, m.cb, (uint32_t)qb);
+    }
   };
   auto deliver = [&](int slot, float r, float g, float b) {
     f32x3 v = acc[slot];
@@ -982,7 +996,9 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
       bidx[rec + 4 + m4] = real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
       continue;
     }
-    bidx[at] = off;
+    // stream layout, FP32: offsets in stream order (the A and B chunks of a
+    // lane adjacent), coefficients in the A-then-B slots
+    bidx[tl.ostride == 16 ? dst + q : at] = off;
     if (coef_bytes == 4)
       reinterpret_cast<float *>(bcoef)[at] =
           real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;
