@@ -114,3 +114,15 @@ def test_gpu_final_light_face():
     assert np.array_equal(lx["value"], (total - direct)[lx["patch"]])
     assert np.array_equal(light, (d + lx["value"]).astype(np.float32))
     assert (lx["patch"] >= 0).all()
+
+
+def test_gpu_luxels_full_level():
+    """The 8x8-room level's 2,000 faces at 16 units per luxel (1.46 M
+    luxels): GPU == restatement, every luxel on its face's plane."""
+    f = scenes.faces_of("level", seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0)
+    lm, lx, patches = _same(f, None, "8x8 level")
+    fi = np.repeat(np.arange(len(f)), lm["n_luxels"])
+    d = np.einsum("ij,ij->i", lx["pos"].astype(np.float64), f["normal"][fi].astype(np.float64))
+    d0 = np.einsum("ij,ij->i", f["points"][fi, 0].astype(np.float64),
+                   f["normal"][fi].astype(np.float64))
+    assert np.max(np.abs(d - d0)) < 1e-2  # FP32 rounding of coordinates up to ~1e4

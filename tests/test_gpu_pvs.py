@@ -191,3 +191,29 @@ def test_gpu_transfers_pvs_argument_errors(pvs_level):
     b = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area)
     assert all(np.array_equal(x, y) for x, y in zip(a.download(), b.download()))
     assert a.info().pairs_considered == b.info().pairs_considered
+
+
+def test_gpu_transfers_pvs_full_level_conservative():
+    """4x4-room level (99,730 patches): a PVS made of the cluster pairs that
+    exchange a transfer reproduces the unculled transfers bit for bit, and
+    considers fewer pairs."""
+    s = scenes.level(seed=2, rooms_x=4, rooms_y=4, chop=32.0, tess=64.0)
+    bsp, _ = scenes.grid_bsp(s.grid_x, s.grid_y)
+    cl = pvs.ClusterFromPoint(bsp, s.p_origin).astype(np.int32)
+    assert (cl >= 0).all()
+    print(f"[pvs] full level: {len(cl)} patches", flush=True)
+    env = _env(s.verts9, s.ids)
+    full = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, normalize=False)
+    rp, col, coef = full.download()
+    row = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    nc = int(cl.max()) + 1
+    used = np.zeros((nc, nc), bool)
+    used[cl[row], cl[col]] = True
+    x = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, normalize=False, cluster=cl,
+                      pvs_rows=np.packbits(used, axis=1, bitorder="little"))
+    g = x.download()
+    print(f"[pvs] full level: {len(col)} transfers, pairs {full.info().pairs_considered} -> "
+          f"{x.info().pairs_considered}", flush=True)
+    assert np.array_equal(g[0], rp) and np.array_equal(g[1], col)
+    assert np.array_equal(g[2].view(np.uint32), coef.view(np.uint32))
+    assert x.info().pairs_considered < full.info().pairs_considered
