@@ -190,3 +190,37 @@ def test_transfers_pvs_culls_exactly_the_invisible_cluster_pairs(small_level):
     a, b = cl2[row], cl2[col]
     keep2 = (a == b) & (a >= 0) | ((a == 0) & (b == 3)) | ((a == 3) & (b == 0)) | (a < 0) | (b < 0)
     assert np.array_equal(g2[1], col[keep2])
+
+
+def test_grid_bsp_structure():
+    """The generated BSP is a tree within the walk's depth bound, every room
+    is exactly one leaf, and each room's PVS row holds itself and its grid
+    neighbours (one doorway)."""
+    xs = np.array([0.0, 300, 700, 1000, 1600])
+    ys = np.array([0.0, 500, 900])
+    bsp, vis = scenes.grid_bsp(xs, ys, pvs_hops=1)
+    nd = bsp.nodes
+    seen, stack, depth = set(), [(0, 1)], 0
+    while stack:
+        n, d = stack.pop()
+        assert n not in seen
+        seen.add(n)
+        depth = max(depth, d)
+        for c in nd[n]["children"]:
+            if c >= 0:
+                stack.append((int(c), d + 1))
+    assert depth <= 64
+    assert sorted(c for c in bsp.leaf_cluster if c >= 0) == list(range(8))
+    rows, over = oracle.decompress_vis(vis.data, vis.pvs_ofs, vis.n_clusters)
+    bits = np.unpackbits(rows, axis=1, bitorder="little")[:, :8].astype(bool)
+    for c in range(8):
+        r, k = divmod(c, 4)
+        want = {c} | {rr * 4 + kk for rr, kk in ((r, k - 1), (r, k + 1), (r - 1, k), (r + 1, k))
+                      if 0 <= rr < 2 and 0 <= kk < 4}
+        assert set(np.flatnonzero(bits[c])) == want
+    # room centres land in their own cluster
+    cx = (xs[:-1] + xs[1:]) / 2
+    cy = (ys[:-1] + ys[1:]) / 2
+    pts = np.array([(x, y, 100.0) for y in cy for x in cx], np.float32)
+    got = oracle.cluster_from_points(bsp.planes, bsp.nodes, bsp.leaf_cluster, pts)
+    assert list(got) == list(range(8))
