@@ -85,9 +85,10 @@ int vr_set_path(int path);
  *                        one before (results unchanged)
  *   "bounce_waves"       persistent waves of the wave-per-row gather
  *   "kd_build"           kd-tree construction in vr_scene_build: 0 = host
- *                        threads, 1 = GPU, 2 (default) = GPU from
- *                        "kd_build_device_min" (default 2048) triangles; the
- *                        tree is the same bit for bit
+ *                        threads, 1 = GPU, 2 (default) = GPU for a tree
+ *                        (main or sky) of "kd_build_device_min" (default
+ *                        2048) triangles or more; the tree is the same bit
+ *                        for bit
  *   "patch_build"        vr_make_patches: 1 (default) = level-parallel (every
  *                        patch of a subdivision level split by its own
  *                        thread, then renumbered), 0 = a thread per face;

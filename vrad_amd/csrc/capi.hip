@@ -441,8 +441,11 @@ static int build_scene_trees(vr_scene *s, int32_t n_threads, bool device) {
   const int nt = host_threads(n_threads);
   s->tris.resize((size_t)s->n_tri);
   precompute_triangles(s->v9.data(), s->ids.data(), s->flags.data(), s->n_tri, s->tris.data(), nt);
+  // kd_build 2: each tree by its own size (the sky tree is usually a few
+  // dozen triangles, where the host is faster than ~20 levels of launches)
   auto build = [&](const float *v9, int64_t n, KdTree *out) {
-    if (device) return build_kdtree_device(v9, n, g_stream, out);
+    const bool dev = device && (tuning().kd_build != 2 || n >= tuning().kd_build_device_min);
+    if (dev) return build_kdtree_device(v9, n, g_stream, out);
     return build_kdtree(v9, n, nt, out);
   };
   s->kd = KdTree();
