@@ -229,15 +229,20 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t n_patches,
  * is stored with its columns in ascending order (stable). */
 int vr_xfers_from_csr(vr_xfers **out, int64_t n_rows, const int64_t *row_ptr,
                       const int32_t *col, const float *coef, int32_t precision);
-/* Synthetic transfers on the device: row lengths lognormal
- * (mu = ln(mean) - sigma^2/2), clipped to [1, max_len] and rescaled to the
- * mean; columns one per equal stratum of [0,n)\{i} (sorted, unique);
+/* Synthetic transfers on the device (no reference counterpart; the
+ * precision workload's CSR): row lengths lognormal (mu = ln(mean) -
+ * sigma^2/2), clipped to [1, max_len] and rescaled to the mean; columns
+ * VR_SYNTH_UNIFORM: a uniformly random subset of [0,n)\{i} drawn without
+ * replacement (the first distinct values of a seeded stream of uniform
+ * draws), or VR_SYNTH_STRATIFIED: one per equal stratum; sorted per row;
  * coef = u/sum(u) * s_i with s_i ~ U[0.3,0.9]. */
+#define VR_SYNTH_UNIFORM 0
+#define VR_SYNTH_STRATIFIED 1
 int vr_xfers_synthetic(vr_xfers **out, uint64_t seed, int64_t n_rows,
                        double mean_len, double sigma, int64_t max_len,
-                       int32_t precision);
+                       int32_t precision, int32_t columns);
 /* Re-encode the coefficients: FP16/BF16 store coef/row_max with the row
- * max as an FP32 per-row scale. */
+ * max as an FP32 per-row scale.  VR_E_STATE while a server holds x. */
 int vr_xfers_set_precision(vr_xfers *x, int32_t precision);
 int vr_xfers_get_info(const vr_xfers *x, vr_xfers_info *out);
 /* Copy out the logical CSR (coef decoded to FP32 without the row scale for
@@ -245,6 +250,7 @@ int vr_xfers_get_info(const vr_xfers *x, vr_xfers_info *out);
  * nullable). */
 int vr_xfers_download(const vr_xfers *x, int64_t *row_ptr, int32_t *col,
                       float *coef, uint16_t *raw_coef16, float *row_scale);
+/* VR_E_STATE (and x stays valid) while a server holds x. */
 int vr_xfers_destroy(vr_xfers *x);
 
 /* ---- direct light: rad/lightmap GatherSampleLight --------------------- */
@@ -268,6 +274,16 @@ typedef struct {
 int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n,
                            const vr_light *lights, int32_t n_lights,
                            float normal_offset, int32_t skip_id, float *rgb);
+
+/* CanLeafTraceToSky (rad/lightmap/lightmap.go:425-451), batched over leaves.
+ * Leaf i's bounds are mins[3i..3i+2], maxs[3i..3i+2] (dleaf_t int16); its
+ * centre is float32(mins + maxs) * 0.5 per axis (32-bit sum).  can_see_sky[i]
+ * = 1 iff, for one of the 162 Anorms directions a (vmath/constants.go:15-184),
+ * TestLineDoesHitSky from the centre to (a * -MAX_TRACE_LENGTH) + centre with
+ * skip id -1 finds the segment visible (a miss, or a TRACE_ID_SKY hit before
+ * any other triangle); else 0.  n < 2^32. */
+int vr_can_leaf_trace_to_sky(vr_scene *s, const int16_t *mins, const int16_t *maxs,
+                             int64_t n, uint8_t *can_see_sky);
 
 /* ---- bounce: rad/lightmap BounceLight --------------------------------- */
 typedef struct {
@@ -318,7 +334,11 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs,
  * on the FP32 coefficients, a host read of the energy after every bounce).
  * A request's result does not depend on what else runs with it.
  * Scenes are added before the first request; the scene and transfer handles
- * must outlive the server. */
+ * must outlive the server.  The launch knobs (vr_set_tuning) are read once,
+ * at vr_server_create; the batched scenes share one gather shape.  While a
+ * server holds a transfer set, vr_xfers_set_precision, vr_xfers_destroy and
+ * a vr_bounce_light / vr_bounce_light_batch needing another gather layout of
+ * it return VR_E_STATE before any work. */
 #define VR_SERVE_BATCHED 0
 #define VR_SERVE_PLAIN 1
 typedef struct vr_server vr_server;

@@ -87,6 +87,8 @@ def _load():
         "or_gather_light": (None, [C.POINTER(_Kd), P, C.c_int64, C.POINTER(_Samples), C.c_int64,
                                    P, C.c_int32, C.c_float, C.c_int32, P, C.c_int]),
         "or_sky_dirs": (None, [P]),
+        "or_can_leaf_trace_to_sky": (None, [C.POINTER(_Kd), P, C.c_int64, P, P, C.c_int64, P,
+                                            C.c_int]),
         "or_bounce_f32": (C.c_int, [C.c_int64, P, P, P, C.c_int, P, P, P, P, P, C.c_int,
                                     C.c_double, P, C.c_int, C.c_int, C.c_int, C.c_int]),
         "or_bounce_f64": (C.c_int, [C.c_int64, P, P, P, P, P, P, P, C.c_int, C.c_double, P]),
@@ -256,6 +258,16 @@ def sky_dirs():
     d = np.zeros((162, 3), np.float32)
     lib.or_sky_dirs(_p(d))
     return d
+
+
+def can_leaf_trace_to_sky(kd, tris, mins, maxs, threads=1):
+    """CanLeafTraceToSky per leaf (int16 bounds, n x 3 each): uint8 0/1."""
+    mn = np.ascontiguousarray(mins, np.int16).reshape(-1, 3)
+    mx = np.ascontiguousarray(maxs, np.int16).reshape(-1, 3)
+    out = np.zeros(len(mn), np.uint8)
+    lib.or_can_leaf_trace_to_sky(C.byref(kd.kd) if kd is not None else None, _p(tris),
+                                 len(tris), _p(mn), _p(mx), len(mn), _p(out), threads)
+    return out
 
 
 ORDERS = {"wave": 0, "sequential": 1, "tiled": 2, "stream": 3}

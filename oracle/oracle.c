@@ -829,18 +829,110 @@ void or_csr_free(or_csr_t *c) {
 /* falloff parameters rad/lightmap/lights.go:259-341).  Frozen semantics in  */
 /* gather_one below; lights are summed in list order.                       */
 /* ------------------------------------------------------------------------ */
-void or_sky_dirs(float *dirs) {
-  /* 162 directions on a Fibonacci sphere (the reference's Anorms table,
-   * vmath/constants.go:15-44, carries only 28 of its 162 entries) */
-  const double ga = 3.14159265358979323846 * (3.0 - sqrt(5.0));
-  for (int k = 0; k < 162; ++k) {
-    double z = 1.0 - (2.0 * k + 1.0) / 162.0;
-    double r = sqrt(1.0 - z * z);
-    double ph = ga * k;
-    dirs[3 * k + 0] = (float)(r * cos(ph));
-    dirs[3 * k + 1] = (float)(r * sin(ph));
-    dirs[3 * k + 2] = (float)z;
+/* The reference's Anorms table (vmath/constants.go:15-184), 162 float32
+ * vectors, in table order: the directions CanLeafTraceToSky walks
+ * (rad/lightmap/lightmap.go:435-448).  Literals round to nearest float32,
+ * as Go's conversion of an untyped constant to float32 does. */
+static const float or_anorms[162][3] = {
+    {-0.525731f, 0.000000f, 0.850651f}, {-0.442863f, 0.238856f, 0.864188f}, {-0.295242f, 0.000000f, 0.955423f},
+    {-0.309017f, 0.500000f, 0.809017f}, {-0.162460f, 0.262866f, 0.951056f}, {0.000000f, 0.000000f, 1.000000f},
+    {0.000000f, 0.850651f, 0.525731f}, {-0.147621f, 0.716567f, 0.681718f}, {0.147621f, 0.716567f, 0.681718f},
+    {0.000000f, 0.525731f, 0.850651f}, {0.309017f, 0.500000f, 0.809017f}, {0.525731f, 0.000000f, 0.850651f},
+    {0.295242f, 0.000000f, 0.955423f}, {0.442863f, 0.238856f, 0.864188f}, {0.162460f, 0.262866f, 0.951056f},
+    {-0.681718f, 0.147621f, 0.716567f}, {-0.809017f, 0.309017f, 0.500000f}, {-0.587785f, 0.425325f, 0.688191f},
+    {-0.850651f, 0.525731f, 0.000000f}, {-0.864188f, 0.442863f, 0.238856f}, {-0.716567f, 0.681718f, 0.147621f},
+    {-0.688191f, 0.587785f, 0.425325f}, {-0.500000f, 0.809017f, 0.309017f}, {-0.238856f, 0.864188f, 0.442863f},
+    {-0.425325f, 0.688191f, 0.587785f}, {-0.716567f, 0.681718f, -0.147621f}, {-0.500000f, 0.809017f, -0.309017f},
+    {-0.525731f, 0.850651f, 0.000000f}, {0.000000f, 0.850651f, -0.525731f}, {-0.238856f, 0.864188f, -0.442863f},
+    {0.000000f, 0.955423f, -0.295242f}, {-0.262866f, 0.951056f, -0.162460f}, {0.000000f, 1.000000f, 0.000000f},
+    {0.000000f, 0.955423f, 0.295242f}, {-0.262866f, 0.951056f, 0.162460f}, {0.238856f, 0.864188f, 0.442863f},
+    {0.262866f, 0.951056f, 0.162460f}, {0.500000f, 0.809017f, 0.309017f}, {0.238856f, 0.864188f, -0.442863f},
+    {0.262866f, 0.951056f, -0.162460f}, {0.500000f, 0.809017f, -0.309017f}, {0.850651f, 0.525731f, 0.000000f},
+    {0.716567f, 0.681718f, 0.147621f}, {0.716567f, 0.681718f, -0.147621f}, {0.525731f, 0.850651f, 0.000000f},
+    {0.425325f, 0.688191f, 0.587785f}, {0.864188f, 0.442863f, 0.238856f}, {0.688191f, 0.587785f, 0.425325f},
+    {0.809017f, 0.309017f, 0.500000f}, {0.681718f, 0.147621f, 0.716567f}, {0.587785f, 0.425325f, 0.688191f},
+    {0.955423f, 0.295242f, 0.000000f}, {1.000000f, 0.000000f, 0.000000f}, {0.951056f, 0.162460f, 0.262866f},
+    {0.850651f, -0.525731f, 0.000000f}, {0.955423f, -0.295242f, 0.000000f}, {0.864188f, -0.442863f, 0.238856f},
+    {0.951056f, -0.162460f, 0.262866f}, {0.809017f, -0.309017f, 0.500000f}, {0.681718f, -0.147621f, 0.716567f},
+    {0.850651f, 0.000000f, 0.525731f}, {0.864188f, 0.442863f, -0.238856f}, {0.809017f, 0.309017f, -0.500000f},
+    {0.951056f, 0.162460f, -0.262866f}, {0.525731f, 0.000000f, -0.850651f}, {0.681718f, 0.147621f, -0.716567f},
+    {0.681718f, -0.147621f, -0.716567f}, {0.850651f, 0.000000f, -0.525731f}, {0.809017f, -0.309017f, -0.500000f},
+    {0.864188f, -0.442863f, -0.238856f}, {0.951056f, -0.162460f, -0.262866f}, {0.147621f, 0.716567f, -0.681718f},
+    {0.309017f, 0.500000f, -0.809017f}, {0.425325f, 0.688191f, -0.587785f}, {0.442863f, 0.238856f, -0.864188f},
+    {0.587785f, 0.425325f, -0.688191f}, {0.688191f, 0.587785f, -0.425325f}, {-0.147621f, 0.716567f, -0.681718f},
+    {-0.309017f, 0.500000f, -0.809017f}, {0.000000f, 0.525731f, -0.850651f}, {-0.525731f, 0.000000f, -0.850651f},
+    {-0.442863f, 0.238856f, -0.864188f}, {-0.295242f, 0.000000f, -0.955423f}, {-0.162460f, 0.262866f, -0.951056f},
+    {0.000000f, 0.000000f, -1.000000f}, {0.295242f, 0.000000f, -0.955423f}, {0.162460f, 0.262866f, -0.951056f},
+    {-0.442863f, -0.238856f, -0.864188f}, {-0.309017f, -0.500000f, -0.809017f}, {-0.162460f, -0.262866f, -0.951056f},
+    {0.000000f, -0.850651f, -0.525731f}, {-0.147621f, -0.716567f, -0.681718f}, {0.147621f, -0.716567f, -0.681718f},
+    {0.000000f, -0.525731f, -0.850651f}, {0.309017f, -0.500000f, -0.809017f}, {0.442863f, -0.238856f, -0.864188f},
+    {0.162460f, -0.262866f, -0.951056f}, {0.238856f, -0.864188f, -0.442863f}, {0.500000f, -0.809017f, -0.309017f},
+    {0.425325f, -0.688191f, -0.587785f}, {0.716567f, -0.681718f, -0.147621f}, {0.688191f, -0.587785f, -0.425325f},
+    {0.587785f, -0.425325f, -0.688191f}, {0.000000f, -0.955423f, -0.295242f}, {0.000000f, -1.000000f, 0.000000f},
+    {0.262866f, -0.951056f, -0.162460f}, {0.000000f, -0.850651f, 0.525731f}, {0.000000f, -0.955423f, 0.295242f},
+    {0.238856f, -0.864188f, 0.442863f}, {0.262866f, -0.951056f, 0.162460f}, {0.500000f, -0.809017f, 0.309017f},
+    {0.716567f, -0.681718f, 0.147621f}, {0.525731f, -0.850651f, 0.000000f}, {-0.238856f, -0.864188f, -0.442863f},
+    {-0.500000f, -0.809017f, -0.309017f}, {-0.262866f, -0.951056f, -0.162460f}, {-0.850651f, -0.525731f, 0.000000f},
+    {-0.716567f, -0.681718f, -0.147621f}, {-0.716567f, -0.681718f, 0.147621f}, {-0.525731f, -0.850651f, 0.000000f},
+    {-0.500000f, -0.809017f, 0.309017f}, {-0.238856f, -0.864188f, 0.442863f}, {-0.262866f, -0.951056f, 0.162460f},
+    {-0.864188f, -0.442863f, 0.238856f}, {-0.809017f, -0.309017f, 0.500000f}, {-0.688191f, -0.587785f, 0.425325f},
+    {-0.681718f, -0.147621f, 0.716567f}, {-0.442863f, -0.238856f, 0.864188f}, {-0.587785f, -0.425325f, 0.688191f},
+    {-0.309017f, -0.500000f, 0.809017f}, {-0.147621f, -0.716567f, 0.681718f}, {-0.425325f, -0.688191f, 0.587785f},
+    {-0.162460f, -0.262866f, 0.951056f}, {0.442863f, -0.238856f, 0.864188f}, {0.162460f, -0.262866f, 0.951056f},
+    {0.309017f, -0.500000f, 0.809017f}, {0.147621f, -0.716567f, 0.681718f}, {0.000000f, -0.525731f, 0.850651f},
+    {0.425325f, -0.688191f, 0.587785f}, {0.587785f, -0.425325f, 0.688191f}, {0.688191f, -0.587785f, 0.425325f},
+    {-0.955423f, 0.295242f, 0.000000f}, {-0.951056f, 0.162460f, 0.262866f}, {-1.000000f, 0.000000f, 0.000000f},
+    {-0.850651f, 0.000000f, 0.525731f}, {-0.955423f, -0.295242f, 0.000000f}, {-0.951056f, -0.162460f, 0.262866f},
+    {-0.864188f, 0.442863f, -0.238856f}, {-0.951056f, 0.162460f, -0.262866f}, {-0.809017f, 0.309017f, -0.500000f},
+    {-0.864188f, -0.442863f, -0.238856f}, {-0.951056f, -0.162460f, -0.262866f}, {-0.809017f, -0.309017f, -0.500000f},
+    {-0.681718f, 0.147621f, -0.716567f}, {-0.681718f, -0.147621f, -0.716567f}, {-0.850651f, 0.000000f, -0.525731f},
+    {-0.688191f, 0.587785f, -0.425325f}, {-0.587785f, 0.425325f, -0.688191f}, {-0.425325f, 0.688191f, -0.587785f},
+    {-0.425325f, -0.688191f, -0.587785f}, {-0.587785f, -0.425325f, -0.688191f}, {-0.688191f, -0.587785f, -0.425325f},
+};
+
+void or_sky_dirs(float *dirs) { memcpy(dirs, or_anorms, sizeof or_anorms); }
+
+/* CanLeafTraceToSky (rad/lightmap/lightmap.go:425-451), with the intent of
+ * its broken SIMD loads: center = float32(mins + maxs) * 0.5 per axis (the
+ * sum taken in 32-bit integers; Go's int16 sum could wrap at the coordinate
+ * limit); for each of the 162 Anorms directions a, TestLineDoesHitSky from
+ * center to (a * -MAX_TRACE_LENGTH) + center with skipId
+ * TRACE_ID_STATICPROP | -1 = -1; true iff some lane's fractionVisible > 0,
+ * i.e. the segment misses every non-sky triangle before its closest sky
+ * hit.  "Any" does not depend on the 4-lane grouping of the directions. */
+typedef struct {
+  const or_kdtree_t *kd;
+  const or_tri_t *tris;
+  int64_t ntri;
+  const int16_t *mins, *maxs;
+  uint8_t *out;
+  float M;
+} sky_ctx_t;
+
+static void leaf_sky_range(void *vc, int64_t b, int64_t e) {
+  const sky_ctx_t *c = (const sky_ctx_t *)vc;
+  const float MTL = (float)OR_MAX_TRACE_LENGTH;
+  for (int64_t i = b; i < e; ++i) {
+    float ctr[3];
+    for (int a = 0; a < 3; ++a)
+      ctr[a] = (float)((int32_t)c->mins[3 * i + a] + (int32_t)c->maxs[3 * i + a]) * 0.5f;
+    uint8_t any = 0;
+    for (int k = 0; k < 162 && !any; ++k) {
+      float en[3];
+      for (int a = 0; a < 3; ++a) en[a] = or_anorms[k][a] * -MTL + ctr[a];
+      uint8_t v;
+      testline_one(c->kd, c->M, c->tris, c->ntri, ctr, en, -1, &v, NULL, NULL, NULL);
+      any = v;
+    }
+    c->out[i] = any;
   }
+}
+
+void or_can_leaf_trace_to_sky(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                              const int16_t *mins, const int16_t *maxs, int64_t n,
+                              uint8_t *can_see_sky, int nthreads) {
+  sky_ctx_t c = {kd, tris, ntri, mins, maxs, can_see_sky, kd ? kd_scene_maxabs(kd) : 1.0f};
+  or_parallel(n, nthreads, leaf_sky_range, &c);
 }
 
 typedef struct {
@@ -903,8 +995,8 @@ static void gather_one(const gl_ctx_t *c, int64_t i, float out[3]) {
         if (!(dot2 > L->stopdot2)) continue;
         if (dot2 <= L->stopdot)
           fall = fall * ((dot2 - L->stopdot2) / (L->stopdot - L->stopdot2));
-        if (L->exponent != 0.0f && L->exponent != 1.0f)
-          fall = fall * powf(dot2, L->exponent);
+        if (L->exponent != 0.0f && L->exponent != 1.0f) /* float64 power, rounded once */
+          fall = fall * (float)pow((double)dot2, (double)L->exponent);
       }
       if (L->end_fade > L->start_fade) {
         if (dist >= L->end_fade) continue;

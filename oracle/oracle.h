@@ -127,6 +127,10 @@ void or_gather_light(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
                      int32_t n_lights, float normal_offset, int32_t skip_id,
                      float *rgb, int nthreads);
 void or_sky_dirs(float *dirs /* 162*3 */);
+/* CanLeafTraceToSky per leaf: mins/maxs are n*3 int16 (dleaf_t bounds) */
+void or_can_leaf_trace_to_sky(const or_kdtree_t *kd, const or_tri_t *tris, int64_t ntri,
+                              const int16_t *mins, const int16_t *maxs, int64_t n,
+                              uint8_t *can_see_sky, int nthreads);
 
 /* ---- BounceLight + FP64 energy ---- */
 /* FP32 in the exact order of a bounce kernel (coef_kind 0=f32, 1=f16, 2=bf16

@@ -23,6 +23,43 @@ def box(mn, mx, id_=0x02000000):
     return t
 
 
+SKY = 0x01000000     # TRACE_ID_SKY (raytracer/constants.go:9)
+OPAQUE = 0x02000000  # TRACE_ID_OPAQUE
+MTL = float(np.float32(1.732050807569 * 32768.0))  # MAX_TRACE_LENGTH as float32
+
+
+def light(kind, origin=(0, 0, 0), normal=(0, 0, 0), inten=(1.0, 2.0, 3.0)):
+    """One light record of the oracle's layout (no fade, no cap)."""
+    import oracle
+    L = np.zeros(1, oracle.LIGHT_DTYPE)
+    L["type"] = kind
+    L["origin"] = origin
+    L["normal"] = normal
+    L["intensity"] = inten
+    L["end_fade"] = -1
+    L["cap_distance"] = 1e22
+    return L
+
+
+def open_box(lo=0.0, hi=100.0, roof=False, sky_quad=True, open_wall=False):
+    """Floor and four walls of the cube [lo, hi]^3 (no top), optionally an
+    opaque roof at z = hi, a sky quad far above, and the x = hi wall left out."""
+    a, b = lo, hi
+    t = []
+    t += quad(OPAQUE, (a, a, a), (b, a, a), (b, b, a), (a, b, a))            # floor
+    t += quad(OPAQUE, (a, a, a), (a, b, a), (a, b, b), (a, a, b))            # x = lo
+    if not open_wall:
+        t += quad(OPAQUE, (b, a, a), (b, b, a), (b, b, b), (b, a, b))        # x = hi
+    t += quad(OPAQUE, (a, a, a), (b, a, a), (b, a, b), (a, a, b))            # y = lo
+    t += quad(OPAQUE, (a, b, a), (b, b, a), (b, b, b), (a, b, b))            # y = hi
+    if roof:
+        t += quad(OPAQUE, (a, a, b), (b, a, b), (b, b, b), (a, b, b))
+    if sky_quad:
+        t += quad(SKY, (-1e4, -1e4, 4 * hi), (1e4, -1e4, 4 * hi), (1e4, 1e4, 4 * hi),
+                  (-1e4, 1e4, 4 * hi))
+    return t
+
+
 def to_arrays(tris):
     v9 = np.array([[*a, *b, *c] for _, a, b, c in tris], np.float32).reshape(-1, 9)
     ids = np.array([i for i, *_ in tris], np.int32)

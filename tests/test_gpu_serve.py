@@ -138,3 +138,37 @@ def test_serve_plain_matches_sequential_restatement(scenes16):
             assert np.array_equal(en, oe)
     finally:
         sv.close()
+
+
+def test_served_transfers_refuse_relayout(scenes16):
+    """While a server holds a transfer set, calls that would reallocate or
+    free its buffers fail with VR_E_STATE before any work, a bounce on the
+    layout the server uses still runs, and the server keeps serving the same
+    results; after the server is gone the calls work again."""
+    print("[gpu] served transfers refuse re-layout", flush=True)
+    b = scenes16[0]
+    s = b.s
+    sv = _server(BATCHED, [b], max_active=2)
+    lib = _lib.lib
+    try:
+        h = b.xf.handle
+        assert lib.vr_xfers_set_precision(h, _lib.VR_COEF_BF16) == _lib.VR_E_STATE
+        assert lib.vr_xfers_destroy(h) == _lib.VR_E_STATE
+        e0 = (s.p_refl * 0.5).astype(np.float32)
+        tot, _, k = lightmap.BounceLight(b.xf, e0, s.p_refl, s.p_area, max_bounces=2,
+                                         stop_frac=0.0)                  # same layout: runs
+        assert k == 2
+        _lib.check(lib.vr_set_tuning(b"bounce_tile_cols", 2048))
+        try:
+            with pytest.raises(_lib.VrError) as ei:
+                lightmap.BounceLight(b.xf, e0, s.p_refl, s.p_area, max_bounces=2, stop_frac=0.0)
+            assert ei.value.code == _lib.VR_E_STATE
+        finally:
+            _lib.check(lib.vr_set_tuning(b"bounce_tile_cols", 4096))
+        moves = _requests([b], 1, seed=9)[0][1]
+        tot, en, _ = sv.wait(sv.submit(0, moves))
+        ot, oe, _ = _expected(b, moves, _lib.VR_COEF_F16)
+        assert np.array_equal(tot.view(np.uint32), ot.view(np.uint32))
+    finally:
+        sv.close()
+    _lib.check(lib.vr_xfers_set_precision(b.xf.handle, _lib.VR_COEF_F16))

@@ -69,6 +69,20 @@ def test_tree_and_precompute_match_oracle_level():
     _same_tree(s.verts9, s.ids)
 
 
+def test_precompute_general_position_goldens(golden_dir):
+    """The product's host precompute against the exact-rational float32
+    emulation of mgl32 (triangles where every operation rounds)."""
+    import json
+    g = json.load(open(os.path.join(golden_dir, "tri_precompute_general.json")))
+    v9 = np.array([t["v"] for t in g["triangles"]], np.float32)
+    ids = np.array([t["id"] for t in g["triangles"]], np.int32)
+    tri = _host_scene(v9, ids)[4]
+    for r, t in zip(tri, g["triangles"]):
+        assert list(np.float32([r["FlNx"], r["FlNy"], r["FlNz"]]).view(np.uint32)) == t["N_bits"]
+        assert int(np.float32(r["FlD"]).view(np.uint32)) == t["D_bits"]
+        assert list(np.asarray(r["ProjectedEdgeEquations"], np.float32).view(np.uint32)) == t["E_bits"]
+
+
 def test_tree_degenerate_inputs():
     _same_tree(np.zeros((0, 9), np.float32), np.zeros(0, np.int32))
     v9, ids = H.to_arrays(H.box((0, 0, 0), (1, 1, 1)))
@@ -206,4 +220,5 @@ def test_light_helpers():
     a, b, c = lightmap.solve_inverse_quadratic(0, 1, 1, 2, 2, 5)   # y = x^2 + 1
     assert (a, b, c) == (1.0, 0.0, 1.0)
     S = lightmap.spot_light((0, 0, 0), (0, 0, -2), (1, 1, 1), 120, 150)
-    assert S["stopdot"][0] == np.float32(np.cos(np.pi / 2))   # clamped to 90 degrees
+    # clamped to 90 degrees; 90 / 180 * pi in float32 is just above pi / 2
+    assert S["stopdot"][0] == np.float32(-4.371139e-08) == S["stopdot2"][0]

@@ -27,6 +27,24 @@ def test_precompute_golden(golden_dir):
         assert rec["id"] == t["id"]
 
 
+def test_precompute_golden_general_position(golden_dir):
+    """Triangles where every operation rounds: the expected bits come from an
+    exact-rational float32 emulation of mgl32's Cross/Len/Normalize/Dot, and
+    each triangle is one that a plausible mistake (N / Len, a float64 Len,
+    another association, a fused multiply-add) gets wrong."""
+    g = json.load(open(os.path.join(golden_dir, "tri_precompute_general.json")))
+    assert len(g["triangles"]) == 12
+    caught = set()
+    for t in g["triangles"]:
+        rec = oracle.precompute(np.array(t["v"], np.float32), np.array([t["id"]], np.int32))[0]
+        assert list(_bits([rec["nx"], rec["ny"], rec["nz"]])) == t["N_bits"], t["v"]
+        assert int(_bits(rec["d"])) == t["D_bits"], t["v"]
+        assert list(_bits(rec["e"])) == t["E_bits"], t["v"]
+        assert (rec["c0"], rec["c1"]) == (t["c0"], t["c1"])
+        caught |= set(t["catches"])
+    assert len(caught) == 5
+
+
 def test_precompute_edge_invariants(room_scene):
     """E1.(p,1): 0 at p1 and p2, 1 at p3; E2.(p,1): 0 at p2, p3, 1 at p1."""
     s = room_scene

@@ -155,13 +155,14 @@ _SIGS = {
                                     C.POINTER(_P)]),
     "vr_xfers_from_csr": (C.c_int, [C.POINTER(_P), C.c_int64, _P, _P, _P, C.c_int32]),
     "vr_xfers_synthetic": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int64, C.c_double,
-                                     C.c_double, C.c_int64, C.c_int32]),
+                                     C.c_double, C.c_int64, C.c_int32, C.c_int32]),
     "vr_xfers_set_precision": (C.c_int, [_P, C.c_int32]),
     "vr_xfers_get_info": (C.c_int, [_P, C.POINTER(XfersInfo)]),
     "vr_xfers_download": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "vr_xfers_destroy": (C.c_int, [_P]),
     "vr_gather_sample_light": (C.c_int, [_P, C.POINTER(Samples), C.c_int64, _P, C.c_int32,
                                          C.c_float, C.c_int32, _P]),
+    "vr_can_leaf_trace_to_sky": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
     "vr_bounce_light": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(BounceParams), _P,
                                   C.POINTER(C.c_int32)]),
     "vr_bounce_light_batch": (C.c_int, [C.POINTER(BounceJob), C.c_int32, C.POINTER(BounceParams),

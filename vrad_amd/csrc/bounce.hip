@@ -1278,22 +1278,29 @@ __device__ __forceinline__ double u01(uint64_t seed, uint64_t a, uint64_t b) {
   return ((double)(z >> 11) + 1.0) * 0x1p-53;  // (0, 1]
 }
 
-// one wave per row: columns one per stratum of [0, n-1) mapped past the row
-// index; coef = u / sum(u) * s_i, sum in FP64 in the wave order
-__global__ __launch_bounds__(256) void k_synthetic_fill(int64_t n, const int64_t *__restrict__ rp,
-                                                        const int32_t *__restrict__ len,
-                                                        uint64_t seed, int32_t *__restrict__ col,
-                                                        float *__restrict__ coef) {
+// Coefficients of row i: coef_p = u_p / sum(u) * s_i, u_p = u01(seed, i, p),
+// the sum in FP64 in wave order (lane partials, then an xor butterfly).
+__device__ __forceinline__ void synthetic_coefs(uint64_t seed, int64_t i, int64_t L, int lane,
+                                                double &sum, double &si) {
+  sum = 0.0;
+  for (int64_t p = lane; p < L; p += 64) sum += u01(seed, (uint64_t)i, (uint64_t)p);
+  for (int off = 32; off >= 1; off >>= 1) sum = sum + __shfl_xor(sum, off);
+  si = 0.3 + 0.6 * u01(seed ^ 0xA5A5A5A5ull, (uint64_t)i, 0xFFFFFFFFull);
+}
+
+// Stratified columns (the named option): one wave per row, column p drawn in
+// stratum [p m / L, (p + 1) m / L) of [0, m), m = n - 1, mapped past i.
+__global__ __launch_bounds__(256) void k_synthetic_fill_strata(
+    int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ len, uint64_t seed,
+    int32_t *__restrict__ col, float *__restrict__ coef) {
   const int lane = threadIdx.x & 63;
   const int64_t wave =
       (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t nwaves = (int64_t)gridDim.x * 4;
   for (int64_t i = wave; i < n; i += nwaves) {
     const int64_t L = len[i], s = rp[i], e = rp[i + 1];
-    double sum = 0.0;
-    for (int64_t p = lane; p < L; p += 64) sum += u01(seed, (uint64_t)i, (uint64_t)p);
-    for (int off = 32; off >= 1; off >>= 1) sum = sum + __shfl_xor(sum, off);
-    const double si = 0.3 + 0.6 * u01(seed ^ 0xA5A5A5A5ull, (uint64_t)i, 0xFFFFFFFFull);
+    double sum, si;
+    synthetic_coefs(seed, i, L, lane, sum, si);
     const int64_t m = n - 1;
     for (int64_t p = lane; p < e - s; p += 64) {
       if (p < L) {
@@ -1312,13 +1319,110 @@ __global__ __launch_bounds__(256) void k_synthetic_fill(int64_t n, const int64_t
   }
 }
 
+// Uniform columns without replacement (the default): one wave per row.  The
+// draws d = 0, 1, 2, ... give candidates c_d = floor(u01(seed ^ K, i, d) m)
+// in [0, m), m = n - 1; the row takes the first L distinct candidates, which
+// is a uniformly random L-subset.  A round takes draws 64 r + lane: a draw is
+// new when no lower lane drew the same value this round and its bit in the
+// wave's bitmap of [0, m) was clear (the atomic OR that sets it returns the
+// old word); only the first L - taken new draws in lane order are kept (the
+// others clear their bit again), so the result equals the sequential
+// process.  The bitmap is then read in column order to write the row sorted
+// (mapped past i) and cleared on the way.  Bitmap words are accessed only
+// with device-scope atomics, loads and stores, which all act at L2.
+__global__ __launch_bounds__(256) void k_synthetic_fill_uniform(
+    int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ len, uint64_t seed,
+    uint32_t *__restrict__ bitmaps, int64_t words, int32_t *__restrict__ col,
+    float *__restrict__ coef) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave =
+      (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  uint32_t *bm = bitmaps + wave * words;
+  const int64_t m = n - 1;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int64_t i = wave; i < n; i += nwaves) {
+    const int64_t L = len[i], s = rp[i], e = rp[i + 1];
+    int64_t taken = 0;
+    for (int64_t r = 0; taken < L; ++r) {
+      const double u = u01(seed ^ 0x5bd1e995ull, (uint64_t)i, (uint64_t)(64 * r + lane));
+      int c = (int)(u * (double)m);
+      if (c >= m) c = (int)m - 1;
+      bool dup = false;
+      for (int j = 0; j < 63; ++j) {
+        const int v = __shfl(c, j);
+        if (j < lane && v == c) dup = true;
+      }
+      const uint32_t bit = 1u << (c & 31);
+      uint32_t old = bit;
+      if (!dup)
+        old = __hip_atomic_fetch_or(&bm[c >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const bool fresh = (old & bit) == 0u;
+      const int rank = __popcll(__ballot(fresh) & lt);
+      const bool keep = fresh && (int64_t)rank < L - taken;
+      if (fresh && !keep)
+        __hip_atomic_fetch_and(&bm[c >> 5], ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      taken += __popcll(__ballot(keep));
+    }
+    double sum, si;
+    synthetic_coefs(seed, i, L, lane, sum, si);
+    // read the bitmap in column order: 64 words per step, a wave prefix of
+    // their bit counts gives each set bit its position in the row
+    int64_t base = 0;
+    for (int64_t w0 = 0; w0 < words; w0 += 64) {
+      const int64_t w = w0 + lane;
+      uint32_t bits =
+          w < words ? __hip_atomic_load(&bm[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+      const int cnt = __popc(bits);
+      int incl = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+      }
+      int64_t pos = base + incl - cnt;
+      if (bits) __hip_atomic_store(&bm[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while (bits) {
+        const int b = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        int64_t c = (w << 5) + b;
+        if (c >= i) ++c;
+        col[s + pos] = (int32_t)c;
+        coef[s + pos] = (float)(u01(seed, (uint64_t)i, (uint64_t)pos) / sum * si);
+        ++pos;
+      }
+      base += __shfl(incl, 63);
+    }
+    for (int64_t p = L + lane; p < e - s; p += 64) {
+      col[s + p] = (int32_t)i;
+      coef[s + p] = 0.0f;
+    }
+    __threadfence();  // the clears land before the next row's atomics
+  }
+}
+
+int64_t synthetic_bitmap_words(int64_t n) { return (n - 1 + 31) / 32; }
+int64_t synthetic_waves(int64_t n) {
+  // up to 8192 waves, the bitmaps held to 512 MiB
+  const int64_t words = synthetic_bitmap_words(n);
+  int64_t w = ((int64_t)512 << 20) / (4 * std::max<int64_t>(words, 1));
+  w = std::min<int64_t>(8192, std::max<int64_t>(4, w)) & ~(int64_t)3;
+  return std::min<int64_t>(w, ((n + 3) / 4) * 4);
+}
+
 void launch_synthetic_fill(int64_t n, const int64_t *rp, const int32_t *len, uint64_t seed,
-                           int32_t *col, float *coef, hipStream_t st) {
+                           int columns, uint32_t *bitmaps, int32_t *col, float *coef,
+                           hipStream_t st) {
   if (n <= 0) return;
-  int64_t blocks = (n + 3) / 4;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_synthetic_fill, dim3((unsigned)blocks), dim3(256), 0, st, n, rp, len, seed,
-                     col, coef);
+  if (columns == 1) {
+    int64_t blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_synthetic_fill_strata, dim3((unsigned)blocks), dim3(256), 0, st, n, rp,
+                       len, seed, col, coef);
+    return;
+  }
+  const int64_t blocks = synthetic_waves(n) / 4;
+  hipLaunchKernelGGL(k_synthetic_fill_uniform, dim3((unsigned)blocks), dim3(256), 0, st, n, rp,
+                     len, seed, bitmaps, synthetic_bitmap_words(n), col, coef);
 }
 
 // ---- 16-bit coefficient encoding: scale = row max, q = rne(coef / scale) --

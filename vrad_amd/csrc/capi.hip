@@ -185,6 +185,9 @@ struct vr_xfers {
   DBuf bidx, bcoef, bstart, boff;
   int blk_cols = 0, blk_prec = -1, blk_stride = 0;
   int64_t blk_chunks = 0;
+  // servers holding this set: while > 0 its buffers are read by a serving
+  // thread, so nothing may reallocate or free them
+  int served = 0;
   DevXfers dev() const {
     DevXfers x;
     x.n = n;
@@ -227,16 +230,70 @@ int finish_layout(vr_xfers *x, hipStream_t st) {
   return rc;
 }
 
+// The 162 fixed sky directions: the reference's Anorms table
+// (vmath/constants.go:15-184), the vertices of a frequency-4 geodesic
+// icosphere rounded to 6 decimals.  CanLeafTraceToSky walks them
+// (rad/lightmap/lightmap.go:435-448); sky ambient weights them by cosine.
+// Each literal converts to the nearest float32, as Go's mgl32.Vec3 literal does.
+static const float kAnorms[162][3] = {
+    {-0.525731f, 0.000000f, 0.850651f}, {-0.442863f, 0.238856f, 0.864188f}, {-0.295242f, 0.000000f, 0.955423f},
+    {-0.309017f, 0.500000f, 0.809017f}, {-0.162460f, 0.262866f, 0.951056f}, {0.000000f, 0.000000f, 1.000000f},
+    {0.000000f, 0.850651f, 0.525731f}, {-0.147621f, 0.716567f, 0.681718f}, {0.147621f, 0.716567f, 0.681718f},
+    {0.000000f, 0.525731f, 0.850651f}, {0.309017f, 0.500000f, 0.809017f}, {0.525731f, 0.000000f, 0.850651f},
+    {0.295242f, 0.000000f, 0.955423f}, {0.442863f, 0.238856f, 0.864188f}, {0.162460f, 0.262866f, 0.951056f},
+    {-0.681718f, 0.147621f, 0.716567f}, {-0.809017f, 0.309017f, 0.500000f}, {-0.587785f, 0.425325f, 0.688191f},
+    {-0.850651f, 0.525731f, 0.000000f}, {-0.864188f, 0.442863f, 0.238856f}, {-0.716567f, 0.681718f, 0.147621f},
+    {-0.688191f, 0.587785f, 0.425325f}, {-0.500000f, 0.809017f, 0.309017f}, {-0.238856f, 0.864188f, 0.442863f},
+    {-0.425325f, 0.688191f, 0.587785f}, {-0.716567f, 0.681718f, -0.147621f}, {-0.500000f, 0.809017f, -0.309017f},
+    {-0.525731f, 0.850651f, 0.000000f}, {0.000000f, 0.850651f, -0.525731f}, {-0.238856f, 0.864188f, -0.442863f},
+    {0.000000f, 0.955423f, -0.295242f}, {-0.262866f, 0.951056f, -0.162460f}, {0.000000f, 1.000000f, 0.000000f},
+    {0.000000f, 0.955423f, 0.295242f}, {-0.262866f, 0.951056f, 0.162460f}, {0.238856f, 0.864188f, 0.442863f},
+    {0.262866f, 0.951056f, 0.162460f}, {0.500000f, 0.809017f, 0.309017f}, {0.238856f, 0.864188f, -0.442863f},
+    {0.262866f, 0.951056f, -0.162460f}, {0.500000f, 0.809017f, -0.309017f}, {0.850651f, 0.525731f, 0.000000f},
+    {0.716567f, 0.681718f, 0.147621f}, {0.716567f, 0.681718f, -0.147621f}, {0.525731f, 0.850651f, 0.000000f},
+    {0.425325f, 0.688191f, 0.587785f}, {0.864188f, 0.442863f, 0.238856f}, {0.688191f, 0.587785f, 0.425325f},
+    {0.809017f, 0.309017f, 0.500000f}, {0.681718f, 0.147621f, 0.716567f}, {0.587785f, 0.425325f, 0.688191f},
+    {0.955423f, 0.295242f, 0.000000f}, {1.000000f, 0.000000f, 0.000000f}, {0.951056f, 0.162460f, 0.262866f},
+    {0.850651f, -0.525731f, 0.000000f}, {0.955423f, -0.295242f, 0.000000f}, {0.864188f, -0.442863f, 0.238856f},
+    {0.951056f, -0.162460f, 0.262866f}, {0.809017f, -0.309017f, 0.500000f}, {0.681718f, -0.147621f, 0.716567f},
+    {0.850651f, 0.000000f, 0.525731f}, {0.864188f, 0.442863f, -0.238856f}, {0.809017f, 0.309017f, -0.500000f},
+    {0.951056f, 0.162460f, -0.262866f}, {0.525731f, 0.000000f, -0.850651f}, {0.681718f, 0.147621f, -0.716567f},
+    {0.681718f, -0.147621f, -0.716567f}, {0.850651f, 0.000000f, -0.525731f}, {0.809017f, -0.309017f, -0.500000f},
+    {0.864188f, -0.442863f, -0.238856f}, {0.951056f, -0.162460f, -0.262866f}, {0.147621f, 0.716567f, -0.681718f},
+    {0.309017f, 0.500000f, -0.809017f}, {0.425325f, 0.688191f, -0.587785f}, {0.442863f, 0.238856f, -0.864188f},
+    {0.587785f, 0.425325f, -0.688191f}, {0.688191f, 0.587785f, -0.425325f}, {-0.147621f, 0.716567f, -0.681718f},
+    {-0.309017f, 0.500000f, -0.809017f}, {0.000000f, 0.525731f, -0.850651f}, {-0.525731f, 0.000000f, -0.850651f},
+    {-0.442863f, 0.238856f, -0.864188f}, {-0.295242f, 0.000000f, -0.955423f}, {-0.162460f, 0.262866f, -0.951056f},
+    {0.000000f, 0.000000f, -1.000000f}, {0.295242f, 0.000000f, -0.955423f}, {0.162460f, 0.262866f, -0.951056f},
+    {-0.442863f, -0.238856f, -0.864188f}, {-0.309017f, -0.500000f, -0.809017f}, {-0.162460f, -0.262866f, -0.951056f},
+    {0.000000f, -0.850651f, -0.525731f}, {-0.147621f, -0.716567f, -0.681718f}, {0.147621f, -0.716567f, -0.681718f},
+    {0.000000f, -0.525731f, -0.850651f}, {0.309017f, -0.500000f, -0.809017f}, {0.442863f, -0.238856f, -0.864188f},
+    {0.162460f, -0.262866f, -0.951056f}, {0.238856f, -0.864188f, -0.442863f}, {0.500000f, -0.809017f, -0.309017f},
+    {0.425325f, -0.688191f, -0.587785f}, {0.716567f, -0.681718f, -0.147621f}, {0.688191f, -0.587785f, -0.425325f},
+    {0.587785f, -0.425325f, -0.688191f}, {0.000000f, -0.955423f, -0.295242f}, {0.000000f, -1.000000f, 0.000000f},
+    {0.262866f, -0.951056f, -0.162460f}, {0.000000f, -0.850651f, 0.525731f}, {0.000000f, -0.955423f, 0.295242f},
+    {0.238856f, -0.864188f, 0.442863f}, {0.262866f, -0.951056f, 0.162460f}, {0.500000f, -0.809017f, 0.309017f},
+    {0.716567f, -0.681718f, 0.147621f}, {0.525731f, -0.850651f, 0.000000f}, {-0.238856f, -0.864188f, -0.442863f},
+    {-0.500000f, -0.809017f, -0.309017f}, {-0.262866f, -0.951056f, -0.162460f}, {-0.850651f, -0.525731f, 0.000000f},
+    {-0.716567f, -0.681718f, -0.147621f}, {-0.716567f, -0.681718f, 0.147621f}, {-0.525731f, -0.850651f, 0.000000f},
+    {-0.500000f, -0.809017f, 0.309017f}, {-0.238856f, -0.864188f, 0.442863f}, {-0.262866f, -0.951056f, 0.162460f},
+    {-0.864188f, -0.442863f, 0.238856f}, {-0.809017f, -0.309017f, 0.500000f}, {-0.688191f, -0.587785f, 0.425325f},
+    {-0.681718f, -0.147621f, 0.716567f}, {-0.442863f, -0.238856f, 0.864188f}, {-0.587785f, -0.425325f, 0.688191f},
+    {-0.309017f, -0.500000f, 0.809017f}, {-0.147621f, -0.716567f, 0.681718f}, {-0.425325f, -0.688191f, 0.587785f},
+    {-0.162460f, -0.262866f, 0.951056f}, {0.442863f, -0.238856f, 0.864188f}, {0.162460f, -0.262866f, 0.951056f},
+    {0.309017f, -0.500000f, 0.809017f}, {0.147621f, -0.716567f, 0.681718f}, {0.000000f, -0.525731f, 0.850651f},
+    {0.425325f, -0.688191f, 0.587785f}, {0.587785f, -0.425325f, 0.688191f}, {0.688191f, -0.587785f, 0.425325f},
+    {-0.955423f, 0.295242f, 0.000000f}, {-0.951056f, 0.162460f, 0.262866f}, {-1.000000f, 0.000000f, 0.000000f},
+    {-0.850651f, 0.000000f, 0.525731f}, {-0.955423f, -0.295242f, 0.000000f}, {-0.951056f, -0.162460f, 0.262866f},
+    {-0.864188f, 0.442863f, -0.238856f}, {-0.951056f, 0.162460f, -0.262866f}, {-0.809017f, 0.309017f, -0.500000f},
+    {-0.864188f, -0.442863f, -0.238856f}, {-0.951056f, -0.162460f, -0.262866f}, {-0.809017f, -0.309017f, -0.500000f},
+    {-0.681718f, 0.147621f, -0.716567f}, {-0.681718f, -0.147621f, -0.716567f}, {-0.850651f, 0.000000f, -0.525731f},
+    {-0.688191f, 0.587785f, -0.425325f}, {-0.587785f, 0.425325f, -0.688191f}, {-0.425325f, 0.688191f, -0.587785f},
+    {-0.425325f, -0.688191f, -0.587785f}, {-0.587785f, -0.425325f, -0.688191f}, {-0.688191f, -0.587785f, -0.425325f},
+};
+
 std::vector<float> sky_dirs() {
-  std::vector<float> d(162 * 3);
-  const double ga = 3.14159265358979323846 * (3.0 - std::sqrt(5.0));
-  for (int k = 0; k < 162; ++k) {
-    double z = 1.0 - (2.0 * k + 1.0) / 162.0, r = std::sqrt(1.0 - z * z), ph = ga * k;
-    d[3 * k + 0] = (float)(r * std::cos(ph));
-    d[3 * k + 1] = (float)(r * std::sin(ph));
-    d[3 * k + 2] = (float)z;
-  }
-  return d;
+  return std::vector<float>(&kAnorms[0][0], &kAnorms[0][0] + 162 * 3);
 }
 
 // staging for host-pointer calls: one device block carved into arrays
@@ -922,12 +979,15 @@ int vr_xfers_from_csr(vr_xfers **out, int64_t n, const int64_t *row_ptr, const i
 }
 
 int vr_xfers_synthetic(vr_xfers **out, uint64_t seed, int64_t n, double mean_len, double sigma,
-                       int64_t max_len, int32_t precision) {
+                       int64_t max_len, int32_t precision, int32_t columns) {
   LOCK;
   if (!out) return fail(VR_E_INVALID, "out is NULL");
   *out = nullptr;
   if (n < 2 || mean_len < 1 || sigma < 0 || max_len < 1)
     return fail(VR_E_INVALID, "bad synthetic parameters");
+  if (columns != VR_SYNTH_UNIFORM && columns != VR_SYNTH_STRATIFIED)
+    return fail(VR_E_INVALID, "unknown column layout");
+  if (n > INT32_MAX) return fail(VR_E_CAPACITY, "columns are int32");
   TRY(ensure_init());
   std::unique_ptr<vr_xfers> x(new vr_xfers);
   x->n = n;
@@ -959,8 +1019,15 @@ int vr_xfers_synthetic(vr_xfers **out, uint64_t seed, int64_t n, double mean_len
   TRY(finish_layout(x.get(), g_stream));
   TRY(x->col.alloc(4 * (size_t)x->stored));
   TRY(x->coef32.alloc(4 * (size_t)x->stored));
-  launch_synthetic_fill(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), seed, x->col.as<int32_t>(),
-                        x->coef32.as<float>(), g_stream);
+  DBuf bitmaps;
+  if (columns == VR_SYNTH_UNIFORM) {
+    const size_t bb = 4 * (size_t)synthetic_waves(n) * (size_t)synthetic_bitmap_words(n);
+    TRY(bitmaps.alloc(bb));
+    VR_HIP(hipMemsetAsync(bitmaps.p, 0, bb, g_stream));
+  }
+  launch_synthetic_fill(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), seed, columns,
+                        bitmaps.as<uint32_t>(), x->col.as<int32_t>(), x->coef32.as<float>(),
+                        g_stream);
   TRY(check_launch());
   VR_HIP(hipStreamSynchronize(g_stream));
   vr_xfers *rx = x.release();
@@ -978,6 +1045,7 @@ int vr_xfers_synthetic(vr_xfers **out, uint64_t seed, int64_t n, double mean_len
 int vr_xfers_set_precision(vr_xfers *x, int32_t precision) {
   LOCK;
   if (!x) return fail(VR_E_INVALID, "xfers is NULL");
+  if (x->served) return fail(VR_E_STATE, "transfers are held by a server");
   if (precision != VR_COEF_F32 && precision != VR_COEF_F16 && precision != VR_COEF_BF16)
     return fail(VR_E_INVALID, "unknown precision");
   TRY(ensure_init());
@@ -1062,6 +1130,7 @@ int vr_xfers_download(const vr_xfers *x, int64_t *row_ptr, int32_t *col, float *
 
 int vr_xfers_destroy(vr_xfers *x) {
   LOCK;
+  if (x && x->served) return fail(VR_E_STATE, "transfers are held by a server");
   delete x;
   return VR_OK;
 }
@@ -1096,10 +1165,41 @@ int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n, const 
   {
     Prof p("gather_light", g_stream);
     launch_gather_light(s->dev, ds, n, S.ptr<vr_light>(7), n_lights, S.ptr<float>(8),
-                        normal_offset, skip_id, S.ptr<float>(6), g_path, g_stream);
+                        normal_offset, skip_id, S.ptr<float>(6), g_path,
+                        tuning().trace_lds_entries, g_stream);
   }
   TRY(check_launch());
   VR_HIP(hipMemcpyAsync(rgb, S.ptr<float>(6), 3 * fb, hipMemcpyDeviceToHost, g_stream));
+  VR_HIP(hipStreamSynchronize(g_stream));
+  return VR_OK;
+}
+
+int vr_can_leaf_trace_to_sky(vr_scene *s, const int16_t *mins, const int16_t *maxs, int64_t n,
+                             uint8_t *can_see_sky) {
+  LOCK;
+  if (!s) return fail(VR_E_INVALID, "NULL scene");
+  if (!s->built) return fail(VR_E_STATE, "scene not built");
+  if (n < 0) return fail(VR_E_INVALID, "negative count");
+  if (n == 0) return VR_OK;
+  if (!mins || !maxs || !can_see_sky) return fail(VR_E_INVALID, "NULL bounds/output");
+  if (n > (int64_t)UINT32_MAX) return fail(VR_E_CAPACITY, "more than 2^32 - 1 leaves per call");
+  TRY(ensure_init());
+  const size_t bb = 6 * (size_t)n;
+  Staging S;
+  TRY(S.make({bb, bb, (size_t)n, 162 * 3 * 4}));
+  VR_HIP(hipMemcpyAsync(S.ptr<int16_t>(0), mins, bb, hipMemcpyHostToDevice, g_stream));
+  VR_HIP(hipMemcpyAsync(S.ptr<int16_t>(1), maxs, bb, hipMemcpyHostToDevice, g_stream));
+  std::vector<float> dirs = sky_dirs();
+  VR_HIP(hipMemcpyAsync(S.ptr<float>(3), dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice,
+                        g_stream));
+  {
+    Prof p("leaf_sky", g_stream);
+    launch_leaf_sky(s->dev, S.ptr<int16_t>(0), S.ptr<int16_t>(1), n, S.ptr<float>(3),
+                    S.ptr<uint8_t>(2), g_stream);
+  }
+  TRY(check_launch());
+  VR_HIP(hipMemcpyAsync(can_see_sky, S.ptr<uint8_t>(2), (size_t)n, hipMemcpyDeviceToHost,
+                        g_stream));
   VR_HIP(hipStreamSynchronize(g_stream));
   return VR_OK;
 }
@@ -1127,6 +1227,16 @@ int tiled_shape(const vr_xfers *x, const Tuning &tu, TiledShape &sh) {
   return VR_OK;
 }
 
+// Whether x's tile pointers and tile-blocked copy already fit shape sh (so
+// prepare_tiles allocates and launches nothing).
+bool tiles_ready(const vr_xfers *x, const TiledShape &sh) {
+  if (x->n == 0) return true;
+  const int ostride = sh.kernel == 2 ? 16 : 12;
+  return x->tp_cols == sh.cols && x->tp.p && x->blk_cols == sh.cols &&
+         x->blk_prec == x->precision && x->blk_chunks == gather_chunks(x->n) &&
+         x->blk_stride == ostride;
+}
+
 // Tile pointers and the tile-blocked copy of x for the tile width, built once
 // and kept with the transfers; fills tl.
 int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t st) {
@@ -1143,6 +1253,8 @@ int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t s
   tl.n_chunks = gather_chunks(n);
   if (n == 0) return VR_OK;
   const DevXfers dx = x->dev();
+  if (x->served && !tiles_ready(x, sh))
+    return fail(VR_E_STATE, "transfers are held by a server with another gather layout");
   if (x->tp_cols != cols || !x->tp.p) {
     TRY(x->tp.alloc(4 * (size_t)n * (size_t)(nt + 1)));
     launch_tile_ptrs(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), x->col.as<int32_t>(), cols, nt,
@@ -1388,6 +1500,10 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs, const vr_bounce_p
   hipStream_t st = g_stream;
   TiledShape sh;
   TRY(tiled_shape(jobs[0].xfers, tu, sh));
+  for (int j = 0; j < n_jobs; ++j)  // refuse before any work
+    if (jobs[j].xfers->served && !tiles_ready(jobs[j].xfers, sh))
+      return fail(VR_E_STATE, "job " + std::to_string(j) +
+                                  ": transfers are held by a server with another gather layout");
   std::vector<BounceJob> hj((size_t)n_jobs);
   std::vector<JobBuffers> bufs((size_t)n_jobs);
   DBuf d_states, d_jobs;
@@ -1535,6 +1651,9 @@ struct vr_server {
   DBuf sky, states, jobs_dev;
   vr::BounceJob *jobs_host = nullptr;  // pinned mirror of the job table
   hipStream_t stream = nullptr;
+  // launch knobs fixed at creation: the serving thread reads these, never the
+  // library-wide tuning a concurrent vr_set_tuning may change
+  vr::Tuning tune{};
   vr::TiledShape shape{};
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -1567,7 +1686,7 @@ int serve_request_setup(vr_server *sv, ServeReq *r) {
   const int path = sv->prm.mode == VR_SERVE_PLAIN ? VR_PATH_PLAIN : VR_PATH_OPT;
   launch_gather_light(sc.scene->dev, ds, n, reinterpret_cast<vr_light *>(base + L.lights),
                       (int32_t)r->lights.size(), sv->sky.as<float>(), sv->prm.normal_offset, -1,
-                      direct, path, st);
+                      direct, path, sv->tune.trace_lds_entries, st);
   launch_emit_init(n, direct, sc.refl.as<float>(), reinterpret_cast<float *>(base + L.emit0),
                    reinterpret_cast<float *>(base + L.total), st);
   VR_HIP(hipMemsetAsync(sv->states.as<BounceState>() + r->slot, 0, sizeof(BounceState), st));
@@ -1759,6 +1878,7 @@ int vr_server_create(vr_server **out, const vr_server_params *p) {
   TRY(ensure_init());
   std::unique_ptr<vr_server> sv(new vr_server);
   sv->prm = *p;
+  sv->tune = tuning();
   if (sv->prm.mode == VR_SERVE_PLAIN) sv->prm.max_active = 1;
   if (sv->prm.hbm_budget <= 0) sv->prm.hbm_budget = (int64_t)8 << 30;
   if (!(sv->prm.normal_offset >= 0)) sv->prm.normal_offset = 0.1f;
@@ -1813,12 +1933,20 @@ int vr_server_add_scene(vr_server *sv, vr_scene *scene, vr_xfers *x, const vr_sa
     VR_HIP(hipMemcpyAsync(sc->area.p, area, 4 * (size_t)n, hipMemcpyHostToDevice, g_stream));
   }
   if (sv->prm.mode == VR_SERVE_BATCHED) {
-    TRY(vr::tiled_shape(x, tuning(), sv->shape));
-    TRY(vr::prepare_tiles(x, sv->shape, sc->tl, g_stream));
+    vr::TiledShape sh{};
+    TRY(vr::tiled_shape(x, sv->tune, sh));
+    if (!sv->scenes.empty() &&
+        (sh.kernel != sv->shape.kernel || sh.cols != sv->shape.cols ||
+         sh.group != sv->shape.group || sh.passes != sv->shape.passes ||
+         sh.prefetch != sv->shape.prefetch))
+      return fail(VR_E_INVALID, "the scenes of a server share one gather shape");
+    TRY(vr::prepare_tiles(x, sh, sc->tl, g_stream));
+    sv->shape = sh;
   } else if (!x->coef32.p) {
     return fail(VR_E_STATE, "plain serving needs the FP32 coefficients");
   }
   VR_HIP(hipStreamSynchronize(g_stream));
+  x->served += 1;
   *scene_id = (int32_t)sv->scenes.size();
   sv->scenes.push_back(std::move(sc));
   return VR_OK;
@@ -1905,6 +2033,10 @@ int vr_server_destroy(vr_server *sv) {
     (void)hipStreamDestroy(sv->stream);
   }
   if (sv->jobs_host) (void)hipHostFree(sv->jobs_host);
+  {
+    LOCK;
+    for (auto &sc : sv->scenes) sc->xfers->served -= 1;
+  }
   delete sv;
   return VR_OK;
 }

@@ -204,8 +204,13 @@ void launch_block_fill(const DevXfers &x, const DevTiles &tl, int G, int coef_by
 // energy of the bounce just gathered; updates state / energy[k]
 void launch_energy(const double *e_row, int64_t n, double *scratch, double *energy, int k,
                    double stop_frac, BounceState *state, hipStream_t st);
+// columns: 0 = uniform without replacement (bitmaps: synthetic_waves(n) x
+// synthetic_bitmap_words(n) zeroed words, left zeroed), 1 = one per stratum
+int64_t synthetic_bitmap_words(int64_t n);
+int64_t synthetic_waves(int64_t n);
 void launch_synthetic_fill(int64_t n, const int64_t *rp, const int32_t *len, uint64_t seed,
-                           int32_t *col, float *coef, hipStream_t st);
+                           int columns, uint32_t *bitmaps, int32_t *col, float *coef,
+                           hipStream_t st);
 void launch_encode16(int64_t n, const int64_t *rp, const float *coef32, uint16_t *coef16,
                      float *scale, int kind, hipStream_t st);
 void launch_decode16(int64_t total, const uint16_t *coef16, float *out, int kind,
@@ -218,8 +223,13 @@ void launch_read_probe(const void *in, int64_t bytes, float *out, int blocks, hi
 struct DevSamples {
   const float *px, *py, *pz, *nx, *ny, *nz;
 };
+// lds_entries: traversal-stack entries per lane in LDS (2/4/8/12/16/24)
 void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
                          const vr_light *lights, int32_t n_lights, const float *sky_dirs,
-                         float eps, int32_t skip, float *rgb, int path, hipStream_t st);
+                         float eps, int32_t skip, float *rgb, int path, int lds_entries,
+                         hipStream_t st);
+// CanLeafTraceToSky per leaf (n waves; n < 2^32): out[i] = 0 / 1
+void launch_leaf_sky(const DevScene &sc, const int16_t *mins, const int16_t *maxs, int64_t n,
+                     const float *sky_dirs, uint8_t *out, hipStream_t st);
 
 }  // namespace vr

@@ -51,7 +51,10 @@ __device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, fl
         const float dot2 = -((L.normal[0] * ux + L.normal[1] * uy) + L.normal[2] * uz);
         if (!(dot2 > L.stopdot2)) continue;
         if (dot2 <= L.stopdot) fall = fall * ((dot2 - L.stopdot2) / (L.stopdot - L.stopdot2));
-        if (L.exponent != 0.0f && L.exponent != 1.0f) fall = fall * powf(dot2, L.exponent);
+        // the cone exponent in float64 then rounded: one rounding of a
+        // near-exact power, so the value does not depend on a float powf
+        if (L.exponent != 0.0f && L.exponent != 1.0f)
+          fall = fall * (float)pow((double)dot2, (double)L.exponent);
       }
       if (L.end_fade > L.start_fade) {
         if (dist >= L.end_fade) continue;
@@ -131,19 +134,75 @@ __global__ __launch_bounds__(256) void k_gather_plain(DevScene sc, DevSamples s,
   rgb[3 * i + 2] = b;
 }
 
+// CanLeafTraceToSky (rad/lightmap/lightmap.go:425-451): one wave per leaf.
+// The leaf's centre is float32(mins + maxs) * 0.5 per axis; lane l traces
+// the segments from it to (a * -MAX_TRACE_LENGTH) + centre for the
+// directions a = dirs[l], dirs[l + 64], dirs[l + 128] (< 162) with skip id
+// -1, and the wave stops after the first round in which some lane's
+// segment is visible (a miss, or a sky hit before any other).  "Any" does
+// not depend on the order the directions are taken in.
+template <int S, int D>
+__global__ __launch_bounds__(64) void k_leaf_sky(DevScene sc, const int16_t *__restrict__ mins,
+                                                 const int16_t *__restrict__ maxs, int64_t n,
+                                                 const float *__restrict__ dirs,
+                                                 uint8_t *__restrict__ out) {
+  __shared__ uint4 s_stk[S * 64];
+  const int64_t leaf = (int64_t)xcd_chunk(blockIdx.x, gridDim.x);
+  if (leaf >= n) return;
+  StackSpill<S, D> spill;
+  HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
+  const float cx = (float)((int32_t)mins[3 * leaf + 0] + (int32_t)maxs[3 * leaf + 0]) * 0.5f;
+  const float cy = (float)((int32_t)mins[3 * leaf + 1] + (int32_t)maxs[3 * leaf + 1]) * 0.5f;
+  const float cz = (float)((int32_t)mins[3 * leaf + 2] + (int32_t)maxs[3 * leaf + 2]) * 0.5f;
+  const float nm = -VR_MAX_TRACE_LENGTH_F;
+  uint8_t any = 0;
+  for (int k0 = 0; k0 < 162; k0 += 64) {
+    const int k = k0 + (int)threadIdx.x;
+    bool vis = false;
+    if (k < 162) {
+      bool sky;
+      vis = test_line<D>(sc, st, cx, cy, cz, dirs[3 * k] * nm + cx, dirs[3 * k + 1] * nm + cy,
+                         dirs[3 * k + 2] * nm + cz, -1, sky);
+    }
+    if (__any(vis)) {
+      any = 1;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) out[leaf] = any;
+}
+
+void launch_leaf_sky(const DevScene &sc, const int16_t *mins, const int16_t *maxs, int64_t n,
+                     const float *sky_dirs, uint8_t *out, hipStream_t st) {
+  if (n <= 0) return;
+#define VR_LEAFSKY(S)                                                                      \
+  hipLaunchKernelGGL((k_leaf_sky<S, kStackDepth>), dim3((unsigned)n), dim3(64), 0, st, sc, \
+                     mins, maxs, n, sky_dirs, out)
+  switch (tuning().trace_lds_entries) {
+    case 2: VR_LEAFSKY(2); break;
+    case 4: VR_LEAFSKY(4); break;
+    case 12: VR_LEAFSKY(12); break;
+    case 16: VR_LEAFSKY(16); break;
+    case 24: VR_LEAFSKY(24); break;
+    default: VR_LEAFSKY(8); break;
+  }
+#undef VR_LEAFSKY
+}
+
 void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
                          const vr_light *lights, int32_t n_lights, const float *sky_dirs,
-                         float eps, int32_t skip, float *rgb, int path, hipStream_t st) {
+                         float eps, int32_t skip, float *rgb, int path, int lds_entries,
+                         hipStream_t st) {
   if (n <= 0) return;
   if (path == VR_PATH_PLAIN)
     hipLaunchKernelGGL(k_gather_plain<kStackDepth>, dim3((unsigned)((n + 255) / 256)), dim3(256),
                        0, st, sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb);
   else {
-    // the traversal-stack entries kept in LDS follow the autotuned knob
+    // the traversal-stack entries kept in LDS: the caller's (autotuned) knob
 #define VR_GATHER(S)                                                                           \
   hipLaunchKernelGGL((k_gather_lds<S, kStackDepth>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, \
                      st, sc, s, n, lights, n_lights, sky_dirs, eps, skip, rgb)
-    switch (tuning().trace_lds_entries) {
+    switch (lds_entries) {
       case 2: VR_GATHER(2); break;
       case 4: VR_GATHER(4); break;
       case 12: VR_GATHER(12); break;

@@ -70,10 +70,13 @@ class Transfers:
 
     @staticmethod
     def synthetic(seed, n_rows, mean_len=2000.0, sigma=1.0, max_len=40000,
-                  precision=_lib.VR_COEF_F32):
+                  precision=_lib.VR_COEF_F32, columns="uniform"):
+        """The precision workload's synthetic CSR; columns "uniform" (drawn
+        without replacement) or "stratified" (one per equal stratum)."""
+        mode = {"uniform": 0, "stratified": 1}[columns]
         h = C.c_void_p()
         check(lib.vr_xfers_synthetic(C.byref(h), int(seed), int(n_rows), float(mean_len),
-                                     float(sigma), int(max_len), int(precision)))
+                                     float(sigma), int(max_len), int(precision), mode))
         return Transfers(h)
 
 
