@@ -300,6 +300,25 @@ int vr_bounce_light(vr_xfers *x, const float *emit0, const float *refl,
                     const vr_bounce_params *params, double *energy,
                     int32_t *n_bounces);
 
+/* vr_bounce_light without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Arguments and results are those of vr_bounce_light, bit for
+ * bit.  The device working set (emit ping-pong, total, reflectivity, area,
+ * per-row energy terms, energy partials, bounce state) lives in one fixed
+ * array per GPU, held until the process ends: at most 524288 rows
+ * (VR_E_CAPACITY above) and 1024 bounces.  Runs the LDS-tiled or stream
+ * gather of the optimised path (else VR_E_UNSUPPORTED) on a transfer set
+ * whose gather layout for the current tuning is already built (else
+ * VR_E_STATE: run vr_bounce_light on it once).  Every call on a GPU shares
+ * that GPU's array, which keeps the last call's contents; the library lock
+ * serialises calls within a process, and a call that overlapped another on
+ * the same array would race. */
+int vr_bounce_light_noalloc(vr_xfers *x, const float *emit0, const float *refl,
+                            const float *area, float *total,
+                            const vr_bounce_params *params, double *energy,
+                            int32_t *n_bounces);
+
 /* Batched BounceLight (no reference counterpart; the batching form of the
  * bake): every job is an independent transfer set with its own host arrays;
  * one launch per bounce covers all jobs' rows (the LDS-tiled gather), each

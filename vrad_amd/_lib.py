@@ -165,6 +165,8 @@ _SIGS = {
     "vr_can_leaf_trace_to_sky": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
     "vr_bounce_light": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(BounceParams), _P,
                                   C.POINTER(C.c_int32)]),
+    "vr_bounce_light_noalloc": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(BounceParams), _P,
+                                          C.POINTER(C.c_int32)]),
     "vr_bounce_light_batch": (C.c_int, [C.POINTER(BounceJob), C.c_int32, C.POINTER(BounceParams),
                                         C.c_int32]),
     "vr_gen_room": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int32, C.c_float, C.c_float]),

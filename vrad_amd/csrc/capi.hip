@@ -91,13 +91,48 @@ struct Prof {
       return fail(VR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
   } while (0)
 
+// ---- the device working set of vr_bounce_light_noalloc ----------------------
+// Callers that cannot allocate device memory (code under stream capture,
+// tight serving loops, harnesses that forbid hipMalloc) get a bounce whose
+// device working set lives here: one file-scope array per GPU, loaded with
+// the code object and held until the process ends.  Layout (each part
+// 256-byte aligned): emit ping-pong and total (12 B per row + 16 each),
+// reflectivity (12 B per row), area (4 B), per-row energy terms (8 B), the
+// energy tree's partials, the per-bounce energies, the bounce state and the
+// one-job table.  In host code `scratch` names a host-side stand-in, never
+// the device memory: host code uses only the address hipGetSymbolAddress
+// returns for the current device (g_scratch).
+constexpr int64_t kNoAllocMaxRows = (int64_t)1 << 19;
+constexpr int kNoAllocMaxBounces = 1024;
+constexpr size_t na_pad(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaVec = na_pad(12 * (size_t)kNoAllocMaxRows + 16);
+constexpr size_t kNaEmit0 = 0, kNaEmit1 = kNaEmit0 + kNaVec, kNaTotal = kNaEmit1 + kNaVec;
+constexpr size_t kNaRefl = kNaTotal + kNaVec;
+constexpr size_t kNaArea = kNaRefl + na_pad(12 * (size_t)kNoAllocMaxRows);
+constexpr size_t kNaERow = kNaArea + na_pad(4 * (size_t)kNoAllocMaxRows);
+constexpr size_t kNaEScr = kNaERow + na_pad(8 * (size_t)kNoAllocMaxRows);
+constexpr size_t kNaEnergy = kNaEScr + na_pad(8 * (size_t)(kNoAllocMaxRows / 2048 + 2));
+constexpr size_t kNaState = kNaEnergy + na_pad(8 * (size_t)kNoAllocMaxBounces);
+constexpr size_t kNaJob = kNaState + 256;
+constexpr size_t kNaBytes = kNaJob + 1024;
+static_assert(sizeof(BounceState) <= 256 && sizeof(BounceJob) <= 1024, "scratch layout");
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+constexpr int kMaxDevices = 64;
+char *g_scratch[kMaxDevices] = {};
+
 int ensure_init() {
   if (g_init) return VR_OK;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
     return fail(VR_E_NODEVICE, "no HIP device available");
+  if (g_device < 0 || g_device >= n || g_device >= kMaxDevices)
+    return fail(VR_E_INVALID, "no such device");
   VR_HIP(hipSetDevice(g_device));
   VR_HIP(hipStreamCreate(&g_stream));
+  // this device's copy of the allocation-free bounce's working set
+  void *sp = nullptr;
+  VR_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  g_scratch[g_device] = static_cast<char *>(sp);
   g_init = true;
   return VR_OK;
 }
@@ -1546,6 +1581,84 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs, const vr_bounce_p
     jobs[j].n_bounces = done;
   }
   VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
+}
+
+int vr_bounce_light_noalloc(vr_xfers *x, const float *emit0, const float *refl,
+                            const float *area, float *total, const vr_bounce_params *prm,
+                            double *energy, int32_t *n_bounces) {
+  LOCK;
+  if (!x || !prm) return fail(VR_E_INVALID, "NULL argument");
+  const int64_t n = x->n;
+  const int K = prm->max_bounces;
+  if (K < 0) return fail(VR_E_INVALID, "max_bounces < 0");
+  if (n > 0 && (!emit0 || !refl || !area || !total)) return fail(VR_E_INVALID, "NULL array");
+  if (K > 0 && !energy) return fail(VR_E_INVALID, "energy is NULL");
+  if (n > kNoAllocMaxRows)
+    return fail(VR_E_CAPACITY, "the allocation-free bounce holds at most 524288 rows");
+  if (K > kNoAllocMaxBounces)
+    return fail(VR_E_CAPACITY, "the allocation-free bounce runs at most 1024 bounces");
+  const Tuning tu = tuning();
+  if (g_path != VR_PATH_OPT || tu.bounce_kernel == 1)
+    return fail(VR_E_UNSUPPORTED,
+                "the allocation-free bounce runs the LDS-tiled or stream gather (optimised path)");
+  TRY(ensure_init());
+  char *base = g_scratch[g_device];
+  if (!base) return fail(VR_E_STATE, "no working set on this device");
+  TiledShape sh;
+  TRY(tiled_shape(x, tu, sh));
+  if (!tiles_ready(x, sh))
+    return fail(VR_E_STATE,
+                "the gather layout of these transfers is not built for the current tuning "
+                "(run vr_bounce_light on them once)");
+  hipStream_t st = g_stream;
+  DevTiles tl;
+  TRY(prepare_tiles(x, sh, tl, st));  // the layout is ready: fills tl only
+  BounceJob jb;
+  jb.x = x->dev();
+  jb.tl = tl;
+  jb.emit[0] = reinterpret_cast<float *>(base + kNaEmit0);
+  jb.emit[1] = reinterpret_cast<float *>(base + kNaEmit1);
+  jb.total = reinterpret_cast<float *>(base + kNaTotal);
+  jb.refl = reinterpret_cast<float *>(base + kNaRefl);
+  jb.area = reinterpret_cast<float *>(base + kNaArea);
+  jb.e_row = reinterpret_cast<double *>(base + kNaERow);
+  jb.scratch = reinterpret_cast<double *>(base + kNaEScr);
+  jb.energy = reinterpret_cast<double *>(base + kNaEnergy);
+  jb.state = reinterpret_cast<BounceState *>(base + kNaState);
+  jb.e_blocks = (n + 2047) / 2048;
+  jb.max_bounces = K;
+  jb.stop_frac = prm->stop_frac;
+  jb.chunk_begin = 0;
+  jb.eblk_begin = 0;
+  BounceJob *d_job = reinterpret_cast<BounceJob *>(base + kNaJob);
+  if (n > 0) {
+    VR_HIP(hipMemcpyAsync(jb.emit[0], emit0, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+    VR_HIP(hipMemcpyAsync(jb.total, total, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+    VR_HIP(hipMemcpyAsync((void *)jb.refl, refl, 12 * (size_t)n, hipMemcpyHostToDevice, st));
+    VR_HIP(hipMemcpyAsync((void *)jb.area, area, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  }
+  // the array keeps the last call's contents: the bounce state starts at zero
+  VR_HIP(hipMemsetAsync(jb.state, 0, sizeof(BounceState), st));
+  VR_HIP(hipMemcpyAsync(d_job, &jb, sizeof jb, hipMemcpyHostToDevice, st));
+  const int kind = x->precision;
+  BounceState hs{0.0, 0, 0};
+  for (int done = 0; done < K; done += 8) {
+    for (int i = 0; i < std::min(8, K - done); ++i) {
+      launch_gather_jobs(kind, sh, d_job, 1, tl.n_chunks, st);
+      launch_energy_jobs(d_job, 1, jb.e_blocks, st);
+    }
+    TRY(check_launch());
+    VR_HIP(hipMemcpyAsync(&hs, jb.state, sizeof hs, hipMemcpyDeviceToHost, st));
+    VR_HIP(hipStreamSynchronize(st));
+    if (hs.converged || hs.k_done >= K) break;
+  }
+  const int nb = hs.k_done;
+  if (nb > 0)
+    VR_HIP(hipMemcpyAsync(energy, jb.energy, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
+  if (n > 0) VR_HIP(hipMemcpyAsync(total, jb.total, 12 * (size_t)n, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  if (n_bounces) *n_bounces = nb;
   return VR_OK;
 }
 
