@@ -80,6 +80,63 @@ def row_batch(s, n_seg, i0):
     return pair_segments(o, nrm, i, j)
 
 
+def pvs_pair_batch(s, n_seg, first_room):
+    """A PVS-culled MakeTransfers visibility batch in the order
+    vr_make_transfers_pvs forms its candidates: patches grouped by room
+    (cluster), then for each room pair a <= b whose PVS bits pass (the room
+    grid's PVS of one doorway), the rows of a against the columns of b (j > i
+    within one room), facing cull; starting at room first_room.  About
+    two thirds of these segments are visible, and a visible segment walks its
+    whole length."""
+    from vrad_amd import pvs, scenes
+    bsp, vis = scenes.grid_bsp(s.grid_x, s.grid_y, pvs_hops=1)
+    cl = pvs.ClusterFromPoint(bsp, s.p_origin).astype(np.int64)
+    rows, _ = pvs.DecompressVis(vis)
+    nc = vis.n_clusters
+    rv = np.unpackbits(rows, axis=1, bitorder="little")[:, :nc].astype(bool)
+    rv = rv | rv.T
+    groups = [np.flatnonzero(cl == c) for c in range(nc)]
+    o, nrm = s.p_origin, s.p_normal
+    out_i, out_j, have = [], [], 0
+    for a in list(range(first_room, nc)) + list(range(first_room)):
+        for b in range(a, nc):
+            if not rv[a, b] or have >= n_seg:
+                continue
+            for i in groups[a]:
+                j = groups[b][groups[b] > i] if a == b else groups[b]
+                dv = o[j] - o[i]
+                j = j[(dv @ nrm[i] > 0) & (-np.einsum("ij,ij->i", nrm[j], dv) > 0)]
+                out_i.append(np.full(len(j), i))
+                out_j.append(j)
+                have += len(j)
+                if have >= n_seg:
+                    break
+        if have >= n_seg:
+            break
+    i = np.concatenate(out_i)[:n_seg]
+    j = np.concatenate(out_j)[:n_seg]
+    return pair_segments(o, nrm, i, j)
+
+
+def hemisphere_rays(s, n_ray, seed):
+    """Closest-hit rays (Trace4Rays' work): from patch centres moved 0.1 along
+    the normal, cosine-distributed about the normal, tmax = MAX_TRACE_LENGTH."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, len(s.p_area), n_ray)
+    nrm = s.p_normal[k].astype(np.float64)
+    u1, u2 = rng.random(n_ray), rng.random(n_ray)
+    r, ph = np.sqrt(u1), 2 * np.pi * u2
+    a = np.where(np.abs(nrm[:, 0:1]) < 0.9, [[1.0, 0, 0]], [[0, 1.0, 0]])
+    t1 = np.cross(nrm, a)
+    t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+    t2 = np.cross(nrm, t1)
+    d = (r * np.cos(ph))[:, None] * t1 + (r * np.sin(ph))[:, None] * t2 + \
+        np.sqrt(1 - u1)[:, None] * nrm
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = (s.p_origin[k] + np.float32(0.1) * s.p_normal[k]).astype(np.float32)
+    return o, d.astype(np.float32)
+
+
 def random_batch(s, n_seg, seed):
     """Random patch pairs passing the facing cull (no ray coherence)."""
     rng = np.random.default_rng(seed)
@@ -193,6 +250,34 @@ def run_visibility(cx, args):
             check(lib.vr_profile_enable(0))
             return el, cx.kernel_ms(b"test_line")
 
+    class RayBatch:
+        def __init__(self, o, d):
+            cols = [o[:, k] for k in range(3)] + [d[:, k] for k in range(3)]
+            cols += [np.zeros(len(o), np.float32),
+                     np.full(len(o), 1.732050807569 * 32768.0, np.float32)]
+            self.cols = [torch.from_numpy(np.ascontiguousarray(c)).to(cx.dev) for c in cols]
+            self.n = len(o)
+            self.hit = torch.empty(self.n, dtype=torch.int32, device=cx.dev)
+            self.t = torch.empty(self.n, dtype=torch.float32, device=cx.dev)
+            self.rays = L.Rays(*[L.C.cast(c.data_ptr(), L.FP) for c in self.cols])
+
+        def step(self):
+            check(lib.vr_trace_closest_dev(env.handle, L.C.byref(self.rays), self.n, -1,
+                                           L.C.c_void_p(self.hit.data_ptr()),
+                                           L.C.c_void_p(self.t.data_ptr()), None, cx.sptr))
+
+        def run(self, steps, warmup):
+            for _ in range(warmup):
+                self.step()
+            torch.cuda.synchronize()
+            check(lib.vr_profile_reset())
+            check(lib.vr_profile_enable(1))
+            for _ in range(steps):
+                self.step()
+            torch.cuda.synchronize()
+            check(lib.vr_profile_enable(0))
+            return None, cx.kernel_ms(b"trace_closest")
+
     batch = Batch(st, en)
     # autotune traversal-stack entries in LDS x rays per wavefront on this batch
     from vrad_amd import autotune
@@ -208,6 +293,38 @@ def run_visibility(cx, args):
     check(lib.vr_set_path(L.VR_PATH_OPT))
     extra = {"autotune_Mrays": sweep, "tuned": best_s,
              "plain_path_Mrays": round(n / (plain_ms * 1e-3) / 1e6, 1), "kd_build": kd_build}
+    # the visible-heavy batch: PVS-culled MakeTransfers rows of the same level
+    progress("visibility: PVS-culled (visible-heavy) batch")
+    pst, pen = pvs_pair_batch(s, args.segments, first_room=27)
+    pb = Batch(pst, pen)
+    prec = autotune.Autotuner(reps=3).tune_visibility(
+        pb.step, key=f"level-pvs:{pb.n}", stacks=(4, 6, 8, 10, 12), rays=(1,))
+    _, pvs_ms = pb.run(args.steps, args.warmup)
+    extra["pvs_batch"] = {
+        "segments": pb.n, "visible_fraction": round(float(pb.vis.float().mean().item()), 4),
+        "Mrays": round(pb.n / (pvs_ms * 1e-3) / 1e6, 1), "kernel_ms": round(pvs_ms, 4),
+        "tuned": prec["best"],
+        "autotune_Mrays": {"S%d" % r["knobs"]["trace_lds_entries"]:
+                           round(pb.n / (r["ms"] * 1e-3) / 1e6, 1) for r in prec["table"]},
+        "note": "the 8x8-room level's PVS-culled MakeTransfers candidates (room-pair "
+                "order of vr_make_transfers_pvs, rooms from 27 on, one-doorway PVS, facing cull)"}
+    del pb
+    # closest hit (Trace4Rays' work at batch scale)
+    progress("visibility: closest-hit rays")
+    ro, rd = hemisphere_rays(s, args.segments, seed=21)
+    rb = RayBatch(ro, rd)
+    crec = autotune.Autotuner(reps=3).tune_visibility(
+        rb.step, key=f"level-closest:{rb.n}", stacks=(4, 6, 8, 10, 12), rays=(1,),
+        family="trace_closest")
+    _, cl_ms = rb.run(args.steps, args.warmup)
+    extra["closest_hit"] = {
+        "rays": rb.n, "Mrays": round(rb.n / (cl_ms * 1e-3) / 1e6, 1),
+        "kernel_ms": round(cl_ms, 4), "tuned": crec["best"],
+        "hit_fraction": round(float((rb.hit >= 0).float().mean().item()), 4),
+        "note": "patch-centre rays, cosine-distributed about the normal, tmax = MAX_TRACE_LENGTH; "
+                "40 B per ray algorithmic"}
+    del rb
+    autotune.Autotuner().apply(best_s)
     if args.random_pairs:
         rb = Batch(*random_batch(s, args.segments, seed=20))
         _, rand_ms = rb.run(3, 1)
@@ -287,10 +404,12 @@ def run_bounce(cx, args):
         return "tiled%dx%d-p%d-d%d" % shp[1:] if shp[0] == 0 else "wave"
 
     # best shapes of earlier sweeps, used when the sweep is skipped
-    default_shape = {"fp32": (2, 4096, 0, 0, 1), "fp16": (2, 4096, 0, 0, 1)}
+    default_shape = {"fp32": (2, 4096, 0, 0, 1), "fp16": (2, 4096, 0, 0, 1),
+                     "bf16": (2, 4096, 0, 0, 1)}
 
     out = {}
-    for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4)):
+    for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4),
+                               ("bf16", L.VR_COEF_BF16, 2, 4)):
         xf.set_precision(prec)
         # stored layout read by the tiled gather: 16-bit column (LDS offset)
         # + coefficient per transfer; per patch row_ptr 8, emit read 12,
@@ -395,7 +514,7 @@ def run_pvs(cx, args):
     from vrad_amd.patches import MakeTransfers
     from vrad_amd.raytracer import Environment
 
-    def level_case(rx, ry, unculled):
+    def level_case(rx, ry, unculled, bounce=False):
         progress(f"PVS: {rx}x{ry}-room level")
         s = scenes.level(seed=2, rooms_x=rx, rooms_y=ry, chop=32.0, tess=64.0)
         env = Environment()
@@ -419,6 +538,10 @@ def run_pvs(cx, args):
         r["pvs"] = {"make_transfers_s": round(time.perf_counter() - t3, 3),
                     "pairs_considered": int(ip.pairs_considered),
                     "segments_traced": int(ip.segments_traced), "transfers": int(ip.nnz)}
+        r["pvs"]["segments_per_s_G"] = round(ip.segments_traced / r["pvs"]["make_transfers_s"]
+                                             / 1e9, 3)
+        if bounce:
+            r["bounce_on_this_csr"] = real_csr_bounce(cx, xp, s)
         if unculled:
             t4 = time.perf_counter()
             xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, normalize=False)
@@ -445,7 +568,36 @@ def run_pvs(cx, args):
 
     return {"note": "PVS = each room and its grid neighbours (one doorway); normalize off so "
                     "culled rows are the unculled rows minus the culled pairs",
-            "level_4x4": level_case(4, 4, True), "level_8x8": level_case(8, 8, False)}
+            "level_4x4": level_case(4, 4, True), "level_8x8": level_case(8, 8, False, True)}
+
+
+def real_csr_bounce(cx, xf, s):
+    """BounceLight GB/s on a transfer set MakeTransfers built (clustered
+    columns, rows of every length), FP32 and FP16: 8 bounces per call, the
+    median of 3 calls after a warm-up call that builds the tile layout."""
+    from vrad_amd import lightmap
+    lib, check, L = cx.lib, cx.check, cx.L
+    inf = xf.info()
+    N, T = int(inf.n_rows), int(inf.stored_nnz)
+    rng = np.random.default_rng(9)
+    e0 = rng.random((N, 3)).astype(np.float32)
+    out = {"rows": N, "transfers": int(inf.nnz), "stored_entries": T}
+    for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4)):
+        xf.set_precision(prec)
+        lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=1, stop_frac=0.0)
+        ms = []
+        for _ in range(3):
+            check(lib.vr_profile_reset())
+            check(lib.vr_profile_enable(1))
+            lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=8, stop_frac=0.0)
+            check(lib.vr_profile_enable(0))
+            ms.append(cx.kernel_ms(b"bounce_gather"))
+        per = float(np.median(ms))
+        B = T * (2 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
+        out[name] = {"ms_per_bounce": round(per, 4), "GBps": round(B / (per * 1e-3) / 1e9, 1),
+                     "frac": round(B / (per * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                     "bytes_per_bounce": B}
+    return out
 
 
 def run_luxel_stage(cx, args):
@@ -587,48 +739,103 @@ def run_luxel_stage(cx, args):
     return out
 
 
+def batching_scenes(n_scenes, seed=64):
+    """The batching form's scenes: triangle and patch targets drawn
+    log-uniformly from [2k, 200k] and [1k, 60k], 1-16 lights, sun on half.
+    Rooms follow the patch target (about 3000 patches per room at chop 32),
+    then the wall tessellation is solved for the triangle target and the chop
+    for the patch target from one calibration scene (triangles ~ 1/tess^2,
+    patches ~ 1/chop^2)."""
+    from vrad_amd import scenes
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_scenes):
+        t_tri = float(np.exp(rng.uniform(np.log(2000.0), np.log(200000.0))))
+        t_pat = float(np.exp(rng.uniform(np.log(1000.0), np.log(60000.0))))
+        nl, sun = int(rng.integers(1, 17)), bool(rng.integers(0, 2))
+        rooms = int(np.clip(round(t_pat / 3000.0), 1, 16))
+        rx = int(np.clip(round(np.sqrt(rooms)), 1, 4))
+        ry = int(np.clip(round(rooms / rx), 1, 4))
+        c = scenes.level(seed=500 + i, rooms_x=rx, rooms_y=ry, chop=64.0, tess=32.0,
+                         n_lights=nl, sun=sun)
+        tess = float(np.clip(32.0 * np.sqrt(len(c.ids) / t_tri), 8.0, 512.0))
+        chop = float(np.clip(64.0 * np.sqrt(len(c.p_area) / t_pat), 12.0, 512.0))
+        out.append(scenes.level(seed=500 + i, rooms_x=rx, rooms_y=ry, chop=chop, tess=tess,
+                                n_lights=nl, sun=sun))
+    return out
+
+
 def run_batch_bake(cx, args):
     """The batching form: 64 procedural scenes of uneven size baked together.
-    Per scene MakeTransfers and direct light, then every scene's FP32 bounces
-    to the stop rule in one batched launch per bounce (a replayed hipGraph of
-    8 bounces), against the same bounces one scene at a time (optimised
-    kernels) and on the plain path; per-scene results compared bit for bit."""
-    from vrad_amd import lightmap, scenes
-    from vrad_amd.patches import MakeTransfers
+    Visibility of every scene's MakeTransfers candidate pairs shares one
+    launch per round (vr_make_transfers_batch), direct light of every scene
+    is one launch (vr_gather_sample_light_batch), and every scene's FP32
+    bounces to the stop rule run in one batched launch per bounce (a replayed
+    hipGraph of 8 bounces).  Each phase is also run one scene at a time, and
+    the bounces on the plain path; results compared bit for bit.  The
+    bounces' tile layouts are built before either bounce leg is timed."""
+    from vrad_amd import lightmap
+    from vrad_amd.patches import MakeTransfers, MakeTransfersBatch
     from vrad_amd.raytracer import Environment
     lib, check, L = cx.lib, cx.check, cx.L
     progress(f"batched bake of {args.batch_scenes} scenes")
-    rng = np.random.default_rng(64)
-    baked, t_xf, t_direct, tris = [], 0.0, 0.0, []
-    for i in range(args.batch_scenes):
-        # uneven sizes, about 1k to 60k patches: log-uniform target, then a room
-        # grid and chop that reach it (about 5300 patches per room at chop 32)
-        target = float(np.exp(rng.uniform(np.log(1000.0), np.log(60000.0))))
-        rooms = int(np.clip(round(target / 5300.0 * 2.0), 1, 16))
-        rx = int(np.clip(round(np.sqrt(rooms)), 1, 4))
-        ry = int(np.clip(round(rooms / rx), 1, 4))
-        chop = float(np.clip(32.0 * np.sqrt(5300.0 * rx * ry / target), 24.0, 256.0))
-        s = scenes.level(seed=500 + i, rooms_x=rx, rooms_y=ry, chop=chop, tess=64.0,
-                         n_lights=int(rng.integers(1, 17)), sun=bool(rng.integers(0, 2)))
+    sc = batching_scenes(args.batch_scenes)
+    envs = []
+    t0 = time.perf_counter()
+    for s in sc:
         env = Environment()
         env.AddTriangles(s.verts9, s.ids)
         env.SetupAccelerationStructure()
-        t0 = time.perf_counter()
-        xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area)
-        t_xf += time.perf_counter() - t0
-        t0 = time.perf_counter()
-        direct = lightmap.GatherSampleLight(env, s.p_origin, s.p_normal, s.lights)
-        t_direct += time.perf_counter() - t0
-        emit0 = (direct * s.p_refl).astype(np.float32)
-        baked.append((xf, emit0, s.p_refl, s.p_area, direct))
-        tris.append(len(s.ids))
-    jobs = [(xf, e0, rf, ar, d) for xf, e0, rf, ar, d in baked]
-    lightmap.BounceLightBatch(jobs[:2], max_bounces=2)          # warm the kernels
+        envs.append(env)
+    t_build = time.perf_counter() - t0
+    items = [(env, s.p_origin, s.p_normal, s.p_area) for s, env in zip(sc, envs)]
+    gitems = [(env, s.p_origin, s.p_normal, s.lights) for s, env in zip(sc, envs)]
+    # warm both paths' kernels on the two smallest scenes
+    small = sorted(range(len(sc)), key=lambda k: len(sc[k].p_area))[:2]
+    for x in MakeTransfersBatch([items[k] for k in small]):
+        x.close()
+    lightmap.GatherSampleLightBatch([gitems[k] for k in small])
+    # one scene at a time
+    progress("batched bake: MakeTransfers and direct light one scene at a time")
+    t0 = time.perf_counter()
+    single = [MakeTransfers(*it) for it in items]
+    t_xf_single = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    d_single = [lightmap.GatherSampleLight(*g) for g in gitems]
+    t_direct_single = time.perf_counter() - t0
+    info_single = [(x.info().nnz, x.info().segments_traced) for x in single]
+    check_ix = sorted(range(len(sc)), key=lambda k: len(sc[k].p_area))
+    check_ix = check_ix[:6] + check_ix[-2:]
+    ref_csr = {k: single[k].download() for k in check_ix}
+    for x in single:
+        x.close()
+    # batched
+    progress("batched bake: MakeTransfers and direct light batched")
+    t0 = time.perf_counter()
+    xfs = MakeTransfersBatch(items)
+    t_xf_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    direct = lightmap.GatherSampleLightBatch(gitems)
+    t_direct_batch = time.perf_counter() - t0
+    same_xf = all((x.info().nnz, x.info().segments_traced) == info_single[k]
+                  for k, x in enumerate(xfs))
+    for k in check_ix:
+        a, b = ref_csr[k], xfs[k].download()
+        same_xf = same_xf and all(np.array_equal(u.view(np.uint32) if u.dtype == np.float32 else u,
+                                                 v.view(np.uint32) if v.dtype == np.float32 else v)
+                                  for u, v in zip(a, b))
+    same_direct = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                      for a, b in zip(direct, d_single))
+    jobs = [(xf, (d * s.p_refl).astype(np.float32), s.p_refl, s.p_area, d)
+            for xf, d, s in zip(xfs, direct, sc)]
+    # tile layouts of every job, outside both timed legs
+    lightmap.BounceLightBatch(jobs, max_bounces=1, stop_frac=0.0, use_graph=False)
+    progress("batched bake: bounces")
     t0 = time.perf_counter()
     batch = lightmap.BounceLightBatch(jobs, max_bounces=100, stop_frac=1e-3, use_graph=True)
     t_batch = time.perf_counter() - t0
     t0 = time.perf_counter()
-    single = [lightmap.BounceLight(xf, e0, rf, ar, total=d) for xf, e0, rf, ar, d in jobs]
+    one = [lightmap.BounceLight(xf, e0, rf, ar, total=d) for xf, e0, rf, ar, d in jobs]
     t_single = time.perf_counter() - t0
     check(lib.vr_set_path(L.VR_PATH_PLAIN))
     try:
@@ -639,20 +846,36 @@ def run_batch_bake(cx, args):
     finally:
         check(lib.vr_set_path(L.VR_PATH_OPT))
     same = all(k1 == k2 and np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
-               for (t1, _, k1), (t2, _, k2) in zip(batch, single))
-    n = [len(j[2]) for j in jobs]
-    nnz = [j[0].info().nnz for j in jobs]
+               for (t1, _, k1), (t2, _, k2) in zip(batch, one))
+    n = [len(s.p_area) for s in sc]
+    tris = [len(s.ids) for s in sc]
+    nnz = [x.info().nnz for x in xfs]
+    segs = sum(x.info().segments_traced for x in xfs)
     ks = [b[2] for b in batch]
-    for j in jobs:
-        j[0].close()
-    return {"scenes": len(jobs), "triangles": [int(min(tris)), int(max(tris))],
+    for x in xfs:
+        x.close()
+    for e in envs:
+        e.close()
+    return {"scenes": len(sc), "triangles": [int(min(tris)), int(max(tris))],
+            "triangles_total": int(sum(tris)),
             "patches": [int(min(n)), int(max(n))], "patches_total": int(sum(n)),
-            "transfers_total": int(sum(nnz)), "bounces": [int(min(ks)), int(max(ks))],
-            "make_transfers_s": round(t_xf, 3), "direct_s": round(t_direct, 3),
+            "lights": [int(min(len(s.lights) for s in sc)), int(max(len(s.lights) for s in sc))],
+            "transfers_total": int(sum(nnz)), "segments_traced": int(segs),
+            "bounces": [int(min(ks)), int(max(ks))],
+            "scene_build_s": round(t_build, 3),
+            "make_transfers_batched_s": round(t_xf_batch, 3),
+            "make_transfers_one_by_one_s": round(t_xf_single, 3),
+            "make_transfers_equal": bool(same_xf),
+            "direct_batched_s": round(t_direct_batch, 4),
+            "direct_one_by_one_s": round(t_direct_single, 4),
+            "direct_equal": bool(same_direct),
             "bounce_batched_graph_s": round(t_batch, 4), "bounce_one_by_one_s": round(t_single, 4),
             "bounce_plain_s": round(t_plain, 4),
             "batched_vs_plain": round(t_plain / t_batch, 1),
-            "batched_equals_one_by_one": bool(same)}
+            "batched_equals_one_by_one": bool(same),
+            "note": "tile layouts built before both timed bounce legs; MakeTransfers equality "
+                    "by nnz and segments for all scenes and full CSR for the 6 smallest and "
+                    "2 largest"}
 
 
 def run_serving(cx, args):

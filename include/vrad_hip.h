@@ -225,6 +225,17 @@ typedef struct {
  * get F_ij = cos_i cos_j A_j / (pi r^2) and the CSR stays resident. */
 int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t n_patches,
                       const vr_xfer_params *params, vr_xfers **out);
+/* MakeTransfers for many scenes at once (the batching form; no reference
+ * counterpart): scene k's patches[k] (n_patches[k] of them) give out[k],
+ * equal bit for bit to vr_make_transfers(scenes[k], ...).  Every round, each
+ * unfinished scene culls its next candidate rows into its own region of one
+ * pair buffer (params->batch_pairs pairs in total, default 16M, at least a
+ * row per scene), and one visibility launch walks all regions through a
+ * table of scene descriptors.  The plain path runs the scenes one by one.
+ * On failure no out[k] is set. */
+int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
+                            const int64_t *n_patches, int32_t n_scenes,
+                            const vr_xfer_params *params, vr_xfers **out);
 /* Upload a host CSR (row_ptr n_rows+1, col/coef nnz) as transfers; each row
  * is stored with its columns in ascending order (stable). */
 int vr_xfers_from_csr(vr_xfers **out, int64_t n_rows, const int64_t *row_ptr,
@@ -274,6 +285,15 @@ typedef struct {
 int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n,
                            const vr_light *lights, int32_t n_lights,
                            float normal_offset, int32_t skip_id, float *rgb);
+
+/* GatherSampleLight for many scenes in one launch (the batching form; no
+ * reference counterpart): scene k's n[k] samples with its lights[k]
+ * (n_lights[k]) write rgb[k] (3 n[k]); per scene equal bit for bit to
+ * vr_gather_sample_light.  The plain path runs the scenes one by one. */
+int vr_gather_sample_light_batch(vr_scene *const *scenes, const vr_samples *samples,
+                                 const int64_t *n, const vr_light *const *lights,
+                                 const int32_t *n_lights, int32_t n_scenes,
+                                 float normal_offset, int32_t skip_id, float *const *rgb);
 
 /* CanLeafTraceToSky (rad/lightmap/lightmap.go:425-451), batched over leaves.
  * Leaf i's bounds are mins[3i..3i+2], maxs[3i..3i+2] (dleaf_t int16); its

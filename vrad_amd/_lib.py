@@ -163,6 +163,9 @@ _SIGS = {
     "vr_gather_sample_light": (C.c_int, [_P, C.POINTER(Samples), C.c_int64, _P, C.c_int32,
                                          C.c_float, C.c_int32, _P]),
     "vr_can_leaf_trace_to_sky": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    "vr_make_transfers_batch": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(XferParams), _P]),
+    "vr_gather_sample_light_batch": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, C.c_float,
+                                               C.c_int32, _P]),
     "vr_bounce_light": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(BounceParams), _P,
                                   C.POINTER(C.c_int32)]),
     "vr_bounce_light_noalloc": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(BounceParams), _P,

@@ -26,7 +26,7 @@ import time
 
 from . import _lib
 
-VIS_STACKS = (2, 4, 8, 12, 16, 24)
+VIS_STACKS = (2, 4, 6, 8, 10, 12, 16, 24)
 VIS_RAYS = (1, 2, 4, 8)
 BOUNCE_KNOBS = ("bounce_kernel", "bounce_tile_cols", "bounce_group", "bounce_passes",
                 "bounce_prefetch")

@@ -386,8 +386,9 @@ int vr_set_tuning(const char *name, int value) {
   if (!name) return fail(VR_E_INVALID, "name is NULL");
   const std::string k(name);
   if (k == "trace_lds_entries") {
-    if (value != 2 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
-      return fail(VR_E_INVALID, "trace_lds_entries must be 2, 4, 8, 12, 16 or 24");
+    if (value != 2 && value != 4 && value != 6 && value != 8 && value != 10 && value != 12 &&
+        value != 16 && value != 24)
+      return fail(VR_E_INVALID, "trace_lds_entries must be 2, 4, 6, 8, 10, 12, 16 or 24");
     tuning().trace_lds_entries = value;
   } else if (k == "trace_rays_per_lane") {
     if (value != 1 && value != 2 && value != 4 && value != 8)
@@ -544,6 +545,8 @@ static int build_scene_trees(vr_scene *s, int32_t n_threads, bool device) {
   TRY(build(s->v9.data(), s->n_tri, &s->kd));
   if (s->kd.max_depth >= kStackDepth)
     return fail(VR_E_CAPACITY, "kd-tree deeper than the traversal stack");
+  if ((int64_t)s->kd.children.size() >= ((int64_t)1 << kStackNodeBits))
+    return fail(VR_E_CAPACITY, "kd-tree of 2^24 nodes or more (traversal-stack records)");
   s->has_sky = 0;
   for (const TriIntersect &t : s->tris)
     if (t.id & VR_TRACE_ID_SKY) s->has_sky = 1;
@@ -561,6 +564,8 @@ static int build_scene_trees(vr_scene *s, int32_t n_threads, bool device) {
     TRY(build(v9s.data(), (int64_t)s->sky_index.size(), &s->kd_sky));
     if (s->kd_sky.max_depth >= kStackDepth)
       return fail(VR_E_CAPACITY, "sky kd-tree deeper than the traversal stack");
+    if ((int64_t)s->kd_sky.children.size() >= ((int64_t)1 << kStackNodeBits))
+      return fail(VR_E_CAPACITY, "sky kd-tree of 2^24 nodes or more");
   }
   s->host_built = true;
   return VR_OK;
@@ -833,6 +838,34 @@ static int pvs_plan(const int32_t *cl, int64_t np, const uint8_t *rows, int32_t 
   return VR_OK;
 }
 
+// The CSR of a finished visibility bit matrix: row counts, padded layout,
+// columns and form factors, then the optional row normalisation.
+static int transfers_from_bits(vr_xfers *x, const DBuf &bits, int64_t words,
+                               const DevPatches &dp, int32_t normalize, hipStream_t st) {
+  const int64_t np = x->n;
+  TRY(x->cnt.alloc(4 * (size_t)std::max<int64_t>(np, 1)));
+  launch_row_count(bits.as<uint32_t>(), np, words, x->cnt.as<int32_t>(), st);
+  TRY(check_launch());
+  x->cnt_host.assign((size_t)np, 0);
+  if (np > 0)
+    VR_HIP(hipMemcpyAsync(x->cnt_host.data(), x->cnt.p, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
+  VR_HIP(hipStreamSynchronize(st));
+  x->nnz = 0;
+  for (int32_t c : x->cnt_host) x->nnz += c;
+  TRY(finish_layout(x, st));
+  TRY(x->col.alloc(4 * (size_t)x->stored));
+  TRY(x->coef32.alloc(4 * (size_t)x->stored));
+  launch_row_fill(bits.as<uint32_t>(), np, words, dp, x->rp.as<int64_t>(), x->cnt.as<int32_t>(),
+                  x->col.as<int32_t>(), x->coef32.as<float>(), st);
+  DBuf rowsum;
+  TRY(rowsum.alloc(8 * (size_t)std::max<int64_t>(np, 1)));
+  launch_row_normalize(np, x->rp.as<int64_t>(), x->coef32.as<float>(), rowsum.as<double>(),
+                       normalize, st);
+  TRY(check_launch());
+  VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
+}
+
 static int make_transfers_impl(vr_scene *s, const vr_patches *p, int64_t np,
                                const vr_xfer_params *prm, const int32_t *cl,
                                const uint8_t *pvs, int32_t nc, vr_xfers **out) {
@@ -913,29 +946,11 @@ static int make_transfers_impl(vr_scene *s, const vr_patches *p, int64_t np,
         i0 = i1;
       }
     }
-    TRY(x->cnt.alloc(fb));
-    launch_row_count(bits.as<uint32_t>(), np, words, x->cnt.as<int32_t>(), st);
-    TRY(check_launch());
-    x->cnt_host.assign((size_t)np, 0);
-    if (np > 0)
-      VR_HIP(hipMemcpyAsync(x->cnt_host.data(), x->cnt.p, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
     unsigned long long segs = 0;
     VR_HIP(hipMemcpyAsync(&segs, segtot.p, 8, hipMemcpyDeviceToHost, st));
     VR_HIP(hipStreamSynchronize(st));
     x->segments = (int64_t)segs;
-    x->nnz = 0;
-    for (int32_t c : x->cnt_host) x->nnz += c;
-    TRY(finish_layout(x.get(), st));
-    TRY(x->col.alloc(4 * (size_t)x->stored));
-    TRY(x->coef32.alloc(4 * (size_t)x->stored));
-    launch_row_fill(bits.as<uint32_t>(), np, words, dp, x->rp.as<int64_t>(), x->cnt.as<int32_t>(),
-                    x->col.as<int32_t>(), x->coef32.as<float>(), st);
-    DBuf rowsum;
-    TRY(rowsum.alloc(8 * (size_t)std::max<int64_t>(np, 1)));
-    launch_row_normalize(np, x->rp.as<int64_t>(), x->coef32.as<float>(), rowsum.as<double>(),
-                         P.normalize, st);
-    TRY(check_launch());
-    VR_HIP(hipStreamSynchronize(st));
+    TRY(transfers_from_bits(x.get(), bits, words, dp, P.normalize, st));
   }
   *out = x.release();
   return VR_OK;
@@ -945,6 +960,225 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfe
                       vr_xfers **out) {
   LOCK;
   return make_transfers_impl(s, p, np, prm, nullptr, nullptr, 0, out);
+}
+
+int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
+                            const int64_t *n_patches, int32_t n_scenes,
+                            const vr_xfer_params *prm, vr_xfers **out) {
+  LOCK;
+  if (n_scenes < 0) return fail(VR_E_INVALID, "n_scenes < 0");
+  if (n_scenes == 0) return VR_OK;
+  if (!scenes || !patches || !n_patches || !out) return fail(VR_E_INVALID, "NULL argument");
+  for (int32_t k = 0; k < n_scenes; ++k) out[k] = nullptr;
+  if (g_path == VR_PATH_PLAIN) {  // the comparison path: one scene at a time
+    for (int32_t k = 0; k < n_scenes; ++k) {
+      const int rc = make_transfers_impl(scenes[k], &patches[k], n_patches[k], prm, nullptr,
+                                         nullptr, 0, &out[k]);
+      if (rc) {
+        for (int32_t q = 0; q < k; ++q) {
+          delete out[q];
+          out[q] = nullptr;
+        }
+        return rc;
+      }
+    }
+    return VR_OK;
+  }
+  for (int32_t k = 0; k < n_scenes; ++k) {
+    const vr_scene *sc = scenes[k];
+    const vr_patches *p = &patches[k];
+    const int64_t np = n_patches[k];
+    const std::string who = "scene " + std::to_string(k) + ": ";
+    if (!sc) return fail(VR_E_INVALID, who + "NULL scene");
+    if (!sc->built) return fail(VR_E_STATE, who + "scene not built");
+    if (np < 0 || np >= (1ll << 31)) return fail(VR_E_INVALID, who + "bad patch count");
+    if (np > 0 && (!p->ox || !p->oy || !p->oz || !p->nx || !p->ny || !p->nz || !p->area))
+      return fail(VR_E_INVALID, who + "NULL patch array");
+  }
+  TRY(ensure_init());
+  vr_xfer_params P{0.1f, -1, 1, 0};
+  if (prm) P = *prm;
+  hipStream_t st = g_stream;
+  // every scene culls its next rows into its own region of one pair buffer;
+  // the regions' pairs are then traced in one launch
+  const int64_t budget = P.batch_pairs > 0 ? P.batch_pairs : (int64_t)16 << 20;
+  struct One {
+    std::unique_ptr<vr_xfers> x;
+    Staging S;
+    DevPatches dp{};
+    DBuf bits;
+    int64_t np = 0, words = 0, cap = 0, off = 0, i0 = 0;
+  };
+  std::vector<One> sc((size_t)n_scenes);
+  int64_t total_cap = 0;
+  for (int32_t k = 0; k < n_scenes; ++k) {
+    One &o = sc[(size_t)k];
+    const vr_patches *p = &patches[k];
+    o.np = n_patches[k];
+    o.x.reset(new vr_xfers);
+    o.x->n = o.np;
+    const size_t fb = 4 * (size_t)std::max<int64_t>(o.np, 1);
+    TRY(o.S.make({fb, fb, fb, fb, fb, fb, fb}));
+    const float *src[7] = {p->ox, p->oy, p->oz, p->nx, p->ny, p->nz, p->area};
+    for (int q = 0; q < 7 && o.np > 0; ++q)
+      VR_HIP(hipMemcpyAsync(o.S.ptr<float>(q), src[q], 4 * (size_t)o.np, hipMemcpyHostToDevice, st));
+    o.dp = DevPatches{o.S.ptr<float>(0), o.S.ptr<float>(1), o.S.ptr<float>(2), o.S.ptr<float>(3),
+                      o.S.ptr<float>(4), o.S.ptr<float>(5), o.S.ptr<float>(6)};
+    o.words = (o.np + 31) / 32;
+    TRY(o.bits.alloc((size_t)std::max<int64_t>(o.np, 1) * (size_t)std::max<int64_t>(o.words, 1) * 4));
+    VR_HIP(hipMemsetAsync(o.bits.p, 0, o.bits.bytes, st));
+    o.cap = std::max<int64_t>(o.np - 1, std::max<int64_t>(1, budget / n_scenes));
+    if (o.cap >= (1ll << 31)) return fail(VR_E_INVALID, "batch_pairs too large");
+    o.off = total_cap;
+    total_cap += o.cap;
+  }
+  DBuf pairs, counts, d_jobs;
+  TRY(pairs.alloc((size_t)std::max<int64_t>(total_cap, 1) * sizeof(int2)));
+  TRY(counts.alloc(4 * (size_t)n_scenes));
+  TRY(d_jobs.alloc(sizeof(TraceJob) * (size_t)n_scenes));
+  std::vector<uint32_t> hc((size_t)n_scenes);
+  std::vector<TraceJob> jobs;
+  const int lds = tuning().trace_lds_entries;
+  {
+    Prof pr("make_transfers", st);
+    for (;;) {
+      bool any = false;
+      VR_HIP(hipMemsetAsync(counts.p, 0, 4 * (size_t)n_scenes, st));
+      for (int32_t k = 0; k < n_scenes; ++k) {
+        One &o = sc[(size_t)k];
+        if (o.i0 + 1 >= o.np) continue;
+        any = true;
+        const int64_t rowlen = o.np - 1 - o.i0;
+        const int64_t rows = std::max<int64_t>(1, o.cap / rowlen);
+        const int64_t i1 = std::min<int64_t>(o.np - 1, o.i0 + rows);
+        o.x->pairs += (2 * o.np - o.i0 - i1 - 1) * (i1 - o.i0) / 2;
+        launch_pair_cull(o.dp, o.np, o.i0, i1, pairs.as<int2>() + o.off,
+                         counts.as<uint32_t>() + k, (uint32_t)o.cap, st);
+        o.i0 = i1;
+      }
+      if (!any) break;
+      TRY(check_launch());
+      VR_HIP(hipMemcpyAsync(hc.data(), counts.p, 4 * (size_t)n_scenes, hipMemcpyDeviceToHost, st));
+      VR_HIP(hipStreamSynchronize(st));
+      jobs.clear();
+      int64_t blocks = 0;
+      for (int32_t k = 0; k < n_scenes; ++k) {
+        One &o = sc[(size_t)k];
+        const int64_t c = std::min<int64_t>(hc[(size_t)k], o.cap);
+        if (c <= 0) continue;
+        o.x->segments += c;
+        TraceJob j;
+        j.sc = scenes[k]->dev;
+        j.p = o.dp;
+        j.pairs = pairs.as<int2>() + o.off;
+        j.bits = o.bits.as<uint32_t>();
+        j.words = o.words;
+        j.count = c;
+        j.blk_begin = blocks;
+        blocks += (c + 63) / 64;
+        jobs.push_back(j);
+      }
+      if (jobs.empty()) continue;
+      if (blocks >= ((int64_t)1 << 32)) return fail(VR_E_CAPACITY, "too many pairs per launch");
+      VR_HIP(hipMemcpyAsync(d_jobs.p, jobs.data(), sizeof(TraceJob) * jobs.size(),
+                            hipMemcpyHostToDevice, st));
+      launch_pair_trace_jobs(d_jobs.as<TraceJob>(), (int)jobs.size(), blocks, P.normal_offset,
+                             P.skip_id, lds, st);
+      TRY(check_launch());
+      // the job table is rewritten next round: wait for this launch
+      VR_HIP(hipStreamSynchronize(st));
+    }
+    for (int32_t k = 0; k < n_scenes; ++k) {
+      One &o = sc[(size_t)k];
+      TRY(transfers_from_bits(o.x.get(), o.bits, o.words, o.dp, P.normalize, st));
+      o.bits.release();
+    }
+  }
+  for (int32_t k = 0; k < n_scenes; ++k) out[k] = sc[(size_t)k].x.release();
+  return VR_OK;
+}
+
+int vr_gather_sample_light_batch(vr_scene *const *scenes, const vr_samples *samples,
+                                 const int64_t *n, const vr_light *const *lights,
+                                 const int32_t *n_lights, int32_t n_scenes, float normal_offset,
+                                 int32_t skip_id, float *const *rgb) {
+  LOCK;
+  if (n_scenes < 0) return fail(VR_E_INVALID, "n_scenes < 0");
+  if (n_scenes == 0) return VR_OK;
+  if (!scenes || !samples || !n || !n_lights || !rgb) return fail(VR_E_INVALID, "NULL argument");
+  int64_t ns_tot = 0, nl_tot = 0;
+  for (int32_t k = 0; k < n_scenes; ++k) {
+    const std::string who = "scene " + std::to_string(k) + ": ";
+    if (!scenes[k]) return fail(VR_E_INVALID, who + "NULL scene");
+    if (!scenes[k]->built) return fail(VR_E_STATE, who + "scene not built");
+    if (n[k] < 0 || n_lights[k] < 0) return fail(VR_E_INVALID, who + "negative count");
+    if (n[k] > 0 && (!rgb[k] || !samples[k].px || !samples[k].py || !samples[k].pz ||
+                     !samples[k].nx || !samples[k].ny || !samples[k].nz))
+      return fail(VR_E_INVALID, who + "NULL sample array or output");
+    if (n_lights[k] > 0 && (!lights || !lights[k])) return fail(VR_E_INVALID, who + "NULL lights");
+    ns_tot += n[k];
+    nl_tot += n_lights[k];
+  }
+  if (ns_tot == 0) return VR_OK;
+  if (g_path == VR_PATH_PLAIN) {  // the comparison path: one scene at a time
+    for (int32_t k = 0; k < n_scenes; ++k)
+      TRY(vr_gather_sample_light(scenes[k], &samples[k], n[k], lights ? lights[k] : nullptr,
+                                 n_lights[k], normal_offset, skip_id, rgb[k]));
+    return VR_OK;
+  }
+  TRY(ensure_init());
+  hipStream_t st = g_stream;
+  Staging S;
+  const size_t fb = 4 * (size_t)ns_tot;
+  TRY(S.make({fb, fb, fb, fb, fb, fb, 3 * fb, sizeof(vr_light) * (size_t)std::max<int64_t>(nl_tot, 1),
+              162 * 3 * 4, sizeof(GatherJob) * (size_t)n_scenes}));
+  std::vector<GatherJob> jobs;
+  int64_t at = 0, lat = 0, blocks = 0;
+  for (int32_t k = 0; k < n_scenes; ++k) {
+    const int64_t m = n[k];
+    const float *src[6] = {samples[k].px, samples[k].py, samples[k].pz,
+                           samples[k].nx, samples[k].ny, samples[k].nz};
+    for (int q = 0; q < 6 && m > 0; ++q)
+      VR_HIP(hipMemcpyAsync(S.ptr<float>(q) + at, src[q], 4 * (size_t)m, hipMemcpyHostToDevice, st));
+    if (n_lights[k] > 0)
+      VR_HIP(hipMemcpyAsync(S.ptr<vr_light>(7) + lat, lights[k], sizeof(vr_light) * (size_t)n_lights[k],
+                            hipMemcpyHostToDevice, st));
+    if (m > 0) {
+      GatherJob j;
+      j.sc = scenes[k]->dev;
+      j.s = DevSamples{S.ptr<float>(0) + at, S.ptr<float>(1) + at, S.ptr<float>(2) + at,
+                       S.ptr<float>(3) + at, S.ptr<float>(4) + at, S.ptr<float>(5) + at};
+      j.lights = S.ptr<vr_light>(7) + lat;
+      j.n_lights = n_lights[k];
+      j.rgb = S.ptr<float>(6) + 3 * at;
+      j.n = m;
+      j.blk_begin = blocks;
+      blocks += (m + 63) / 64;
+      jobs.push_back(j);
+    }
+    at += m;
+    lat += n_lights[k];
+  }
+  if (blocks >= ((int64_t)1 << 32)) return fail(VR_E_CAPACITY, "too many samples per launch");
+  std::vector<float> dirs = sky_dirs();
+  VR_HIP(hipMemcpyAsync(S.ptr<float>(8), dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice, st));
+  VR_HIP(hipMemcpyAsync(S.ptr<GatherJob>(9), jobs.data(), sizeof(GatherJob) * jobs.size(),
+                        hipMemcpyHostToDevice, st));
+  {
+    Prof p("gather_light", st);
+    launch_gather_jobs(S.ptr<GatherJob>(9), (int)jobs.size(), blocks, S.ptr<float>(8),
+                       normal_offset, skip_id, tuning().trace_lds_entries, st);
+  }
+  TRY(check_launch());
+  at = 0;
+  for (int32_t k = 0; k < n_scenes; ++k) {
+    if (n[k] > 0)
+      VR_HIP(hipMemcpyAsync(rgb[k], S.ptr<float>(6) + 3 * at, 12 * (size_t)n[k],
+                            hipMemcpyDeviceToHost, st));
+    at += n[k];
+  }
+  VR_HIP(hipStreamSynchronize(st));
+  return VR_OK;
 }
 
 int vr_make_transfers_pvs(vr_scene *s, const vr_patches *p, int64_t np,

@@ -66,6 +66,20 @@ void launch_pair_cull(const DevPatches &p, int64_t np, int64_t i0, int64_t i1, i
 void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pairs,
                        const uint32_t *count, uint32_t cap, float eps, int32_t skip,
                        uint32_t *bits, int64_t words, int path, hipStream_t st);
+// One scene's candidate pairs in a many-scene visibility launch: blocks
+// [blk_begin, blk_begin + ceil(count / 64)) of the launch trace pairs[0, count)
+// on sc with patches p, setting bits of the scene's N x N matrix (words per row).
+struct TraceJob {
+  DevScene sc;
+  DevPatches p;
+  const int2 *pairs;
+  uint32_t *bits;
+  int64_t words;
+  int64_t count;
+  int64_t blk_begin;
+};
+void launch_pair_trace_jobs(const TraceJob *jobs, int n_jobs, int64_t total_blocks, float eps,
+                            int32_t skip, int lds_entries, hipStream_t st);
 void launch_row_count(const uint32_t *bits, int64_t np, int64_t words, int32_t *cnt,
                       hipStream_t st);
 // exclusive scan of padded lengths (multiple of 8) into rp[0..n]
@@ -228,6 +242,21 @@ void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
                          const vr_light *lights, int32_t n_lights, const float *sky_dirs,
                          float eps, int32_t skip, float *rgb, int path, int lds_entries,
                          hipStream_t st);
+// One scene's samples in a many-scene direct-light launch: blocks
+// [blk_begin, blk_begin + ceil(n / 64)) gather samples [0, n) of s on sc with
+// lights[0, n_lights), writing rgb (3 n).
+struct GatherJob {
+  DevScene sc;
+  DevSamples s;
+  const vr_light *lights;
+  int32_t n_lights;
+  float *rgb;
+  int64_t n;
+  int64_t blk_begin;
+};
+void launch_gather_jobs(const GatherJob *jobs, int n_jobs, int64_t total_blocks,
+                        const float *sky_dirs, float eps, int32_t skip, int lds_entries,
+                        hipStream_t st);
 // CanLeafTraceToSky per leaf (n waves; n < 2^32): out[i] = 0 / 1
 void launch_leaf_sky(const DevScene &sc, const int16_t *mins, const int16_t *maxs, int64_t n,
                      const float *sky_dirs, uint8_t *out, hipStream_t st);

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) void k_gather_lds(DevScene sc, DevSamples s, in
                                                    int32_t n_lights, const float *__restrict__ dirs,
                                                    float eps, int32_t skip,
                                                    float *__restrict__ rgb) {
-  __shared__ uint4 s_stk[S * 64];
+  __shared__ uint2 s_stk[S * 64];
   const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (i >= n) return;
   StackSpill<S, D> spill;
@@ -134,6 +134,57 @@ __global__ __launch_bounds__(256) void k_gather_plain(DevScene sc, DevSamples s,
   rgb[3 * i + 2] = b;
 }
 
+// Many scenes' samples in one launch: block b (after the XCD remap) belongs
+// to the job whose block range holds it (binary search; the job index is
+// uniform per wave), and lane l gathers that job's sample
+// (b - first block) * 64 + l from the job's scene and lights.
+template <int S, int D>
+__global__ __launch_bounds__(64) void k_gather_jobs(const GatherJob *__restrict__ jobs, int n_jobs,
+                                                    const float *__restrict__ dirs, float eps,
+                                                    int32_t skip) {
+  __shared__ uint2 s_stk[S * 64];
+  const int64_t b = (int64_t)xcd_chunk(blockIdx.x, gridDim.x);
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].blk_begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const int jx = __builtin_amdgcn_readfirstlane(lo);
+  const GatherJob &J = jobs[jx];
+  const int64_t i = (b - J.blk_begin) * 64 + threadIdx.x;
+  if (i >= J.n) return;
+  StackSpill<S, D> spill;
+  HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
+  const DevSamples &s = J.s;
+  float r, g, bb;
+  gather_one<D>(J.sc, st, s.px[i], s.py[i], s.pz[i], s.nx[i], s.ny[i], s.nz[i], J.lights,
+                J.n_lights, dirs, eps, skip, r, g, bb);
+  J.rgb[3 * i + 0] = r;
+  J.rgb[3 * i + 1] = g;
+  J.rgb[3 * i + 2] = bb;
+}
+
+void launch_gather_jobs(const GatherJob *jobs, int n_jobs, int64_t total_blocks,
+                        const float *sky_dirs, float eps, int32_t skip, int lds_entries,
+                        hipStream_t st) {
+  if (n_jobs <= 0 || total_blocks <= 0) return;
+#define VR_G_JOBS(S)                                                                          \
+  hipLaunchKernelGGL((k_gather_jobs<S, kStackDepth>), dim3((unsigned)total_blocks), dim3(64), 0, \
+                     st, jobs, n_jobs, sky_dirs, eps, skip)
+  switch (lds_entries) {
+    case 2: VR_G_JOBS(2); break;
+    case 4: VR_G_JOBS(4); break;
+    case 6: VR_G_JOBS(6); break;
+    case 10: VR_G_JOBS(10); break;
+    case 12: VR_G_JOBS(12); break;
+    case 16: VR_G_JOBS(16); break;
+    case 24: VR_G_JOBS(24); break;
+    default: VR_G_JOBS(8); break;
+  }
+#undef VR_G_JOBS
+}
+
 // CanLeafTraceToSky (rad/lightmap/lightmap.go:425-451): one wave per leaf.
 // The leaf's centre is float32(mins + maxs) * 0.5 per axis; lane l traces
 // the segments from it to (a * -MAX_TRACE_LENGTH) + centre for the
@@ -146,7 +197,7 @@ __global__ __launch_bounds__(64) void k_leaf_sky(DevScene sc, const int16_t *__r
                                                  const int16_t *__restrict__ maxs, int64_t n,
                                                  const float *__restrict__ dirs,
                                                  uint8_t *__restrict__ out) {
-  __shared__ uint4 s_stk[S * 64];
+  __shared__ uint2 s_stk[S * 64];
   const int64_t leaf = (int64_t)xcd_chunk(blockIdx.x, gridDim.x);
   if (leaf >= n) return;
   StackSpill<S, D> spill;
@@ -181,6 +232,8 @@ void launch_leaf_sky(const DevScene &sc, const int16_t *mins, const int16_t *max
   switch (tuning().trace_lds_entries) {
     case 2: VR_LEAFSKY(2); break;
     case 4: VR_LEAFSKY(4); break;
+    case 6: VR_LEAFSKY(6); break;
+    case 10: VR_LEAFSKY(10); break;
     case 12: VR_LEAFSKY(12); break;
     case 16: VR_LEAFSKY(16); break;
     case 24: VR_LEAFSKY(24); break;
@@ -205,6 +258,8 @@ void launch_gather_light(const DevScene &sc, const DevSamples &s, int64_t n,
     switch (lds_entries) {
       case 2: VR_GATHER(2); break;
       case 4: VR_GATHER(4); break;
+      case 6: VR_GATHER(6); break;
+      case 10: VR_GATHER(10); break;
       case 12: VR_GATHER(12); break;
       case 16: VR_GATHER(16); break;
       case 24: VR_GATHER(24); break;

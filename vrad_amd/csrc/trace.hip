@@ -2,7 +2,7 @@
 // (segment visibility) over the resident kd-tree.
 //
 // Optimised path: one wave per block, XCD-aware ray ranges, per-lane
-// traversal stacks with the top S entries in LDS (S x 16 B x 64 lanes per
+// traversal stacks with the top S entries in LDS (S x 8 B x 64 lanes per
 // wave) and the rest in scratch.  Plain path: one thread per ray, 256-thread
 // blocks, the whole stack in private scratch memory.
 #include "kernels.h"
@@ -16,7 +16,7 @@ __global__ __launch_bounds__(64) void k_trace_closest_lds(DevScene sc, DevRays r
                                                           int32_t skip, int32_t *__restrict__ hit,
                                                           float *__restrict__ hit_t,
                                                           float *__restrict__ hit_n) {
-  __shared__ uint4 s_stk[S * 64];
+  __shared__ uint2 s_stk[S * 64];
   const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R + threadIdx.x;
   StackSpill<S, D> spill;
   HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
@@ -71,7 +71,7 @@ template <int S, int D, int R>
 __global__ __launch_bounds__(64) void k_test_line_lds(DevScene sc, DevSegs s, int64_t n,
                                                       int32_t skip, uint8_t *__restrict__ vis,
                                                       uint8_t *__restrict__ sky) {
-  __shared__ uint4 s_stk[S * 64];
+  __shared__ uint2 s_stk[S * 64];
   __shared__ int32_t s_q[R > 1 ? 64 * (R - 1) : 1];  // queued segments (offsets from base)
   const int lane = (int)threadIdx.x;
   const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R;
@@ -139,9 +139,10 @@ __global__ __launch_bounds__(256) void k_test_line_plain(DevScene sc, DevSegs s,
 
 // (traversal-stack entries in LDS, rays per lane)
 #define VR_LDS_VARIANTS(X)                                                                     \
-  X(2, 1) X(2, 2) X(2, 4) X(2, 8)                                                              \
-  X(4, 1) X(8, 1) X(12, 1) X(16, 1) X(24, 1) X(4, 2) X(8, 2) X(12, 2) X(16, 2) X(24, 2) X(4, 4) \
-  X(8, 4) X(12, 4) X(16, 4) X(24, 4) X(4, 8) X(8, 8) X(12, 8) X(16, 8) X(24, 8)
+  X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4) X(4, 8) X(6, 1) X(6, 2) X(6, 4)       \
+  X(6, 8) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(10, 1) X(10, 2) X(10, 4) X(10, 8) X(12, 1)          \
+  X(12, 2) X(12, 4) X(12, 8) X(16, 1) X(16, 2) X(16, 4) X(16, 8) X(24, 1) X(24, 2) X(24, 4)    \
+  X(24, 8)
 
 void launch_trace_closest(const DevScene &sc, const DevRays &r, int64_t n, int32_t skip,
                           int32_t *hit, float *hit_t, float *hit_n, int path, hipStream_t st) {

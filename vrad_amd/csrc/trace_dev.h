@@ -78,49 +78,64 @@ __device__ __forceinline__ void tri_test(const float4 *__restrict__ refs, uint32
 }
 constexpr uint32_t kNoRef = 0xffffffffu;
 
-// Per-lane traversal stack: the first S entries in LDS as 16-byte records
-// (entry e of lane l at [e * 64 + l]: one ds_write_b128 / ds_read_b128 per
-// push / pop, conflict-free), deeper entries in private scratch.  S = D
-// keeps the whole stack in LDS; S = 0 is the plain path's scratch stack.
-// The deeper entries live in a separate object (StackSpill) the stack points
-// to: an array indexed at run time sits in private memory, and keeping it out
-// of this struct lets the LDS pointer stay in a register.
+// Per-lane traversal stack entries are 8-byte records: the node index in
+// the low 24 bits, then the entry's interval [a, b] as two 20-bit floats
+// (sign, exponent, the top 11 mantissa bits), a rounded toward -inf and b
+// toward +inf.  A popped interval can only be wider than the pushed one, so
+// a walk may visit a node more, never fewer, and its result is unchanged.
+// Trees hold fewer than 2^24 nodes (the scene build checks).
+constexpr uint32_t kStackNodeBits = 24;
+__device__ __forceinline__ uint32_t f20_down(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u >> 12) + ((u >> 31) & ((u & 0xfffu) != 0u));  // negative: magnitude up
+}
+__device__ __forceinline__ uint32_t f20_up(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u >> 12) + ((~u >> 31) & ((u & 0xfffu) != 0u));  // positive: magnitude up
+}
+__device__ __forceinline__ uint2 stack_pack(uint32_t n, float a, float b) {
+  const uint32_t ka = f20_down(a), kb = f20_up(b);
+  return make_uint2(n | (ka << kStackNodeBits), (ka >> 8) | (kb << 12));
+}
+__device__ __forceinline__ void stack_unpack(uint2 w, uint32_t &n, float &a, float &b) {
+  n = w.x & ((1u << kStackNodeBits) - 1u);
+  a = __uint_as_float(((w.x >> kStackNodeBits) | ((w.y & 0xfffu) << 8)) << 12);
+  b = __uint_as_float((w.y >> 12) << 12);
+}
+
+// The first S entries of a lane's stack sit in LDS (entry e of lane l at
+// [e * 64 + l]: one ds_write_b64 / ds_read_b64 per push / pop, conflict
+// free), deeper entries in private scratch.  S = D keeps the whole stack in
+// LDS.  The deeper entries live in a separate object (StackSpill) the stack
+// points to: an array indexed at run time sits in private memory, and keeping
+// it out of this struct lets the LDS pointer stay in a register.
 template <int S, int D>
 struct StackSpill {
-  uint32_t n[D - S > 0 ? D - S : 1];
-  float a[D - S > 0 ? D - S : 1], b[D - S > 0 ? D - S : 1];
+  uint64_t v[D - S > 0 ? D - S : 1];
 };
 #define VR_PRIVATE __attribute__((address_space(5)))
 template <int S, int D>
 struct HybridStack {
-  uint4 *e;  // this lane's entry 0 in LDS; entry k at e + 64 k
-  VR_PRIVATE uint32_t *pn;  // private-memory pointers: scratch (not flat) accesses
-  VR_PRIVATE float *pa, *pb;
-  __device__ HybridStack(uint4 *lds, int lane, StackSpill<S, D> &sp)
-      : e(lds + lane),
-        pn((VR_PRIVATE uint32_t *)sp.n),
-        pa((VR_PRIVATE float *)sp.a),
-        pb((VR_PRIVATE float *)sp.b) {}
+  uint2 *e;  // this lane's entry 0 in LDS; entry k at e + 64 k
+  VR_PRIVATE uint64_t *pv;  // private-memory pointer: scratch (not flat) accesses
+  __device__ HybridStack(uint2 *lds, int lane, StackSpill<S, D> &sp)
+      : e(lds + lane), pv((VR_PRIVATE uint64_t *)sp.v) {}
   __device__ void push(int sp, uint32_t n, float a, float b) {
-    if (S == D || sp < S) {
-      e[64 * sp] = make_uint4(n, __float_as_uint(a), __float_as_uint(b), 0u);
-    } else {
-      pn[sp - S] = n;
-      pa[sp - S] = a;
-      pb[sp - S] = b;
-    }
+    const uint2 w = stack_pack(n, a, b);
+    if (S == D || sp < S)
+      e[64 * sp] = w;
+    else
+      pv[sp - S] = (uint64_t)w.x | ((uint64_t)w.y << 32);
   }
   __device__ void get(int sp, uint32_t &n, float &a, float &b) const {
+    uint2 w;
     if (S == D || sp < S) {
-      const uint4 v = e[64 * sp];
-      n = v.x;
-      a = __uint_as_float(v.y);
-      b = __uint_as_float(v.z);
+      w = e[64 * sp];
     } else {
-      n = pn[sp - S];
-      a = pa[sp - S];
-      b = pb[sp - S];
+      const uint64_t v = pv[sp - S];
+      w = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
     }
+    stack_unpack(w, n, a, b);
   }
 };
 

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p
                                                        uint32_t cap, float eps, int32_t skip,
                                                        uint32_t *__restrict__ bits,
                                                        int64_t words) {
-  __shared__ uint4 s_stk[S * 64];
+  __shared__ uint2 s_stk[S * 64];
   const uint32_t n = min(*count, cap);
   const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   if (k >= n) return;
@@ -138,6 +138,8 @@ void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pair
     switch (tuning().trace_lds_entries) {
       case 2: VR_PAIR_TRACE(2); break;
       case 4: VR_PAIR_TRACE(4); break;
+      case 6: VR_PAIR_TRACE(6); break;
+      case 10: VR_PAIR_TRACE(10); break;
       case 12: VR_PAIR_TRACE(12); break;
       case 16: VR_PAIR_TRACE(16); break;
       case 24: VR_PAIR_TRACE(24); break;
@@ -145,6 +147,57 @@ void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pair
     }
 #undef VR_PAIR_TRACE
   }
+}
+
+// Many scenes' candidate pairs in one launch: block b (after the XCD remap)
+// belongs to the job whose block range holds it (binary search over the
+// jobs' first blocks; the job index is the same for the whole wave, so its
+// descriptor comes through scalar loads), and lane l traces that job's pair
+// (b - first block) * 64 + l on the job's scene.
+template <int S, int D>
+__global__ __launch_bounds__(64) void k_pair_trace_jobs(const TraceJob *__restrict__ jobs,
+                                                        int n_jobs, float eps, int32_t skip) {
+  __shared__ uint2 s_stk[S * 64];
+  const int64_t b = (int64_t)xcd_chunk(blockIdx.x, gridDim.x);
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {  // last job with blk_begin <= b
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].blk_begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const int jx = __builtin_amdgcn_readfirstlane(lo);
+  const TraceJob &J = jobs[jx];
+  const int64_t k = (b - J.blk_begin) * 64 + threadIdx.x;
+  if (k >= (int64_t)J.count) return;
+  const int2 ij = J.pairs[k];
+  const int i = ij.x, j = ij.y;
+  const DevPatches &p = J.p;
+  StackSpill<S, D> spill;
+  HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
+  bool sky;
+  const bool vis = test_line<D>(J.sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
+                                p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
+                                p.oy[j] + eps * p.ny[j], p.oz[j] + eps * p.nz[j], skip, sky);
+  if (vis) set_pair_bits(J.bits, J.words, i, j);
+}
+
+void launch_pair_trace_jobs(const TraceJob *jobs, int n_jobs, int64_t total_blocks, float eps,
+                            int32_t skip, int lds_entries, hipStream_t st) {
+  if (n_jobs <= 0 || total_blocks <= 0) return;
+#define VR_PT_JOBS(S)                                                                          \
+  hipLaunchKernelGGL((k_pair_trace_jobs<S, kStackDepth>), dim3((unsigned)total_blocks), dim3(64), \
+                     0, st, jobs, n_jobs, eps, skip)
+  switch (lds_entries) {
+    case 2: VR_PT_JOBS(2); break;
+    case 4: VR_PT_JOBS(4); break;
+    case 6: VR_PT_JOBS(6); break;
+    case 10: VR_PT_JOBS(10); break;
+    case 12: VR_PT_JOBS(12); break;
+    case 16: VR_PT_JOBS(16); break;
+    case 24: VR_PT_JOBS(24); break;
+    default: VR_PT_JOBS(8); break;
+  }
+#undef VR_PT_JOBS
 }
 
 // one wave per row

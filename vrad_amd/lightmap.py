@@ -196,6 +196,35 @@ def GatherSampleLight(env, positions, normals, lights, normal_offset=0.1, skip_i
     return rgb
 
 
+def GatherSampleLightBatch(items, normal_offset=0.1, skip_id=-1):
+    """GatherSampleLight for many scenes in one launch: items is a list of
+    (env, positions, normals, lights); returns one rgb (n,3) per item, each
+    equal to GatherSampleLight on that item alone."""
+    k = len(items)
+    if k == 0:
+        return []
+    smp, keep, outs = (_lib.Samples * k)(), [], []
+    hs = (C.c_void_p * k)()
+    lp = (C.c_void_p * k)()
+    rp = (C.c_void_p * k)()
+    n = np.zeros(k, np.int64)
+    nl = np.zeros(k, np.int32)
+    for q, (env, pos, nrm, lights) in enumerate(items):
+        s = _Samples(pos, nrm)
+        L = np.ascontiguousarray(lights, LIGHT_DTYPE)
+        rgb = np.zeros((s.n, 3), np.float32)
+        keep.append((s, L))
+        outs.append(rgb)
+        smp[q] = s.s
+        n[q], nl[q] = s.n, len(L)
+        hs[q] = env.handle.value if isinstance(env.handle, C.c_void_p) else env.handle
+        lp[q] = L.ctypes.data if len(L) else None
+        rp[q] = rgb.ctypes.data
+    check(lib.vr_gather_sample_light_batch(hs, smp, ptr(n), lp, ptr(nl), k, float(normal_offset),
+                                           int(skip_id), rp))
+    return outs
+
+
 def CanLeafTraceToSky(env, mins, maxs):
     """CanLeafTraceToSky (rad/lightmap/lightmap.go:425-451) for many leaves at
     once: mins, maxs are (n, 3) int16 leaf bounds; returns bool (n,)."""

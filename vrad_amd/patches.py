@@ -116,6 +116,30 @@ def MakeTransfers(env, origin, normal, area, normal_offset=ON_EPSILON, skip_id=-
     return Transfers(h)
 
 
+def MakeTransfersBatch(items, normal_offset=ON_EPSILON, skip_id=-1, normalize=True,
+                       batch_pairs=0):
+    """MakeTransfers for many scenes in shared visibility launches: items is a
+    list of (env, origin, normal, area); returns one Transfers per item, each
+    equal to MakeTransfers on that item alone."""
+    k = len(items)
+    if k == 0:
+        return []
+    structs, keep = (_lib.Patches * k)(), []
+    hs = (C.c_void_p * k)()
+    n = np.zeros(k, np.int64)
+    for q, (env, origin, normal, area) in enumerate(items):
+        p, cols = patch_struct(origin, normal, area)
+        structs[q] = p
+        keep.append(cols)
+        n[q] = len(cols[-1])
+        hs[q] = env.handle.value if isinstance(env.handle, C.c_void_p) else env.handle
+    prm = _lib.XferParams(float(normal_offset), int(skip_id), 1 if normalize else 0,
+                          int(batch_pairs))
+    out = (C.c_void_p * k)()
+    check(lib.vr_make_transfers_batch(hs, structs, ptr(n), k, C.byref(prm), out))
+    return [Transfers(C.c_void_p(out[q])) for q in range(k)]
+
+
 MAX_CHOP = 4.0  # maxChop, rad/patches/face.go:24 (luxel widths)
 MIN_CHOP = 4.0  # minChop, rad/patches/face.go:25
 
