@@ -80,6 +80,18 @@ def row_batch(s, n_seg, i0):
     return pair_segments(o, nrm, i, j)
 
 
+def pvs_groups(s):
+    """Patch clusters of the level's room grid, the one-doorway PVS (either
+    direction), and the patches of each room."""
+    from vrad_amd import pvs, scenes
+    bsp, vis = scenes.grid_bsp(s.grid_x, s.grid_y, pvs_hops=1)
+    cl = pvs.ClusterFromPoint(bsp, s.p_origin).astype(np.int64)
+    rows, _ = pvs.DecompressVis(vis)
+    nc = vis.n_clusters
+    rv = np.unpackbits(rows, axis=1, bitorder="little")[:, :nc].astype(bool)
+    return cl, rv | rv.T, [np.flatnonzero(cl == c) for c in range(nc)], rows
+
+
 def pvs_pair_batch(s, n_seg, first_room):
     """A PVS-culled MakeTransfers visibility batch in the order
     vr_make_transfers_pvs forms its candidates: patches grouped by room
@@ -88,14 +100,8 @@ def pvs_pair_batch(s, n_seg, first_room):
     within one room), facing cull; starting at room first_room.  About
     two thirds of these segments are visible, and a visible segment walks its
     whole length."""
-    from vrad_amd import pvs, scenes
-    bsp, vis = scenes.grid_bsp(s.grid_x, s.grid_y, pvs_hops=1)
-    cl = pvs.ClusterFromPoint(bsp, s.p_origin).astype(np.int64)
-    rows, _ = pvs.DecompressVis(vis)
-    nc = vis.n_clusters
-    rv = np.unpackbits(rows, axis=1, bitorder="little")[:, :nc].astype(bool)
-    rv = rv | rv.T
-    groups = [np.flatnonzero(cl == c) for c in range(nc)]
+    cl, rv, groups, _ = pvs_groups(s)
+    nc = len(groups)
     o, nrm = s.p_origin, s.p_normal
     out_i, out_j, have = [], [], 0
     for a in list(range(first_room, nc)) + list(range(first_room)):
@@ -568,7 +574,90 @@ def run_pvs(cx, args):
 
     return {"note": "PVS = each room and its grid neighbours (one doorway); normalize off so "
                     "culled rows are the unculled rows minus the culled pairs",
-            "level_4x4": level_case(4, 4, True), "level_8x8": level_case(8, 8, False, True)}
+            "level_4x4": level_case(4, 4, True)}
+
+
+def run_level_bake(cx, args):
+    """The plain GPU path at scale, end to end: the 8x8-room level (16 point
+    lights + sun), scene build, clusters and PVS, PVS-culled MakeTransfers
+    (row-normalised), direct light, 100 FP32 bounces.  Checked on the CPU
+    (oracle) on a sampled subset: the direct light of 4096 patches, and the
+    transfer columns of 64 rows against TestLine over every PVS candidate of
+    those rows.  Then the bounce rate on this CSR in FP32 and FP16."""
+    import oracle
+    from vrad_amd import lightmap, pvs, scenes
+    from vrad_amd.patches import MakeTransfers
+    from vrad_amd.raytracer import Environment
+    progress("level bake: 8x8-room level end to end")
+    s = scenes.level(**LEVEL)
+    np_ = len(s.p_area)
+    times = {}
+    t0 = time.perf_counter()
+    env = Environment()
+    env.AddTriangles(s.verts9, s.ids)
+    env.SetupAccelerationStructure()
+    times["scene_build_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    bsp, vis = scenes.grid_bsp(s.grid_x, s.grid_y, pvs_hops=1)
+    cl = pvs.ClusterFromPoint(bsp, s.p_origin).astype(np.int32)
+    rows, _ = pvs.DecompressVis(vis)
+    times["clusters_and_pvs_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area, cluster=cl, pvs_rows=rows)
+    times["make_transfers_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    direct = lightmap.GatherSampleLight(env, s.p_origin, s.p_normal, s.lights)
+    times["direct_light_ms"] = (time.perf_counter() - t0) * 1e3
+    emit0 = (direct * s.p_refl).astype(np.float32)
+    t0 = time.perf_counter()
+    tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct,
+                                      max_bounces=100, stop_frac=0.0)
+    times["bounces_ms"] = (time.perf_counter() - t0) * 1e3
+    times["total_ms"] = sum(times.values())
+    times = {a: round(b, 1) for a, b in times.items()}
+    inf = xf.info()
+    progress("level bake: CPU checks on a sample")
+    tri = oracle.precompute(s.verts9, s.ids)
+    kd = oracle.KdTree(s.verts9)
+    rng = np.random.default_rng(77)
+    pick = rng.choice(np_, 4096, replace=False)
+    want = oracle.gather_light(kd, tri, s.p_origin[pick], s.p_normal[pick], s.lights, threads=16)
+    direct_ok = bool(np.array_equal(want.view(np.uint32), direct[pick].view(np.uint32)))
+    rp, col, _ = xf.download()
+    cl64, rv, groups, _ = pvs_groups(s)
+    o, nrm = s.p_origin, s.p_normal
+    rows_ok, seg_checked = True, 0
+    for i in rng.choice(np_, 64, replace=False):
+        j = np.arange(np_)
+        ok = rv[cl64[i], np.maximum(cl64, 0)] | (cl64 < 0) | (cl64[i] < 0)
+        j = j[ok & (j != i)]
+        # the kernels cull and trace a pair in ascending patch order, with
+        # one float32 rounding per operation
+        lo, hi = np.minimum(i, j), np.maximum(i, j)
+        d = [o[hi, c] - o[lo, c] for c in range(3)]
+        di = (nrm[lo, 0] * d[0] + nrm[lo, 1] * d[1]) + nrm[lo, 2] * d[2]
+        dj = -((nrm[hi, 0] * d[0] + nrm[hi, 1] * d[1]) + nrm[hi, 2] * d[2])
+        keep = (di > 0) & (dj > 0)
+        j, lo, hi = j[keep], lo[keep], hi[keep]
+        a, b = pair_segments(o, nrm, lo, hi)
+        v, _ = oracle.testline(kd, tri, a, b, threads=16)
+        seg_checked += len(j)
+        got = col[rp[i]:rp[i + 1]]
+        got = got[got != i]             # padding entries (col = row, coef = 0)
+        rows_ok = rows_ok and np.array_equal(np.sort(got), np.sort(j[v.astype(bool)]))
+    del rp, col
+    out = {"triangles": int(len(s.ids)), "patches": int(np_),
+           "lights": int(len(s.lights)), "sun": bool((s.lights["type"] == 3).any()),
+           "transfers": int(inf.nnz), "segments_traced": int(inf.segments_traced),
+           "bounces": int(k), "energy_first_last": [float(en[0]), float(en[-1])],
+           "times": times,
+           "cpu_check": {"direct_light_4096_patches_bitexact": direct_ok,
+                         "rows_checked": 64, "segments_checked": int(seg_checked),
+                         "transfer_columns_equal_oracle_visibility": bool(rows_ok)}}
+    out["bounce_rate_on_this_csr"] = real_csr_bounce(cx, xf, s)
+    xf.close()
+    env.close()
+    return out
 
 
 def real_csr_bounce(cx, xf, s):
@@ -912,21 +1001,28 @@ def run_serving(cx, args):
     xs.set_precision(L.VR_COEF_F32)
     csr32 = xs.download()
     xs.set_precision(L.VR_COEF_F16)
-    trace, t = [], 0.0
-    for _ in range(args.serve_requests):
-        t += float(rng.exponential(1.0 / args.serve_rate))
-        sid = int(rng.integers(len(resident)))
-        nl = len(resident[sid][0].lights)
-        k = int(rng.integers(1, min(4, nl) + 1))
-        moves = [(int(li), rng.uniform(-64, 64, 3).astype(np.float32))
-                 for li in rng.choice(nl, size=k, replace=False)]
-        trace.append((t, sid, moves))
+    def make_trace(n_req, rate, seed):
+        g = np.random.default_rng(seed)
+        out, t = [], 0.0
+        for _ in range(n_req):
+            t += float(g.exponential(1.0 / rate))
+            sid = int(g.integers(len(resident)))
+            nl = len(resident[sid][0].lights)
+            k = int(g.integers(1, min(4, nl) + 1))
+            moves = [(int(li), g.uniform(-64, 64, 3).astype(np.float32))
+                     for li in g.choice(nl, size=k, replace=False)]
+            out.append((t, sid, moves))
+        return out
 
-    def serve(mode, max_active):
-        sv = Server(mode=mode, max_active=max_active, hbm_budget=16 << 30, bounces=8)
-        for s, env, xf in resident:
-            sv.add_scene(env, xf, s.p_origin, s.p_normal, s.p_refl, s.p_area, s.lights)
-        sv.wait(sv.submit(0, []))                       # first request pays set-up
+    trace = make_trace(args.serve_requests, args.serve_rate, 2027)
+
+    def serve(mode, max_active, trace=trace, keep=None):
+        sv = keep or Server(mode=mode, max_active=max_active, hbm_budget=16 << 30, bounces=8)
+        if keep is None:
+            for s, env, xf in resident:
+                sv.add_scene(env, xf, s.p_origin, s.p_normal, s.p_refl, s.p_area, s.lights)
+            sv.wait(sv.submit(0, []))                   # first request pays set-up
+        st0 = sv.stats()
         progress(f"serving trace, mode {mode}: {len(trace)} requests")
         tickets, t0 = [], time.perf_counter()
         for i, (at, sid, moves) in enumerate(trace):
@@ -940,12 +1036,27 @@ def run_serving(cx, args):
         wall = time.perf_counter() - t0
         st = sv.stats()
         lat = np.array([r[2] for r in res])
+        it = int(st["iterations"] - st0["iterations"])
         out = {"p50_ms": round(float(np.percentile(lat, 50)), 2),
                "p99_ms": round(float(np.percentile(lat, 99)), 2),
                "mean_ms": round(float(lat.mean()), 2),
                "throughput_rps": round(len(res) / wall, 2), "wall_s": round(wall, 2),
-               "max_active_seen": int(st["max_active_seen"]), "iterations": int(st["iterations"])}
+               "max_active_seen": int(st["max_active_seen"]), "iterations": it}
+        if mode == BATCHED and it > 0:
+            # request-bounces per batched launch (8 bounces per request)
+            out["mean_requests_per_launch"] = round(8 * len(res) / it, 2)
         return sv, out, res
+
+    def service_times(sv, n_req):
+        """Closed loop: each request submitted after the previous finished,
+        so its latency is its service time (no queueing)."""
+        lat = []
+        for _, sid, moves in make_trace(n_req, 1.0, 4242):
+            lat.append(sv.wait(sv.submit(sid, moves))[2])
+        lat = np.array(lat)
+        return {"requests": n_req, "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "mean_ms": round(float(lat.mean()), 2)}
 
     sv, batched, res_b = serve(BATCHED, 16)
     # alone vs mix: a sample of the trace's requests re-run one at a time
@@ -955,6 +1066,13 @@ def run_serving(cx, args):
         tot, en, _ = sv.wait(sv.submit(sid, moves))
         same = same and np.array_equal(tot.view(np.uint32), res_b[j][0].view(np.uint32)) \
             and np.array_equal(en, res_b[j][1])
+    # heavier arrival rates, where continuous batching engages
+    sweep = {}
+    for rate in args.serve_sweep:
+        _, r, _ = serve(BATCHED, 16, make_trace(args.serve_sweep_requests, rate, 3000 + int(rate)),
+                        keep=sv)
+        sweep[f"{rate:g}_rps"] = r
+    svc_batched = service_times(sv, args.serve_service_requests)
     sv.close()
     # luxel error of a served FP16 re-bake against the FP64 recurrence (same
     # direct light, FP32 coefficients), on the smallest scene
@@ -975,7 +1093,11 @@ def run_serving(cx, args):
     lux_err = float(np.max(np.abs(got[nz] - t64[nz]) / np.abs(t64[nz]))) if nz.any() else 0.0
     lux_err_zero_ok = bool(np.all(got[~nz] == 0))
     del res_b
-    svp, plain, _ = serve(PLAIN, 1)
+    # the plain server on the head of the same trace: its service time is
+    # above the mean arrival gap, so its queue (and p99) grows with the
+    # trace's length; the closed-loop service times compare without queueing
+    svp, plain, _ = serve(PLAIN, 1, trace[:args.serve_plain_requests])
+    svc_plain = service_times(svp, args.serve_service_requests)
     svp.close()
     sizes = [len(r[0].p_area) for r in resident]
     nnz = [r[2].info().nnz for r in resident]
@@ -983,8 +1105,18 @@ def run_serving(cx, args):
         xf.close()
     return {"requests": len(trace), "rate_rps": args.serve_rate, "scenes": len(resident),
             "patches": [int(min(sizes)), int(max(sizes))], "transfers_total": int(sum(nnz)),
-            "batched": batched, "plain_sequential": plain,
+            "batched": batched, "batched_rate_sweep": sweep,
+            "plain_sequential": dict(plain, trace_requests=len(trace[:args.serve_plain_requests]),
+                                     note="arrivals exceed its capacity: p99 grows with the "
+                                          "trace length"),
             "p99_ratio_plain_over_batched": round(plain["p99_ms"] / batched["p99_ms"], 1),
+            "service_time": {"batched": svc_batched, "plain": svc_plain,
+                             "p99_ratio_plain_over_batched":
+                                 round(svc_plain["p99_ms"] / svc_batched["p99_ms"], 1),
+                             "p50_ratio_plain_over_batched":
+                                 round(svc_plain["p50_ms"] / svc_batched["p50_ms"], 1)},
+            "p99_ratio_plain_service_over_batched_under_load":
+                round(svc_plain["p99_ms"] / batched["p99_ms"], 1),
             "alone_equals_mix": bool(same),
             "luxel_rel_err_fp16_vs_fp64": lux_err, "luxel_zero_where_fp64_zero": lux_err_zero_ok,
             "luxel_err_sample": f"request {js} of the trace on the smallest scene "
@@ -1019,10 +1151,17 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
     ap.add_argument("--no-pvs", action="store_true", help="skip the PVS-culled MakeTransfers")
     ap.add_argument("--no-luxels", action="store_true", help="skip the luxel stage")
+    ap.add_argument("--no-level", action="store_true",
+                    help="skip the 8x8-room level bake end to end")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--serve-scenes", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[100.0, 300.0],
+                    help="further arrival rates for the batched server")
+    ap.add_argument("--serve-sweep-requests", type=int, default=600)
+    ap.add_argument("--serve-plain-requests", type=int, default=300)
+    ap.add_argument("--serve-service-requests", type=int, default=60)
     ap.add_argument("--random-pairs", action="store_true",
                     help="also time an incoherent batch of random patch pairs")
     ap.add_argument("--bounce-sweep", type=int, default=1,
@@ -1043,6 +1182,7 @@ def main():
     batch = None if args.no_batch else run_batch_bake(cx, args)
     serving = None if args.no_serve else run_serving(cx, args)
     pvs_out = None if args.no_pvs else run_pvs(cx, args)
+    level = None if args.no_level else run_level_bake(cx, args)
     lux = None if args.no_luxels else run_luxel_stage(cx, args)
 
     if args.calib_copy_mb > 0:
@@ -1062,6 +1202,7 @@ def main():
                                "4M MakeTransfers TestLine segments per batch (row order)"},
         "roofline": None, "cpu_baseline": cpu, "visibility": None, "bounce": bounce,
         "room_bake": room, "batch_bake": batch, "serving": serving, "pvs_culling": pvs_out,
+        "level_bake": level,
         "luxel_stage": lux,
     }
     if v is not None:
