@@ -449,6 +449,27 @@ typedef struct {
 int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
                     vr_patch *out, int64_t cap, int64_t *n_out);
 
+/* PairEdges' vertex normals (rad/lightmap/lightmap.go:37-228): for vertex j
+ * of face i, the normalised sum of the normals of the other faces that share
+ * that vertex (same position, bit for bit) and whose normal's dot with face
+ * i's is >= smoothing_threshold (smoothingThreshold, lightmap.go:29,
+ * 0.7071067), taken in face order, plus face i's own normal last.  Writes
+ * VR_FACE_MAX_POINTS x 3 floats per face (unused vertices 0).  Faces carry no
+ * smoothing groups or displacements (both 0 / absent in the reference's
+ * test maps).  Runs on the host: it is a hash join over vertices, O(points). */
+int vr_pair_edges(const vr_face *faces, int64_t n_faces, float smoothing_threshold,
+                  float *vertex_normals);
+/* vr_make_patches with GetPhongNormal (rad/lightmap/normallist.go:51-144,
+ * called at rad/patches/subdivide.go:387): each child patch's normal is the
+ * face normal bent toward vertex_normals (vr_pair_edges' output, or any
+ * VR_FACE_MAX_POINTS x 3 per face) at the child's centre, interpolated in
+ * the fan triangle (face centroid, vertex j, vertex j+1) that holds it; the
+ * face centroid is the base patch's centre (face.go:153).  vertex_normals
+ * NULL is vr_make_patches. */
+int vr_make_patches_phong(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
+                          const float *vertex_normals, vr_patch *out, int64_t cap,
+                          int64_t *n_out);
+
 /* ---- luxel stage: lightmap extents and luxels ------------------------- */
 #define VR_FACE_NOLIGHT 4u /* SURF_NOLIGHT: no lightmap, like SURF_SKY */
 /* MAX_LIGHTMAP_DIM_WITHOUT_BORDER = MAX_DISP_LIGHTMAP_DIM_WITHOUT_BORDER
@@ -492,7 +513,24 @@ int vr_face_luxels(const vr_face *faces, int64_t n_faces, const float *lux_ofs,
                    const float *patch_value, float *pos, float *normal, int32_t *patch,
                    float *value);
 
-/* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- */
+/* The BSP lighting lump of the luxel light (the on-disk output; the
+ * reference's writer is a stub, cmd/tasks/finish/main.go:15-18).  For each
+ * face (in order) with n_luxels > 0: its average colour, then its luxels,
+ * each a ColorRGBExp32 (bytes r, g, b, signed exponent e: colour = bytes x
+ * 2^e; e = floor(log2(max channel)) - 7, channels truncated, as Source's
+ * VectorToColorRGBExp32 named at compute/leafambient/lighting.go:67).
+ * lightofs[f] (nullable) = byte offset of face f's first luxel, -1 without
+ * luxels (dface_t.lightofs; the average sits in the 4 bytes before it).
+ * luxel_rgb: 3 floats per luxel (n_luxels), indexed by lm's luxel_ofs.  The
+ * average is the mean of the face's luxels, summed in float64 (64 lane
+ * partials, luxel k to lane k mod 64, xor butterfly 32 .. 1) and rounded to
+ * float32.  *lump_bytes gets the size; lump NULL only sizes it (lump_cap:
+ * bytes available).  Runs on the GPU. */
+int vr_lighting_lump(const float *luxel_rgb, int64_t n_luxels, const vr_face_lightmap *lm,
+                     int64_t n_faces, uint8_t *lump, int64_t lump_cap, int32_t *lightofs,
+                     int64_t *lump_bytes);
+
+/* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- *//* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- */
 /* The BSP lumps PointInLeaf reads (cache.GetLumpCache().Planes / Nodes /
  * Leafs, rad/clustertable/point.go:14-36): a plane's normal and distance, a
  * node's plane and children (child < 0: leaf -1-child), a leaf's cluster.

@@ -99,8 +99,14 @@ def _load():
         "or_bf16_to_f32": (C.c_float, [C.c_uint16]),
         "or_make_patches": (C.c_int, [P, C.c_int64, C.c_float, C.c_float, P, C.c_int64,
                                       C.POINTER(C.c_int64)]),
+        "or_make_patches_phong": (C.c_int, [P, C.c_int64, C.c_float, C.c_float, P, P,
+                                            C.c_int64, C.POINTER(C.c_int64)]),
+        "or_pair_edges": (None, [P, C.c_int64, C.c_float, P]),
+        "or_phong_normal": (None, [P, P, P, P, P]),
         "or_calc_face_extents": (C.c_int64, [P, C.c_int64, P, P]),
         "or_face_luxels": (None, [P, C.c_int64, P, P, P, C.c_int64, P, P]),
+        "or_rgbexp32": (C.c_uint32, [C.c_float, C.c_float, C.c_float]),
+        "or_lighting_lump": (C.c_int64, [P, P, C.c_int64, P, P]),
         "or_cluster_from_point": (C.c_int16, [P, P, P, P]),
         "or_patch_clusters": (None, [P, P, P, P, C.c_int64, P]),
         "or_decompress_vis": (C.c_int, [P, C.c_int64, P, C.c_int32, P, C.POINTER(C.c_int64)]),
@@ -332,17 +338,46 @@ def f32_to_bf16(x):
     return lib.or_f32_to_bf16(float(x))
 
 
-def make_patches(faces, max_chop=4.0, min_chop=4.0):
+def make_patches(faces, max_chop=4.0, min_chop=4.0, vertex_normals=None):
     """MakePatchForFace + SubdividePatches over faces (FACE_DTYPE): patches
-    (PATCH_DTYPE) in the reference's array order."""
+    (PATCH_DTYPE) in the reference's array order.  vertex_normals (n_faces x
+    16 x 3 float32, e.g. pair_edges()) gives children GetPhongNormal normals."""
     f = np.ascontiguousarray(faces, FACE_DTYPE)
+    vn = None
+    if vertex_normals is not None:
+        vn = np.ascontiguousarray(vertex_normals, np.float32)
+        assert vn.shape == (len(f), 16, 3)
     n = C.c_int64()
-    if lib.or_make_patches(_p(f), len(f), max_chop, min_chop, None, 0, C.byref(n)) != 0:
+    if lib.or_make_patches_phong(_p(f), len(f), max_chop, min_chop, _p(vn), None, 0,
+                                 C.byref(n)) != 0:
         raise ValueError("a winding exceeds 16 points")
     out = np.zeros(n.value, PATCH_DTYPE)
-    if lib.or_make_patches(_p(f), len(f), max_chop, min_chop, _p(out), len(out),
-                           C.byref(n)) != 0:
+    if lib.or_make_patches_phong(_p(f), len(f), max_chop, min_chop, _p(vn), _p(out), len(out),
+                                 C.byref(n)) != 0:
         raise ValueError("a winding exceeds 16 points")
+    return out
+
+
+SMOOTHING_THRESHOLD = np.float32(0.7071067)  # rad/lightmap/lightmap.go:29
+
+
+def phong_normal(face, centroid, vertex_normals, spot):
+    """GetPhongNormal for one face (FACE_DTYPE record), its centroid, its
+    16 x 3 vertex normals and a point: the bent unit normal (float32[3])."""
+    f = np.ascontiguousarray(np.atleast_1d(face)[:1], FACE_DTYPE)
+    c = np.ascontiguousarray(centroid, np.float32)
+    vn = np.ascontiguousarray(vertex_normals, np.float32).reshape(16, 3)
+    sp = np.ascontiguousarray(spot, np.float32)
+    out = np.zeros(3, np.float32)
+    lib.or_phong_normal(_p(f), _p(c), _p(vn), _p(sp), _p(out))
+    return out
+
+
+def pair_edges(faces, threshold=SMOOTHING_THRESHOLD):
+    """PairEdges' per-vertex normals: n_faces x 16 x 3 float32."""
+    f = np.ascontiguousarray(faces, FACE_DTYPE)
+    out = np.zeros((len(f), 16, 3), np.float32)
+    lib.or_pair_edges(_p(f), len(f), float(threshold), _p(out))
     return out
 
 
@@ -408,3 +443,20 @@ def face_luxels(faces, lm, patches, lux_ofs=None):
     pat = np.zeros(n, np.int32)
     lib.or_face_luxels(_p(f), len(f), _p(o), _p(L), _p(p), len(p), _p(pos), _p(pat))
     return pos, pat
+
+
+def rgbexp32(r, g, b):
+    """ColorRGBExp32 of one colour as its 4 bytes (r, g, b, exponent)."""
+    v = lib.or_rgbexp32(float(r), float(g), float(b))
+    return [(v >> s) & 0xFF for s in (0, 8, 16, 24)]
+
+
+def lighting_lump(luxel_rgb, lm):
+    """(lump bytes (uint8), lightofs (int32 per face))."""
+    v = np.ascontiguousarray(luxel_rgb, np.float32).reshape(-1, 3)
+    L = np.ascontiguousarray(lm, FACE_LIGHTMAP_DTYPE)
+    nb = lib.or_lighting_lump(_p(v), _p(L), len(L), None, None)
+    lump = np.zeros(nb // 4, np.uint32)
+    ofs = np.zeros(len(L), np.int32)
+    lib.or_lighting_lump(_p(v), _p(L), len(L), _p(lump), _p(ofs))
+    return lump.view(np.uint8), ofs

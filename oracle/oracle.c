@@ -1621,8 +1621,87 @@ typedef struct {
   const or_face_t *faces;
   float face_mn[3], face_mx[3]; /* of the face being subdivided */
   float min_chop;
+  const float *vn;     /* the face's vertex normals (NULL: children keep the face normal) */
+  float centroid[3];   /* the face's base-patch centre */
   int err;
 } pw_ctx_t;
+
+/* GetPhongNormal (rad/lightmap/normallist.go:51-144): the face normal, or,
+   for the first edge j whose fan triangle (centroid, vertex j, vertex j+1)
+   holds spot in barycentric terms (a1 >= 0 and a2 >= 0), the normalised
+   (1 - a1 - a2) face normal + a1 vn[j] + a2 vn[j+1].  mgl32 Dot is
+   left to right; Len is float32(math.Sqrt(float64(.))), a correctly rounded
+   float sqrt; Normalize multiplies by 1 / Len.  Below 1e-20 the reference
+   stops the bake; this keeps the face normal. */
+void or_phong_normal(const or_face_t *fc, const float c[3], const float *vn, const float spot[3],
+                     float out[3]) {
+#define PDOT(a, b) (((a)[0] * (b)[0] + (a)[1] * (b)[1]) + (a)[2] * (b)[2])
+  memcpy(out, fc->normal, 12);
+  const int E = fc->n_points;
+  for (int j = 0; j < E; ++j) {
+    const float *p1 = fc->points[j], *p2 = fc->points[(j + 1) % E];
+    const float *n1 = vn + 3 * j, *n2 = vn + 3 * ((j + 1) % E);
+    float v1[3], v2[3], vs[3];
+    for (int k = 0; k < 3; ++k) {
+      v1[k] = p1[k] - c[k];
+      v2[k] = p2[k] - c[k];
+      vs[k] = spot[k] - c[k];
+    }
+    const float aa = PDOT(v1, v1), bb = PDOT(v2, v2), ab = PDOT(v1, v2);
+    const float num = bb * PDOT(v1, vs) - ab * PDOT(vs, v2);
+    const float a1 = num / (aa * bb - ab * ab);
+    const float a2 = (PDOT(vs, v2) - a1 * ab) / bb;
+    if (a1 >= 0.0f && a2 >= 0.0f) {
+      const float scale = (1.0f - a1) - a2;
+      float ph[3], t[3];
+      for (int k = 0; k < 3; ++k) ph[k] = fc->normal[k] * scale;
+      for (int k = 0; k < 3; ++k) t[k] = n1[k] * a1;
+      for (int k = 0; k < 3; ++k) ph[k] = ph[k] + t[k];
+      for (int k = 0; k < 3; ++k) t[k] = n2[k] * a2;
+      for (int k = 0; k < 3; ++k) ph[k] = ph[k] + t[k];
+      const float len = sqrtf(PDOT(ph, ph));
+      if (len < 1.0e-20f) return;
+      const float inv = 1.0f / len;
+      for (int k = 0; k < 3; ++k) out[k] = ph[k] * inv;
+      return;
+    }
+  }
+#undef PDOT
+}
+
+/* PairEdges (rad/lightmap/lightmap.go:37-228) with smoothing groups 0 and
+   no displacements: vertex j of face i sums, in face order, the normals of
+   every other face with a point at the same position (bit for bit) whose
+   normal's dot with face i's is >= threshold, then adds face i's normal and
+   normalises.  Brute force over face pairs (the reference indexes shared
+   vertices; the reference's loop at :74 runs to len(Edges), not
+   f.NumEdges: this follows the per-face vertex loop it stands for). */
+void or_pair_edges(const or_face_t *faces, int64_t n, float threshold, float *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const float *fn = faces[i].normal;
+    float *o = out + (size_t)i * OR_FACE_MAX_POINTS * 3;
+    memset(o, 0, sizeof(float) * OR_FACE_MAX_POINTS * 3);
+    for (int j = 0; j < faces[i].n_points; ++j) {
+      const float *v = faces[i].points[j];
+      float sum[3] = {0.0f, 0.0f, 0.0f};
+      for (int64_t k = 0; k < n; ++k) {
+        if (k == i) continue;
+        int shares = 0;
+        for (int q = 0; q < faces[k].n_points && !shares; ++q)
+          shares = memcmp(faces[k].points[q], v, 12) == 0;
+        if (!shares) continue;
+        const float *nn = faces[k].normal;
+        const float cosang = (nn[0] * fn[0] + nn[1] * fn[1]) + nn[2] * fn[2];
+        if (cosang >= threshold)
+          for (int q = 0; q < 3; ++q) sum[q] = sum[q] + nn[q];
+      }
+      for (int q = 0; q < 3; ++q) sum[q] = sum[q] + fn[q];
+      const float len = sqrtf((sum[0] * sum[0] + sum[1] * sum[1]) + sum[2] * sum[2]);
+      const float inv = 1.0f / len;
+      for (int q = 0; q < 3; ++q) o[3 * j + q] = sum[q] * inv;
+    }
+  }
+}
 
 static int pw_zero_light(const float v[3]) {
   return fabsf(v[0]) < 1e-20f && fabsf(v[1]) < 1e-20f && fabsf(v[2]) < 1e-20f;
@@ -1687,7 +1766,10 @@ static void pw_subdivide(pw_ctx_t *c, int64_t k, or_patch_t *P) {
     memcpy(C->points, ow[q]->p, 12 * (size_t)ow[q]->n);
     C->area = ar[q];
     memcpy(C->origin, ca[q], 12);
-    memcpy(C->normal, fc->normal, 12);
+    if (c->vn)
+      or_phong_normal(fc, c->centroid, c->vn, C->origin, C->normal);
+    else
+      memcpy(C->normal, fc->normal, 12);
     pw_bounds(ow[q], C->mins, C->maxs);
     if (pw_zero_light(fc->base_light)) {
       float t[3];
@@ -1713,6 +1795,12 @@ static void pw_subdivide(pw_ctx_t *c, int64_t k, or_patch_t *P) {
 
 int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, float min_chop,
                     or_patch_t *out, int64_t cap, int64_t *n_out) {
+  return or_make_patches_phong(faces, n_faces, max_chop, min_chop, NULL, out, cap, n_out);
+}
+
+int or_make_patches_phong(const or_face_t *faces, int64_t n_faces, float max_chop,
+                          float min_chop, const float *vnorm, or_patch_t *out, int64_t cap,
+                          int64_t *n_out) {
   pw_ctx_t c;
   memset(&c, 0, sizeof c);
   c.out = out;
@@ -1762,6 +1850,8 @@ int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, flo
     if (faces[base[i].face].flags & 2u) continue; /* PreventSubdivision */
     memcpy(c.face_mn, base[i].mins, 12); /* the face's bounds (FaceMins / FaceMaxs) */
     memcpy(c.face_mx, base[i].maxs, 12);
+    memcpy(c.centroid, base[i].origin, 12); /* face.go:153 (face offsets 0) */
+    c.vn = vnorm ? vnorm + (size_t)base[i].face * OR_FACE_MAX_POINTS * 3 : NULL;
     pw_subdivide(&c, i, &base[i]);
     if (c.err) break;
   }
@@ -1999,4 +2089,64 @@ void or_face_luxels(const or_face_t *faces, int64_t n_faces, const float *lux_of
   free(lo);
   free(li);
   free(fill);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Lighting lump: ColorRGBExp32 luxels and per-face offsets.  The reference's */
+/* writer is a stub (cmd/tasks/finish/main.go:15-18); the encoding is the     */
+/* Source engine's VectorToColorRGBExp32 (named in the reference's            */
+/* compute/leafambient/lighting.go:67), restated from its definition: an     */
+/* exponent that puts the largest channel in [128, 256), channels truncated.  */
+/* ------------------------------------------------------------------------ */
+uint32_t or_rgbexp32(float r, float g, float b) {
+  float m = r > g ? r : g;
+  m = m > b ? m : b;
+  if (!(m > 0.0f) || !(m <= 3.402823466e38f)) return 0u;
+  int ex;
+  (void)frexpf(m, &ex);         /* m = f 2^ex, f in [0.5, 1): floor(log2 m) = ex - 1 */
+  const int e = (ex - 1) - 7;
+  if (e < -127 || e > 127) return 0u;
+  const float scale = ldexpf(1.0f, -e);
+  const float c[3] = {r, g, b};
+  uint32_t out = 0u;
+  for (int k = 0; k < 3; ++k) {
+    float v = c[k] > 0.0f ? c[k] * scale : 0.0f;
+    int q = (int)v;               /* truncation toward zero */
+    if (q > 255) q = 255;
+    out |= (uint32_t)q << (8 * k);
+  }
+  return out | ((uint32_t)(uint8_t)(int8_t)e << 24);
+}
+
+int64_t or_lighting_lump(const float *luxel_rgb, const or_face_lightmap_t *lm, int64_t n_faces,
+                         uint32_t *lump, int32_t *lightofs) {
+  int64_t at = 0; /* in 4-byte entries */
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const int64_t n = lm[f].n_luxels;
+    if (n <= 0) {
+      if (lightofs) lightofs[f] = -1;
+      continue;
+    }
+    if (lightofs) lightofs[f] = (int32_t)(4 * (at + 1));
+    if (lump) {
+      const float *v = luxel_rgb + 3 * lm[f].luxel_ofs;
+      double part[64][3];
+      for (int l = 0; l < 64; ++l) part[l][0] = part[l][1] = part[l][2] = 0.0;
+      for (int64_t k = 0; k < n; ++k)
+        for (int c = 0; c < 3; ++c) part[k % 64][c] = part[k % 64][c] + (double)v[3 * k + c];
+      for (int off = 32; off >= 1; off >>= 1) {
+        double nx[64][3];
+        for (int l = 0; l < 64; ++l)
+          for (int c = 0; c < 3; ++c) nx[l][c] = part[l][c] + part[l ^ off][c];
+        memcpy(part, nx, sizeof part);
+      }
+      const float avg[3] = {(float)(part[0][0] / (double)n), (float)(part[0][1] / (double)n),
+                            (float)(part[0][2] / (double)n)};
+      lump[at] = or_rgbexp32(avg[0], avg[1], avg[2]);
+      for (int64_t k = 0; k < n; ++k)
+        lump[at + 1 + k] = or_rgbexp32(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+    }
+    at += 1 + n;
+  }
+  return 4 * at;
 }

@@ -175,6 +175,16 @@ typedef struct {
  * past OR_FACE_MAX_POINTS). */
 int or_make_patches(const or_face_t *faces, int64_t n_faces, float max_chop, float min_chop,
                     or_patch_t *out, int64_t cap, int64_t *n_out);
+/* with GetPhongNormal child normals from per-face vertex normals
+ * (OR_FACE_MAX_POINTS x 3 floats per face; NULL = or_make_patches) */
+int or_make_patches_phong(const or_face_t *faces, int64_t n_faces, float max_chop,
+                          float min_chop, const float *vnorm, or_patch_t *out, int64_t cap,
+                          int64_t *n_out);
+/* GetPhongNormal at spot for a face with centroid c and vertex normals vn */
+void or_phong_normal(const or_face_t *fc, const float c[3], const float *vn, const float spot[3],
+                     float out[3]);
+/* PairEdges' vertex normals, OR_FACE_MAX_POINTS x 3 floats per face */
+void or_pair_edges(const or_face_t *faces, int64_t n, float threshold, float *out);
 
 /* ---- luxel stage ---- */
 typedef struct {
@@ -194,6 +204,19 @@ int64_t or_calc_face_extents(const or_face_t *faces, int64_t n, const float *lux
 void or_face_luxels(const or_face_t *faces, int64_t n_faces, const float *lux_ofs,
                     const or_face_lightmap_t *lm, const or_patch_t *patches, int64_t n_patches,
                     float *pos, int32_t *patch);
+/* ColorRGBExp32 (the Source lighting lump's luxel: r, g, b bytes and a
+ * signed exponent byte, packed little endian into a uint32): e =
+ * floor(log2(max channel)) - 7, each channel trunc(c * 2^-e); 0 for a zero,
+ * negative or non-finite maximum or e below -127. */
+uint32_t or_rgbexp32(float r, float g, float b);
+/* The lighting lump of a set of faces: for each face (in order) with
+ * n_luxels > 0, its average colour (ColorRGBExp32) and then its luxels;
+ * lightofs[f] = byte offset of face f's first luxel, -1 without luxels.
+ * Average: float64 channel sums, 64 lane partials (luxel k to lane k % 64),
+ * combined by xor butterfly (32 .. 1), divided by n in float64, rounded to
+ * float32.  Returns the lump size in bytes; lump may be NULL to size it. */
+int64_t or_lighting_lump(const float *luxel_rgb, const or_face_lightmap_t *lm, int64_t n_faces,
+                         uint32_t *lump, int32_t *lightofs);
 
 /* ---- BSP clusters and PVS (rad/clustertable/point.go, rad/lightmap/vis.go) */
 typedef struct {

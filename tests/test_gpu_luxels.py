@@ -116,6 +116,41 @@ def test_gpu_final_light_face():
     assert (lx["patch"] >= 0).all()
 
 
+def test_gpu_lighting_lump():
+    """The room's final luxel light encoded as the lighting lump: equal to the
+    restatement byte for byte, lightofs -1 exactly on the faces without
+    luxels, every luxel within one unit of its light, and hand-chosen luxels
+    encoded to their hand-derived bytes."""
+    s = scenes.room(seed=1, n_pillars=40, chop=64.0, tess=32.0)
+    faces = scenes.faces_of("room")
+    patches = MakePatches(faces)
+    env = Environment()
+    env.AddTriangles(s.verts9, s.ids)
+    env.SetupAccelerationStructure()
+    rng = np.random.default_rng(3)
+    direct = rng.uniform(0, 4, (len(patches), 3)).astype(np.float32)
+    lm, light, _ = luxels.FinalLightFace(env, faces, patches, direct * np.float32(1.5), direct,
+                                         s.lights)
+    print(f"[gpu] lighting lump: {len(faces)} faces, {len(light)} luxels", flush=True)
+    lump, ofs = luxels.LightingLump(light, lm)
+    o_lump, o_ofs = oracle.lighting_lump(light, lm)
+    assert np.array_equal(lump, o_lump) and np.array_equal(ofs, o_ofs)
+    assert np.array_equal(ofs < 0, lm["n_luxels"] == 0)
+    for f in np.flatnonzero(lm["n_luxels"] > 0)[:50]:
+        a, n = int(ofs[f]), int(lm["n_luxels"][f])
+        dec = luxels.decode_rgbexp32(lump[a:a + 4 * n]).astype(np.float64)
+        src = light[lm["luxel_ofs"][f]:lm["luxel_ofs"][f] + n].astype(np.float64)
+        unit = np.ldexp(1.0, lump[a + 3:a + 4 * n:4].view(np.int8).astype(int))
+        assert np.all(dec <= src) and np.all(src - dec < unit[:, None])
+    hand = np.array([[1.0, 0.5, 0.25], [255.0, 0.0, 0.0], [256.0, 1.0, 0.0], [1e-3, 2e-3, 0.0]],
+                    np.float32)
+    lmh = np.zeros(1, _lib.FACE_LIGHTMAP_DTYPE)
+    lmh["n_luxels"] = 4
+    lh, oh = luxels.LightingLump(hand, lmh)
+    assert list(oh) == [4]
+    assert lh[4:].tolist() == [128, 64, 32, 0xF9, 255, 0, 0, 0, 128, 0, 0, 1, 65, 131, 0, 0xF0]
+
+
 def test_gpu_luxels_full_level():
     """The 8x8-room level's 2,000 faces at 16 units per luxel (1.46 M
     luxels): GPU == restatement, every luxel on its face's plane."""

@@ -94,3 +94,24 @@ def test_gpu_patches_count_only_and_capacity(build):
     rc = _lib.lib.vr_make_patches(_lib.ptr(fa), len(fa), 4.0, 4.0, _lib.ptr(small), len(small),
                                   C.byref(n))
     assert rc == _lib.VR_E_CAPACITY
+
+
+def test_gpu_patches_phong_normals(build):
+    """Children take GetPhongNormal normals from PairEdges' vertex normals
+    (rad/lightmap/normallist.go:51-144): bit for bit against the oracle, on
+    a generated level with some normals jittered into shallow creases and on
+    random polygons with random unit vertex normals."""
+    from vrad_amd.patches import PairEdges
+    rng = np.random.default_rng(23)
+    f = scenes.faces_of("level", rooms_x=3, rooms_y=2, chop=64.0, tess=64.0)
+    pick = rng.random(len(f)) < 0.3
+    nn = f["normal"][pick] + rng.normal(scale=0.2, size=(pick.sum(), 3)).astype(np.float32)
+    f["normal"][pick] = (nn / np.linalg.norm(nn, axis=1, keepdims=True)).astype(np.float32)
+    vn = PairEdges(f)
+    assert np.array_equal(vn.view(np.uint32), oracle.pair_edges(f).view(np.uint32))
+    g = _same(f, "level 3x2, phong normals", vertex_normals=vn)
+    assert (g["normal"][g["parent"] >= 0] != f["normal"][g["face"][g["parent"] >= 0]]).any()
+    r = _random_convex(rng, 1500)
+    rv = rng.normal(size=(len(r), 16, 3))
+    rv = (rv / np.linalg.norm(rv, axis=2, keepdims=True)).astype(np.float32)
+    _same(r, "random convex polygons, random vertex normals", vertex_normals=rv)

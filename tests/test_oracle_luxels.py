@@ -102,3 +102,72 @@ def test_luxel_patch_choice_independent_check():
                 d = np.sum((p - ctr) ** 2, axis=1)
                 assert pat[k] == mine[np.flatnonzero(d == d.min())[0]]
     assert outside > 0
+
+
+# ---- the lighting lump (ColorRGBExp32) ------------------------------------------------
+def test_rgbexp32_hand_encoded():
+    """Hand-derived: (1, .5, .25): max 1 = 2^0, e = -7, x 128 -> 128, 64, 32;
+    255 fits with e = 0; 256 needs e = 1 (1 -> 0.5 truncates to 0); 2e-3:
+    floor(log2) = -9, e = -16, x 65536 -> 131 and 65 (1e-3)."""
+    assert oracle.rgbexp32(1.0, 0.5, 0.25) == [128, 64, 32, 0xF9]
+    assert oracle.rgbexp32(255.0, 0.0, 0.0) == [255, 0, 0, 0]
+    assert oracle.rgbexp32(256.0, 1.0, 0.0) == [128, 0, 0, 1]
+    assert oracle.rgbexp32(1e-3, 2e-3, 0.0) == [65, 131, 0, 0xF0]
+    assert oracle.rgbexp32(0.0, 0.0, 0.0) == [0, 0, 0, 0]
+    assert oracle.rgbexp32(-1.0, -2.0, -3.0) == [0, 0, 0, 0]
+    assert oracle.rgbexp32(1e-39, 0, 0) == [0, 0, 0, 0]           # e below -127
+    assert oracle.rgbexp32(float("inf"), 1, 1) == [0, 0, 0, 0]
+
+
+def test_rgbexp32_round_trip_bound():
+    """Decoded = bytes x 2^e: the largest channel lands in [128, 256) (so its
+    relative error is below 1/128), every channel is at most the input and
+    within one unit 2^e of it."""
+    from vrad_amd.luxels import decode_rgbexp32
+    rng = np.random.default_rng(5)
+    c = (rng.random((5000, 3)) * np.exp(rng.uniform(-20, 20, (5000, 1)))).astype(np.float32)
+    enc = np.array([oracle.rgbexp32(*v) for v in c], np.uint8)
+    dec = decode_rgbexp32(enc).astype(np.float64)
+    e = enc[:, 3].view(np.int8).astype(np.float64)
+    m = c.max(axis=1).astype(np.float64)
+    assert np.all(enc[:, :3].max(axis=1) >= 128)
+    assert np.all(dec <= c + 0.0) and np.all(c - dec < np.ldexp(1.0, e.astype(int))[:, None])
+    assert np.all((m - dec.max(axis=1)) / m < 1.0 / 128)
+
+
+def test_lighting_lump_layout():
+    """Two lit faces around an unlit one: each lit face's average then its
+    luxels; lightofs points past the average (-1 when unlit).  Constant luxels
+    average to themselves."""
+    lm = np.zeros(3, oracle.FACE_LIGHTMAP_DTYPE)
+    lm["n_luxels"] = [4, 0, 70]
+    lm["luxel_ofs"] = [0, 4, 4]
+    lux = np.zeros((74, 3), np.float32)
+    lux[:4] = (2.0, 1.0, 0.5)
+    lux[4:] = np.linspace(0, 8, 70 * 3, dtype=np.float32).reshape(70, 3)
+    lump, ofs = oracle.lighting_lump(lux, lm)
+    assert list(ofs) == [4, -1, 4 + 16 + 4] and len(lump) == 4 * (5 + 71)
+    words = lump.reshape(-1, 4)
+    assert list(words[0]) == oracle.rgbexp32(2.0, 1.0, 0.5)        # the average
+    assert all(list(words[1 + k]) == oracle.rgbexp32(2.0, 1.0, 0.5) for k in range(4))
+    avg = lux[4:].astype(np.float64).mean(axis=0).astype(np.float32)
+    assert list(words[5]) == oracle.rgbexp32(*avg)
+    assert all(list(words[6 + k]) == oracle.rgbexp32(*lux[4 + k]) for k in range(70))
+
+
+def test_lighting_lump_sizing_on_host():
+    """The product sizes the lump and fills lightofs on the host (no GPU)."""
+    import ctypes as C
+    from vrad_amd import _lib
+    lm = np.zeros(4, _lib.FACE_LIGHTMAP_DTYPE)
+    lm["n_luxels"] = [9, 0, 1, 25]
+    lm["luxel_ofs"] = [0, 9, 9, 10]
+    nb = C.c_int64()
+    ofs = np.zeros(4, np.int32)
+    _lib.check(_lib.lib.vr_lighting_lump(None, 35, _lib.ptr(lm), 4, None, 0, _lib.ptr(ofs),
+                                         C.byref(nb)))
+    o_lump, o_ofs = oracle.lighting_lump(np.zeros((35, 3), np.float32), lm)
+    assert nb.value == len(o_lump) == 4 * (10 + 2 + 26) and np.array_equal(ofs, o_ofs)
+    lm["luxel_ofs"][3] = 11                                    # runs past the luxels
+    assert _lib.lib.vr_lighting_lump(None, 35, _lib.ptr(lm), 4, None, 0, _lib.ptr(ofs),
+                                     C.byref(nb)) == _lib.VR_E_INVALID

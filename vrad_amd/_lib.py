@@ -74,6 +74,7 @@ class Light(C.Structure):
 
 
 # vr_face / vr_patch records
+VR_FACE_MAX_POINTS = 16  # points per vr_face / vr_patch winding
 FACE_DTYPE = np.dtype([("n_points", "<i4"), ("points", "<f4", (16, 3)), ("normal", "<f4", 3),
                        ("lux_s", "<f4", 3), ("lux_t", "<f4", 3), ("base_light", "<f4", 3),
                        ("flags", "<u4")])
@@ -163,6 +164,8 @@ _SIGS = {
     "vr_gather_sample_light": (C.c_int, [_P, C.POINTER(Samples), C.c_int64, _P, C.c_int32,
                                          C.c_float, C.c_int32, _P]),
     "vr_can_leaf_trace_to_sky": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    "vr_lighting_lump": (C.c_int, [_P, C.c_int64, _P, C.c_int64, _P, C.c_int64, _P,
+                                   C.POINTER(C.c_int64)]),
     "vr_make_transfers_batch": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(XferParams), _P]),
     "vr_gather_sample_light_batch": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, C.c_float,
                                                C.c_int32, _P]),
@@ -181,6 +184,9 @@ _SIGS = {
     "vr_gen_faces": (C.c_int, [_P, C.c_float, _P, C.POINTER(C.c_int64)]),
     "vr_make_patches": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, _P, C.c_int64,
                                   C.POINTER(C.c_int64)]),
+    "vr_make_patches_phong": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, _P, _P,
+                                        C.c_int64, C.POINTER(C.c_int64)]),
+    "vr_pair_edges": (C.c_int, [_P, C.c_int64, C.c_float, _P]),
     "vr_gen_grid": (C.c_int, [_P, _P, C.POINTER(C.c_int32), _P, C.POINTER(C.c_int32)]),
     "vr_gen_destroy": (C.c_int, [_P]),
     "vr_calc_face_extents": (C.c_int, [_P, C.c_int64, _P, _P, C.POINTER(C.c_int64)]),

@@ -2436,6 +2436,25 @@ int vr_gen_copy(const vr_gen *g, float *verts9, int32_t *ids, uint8_t *flags, fl
 
 int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
                     vr_patch *out, int64_t cap, int64_t *n_out) {
+  return vr_make_patches_phong(faces, n_faces, max_chop, min_chop, nullptr, out, cap, n_out);
+}
+
+int vr_pair_edges(const vr_face *faces, int64_t n_faces, float smoothing_threshold,
+                  float *vertex_normals) {
+  LOCK;
+  if (n_faces < 0) return fail(VR_E_INVALID, "n_faces < 0");
+  if (n_faces > 0 && (!faces || !vertex_normals)) return fail(VR_E_INVALID, "NULL argument");
+  for (int64_t f = 0; f < n_faces; ++f)
+    if (faces[f].n_points < 0 || faces[f].n_points > VR_FACE_MAX_POINTS)
+      return fail(VR_E_INVALID, "face " + std::to_string(f) + ": n_points outside 0.." +
+                                    std::to_string(VR_FACE_MAX_POINTS));
+  pair_edges(faces, n_faces, smoothing_threshold, vertex_normals);
+  return VR_OK;
+}
+
+int vr_make_patches_phong(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
+                          const float *vertex_normals, vr_patch *out, int64_t cap,
+                          int64_t *n_out) {
   LOCK;
   if (!n_out) return fail(VR_E_INVALID, "n_out is NULL");
   *n_out = 0;
@@ -2450,7 +2469,8 @@ int vr_make_patches(const vr_face *faces, int64_t n_faces, float max_chop, float
                                     std::to_string(VR_FACE_MAX_POINTS));
   TRY(ensure_init());
   Prof p("make_patches", g_stream);
-  return make_patches_device(faces, n_faces, max_chop, min_chop, out, cap, n_out, g_stream);
+  return make_patches_device(faces, n_faces, max_chop, min_chop, vertex_normals, out, cap, n_out,
+                             g_stream);
 }
 
 // Checks the BSP the kernels walk: every plane and child index reachable
@@ -2536,6 +2556,55 @@ int vr_calc_face_extents(const vr_face *faces, int64_t n, const float *lux_ofs,
     at += out[f].n_luxels;
   }
   *n_luxels = at;
+  return VR_OK;
+}
+
+int vr_lighting_lump(const float *luxel_rgb, int64_t n_luxels, const vr_face_lightmap *lm,
+                     int64_t n_faces, uint8_t *lump, int64_t lump_cap, int32_t *lightofs,
+                     int64_t *lump_bytes) {
+  LOCK;
+  if (!lm || !lump_bytes) return fail(VR_E_INVALID, "NULL argument");
+  if (n_faces < 0 || n_luxels < 0) return fail(VR_E_INVALID, "negative count");
+  std::vector<int64_t> at((size_t)n_faces);
+  int64_t e = 0;  // 4-byte entries
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const int64_t n = lm[f].n_luxels;
+    if (n <= 0) {
+      at[(size_t)f] = -1;
+      continue;
+    }
+    if (lm[f].luxel_ofs < 0 || lm[f].luxel_ofs > n_luxels - n)
+      return fail(VR_E_INVALID, "face " + std::to_string(f) + ": luxels outside luxel_rgb");
+    at[(size_t)f] = e;
+    e += 1 + n;
+  }
+  if (4 * e > (int64_t)INT32_MAX)
+    return fail(VR_E_CAPACITY, "lighting lump above 2 GiB (int32 face offsets)");
+  *lump_bytes = 4 * e;
+  if (lightofs)
+    for (int64_t f = 0; f < n_faces; ++f)
+      lightofs[f] = at[(size_t)f] < 0 ? -1 : (int32_t)(4 * (at[(size_t)f] + 1));
+  if (!lump) return VR_OK;
+  if (lump_cap < 4 * e) return fail(VR_E_CAPACITY, "lump buffer smaller than lump_bytes");
+  if (e == 0) return VR_OK;
+  if (!luxel_rgb) return fail(VR_E_INVALID, "luxel_rgb is NULL");
+  TRY(ensure_init());
+  Staging S;
+  TRY(S.make({12 * (size_t)std::max<int64_t>(n_luxels, 1),
+              sizeof(vr_face_lightmap) * (size_t)std::max<int64_t>(n_faces, 1),
+              8 * (size_t)std::max<int64_t>(n_faces, 1), 4 * (size_t)e}));
+  if (n_luxels > 0)
+    VR_HIP(hipMemcpyAsync(S.ptr<float>(0), luxel_rgb, 12 * (size_t)n_luxels,
+                          hipMemcpyHostToDevice, g_stream));
+  VR_HIP(hipMemcpyAsync(S.ptr<vr_face_lightmap>(1), lm, sizeof(vr_face_lightmap) * (size_t)n_faces,
+                        hipMemcpyHostToDevice, g_stream));
+  VR_HIP(hipMemcpyAsync(S.ptr<int64_t>(2), at.data(), 8 * (size_t)n_faces, hipMemcpyHostToDevice,
+                        g_stream));
+  launch_lighting_lump(S.ptr<float>(0), S.ptr<vr_face_lightmap>(1), S.ptr<int64_t>(2), n_faces,
+                       S.ptr<uint32_t>(3), g_stream);
+  TRY(check_launch());
+  VR_HIP(hipMemcpyAsync(lump, S.ptr<uint32_t>(3), 4 * (size_t)e, hipMemcpyDeviceToHost, g_stream));
+  VR_HIP(hipStreamSynchronize(g_stream));
   return VR_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <future>
 #include <thread>
+#include <unordered_map>
 
 #include "vr_internal.h"
 
@@ -341,6 +342,66 @@ int build_kdtree(const float *v9, int64_t n, int n_threads, KdTree *out) {
   std::memcpy(out->bmax, mx, sizeof mx);
   out->max_depth = S.max_depth;
   return VR_OK;
+}
+
+}  // namespace vr
+
+namespace vr {
+
+// PairEdges (rad/lightmap/lightmap.go:37-216) without displacements or
+// smoothing groups (every face's SmoothingGroups is 0): for vertex j of face
+// i, the face normals of the other faces that use the same vertex, when
+// their cosine with face i's normal is at least the threshold, are added in
+// face order to a zero vector, then face i's normal, and the sum is
+// normalised (mgl32 Add, Dot, Normalize).  A vertex is identified by its
+// position, bit for bit (the BSP's shared vertex index).  The loop over
+// faces per vertex takes j below the face's own edge count, as the Go code's
+// comment intends (it iterates over the map's edge count).
+void pair_edges(const vr_face *faces, int64_t n, float threshold, float *out) {
+  struct Key {
+    uint32_t x, y, z;
+    bool operator==(const Key &o) const { return x == o.x && y == o.y && z == o.z; }
+  };
+  struct Hash {
+    size_t operator()(const Key &k) const {
+      uint64_t h = (uint64_t)k.x * 0x9E3779B97F4A7C15ull;
+      h ^= (uint64_t)k.y * 0xC2B2AE3D27D4EB4Full + (h >> 29);
+      h ^= (uint64_t)k.z * 0x165667B19E3779F9ull + (h >> 31);
+      return (size_t)h;
+    }
+  };
+  auto key = [](const float p[3]) {
+    Key k;
+    std::memcpy(&k.x, &p[0], 4);
+    std::memcpy(&k.y, &p[1], 4);
+    std::memcpy(&k.z, &p[2], 4);
+    return k;
+  };
+  std::unordered_map<Key, std::vector<int64_t>, Hash> users;
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = 0; j < faces[i].n_points; ++j) {
+      std::vector<int64_t> &u = users[key(faces[i].points[j])];
+      if (u.empty() || u.back() != i) u.push_back(i);
+    }
+  for (int64_t i = 0; i < n; ++i) {
+    const float *fn = faces[i].normal;
+    float *o = out + (size_t)i * VR_FACE_MAX_POINTS * 3;
+    std::fill(o, o + VR_FACE_MAX_POINTS * 3, 0.0f);
+    for (int j = 0; j < faces[i].n_points; ++j) {
+      float s[3] = {0.0f, 0.0f, 0.0f};
+      for (int64_t k : users[key(faces[i].points[j])]) {
+        if (k == i) continue;
+        const float *nn = faces[k].normal;
+        const float c = (nn[0] * fn[0] + nn[1] * fn[1]) + nn[2] * fn[2];
+        if (!(c >= threshold)) continue;
+        for (int q = 0; q < 3; ++q) s[q] = s[q] + nn[q];
+      }
+      for (int q = 0; q < 3; ++q) s[q] = s[q] + fn[q];
+      const float len = (float)std::sqrt((double)((s[0] * s[0] + s[1] * s[1]) + s[2] * s[2]));
+      const float inv = 1.0f / len;
+      for (int q = 0; q < 3; ++q) o[3 * j + q] = s[q] * inv;
+    }
+  }
 }
 
 }  // namespace vr

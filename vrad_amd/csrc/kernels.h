@@ -35,8 +35,11 @@ Tuning &tuning();
 // ---- patch generation (patches.hip) ----
 // MakePatchForFace + SubdividePatches on the GPU; *n_out = patch count,
 // written to out when cap allows (out == NULL: count only).
+// vnorm (nullable): PairEdges' per-vertex normals, VR_FACE_MAX_POINTS x 3 per
+// face; children then take GetPhongNormal at their centre
 int make_patches_device(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
-                        vr_patch *out, int64_t cap, int64_t *n_out, hipStream_t st);
+                        const float *vnorm, vr_patch *out, int64_t cap, int64_t *n_out,
+                        hipStream_t st);
 
 // ---- kd-tree construction on the device (kdbuild.hip) ----
 struct KdTree;
@@ -100,6 +103,11 @@ void launch_face_luxels(const vr_face *faces, const float *lux_ofs, const vr_fac
                         const int32_t *leaf_idx, const vr_patch *patches,
                         const float *patch_value, float *pos, float *nrm, int32_t *lpatch,
                         float *value, hipStream_t st);
+
+// the lighting lump: per face with luxels, entry at[f] = its average colour,
+// then its luxels as ColorRGBExp32
+void launch_lighting_lump(const float *lux, const vr_face_lightmap *lm, const int64_t *at,
+                          int64_t n_faces, uint32_t *lump, hipStream_t st);
 
 // ---- BSP clusters and PVS (pvs.hip) ----
 struct DevBsp {

@@ -202,4 +202,71 @@ void launch_face_luxels(const vr_face *faces, const float *lux_ofs, const vr_fac
                      blocks, leaf_ofs, leaf_idx, patches, patch_value, pos, nrm, lpatch, value);
 }
 
+// ---- the lighting lump: ColorRGBExp32 luxels -------------------------------
+// VectorToColorRGBExp32, the Source engine's luxel encoding (named in the
+// reference's compute/leafambient/lighting.go:67; the reference's lump writer
+// is a stub, cmd/tasks/finish/main.go:15-18): e = floor(log2(max channel)) - 7
+// puts the largest channel in [128, 256) after scaling by 2^-e, and each
+// channel is truncated to a byte; bytes r, g, b, e (signed) little endian.
+// Zero for a zero, negative or non-finite maximum, or e below -127.
+__device__ __forceinline__ uint32_t rgbexp32(float r, float g, float b) {
+  float m = r > g ? r : g;
+  m = m > b ? m : b;
+  if (!(m > 0.0f) || !(m <= 3.402823466e38f)) return 0u;
+  int ex;
+  (void)frexpf(m, &ex);
+  const int e = (ex - 1) - 7;
+  if (e < -127 || e > 127) return 0u;
+  const float scale = ldexpf(1.0f, -e);
+  const float c[3] = {r, g, b};
+  uint32_t out = 0u;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float v = c[k] > 0.0f ? c[k] * scale : 0.0f;
+    int q = (int)v;
+    q = q > 255 ? 255 : q;
+    out |= (uint32_t)q << (8 * k);
+  }
+  return out | ((uint32_t)(uint8_t)(int8_t)e << 24);
+}
+
+// One wave per face with luxels: its average colour (float64 lane partials of
+// luxels k = lane mod 64, an xor butterfly 32 .. 1, the mean rounded to
+// float32) at entry at[f], then its luxels.
+__global__ __launch_bounds__(256) void k_lighting_lump(const float *__restrict__ lux,
+                                                       const vr_face_lightmap *__restrict__ lm,
+                                                       const int64_t *__restrict__ at,
+                                                       int64_t n_faces,
+                                                       uint32_t *__restrict__ lump) {
+  const int lane = threadIdx.x & 63;
+  const int64_t f = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (f >= n_faces) return;
+  const int64_t n = lm[f].n_luxels;
+  if (n <= 0) return;
+  const float *v = lux + 3 * lm[f].luxel_ofs;
+  uint32_t *out = lump + at[f];
+  double sr = 0.0, sg = 0.0, sb = 0.0;
+  for (int64_t k = lane; k < n; k += 64) {
+    const float r = v[3 * k], g = v[3 * k + 1], b = v[3 * k + 2];
+    sr = sr + (double)r;
+    sg = sg + (double)g;
+    sb = sb + (double)b;
+    out[1 + k] = rgbexp32(r, g, b);
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    sr = sr + __shfl_xor(sr, off);
+    sg = sg + __shfl_xor(sg, off);
+    sb = sb + __shfl_xor(sb, off);
+  }
+  if (lane == 0)
+    out[0] = rgbexp32((float)(sr / (double)n), (float)(sg / (double)n), (float)(sb / (double)n));
+}
+
+void launch_lighting_lump(const float *lux, const vr_face_lightmap *lm, const int64_t *at,
+                          int64_t n_faces, uint32_t *lump, hipStream_t st) {
+  if (n_faces <= 0) return;
+  hipLaunchKernelGGL(k_lighting_lump, dim3((unsigned)((n_faces + 3) / 4)), dim3(256), 0, st, lux,
+                     lm, at, n_faces, lump);
+}
+
 }  // namespace vr

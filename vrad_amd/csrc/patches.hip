@@ -19,9 +19,9 @@
 // code does not do what it intends, the intent: WindingCenter
 // (vmath/polygon/winding.go:231-239) averages the points (scale 1 / n), and
 // WindingAreaAndBalancePoint (subdivide.go:412-438) returns its area-weighted
-// centre.  A child's normal is its face's (GetPhongNormal,
-// rad/lightmap/normallist.go:51-144, with no neighbour within the smoothing
-// threshold).
+// centre.  A child's normal is GetPhongNormal (rad/lightmap/
+// normallist.go:51-144) at its centre when per-vertex normals (PairEdges)
+// are given, else its face's normal.
 #include <cstring>
 #include <vector>
 
@@ -200,13 +200,54 @@ __device__ bool base_patch(const vr_face &fc, int32_t face, float max_chop, vr_p
   return true;
 }
 
+// GetPhongNormal (rad/lightmap/normallist.go:51-144) at spot on face fc:
+// c is the face centroid (the base patch's centre, GetFaceCentroids), vn the
+// face's per-vertex normals (PairEdges).  For the first edge (j, j + 1)
+// whose wedge from the centroid holds the spot (a1 >= 0 and a2 >= 0 in the
+// wedge's coordinates), the face normal, vn[j] and vn[j + 1] are blended
+// with weights 1 - a1 - a2, a1, a2 and normalised (mgl32 ops, one rounding
+// each); otherwise the face normal.  A blend of length below 1e-20 (where
+// the Go code stops with log.Fatalf) keeps the face normal.
+__device__ void phong_normal(const vr_face &fc, const float c[3], const float *__restrict__ vn,
+                             const float spot[3], float out[3]) {
+  auto dot = [](const float *a, const float *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; };
+  for (int k = 0; k < 3; ++k) out[k] = fc.normal[k];
+  const int E = fc.n_points;
+  for (int j = 0; j < E; ++j) {
+    const int j2 = (j + 1) % E;
+    float v1[3], v2[3], vs[3];
+    for (int k = 0; k < 3; ++k) {
+      v1[k] = fc.points[j][k] - c[k];
+      v2[k] = fc.points[j2][k] - c[k];
+      vs[k] = spot[k] - c[k];
+    }
+    const float aa = dot(v1, v1), bb = dot(v2, v2), ab = dot(v1, v2);
+    const float a1 = (bb * dot(v1, vs) - ab * dot(vs, v2)) / (aa * bb - ab * ab);
+    const float a2 = (dot(vs, v2) - a1 * ab) / bb;
+    if (a1 >= 0.0f && a2 >= 0.0f) {
+      const float sc = (1.0f - a1) - a2;
+      float ph[3];
+      for (int k = 0; k < 3; ++k) ph[k] = fc.normal[k] * sc;
+      for (int k = 0; k < 3; ++k) ph[k] = ph[k] + vn[3 * j + k] * a1;
+      for (int k = 0; k < 3; ++k) ph[k] = ph[k] + vn[3 * j2 + k] * a2;
+      const float len = __builtin_sqrtf(dot(ph, ph));
+      if (len < 1.0e-20f) return;  // the reference stops the bake here
+      const float inv = 1.0f / len;
+      for (int k = 0; k < 3; ++k) out[k] = ph[k] * inv;
+      return;
+    }
+  }
+}
+
 // SubdividePatch's step for one patch P (subdivide.go:166-262 with
 // CreateChildPatch, :353-410): decides the split (updating P.chop as the
 // reference does) and fills the two children (parent / child indices left
-// to the caller).  face_mn / face_mx are the face's base-patch bounds.
+// to the caller).  face_mn / face_mx are the face's base-patch bounds,
+// centroid its base patch's centre, vn its per-vertex normals (nullable).
 // Returns 1 split, 0 leaf, -1 a winding past the point capacity.
 __device__ int split_step(const vr_face &fc, vr_patch &P, const float face_mn[3],
-                          const float face_mx[3], float min_chop, vr_patch ch[2]) {
+                          const float face_mx[3], const float centroid[3],
+                          const float *__restrict__ vn, float min_chop, vr_patch ch[2]) {
   bool split = false;
   Wind o1, o2;
   float c1[3], c2[3], a1 = 0.0f, a2 = 0.0f;
@@ -258,7 +299,10 @@ __device__ int split_step(const vr_face &fc, vr_patch &P, const float face_mn[3]
       for (int z = 0; z < 3; ++z) C.points[i][z] = i < ow[q]->n ? ow[q]->p[i][z] : 0.0f;
     C.area = ar[q];
     for (int z = 0; z < 3; ++z) C.origin[z] = ca[q][z];
-    for (int z = 0; z < 3; ++z) C.normal[z] = fc.normal[z];
+    if (vn)
+      phong_normal(fc, centroid, vn, C.origin, C.normal);
+    else
+      for (int z = 0; z < 3; ++z) C.normal[z] = fc.normal[z];
     wind_bounds(*ow[q], C.mins, C.maxs);
     if (zero_light(fc.base_light)) {
       float t[3];
@@ -287,14 +331,15 @@ struct Pending {  // a second child waiting for its subdivision
 // or the stack overflows.
 template <bool WRITE>
 __device__ int64_t subdivide_face(const vr_face &fc, vr_patch P, int64_t bidx, int64_t cbase,
-                                  float min_chop, vr_patch *out, Pending *stk) {
+                                  float min_chop, const float *vn, vr_patch *out, Pending *stk) {
   const float face_mn[3] = {P.mins[0], P.mins[1], P.mins[2]};
   const float face_mx[3] = {P.maxs[0], P.maxs[1], P.maxs[2]};
+  const float centroid[3] = {P.origin[0], P.origin[1], P.origin[2]};
   int64_t made = 0, k = bidx;
   int sp = 0;
   for (;;) {
     vr_patch ch[2];
-    const int r = split_step(fc, P, face_mn, face_mx, min_chop, ch);
+    const int r = split_step(fc, P, face_mn, face_mx, centroid, vn, min_chop, ch);
     if (r < 0) return -1;
     if (r) {
       const int64_t i1 = cbase + made, i2 = cbase + made + 1;
@@ -335,7 +380,8 @@ __global__ __launch_bounds__(64) void k_make_patches(const vr_face *__restrict__
                                                      const int64_t *__restrict__ bofs,
                                                      const int64_t *__restrict__ cofs,
                                                      vr_patch *__restrict__ out,
-                                                     Pending *__restrict__ stacks) {
+                                                     Pending *__restrict__ stacks,
+                                                     const float *__restrict__ vnorm) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n_faces) return;
   const vr_face fc = faces[f];
@@ -360,7 +406,8 @@ __global__ __launch_bounds__(64) void k_make_patches(const vr_face *__restrict__
     return;
   }
   const int64_t m = subdivide_face<WRITE>(fc, P, WRITE ? bofs[f] : 0, WRITE ? cofs[f] : 0,
-                                          min_chop, out, stk);
+                                          min_chop,
+                                          vnorm ? vnorm + f * (kMaxPts * 3) : nullptr, out, stk);
   if (!WRITE) cnt[f] = m;
 }
 
@@ -397,9 +444,10 @@ __global__ __launch_bounds__(256) void k_lv_gather(const vr_patch *__restrict__ 
   if (i >= n) return;
   const vr_patch P = tmp[face_of[i]];
   lv[i] = P;
-  for (int k = 0; k < 3; ++k) {
-    fbounds[6 * (int64_t)P.face + k] = P.mins[k];
-    fbounds[6 * (int64_t)P.face + 3 + k] = P.maxs[k];
+  for (int k = 0; k < 3; ++k) {  // base bounds and centre (the face centroid)
+    fbounds[9 * (int64_t)P.face + k] = P.mins[k];
+    fbounds[9 * (int64_t)P.face + 3 + k] = P.maxs[k];
+    fbounds[9 * (int64_t)P.face + 6 + k] = P.origin[k];
   }
 }
 
@@ -410,16 +458,18 @@ __global__ __launch_bounds__(64) void k_lv_split(const vr_face *__restrict__ fac
                                                  vr_patch *__restrict__ lv, int64_t n,
                                                  float min_chop, vr_patch *__restrict__ kids,
                                                  uint8_t *__restrict__ split,
-                                                 uint32_t *__restrict__ err) {
+                                                 uint32_t *__restrict__ err,
+                                                 const float *__restrict__ vnorm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   vr_patch P = lv[i];
   const vr_face &fc = faces[P.face];
   split[i] = 0;
   if (fc.flags & VR_FACE_NOCHOP) return;  // PreventSubdivision: never subdivided
-  const float *b = fbounds + 6 * (int64_t)P.face;
+  const float *b = fbounds + 9 * (int64_t)P.face;
   vr_patch ch[2];
-  const int r = split_step(fc, P, b, b + 3, min_chop, ch);
+  const int r = split_step(fc, P, b, b + 3, b + 6,
+                           vnorm ? vnorm + (int64_t)P.face * (kMaxPts * 3) : nullptr, min_chop, ch);
   if (r < 0) {
     atomicOr(err, 1u);
     return;
@@ -502,8 +552,8 @@ __global__ __launch_bounds__(256) void k_lv_scatter(const vr_patch *__restrict__
 // Level-parallel MakePatches + SubdividePatches; the same patches, in the
 // same order, as the thread-per-face kernel.
 static int make_patches_levels(const vr_face *faces, int64_t n_faces, float max_chop,
-                               float min_chop, vr_patch *out, int64_t cap, int64_t *n_out,
-                               hipStream_t st) {
+                               float min_chop, const float *vnorm, vr_patch *out, int64_t cap,
+                               int64_t *n_out, hipStream_t st) {
   struct Buf {  // owns one device allocation; moves, never copies
     void *p = nullptr;
     Buf() = default;
@@ -521,7 +571,7 @@ static int make_patches_levels(const vr_face *faces, int64_t n_faces, float max_
   constexpr int kMaxLevels = 64;
   std::vector<Level> L(1);
   L.reserve(kMaxLevels + 1);
-  Buf d_faces, d_tmp, d_has, d_fof, d_fb, d_kids, d_err, d_out;
+  Buf d_faces, d_tmp, d_has, d_fof, d_fb, d_kids, d_err, d_out, d_vn;
   int rc = VR_OK;
   auto H = [&](hipError_t e, const char *what) {
     if (e == hipSuccess) return VR_OK;
@@ -532,10 +582,14 @@ static int make_patches_levels(const vr_face *faces, int64_t n_faces, float max_
   auto M = [&](Buf &b, size_t bytes) { return H(hipMalloc(&b.p, bytes ? bytes : 16), "hipMalloc"); };
   const unsigned gf = (unsigned)((n_faces + 63) / 64);
   if ((rc = M(d_faces, sizeof(vr_face) * n_faces)) || (rc = M(d_tmp, sizeof(vr_patch) * n_faces)) ||
-      (rc = M(d_has, n_faces)) || (rc = M(d_fb, 24 * n_faces)) || (rc = M(d_err, 4)) ||
+      (rc = M(d_has, n_faces)) || (rc = M(d_fb, 36 * n_faces)) || (rc = M(d_err, 4)) ||
       (rc = H(hipMemsetAsync(d_err.p, 0, 4, st), "memset")) ||
       (rc = H(hipMemcpyAsync(d_faces.p, faces, sizeof(vr_face) * n_faces, hipMemcpyHostToDevice, st),
               "upload")))
+    return rc;
+  const size_t vnb = sizeof(float) * kMaxPts * 3 * (size_t)n_faces;
+  if (vnorm && ((rc = M(d_vn, vnb)) ||
+                (rc = H(hipMemcpyAsync(d_vn.p, vnorm, vnb, hipMemcpyHostToDevice, st), "upload"))))
     return rc;
   hipLaunchKernelGGL(k_lv_base, dim3(gf), dim3(64), 0, st, (const vr_face *)d_faces.p, n_faces,
                      max_chop, (vr_patch *)d_tmp.p, (uint8_t *)d_has.p);
@@ -570,7 +624,8 @@ static int make_patches_levels(const vr_face *faces, int64_t n_faces, float max_
       return rc;
     hipLaunchKernelGGL(k_lv_split, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
                        (const vr_face *)d_faces.p, (const float *)d_fb.p, (vr_patch *)C.lv.p, n,
-                       min_chop, (vr_patch *)kids.p, (uint8_t *)C.split.p, (uint32_t *)d_err.p);
+                       min_chop, (vr_patch *)kids.p, (uint8_t *)C.split.p, (uint32_t *)d_err.p,
+                       (const float *)d_vn.p);
     sp.resize((size_t)n);
     uint32_t err = 0;
     if ((rc = H(hipGetLastError(), "launch")) ||
@@ -670,17 +725,18 @@ static int make_patches_levels(const vr_face *faces, int64_t n_faces, float max_
 }
 
 int make_patches_device(const vr_face *faces, int64_t n_faces, float max_chop, float min_chop,
-                        vr_patch *out, int64_t cap, int64_t *n_out, hipStream_t st) {
+                        const float *vnorm, vr_patch *out, int64_t cap, int64_t *n_out,
+                        hipStream_t st) {
   *n_out = 0;
   if (n_faces == 0) return VR_OK;
   if (tuning().patch_build == 1)
-    return make_patches_levels(faces, n_faces, max_chop, min_chop, out, cap, n_out, st);
+    return make_patches_levels(faces, n_faces, max_chop, min_chop, vnorm, out, cap, n_out, st);
   void *d_faces = nullptr, *d_base = nullptr, *d_cnt = nullptr, *d_ofs = nullptr,
-       *d_out = nullptr, *d_stk = nullptr;
+       *d_out = nullptr, *d_stk = nullptr, *d_vn = nullptr;
   int rc = VR_OK;
   auto release = [&]() {
     (void)hipStreamSynchronize(st);
-    for (void *p : {d_faces, d_base, d_cnt, d_ofs, d_out, d_stk})
+    for (void *p : {d_faces, d_base, d_cnt, d_ofs, d_out, d_stk, d_vn})
       if (p) (void)hipFree(p);
   };
   auto H = [&](hipError_t e, const char *what) {
@@ -698,11 +754,17 @@ int make_patches_device(const vr_face *faces, int64_t n_faces, float max_chop, f
     release();
     return rc == VR_E_HIP ? VR_E_NOMEM : rc;
   }
+  const size_t vnb = sizeof(float) * kMaxPts * 3 * (size_t)n_faces;
+  if (vnorm && ((rc = H(hipMalloc(&d_vn, vnb), "hipMalloc")) ||
+                (rc = H(hipMemcpyAsync(d_vn, vnorm, vnb, hipMemcpyHostToDevice, st), "upload")))) {
+    release();
+    return rc;
+  }
   const dim3 g((unsigned)((n_faces + 63) / 64));
   hipLaunchKernelGGL(k_make_patches<false>, g, dim3(64), 0, st, (const vr_face *)d_faces, n_faces,
                      max_chop, min_chop, (int32_t *)d_base, (int64_t *)d_cnt,
                      (const int64_t *)nullptr, (const int64_t *)nullptr, (vr_patch *)nullptr,
-                     (Pending *)d_stk);
+                     (Pending *)d_stk, (const float *)d_vn);
   std::vector<int32_t> hb((size_t)n_faces);
   std::vector<int64_t> hc((size_t)n_faces);
   if ((rc = H(hipGetLastError(), "count launch")) ||
@@ -747,7 +809,7 @@ int make_patches_device(const vr_face *faces, int64_t n_faces, float max_chop, f
   hipLaunchKernelGGL(k_make_patches<true>, g, dim3(64), 0, st, (const vr_face *)d_faces, n_faces,
                      max_chop, min_chop, (int32_t *)nullptr, (int64_t *)nullptr,
                      (const int64_t *)d_ofs, (const int64_t *)d_ofs + n_faces, (vr_patch *)d_out,
-                     (Pending *)d_stk);
+                     (Pending *)d_stk, (const float *)d_vn);
   if ((rc = H(hipGetLastError(), "fill launch")) ||
       (rc = H(hipMemcpyAsync(out, d_out, sizeof(vr_patch) * (size_t)total, hipMemcpyDeviceToHost, st),
               "download")) ||

@@ -40,6 +40,10 @@ struct KdTree {
 // n_threads host threads and stitched in the sequential allocation order.
 int build_kdtree(const float *v9, int64_t n, int n_threads, KdTree *out);
 
+// PairEdges' per-vertex normals (rad/lightmap/lightmap.go:37-216) of n faces:
+// out holds VR_FACE_MAX_POINTS x 3 floats per face.
+void pair_edges(const vr_face *faces, int64_t n, float threshold, float *out);
+
 // ---- procedural scenes -----------------------------------------------------
 struct GenScene {
   std::vector<float> v9;

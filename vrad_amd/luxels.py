@@ -66,3 +66,26 @@ def FinalLightFace(env, faces, patches, patch_total, patch_direct, lights, lux_o
     lx = FaceLuxels(faces, lm, patches, bounced, lux_ofs)
     direct = GatherSampleLight(env, lx["pos"], lx["normal"], lights)
     return lm, (direct + lx["value"]).astype(np.float32), lx
+
+
+def LightingLump(luxel_rgb, lm):
+    """The BSP lighting lump of the luxel light (luxel_rgb (n, 3) in the
+    luxel order of lm, the CalcFaceExtents records): (lump bytes (uint8),
+    lightofs (int32 per face, -1 for a face without luxels)).  Per face: its
+    average colour, then its luxels, as ColorRGBExp32."""
+    v = np.ascontiguousarray(luxel_rgb, np.float32).reshape(-1, 3)
+    L = np.ascontiguousarray(lm, _lib.FACE_LIGHTMAP_DTYPE)
+    nb = C.c_int64()
+    ofs = np.zeros(len(L), np.int32)
+    check(lib.vr_lighting_lump(ptr(v), len(v), ptr(L), len(L), None, 0, ptr(ofs), C.byref(nb)))
+    lump = np.zeros(nb.value, np.uint8)
+    check(lib.vr_lighting_lump(ptr(v), len(v), ptr(L), len(L), ptr(lump), len(lump), ptr(ofs),
+                               C.byref(nb)))
+    return lump, ofs
+
+
+def decode_rgbexp32(lump):
+    """Colours of ColorRGBExp32 entries (4 bytes each): bytes x 2^exponent."""
+    b = np.asarray(lump, np.uint8).reshape(-1, 4)
+    e = b[:, 3].view(np.int8).astype(np.int32)
+    return (b[:, :3].astype(np.float64) * np.ldexp(1.0, e)[:, None]).astype(np.float32)

@@ -144,18 +144,42 @@ MAX_CHOP = 4.0  # maxChop, rad/patches/face.go:24 (luxel widths)
 MIN_CHOP = 4.0  # minChop, rad/patches/face.go:25
 
 
-def MakePatches(faces, max_chop=MAX_CHOP, min_chop=MIN_CHOP):
+SMOOTHING_THRESHOLD = np.float32(0.7071067)  # smoothingThreshold, rad/lightmap/lightmap.go:29
+
+
+def PairEdges(faces, smoothing_threshold=SMOOTHING_THRESHOLD):
+    """PairEdges (rad/lightmap/lightmap.go:37-228): per-vertex normals of the
+    faces (_lib.FACE_DTYPE), n_faces x VR_FACE_MAX_POINTS x 3 float32 -- the
+    normalised sum of the face's own normal and those of faces sharing the
+    vertex within the smoothing threshold."""
+    f = np.ascontiguousarray(faces, _lib.FACE_DTYPE)
+    out = np.zeros((len(f), _lib.VR_FACE_MAX_POINTS, 3), np.float32)
+    check(lib.vr_pair_edges(ptr(f), len(f), float(smoothing_threshold), ptr(out)))
+    return out
+
+
+def MakePatches(faces, max_chop=MAX_CHOP, min_chop=MIN_CHOP, vertex_normals=None):
     """MakePatches / MakePatchForFace (rad/patches/build.go:21-68,
     rad/patches/face.go:30-197) and SubdividePatches (rad/patches/subdivide.go
     :22-438) on the GPU: faces is an array of _lib.FACE_DTYPE; returns the
     patches (_lib.PATCH_DTYPE) in the reference's array order: a base patch
     per face of positive area, then each base patch's children in creation
-    order.  Leaves (child1 == -1) are the patches the bake uses."""
+    order.  Leaves (child1 == -1) are the patches the bake uses.
+
+    vertex_normals (PairEdges' output) gives each child the GetPhongNormal
+    normal at its centre (rad/lightmap/normallist.go:51-144, called at
+    subdivide.go:387); without it children keep the face normal."""
     f = np.ascontiguousarray(faces, _lib.FACE_DTYPE)
+    vn = None
+    if vertex_normals is not None:
+        vn = np.ascontiguousarray(vertex_normals, np.float32)
+        if vn.shape != (len(f), _lib.VR_FACE_MAX_POINTS, 3):
+            raise ValueError(f"vertex_normals: shape {vn.shape}, want "
+                             f"({len(f)}, {_lib.VR_FACE_MAX_POINTS}, 3)")
     n = C.c_int64()
-    check(lib.vr_make_patches(ptr(f), len(f), float(max_chop), float(min_chop), None, 0,
-                              C.byref(n)))
+    check(lib.vr_make_patches_phong(ptr(f), len(f), float(max_chop), float(min_chop), ptr(vn),
+                                    None, 0, C.byref(n)))
     out = np.zeros(n.value, _lib.PATCH_DTYPE)
-    check(lib.vr_make_patches(ptr(f), len(f), float(max_chop), float(min_chop), ptr(out),
-                              len(out), C.byref(n)))
+    check(lib.vr_make_patches_phong(ptr(f), len(f), float(max_chop), float(min_chop), ptr(vn),
+                                    ptr(out), len(out), C.byref(n)))
     return out
