@@ -48,6 +48,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
 LEVEL = dict(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
+# the precision form's scene: ~1.5 M triangles, ~250k patches
+C3_LEVEL = dict(seed=3, rooms_x=16, rooms_y=16, chop=72.0, tess=40.0, n_lights=16, sun=True)
 
 
 def progress(msg):
@@ -211,6 +213,77 @@ def kd_build_compare(s, env):
     return out
 
 
+class Batch:
+    """A TestLine batch resident on the device (vr_test_line_dev)."""
+
+    def __init__(self, cx, env, a, b):
+        self.cx, self.env = cx, env
+        torch, L = cx.torch, cx.L
+        self.cols = [torch.from_numpy(np.ascontiguousarray(x[:, k])).to(cx.dev)
+                     for x in (a, b) for k in range(3)]
+        self.n = len(a)
+        self.vis = torch.empty(self.n, dtype=torch.uint8, device=cx.dev)
+        self.sky = torch.empty(self.n, dtype=torch.uint8, device=cx.dev)
+        self.segs = L.Segs(*[L.C.cast(c.data_ptr(), L.FP) for c in self.cols])
+
+    def step(self):
+        cx, L = self.cx, self.cx.L
+        cx.check(cx.lib.vr_test_line_dev(self.env.handle, L.C.byref(self.segs), self.n, -1,
+                                         L.C.c_void_p(self.vis.data_ptr()),
+                                         L.C.c_void_p(self.sky.data_ptr()), cx.sptr))
+
+    def run(self, steps, warmup):
+        cx, torch, check, lib = self.cx, self.cx.torch, self.cx.check, self.cx.lib
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        check(lib.vr_profile_reset())
+        check(lib.vr_profile_enable(1))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        check(lib.vr_profile_enable(0))
+        return el, cx.kernel_ms(b"test_line")
+
+
+class RayBatch:
+    """A closest-hit batch resident on the device (vr_trace_closest_dev)."""
+
+    def __init__(self, cx, env, o, d):
+        self.cx, self.env = cx, env
+        torch, L = cx.torch, cx.L
+        cols = [o[:, k] for k in range(3)] + [d[:, k] for k in range(3)]
+        cols += [np.zeros(len(o), np.float32),
+                 np.full(len(o), 1.732050807569 * 32768.0, np.float32)]
+        self.cols = [torch.from_numpy(np.ascontiguousarray(c)).to(cx.dev) for c in cols]
+        self.n = len(o)
+        self.hit = torch.empty(self.n, dtype=torch.int32, device=cx.dev)
+        self.t = torch.empty(self.n, dtype=torch.float32, device=cx.dev)
+        self.rays = L.Rays(*[L.C.cast(c.data_ptr(), L.FP) for c in self.cols])
+
+    def step(self):
+        cx, L = self.cx, self.cx.L
+        cx.check(cx.lib.vr_trace_closest_dev(self.env.handle, L.C.byref(self.rays), self.n,
+                                             -1, L.C.c_void_p(self.hit.data_ptr()),
+                                             L.C.c_void_p(self.t.data_ptr()), None, cx.sptr))
+
+    def run(self, steps, warmup):
+        cx, torch, check, lib = self.cx, self.cx.torch, self.cx.check, self.cx.lib
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        check(lib.vr_profile_reset())
+        check(lib.vr_profile_enable(1))
+        for _ in range(steps):
+            self.step()
+        torch.cuda.synchronize()
+        check(lib.vr_profile_enable(0))
+        return None, cx.kernel_ms(b"trace_closest")
+
+
 def run_visibility(cx, args):
     progress("visibility")
     from vrad_amd import scenes
@@ -227,64 +300,7 @@ def run_visibility(cx, args):
     st, en = row_batch(s, args.segments, i0=len(s.p_area) // 2)
     n = len(st)
 
-    class Batch:
-        def __init__(self, a, b):
-            self.cols = [torch.from_numpy(np.ascontiguousarray(x[:, k])).to(cx.dev)
-                         for x in (a, b) for k in range(3)]
-            self.n = len(a)
-            self.vis = torch.empty(self.n, dtype=torch.uint8, device=cx.dev)
-            self.sky = torch.empty(self.n, dtype=torch.uint8, device=cx.dev)
-            self.segs = L.Segs(*[L.C.cast(c.data_ptr(), L.FP) for c in self.cols])
-
-        def step(self):
-            check(lib.vr_test_line_dev(env.handle, L.C.byref(self.segs), self.n, -1,
-                                       L.C.c_void_p(self.vis.data_ptr()),
-                                       L.C.c_void_p(self.sky.data_ptr()), cx.sptr))
-
-        def run(self, steps, warmup):
-            for _ in range(warmup):
-                self.step()
-            torch.cuda.synchronize()
-            check(lib.vr_profile_reset())
-            check(lib.vr_profile_enable(1))
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            for _ in range(steps):
-                self.step()
-            torch.cuda.synchronize()
-            el = time.perf_counter() - t
-            check(lib.vr_profile_enable(0))
-            return el, cx.kernel_ms(b"test_line")
-
-    class RayBatch:
-        def __init__(self, o, d):
-            cols = [o[:, k] for k in range(3)] + [d[:, k] for k in range(3)]
-            cols += [np.zeros(len(o), np.float32),
-                     np.full(len(o), 1.732050807569 * 32768.0, np.float32)]
-            self.cols = [torch.from_numpy(np.ascontiguousarray(c)).to(cx.dev) for c in cols]
-            self.n = len(o)
-            self.hit = torch.empty(self.n, dtype=torch.int32, device=cx.dev)
-            self.t = torch.empty(self.n, dtype=torch.float32, device=cx.dev)
-            self.rays = L.Rays(*[L.C.cast(c.data_ptr(), L.FP) for c in self.cols])
-
-        def step(self):
-            check(lib.vr_trace_closest_dev(env.handle, L.C.byref(self.rays), self.n, -1,
-                                           L.C.c_void_p(self.hit.data_ptr()),
-                                           L.C.c_void_p(self.t.data_ptr()), None, cx.sptr))
-
-        def run(self, steps, warmup):
-            for _ in range(warmup):
-                self.step()
-            torch.cuda.synchronize()
-            check(lib.vr_profile_reset())
-            check(lib.vr_profile_enable(1))
-            for _ in range(steps):
-                self.step()
-            torch.cuda.synchronize()
-            check(lib.vr_profile_enable(0))
-            return None, cx.kernel_ms(b"trace_closest")
-
-    batch = Batch(st, en)
+    batch = Batch(cx, env, st, en)
     # autotune traversal-stack entries in LDS x rays per wavefront on this batch
     from vrad_amd import autotune
     rec = autotune.Autotuner(reps=3).tune_visibility(batch.step, key=f"level:{n}")
@@ -302,7 +318,7 @@ def run_visibility(cx, args):
     # the visible-heavy batch: PVS-culled MakeTransfers rows of the same level
     progress("visibility: PVS-culled (visible-heavy) batch")
     pst, pen = pvs_pair_batch(s, args.segments, first_room=27)
-    pb = Batch(pst, pen)
+    pb = Batch(cx, env, pst, pen)
     prec = autotune.Autotuner(reps=3).tune_visibility(
         pb.step, key=f"level-pvs:{pb.n}", stacks=(4, 6, 8, 10, 12), rays=(1,))
     _, pvs_ms = pb.run(args.steps, args.warmup)
@@ -318,7 +334,7 @@ def run_visibility(cx, args):
     # closest hit (Trace4Rays' work at batch scale)
     progress("visibility: closest-hit rays")
     ro, rd = hemisphere_rays(s, args.segments, seed=21)
-    rb = RayBatch(ro, rd)
+    rb = RayBatch(cx, env, ro, rd)
     crec = autotune.Autotuner(reps=3).tune_visibility(
         rb.step, key=f"level-closest:{rb.n}", stacks=(4, 6, 8, 10, 12), rays=(1,),
         family="trace_closest")
@@ -332,12 +348,58 @@ def run_visibility(cx, args):
     del rb
     autotune.Autotuner().apply(best_s)
     if args.random_pairs:
-        rb = Batch(*random_batch(s, args.segments, seed=20))
+        rb = Batch(cx, env, *random_batch(s, args.segments, seed=20))
         _, rand_ms = rb.run(3, 1)
         del rb
         extra["random_pairs_Mrays"] = round(n / (rand_ms * 1e-3) / 1e6, 1)
     return dict(s=s, st=st, en=en, n=n, vis=vis, elapsed=elapsed, kern_ms=kern_ms,
                 info=info, build_s=build_s, extra=extra)
+
+
+def run_c3_scene(cx, args):
+    """Visibility on the precision form's scene at its stated size: the level
+    generator at ~1.55 M triangles / ~264k patches.  Scene build with the tree
+    on the GPU and on the host (same tree), then a 4 M-segment MakeTransfers
+    batch in row order and 4 M closest-hit rays, each at its autotuned shape."""
+    progress("C3 scene: build")
+    from vrad_amd import autotune, scenes
+    from vrad_amd.raytracer import Environment
+    s = scenes.level(**C3_LEVEL)
+    env = Environment()
+    env.AddTriangles(s.verts9, s.ids)
+    t0 = time.perf_counter()
+    env.SetupAccelerationStructure(n_threads=16)
+    build_ms = (time.perf_counter() - t0) * 1e3
+    info = env.info()
+    kd = kd_build_compare(s, env)
+    out = {"generator": dict(C3_LEVEL), "triangles": int(info.n_tri),
+           "patches": int(len(s.p_area)), "kd_nodes": int(info.n_nodes),
+           "kd_leaf_refs": int(info.n_leaf_refs), "scene_build_ms": round(build_ms, 1),
+           "kd_build": kd}
+    progress("C3 scene: TestLine batch")
+    st, en = row_batch(s, args.segments, i0=len(s.p_area) // 2)
+    b = Batch(cx, env, st, en)
+    rec = autotune.Autotuner(reps=3).tune_visibility(b.step, key=f"c3:{b.n}",
+                                                     stacks=(6, 8, 10, 12), rays=(1,))
+    _, ms = b.run(args.steps, args.warmup)
+    out["test_line"] = {"segments": b.n, "Mrays": round(b.n / (ms * 1e-3) / 1e6, 1),
+                        "kernel_ms": round(ms, 4), "tuned": rec["best"],
+                        "visible_fraction": round(float(b.vis.float().mean().item()), 4),
+                        "note": "MakeTransfers rows from the middle patch on, facing cull"}
+    del b
+    progress("C3 scene: closest-hit rays")
+    ro, rd = hemisphere_rays(s, args.segments, seed=33)
+    rb = RayBatch(cx, env, ro, rd)
+    rec = autotune.Autotuner(reps=3).tune_visibility(rb.step, key=f"c3-closest:{rb.n}",
+                                                     stacks=(6, 8, 10, 12), rays=(1,),
+                                                     family="trace_closest")
+    _, ms = rb.run(args.steps, args.warmup)
+    out["closest_hit"] = {"rays": rb.n, "Mrays": round(rb.n / (ms * 1e-3) / 1e6, 1),
+                          "kernel_ms": round(ms, 4), "tuned": rec["best"],
+                          "hit_fraction": round(float((rb.hit >= 0).float().mean().item()), 4)}
+    del rb
+    env.close()
+    return out
 
 
 def run_cpu(v, args):
@@ -662,30 +724,44 @@ def run_level_bake(cx, args):
 
 def real_csr_bounce(cx, xf, s):
     """BounceLight GB/s on a transfer set MakeTransfers built (clustered
-    columns, rows of every length), FP32 and FP16: 8 bounces per call, the
-    median of 3 calls after a warm-up call that builds the tile layout."""
-    from vrad_amd import lightmap
+    columns, rows of every length), FP32 and FP16, with both row placements:
+    8 bounces per call, the median of 3 calls after a warm-up call that
+    builds the tile layout."""
+    from vrad_amd import autotune, lightmap
     lib, check, L = cx.lib, cx.check, cx.L
     inf = xf.info()
     N, T = int(inf.n_rows), int(inf.stored_nnz)
     rng = np.random.default_rng(9)
     e0 = rng.random((N, 3)).astype(np.float32)
     out = {"rows": N, "transfers": int(inf.nnz), "stored_entries": T}
-    for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0), ("fp16", L.VR_COEF_F16, 2, 4)):
-        xf.set_precision(prec)
-        lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=1, stop_frac=0.0)
-        ms = []
-        for _ in range(3):
-            check(lib.vr_profile_reset())
-            check(lib.vr_profile_enable(1))
-            lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=8, stop_frac=0.0)
-            check(lib.vr_profile_enable(0))
-            ms.append(cx.kernel_ms(b"bounce_gather"))
-        per = float(np.median(ms))
-        B = T * (2 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
-        out[name] = {"ms_per_bounce": round(per, 4), "GBps": round(B / (per * 1e-3) / 1e9, 1),
-                     "frac": round(B / (per * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                     "bytes_per_bounce": B}
+    old = autotune.get_knob("bounce_placement")
+    try:
+        for placement, pname in ((0, "strided"), (1, "row_runs"), (2, "strided_by_index")):
+            autotune.set_knob("bounce_placement", placement)
+            for name, prec, cb, sb in (("fp32", L.VR_COEF_F32, 4, 0),
+                                       ("fp16", L.VR_COEF_F16, 2, 4)):
+                xf.set_precision(prec)
+                lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=1, stop_frac=0.0)
+                ms = []
+                for _ in range(3):
+                    check(lib.vr_profile_reset())
+                    check(lib.vr_profile_enable(1))
+                    lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=8,
+                                         stop_frac=0.0)
+                    check(lib.vr_profile_enable(0))
+                    ms.append(cx.kernel_ms(b"bounce_gather"))
+                per = float(np.median(ms))
+                B = T * (2 + cb) + N * (8 + 12 + 12 + 12 + 24 + sb)
+                out.setdefault(pname, {})[name] = {
+                    "ms_per_bounce": round(per, 4), "GBps": round(B / (per * 1e-3) / 1e9, 1),
+                    "frac": round(B / (per * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                    "bytes_per_bounce": B}
+    finally:
+        autotune.set_knob("bounce_placement", old)
+        xf.set_precision(L.VR_COEF_F32)
+    out["note"] = ("strided: each gather chunk takes a stride through all rows longest-first; "
+                   "row_runs: a run of consecutive rows (bounce_placement = 1); "
+                   "strided_by_index: strided chunks, rows dealt to waves by index (2)")
     return out
 
 
@@ -1151,6 +1227,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
     ap.add_argument("--no-pvs", action="store_true", help="skip the PVS-culled MakeTransfers")
     ap.add_argument("--no-luxels", action="store_true", help="skip the luxel stage")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 1.5M-triangle scene")
     ap.add_argument("--no-level", action="store_true",
                     help="skip the 8x8-room level bake end to end")
     ap.add_argument("--batch-scenes", type=int, default=64)
@@ -1177,6 +1254,7 @@ def main():
     cx = Ctx()
     v = None if args.no_trace else run_visibility(cx, args)
     cpu = None if (v is None or args.no_cpu) else run_cpu(v, args)
+    c3 = None if (args.no_trace or args.no_c3) else run_c3_scene(cx, args)
     bounce = None if args.no_bounce else run_bounce(cx, args)
     room = None if args.no_room else run_room_bake(cx, args)
     batch = None if args.no_batch else run_batch_bake(cx, args)
@@ -1202,7 +1280,7 @@ def main():
                                "4M MakeTransfers TestLine segments per batch (row order)"},
         "roofline": None, "cpu_baseline": cpu, "visibility": None, "bounce": bounce,
         "room_bake": room, "batch_bake": batch, "serving": serving, "pvs_culling": pvs_out,
-        "level_bake": level,
+        "level_bake": level, "c3_scene": c3,
         "luxel_stage": lux,
     }
     if v is not None:

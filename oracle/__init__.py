@@ -276,7 +276,8 @@ def can_leaf_trace_to_sky(kd, tris, mins, maxs, threads=1):
     return out
 
 
-ORDERS = {"wave": 0, "sequential": 1, "tiled": 2, "stream": 3}
+ORDERS = {"wave": 0, "sequential": 1, "tiled": 2, "stream": 3, "stream-runs": 4,
+          "stream-by-index": 5}
 
 
 def bounce_f32(rp, col, coef, emit0, refl, area, total=None, max_bounces=100, stop_frac=1e-3,

@@ -1240,7 +1240,8 @@ static void bg_range(void *vc, int64_t b, int64_t e) {
 }
 
 /* Stream order, one chunk of rows per call (chunks are independent).
- * Chunking: n_chunks = max(ceil(n / 1024), min(256, ceil(n / 256))); chunk c
+ * Chunking: n_chunks = min(256, ceil(n / 256)) for n <= 256 x 1024, else
+ * 256 ceil(n / (256 x 1024)) (at most 1024 rows a chunk either way); chunk c
  * holds rows rows_by_len[c + r n_chunks]; wave w (0..15), slot j (0..63)
  * takes rank r = 16 j + (j odd ? 15 - w : w).  In tile t a slot's segment
  * is its row's entries with columns in [t C, (t + 1) C), padded with zero
@@ -1391,6 +1392,11 @@ static void bg_stream_range(void *vc, int64_t cb, int64_t ce) {
   }
 }
 
+static int cmp_i32(const void *a, const void *b) {
+  const int32_t i = *(const int32_t *)a, j = *(const int32_t *)b;
+  return i < j ? -1 : (i > j);
+}
+
 static int cmp_len_desc(const void *a, const void *b, void *rp) {
   const int64_t *r = (const int64_t *)rp;
   const int32_t i = *(const int32_t *)a, j = *(const int32_t *)b;
@@ -1411,12 +1417,45 @@ int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
   bg_ctx_t c = {n, row_ptr, col, coef, coef_kind, order, tile_cols, group, row_scale, emit, add,
                 NULL, 0};
   int32_t *by_len = NULL;
+  const int runs = order == 4;   /* stream order with rows dealt in runs */
+  const int by_index = order == 5; /* stream order, chunks' rows ranked by index */
+  if (runs || by_index) c.order = order = 3;
   if (order == 3 && n > 0) {
     by_len = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
-    for (int64_t i = 0; i < n; ++i) by_len[i] = (int32_t)i;
-    qsort_r(by_len, (size_t)n, sizeof(int32_t), cmp_len_desc, (void *)row_ptr);
-    const int64_t a = (n + 1023) / 1024, b = (n + 255) / 256;
-    c.n_chunks = a > (b < 256 ? b : 256) ? a : (b < 256 ? b : 256);
+    /* at most 1024 rows a chunk: min(256, ceil(n / 256)) chunks up to 256
+       full chunks, beyond that a whole multiple of 256 */
+    const int64_t b = (n + 255) / 256;
+    c.n_chunks = n > 256 * 1024 ? 256 * ((n + 256 * 1024 - 1) / (256 * 1024)) : (b < 256 ? b : 256);
+    if (!runs) {
+      /* every row longest first; chunk c takes by_len[c + k n_chunks] */
+      for (int64_t i = 0; i < n; ++i) by_len[i] = (int32_t)i;
+      qsort_r(by_len, (size_t)n, sizeof(int32_t), cmp_len_desc, (void *)row_ptr);
+      if (by_index) {
+        /* chunk c keeps its rows, ranked again by row index */
+        int32_t *mine = (int32_t *)malloc(sizeof(int32_t) * 1024);
+        for (int64_t ck = 0; ck < c.n_chunks; ++ck) {
+          int64_t m = 0;
+          for (int64_t k = ck; k < n; k += c.n_chunks) mine[m++] = by_len[k];
+          qsort(mine, (size_t)m, sizeof(int32_t), cmp_i32);
+          for (int64_t k = 0; k < m; ++k) by_len[ck + k * c.n_chunks] = mine[k];
+        }
+        free(mine);
+      }
+    } else {
+      /* chunk c's rows are the c-th run of consecutive rows, runs of
+         ceil((n - c) / n_chunks) rows, longest first inside the run, and
+         its k-th row sits at by_len[c + k n_chunks] as above */
+      int32_t *run = (int32_t *)malloc(sizeof(int32_t) * 1024);
+      int64_t first = 0;
+      for (int64_t ck = 0; ck < c.n_chunks && first < n; ++ck) {
+        const int64_t size = (n - ck + c.n_chunks - 1) / c.n_chunks;
+        for (int64_t k = 0; k < size; ++k) run[k] = (int32_t)(first + k);
+        qsort_r(run, (size_t)size, sizeof(int32_t), cmp_len_desc, (void *)row_ptr);
+        for (int64_t k = 0; k < size; ++k) by_len[ck + k * c.n_chunks] = run[k];
+        first += size;
+      }
+      free(run);
+    }
     c.rows_by_len = by_len;
   }
   double cum = 0.0;

@@ -135,7 +135,9 @@ void or_can_leaf_trace_to_sky(const or_kdtree_t *kd, const or_tri_t *tris, int64
 /* ---- BounceLight + FP64 energy ---- */
 /* FP32 in the exact order of a bounce kernel (coef_kind 0=f32, 1=f16, 2=bf16
  * with per-row scale; order 0 = wave per row, 1 = sequential, 2 = LDS-tiled
- * with tile_cols / group); energy reduced in the kernels' fixed tree order.
+ * with tile_cols / group, 3 = stream, 4 = stream with rows dealt to chunks in
+ * runs of consecutive rows, 5 = stream with each chunk's rows ranked by row
+ * index); energy reduced in the kernels' fixed tree order.
  * Order 2 needs each row's columns in ascending order. */
 int or_bounce_f32(int64_t n, const int64_t *row_ptr, const int32_t *col,
                   const void *coef, int coef_kind, const float *row_scale,

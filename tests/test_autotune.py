@@ -58,3 +58,29 @@ def test_bounce_sweep_keeps_the_summation_order():
     rec = tu.tune_bounce(run=None, key="250k", keep_order=False, measure=measure)
     assert len(tried) == len(autotune.BOUNCE_SHAPES)
     assert {t[0] for t in tried} == {0, 1, 2}
+
+
+def test_bounce_sweep_of_row_placements():
+    kn = Knobs()
+    tu = autotune.Autotuner(setter=kn.set)
+    seen = []
+
+    def measure(k):
+        seen.append((k["bounce_kernel"], k["bounce_placement"]))
+        return 1.0 + k["bounce_placement"] * (-0.1 if k["bounce_kernel"] == 0 else 0.1)
+
+    # keeping the order: the stream gather stays at the current placement,
+    # the tiled gather (placement-independent order) tries every placement
+    rec = tu.tune_bounce(run=None, key="p", measure=measure, current=(2, 4096, 0),
+                         placements=(0, 1, 2))
+    assert seen and {p for k, p in seen} == {0}
+    assert rec["best"]["bounce_placement"] == 0
+    seen.clear()
+    rec = tu.tune_bounce(run=None, key="p-tiled", measure=measure, current=(0, 4096, 0),
+                         placements=(0, 1, 2))
+    assert {p for k, p in seen} == {0, 1, 2} and all(k == 0 for k, _ in seen)
+    assert rec["best"]["bounce_placement"] == 2
+    seen.clear()
+    rec = tu.tune_bounce(run=None, key="q", keep_order=False, measure=measure,
+                         placements=(0, 1, 2))
+    assert len(seen) == 3 * len(autotune.BOUNCE_SHAPES)

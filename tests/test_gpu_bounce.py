@@ -191,3 +191,87 @@ def test_synthetic_csr_bitexact():
                                       stop_frac=0.0, threads=8)
     assert k == ok == 8 and np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
     assert np.array_equal(en, oen)
+
+
+PLACEMENT_ORDER = {1: "stream-runs", 2: "stream-by-index"}
+
+
+@pytest.mark.parametrize("placement", [1, 2], ids=["row-runs", "by-index"])
+@pytest.mark.parametrize("cfg", [(2, 4096, 0, 0, 1), (2, 2048, 0, 0, 1), (0, 4096, 16, 1, 3)],
+                         ids=["stream4096", "stream2048", "tiled4096x16"])
+def test_row_placement_bitexact(cfg, placement):
+    """bounce_placement = 1 deals each gather chunk a run of consecutive rows
+    (longest first inside the run), 2 ranks each chunk's rows of placement 0
+    by row index: the stream gather then sums in the order the oracle
+    restates ("stream-runs", "stream-by-index"), the tiled gather's order
+    does not change, and switching the knob back relays the rows out again."""
+    P = placement
+    from vrad_amd import scenes
+    s = scenes.level(seed=5, rooms_x=3, rooms_y=2, chop=64.0, tess=64.0, n_lights=4)
+    env = Environment()
+    env.AddTriangles(s.verts9, s.ids)
+    env.SetupAccelerationStructure()
+    xf = MakeTransfers(env, s.p_origin, s.p_normal, s.p_area)
+    rp, col, coef = xf.download()
+    n = len(s.p_area)
+    rng = np.random.default_rng(8)
+    e0 = rng.random((n, 3)).astype(np.float32)
+    print(f"[gpu] row runs: {n} rows, {len(col)} transfers, kernel {cfg}", flush=True)
+    _set_kernel(*cfg)
+    try:
+        out = {}
+        for placement in (P, 0, P):
+            _lib.check(_lib.lib.vr_set_tuning(b"bounce_placement", placement))
+            tot, en, k = lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=4,
+                                              stop_frac=0.0)
+            assert k == 4
+            if placement in out:
+                assert np.array_equal(tot.view(np.uint32), out[placement][0].view(np.uint32))
+            out[placement] = (tot, en)
+    finally:
+        _lib.check(_lib.lib.vr_set_tuning(b"bounce_placement", 0))
+        _set_kernel(*DEFAULT_KNOBS)
+    kw = _oracle_kw(*cfg)
+    if cfg[0] == 2:
+        kw["order"] = PLACEMENT_ORDER[P]
+    otot, oen, ok = oracle.bounce_f32(rp, col, coef, e0, s.p_refl, s.p_area, max_bounces=4,
+                                      stop_frac=0.0, threads=8, **kw)
+    assert np.array_equal(out[P][0].view(np.uint32), otot.view(np.uint32))
+    assert np.array_equal(out[P][1], oen)
+    if cfg[0] == 0:   # the tiled order does not depend on the placement
+        assert np.array_equal(out[0][0].view(np.uint32), out[P][0].view(np.uint32))
+    else:             # and the placed order differs from the strided one somewhere
+        assert not np.array_equal(out[0][0].view(np.uint32), out[P][0].view(np.uint32))
+
+
+@pytest.mark.parametrize("placement", [0, 1, 2], ids=["strided", "row-runs", "by-index"])
+def test_stream_order_beyond_256_full_chunks(placement):
+    """More rows than 256 chunks of 1024 hold: the gather runs 512 chunks (two
+    full rounds of one workgroup per CU) and still sums in the order the
+    oracle restates, FP32 and FP16, with either row placement."""
+    n = 300_000
+    xf = Transfers.synthetic(seed=21, n_rows=n, mean_len=40.0, sigma=1.0, max_len=2000)
+    rp, col, coef = xf.download()
+    rng = np.random.default_rng(6)
+    e0 = rng.random((n, 3)).astype(np.float32)
+    refl = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+    area = rng.uniform(1, 4, n).astype(np.float32)
+    print(f"[gpu] {n} rows, {len(col)} transfers, placement {placement}", flush=True)
+    _lib.check(_lib.lib.vr_set_tuning(b"bounce_placement", placement))
+    try:
+        tot, en, k = lightmap.BounceLight(xf, e0, refl, area, max_bounces=3, stop_frac=0.0)
+        xf.set_precision(_lib.VR_COEF_F16)
+        tot16, en16, _ = lightmap.BounceLight(xf, e0, refl, area, max_bounces=3, stop_frac=0.0)
+        _, _, _, raw, sc = xf.download(raw16=True)
+    finally:
+        _lib.check(_lib.lib.vr_set_tuning(b"bounce_placement", 0))
+    order = PLACEMENT_ORDER.get(placement, "stream")
+    otot, oen, ok = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=3,
+                                      stop_frac=0.0, threads=16, order=order)
+    assert k == ok == 3 and np.array_equal(tot.view(np.uint32), otot.view(np.uint32))
+    assert np.array_equal(en, oen)
+    o16, oen16, _ = oracle.bounce_f32(rp, col, raw, e0, refl, area, max_bounces=3,
+                                      stop_frac=0.0, kind=_lib.VR_COEF_F16, row_scale=sc,
+                                      threads=16, order=order)
+    assert np.array_equal(tot16.view(np.uint32), o16.view(np.uint32))
+    assert np.array_equal(en16, oen16)

@@ -165,6 +165,13 @@ def test_served_transfers_refuse_relayout(scenes16):
             assert ei.value.code == _lib.VR_E_STATE
         finally:
             _lib.check(lib.vr_set_tuning(b"bounce_tile_cols", 4096))
+        _lib.check(lib.vr_set_tuning(b"bounce_placement", 1))   # deals the rows again
+        try:
+            with pytest.raises(_lib.VrError) as ei:
+                lightmap.BounceLight(b.xf, e0, s.p_refl, s.p_area, max_bounces=2, stop_frac=0.0)
+            assert ei.value.code == _lib.VR_E_STATE
+        finally:
+            _lib.check(lib.vr_set_tuning(b"bounce_placement", 0))
         moves = _requests([b], 1, seed=9)[0][1]
         tot, en, _ = sv.wait(sv.submit(0, moves))
         ot, oe, _ = _expected(b, moves, _lib.VR_COEF_F16)

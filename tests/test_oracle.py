@@ -429,13 +429,16 @@ def test_bounce_orders_agree():
     outs = []
     for kw in (dict(order="wave"), dict(order="sequential"), dict(order="tiled"),
                dict(order="tiled", tile_cols=512, group=16), dict(order="stream"),
-               dict(order="stream", tile_cols=512)):
+               dict(order="stream", tile_cols=512), dict(order="stream-runs"),
+               dict(order="stream-by-index")):
         t, _, k = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=5, stop_frac=0.0,
                                     **kw)
         assert k == 5 and np.max(np.abs(t - ref) / ref) < 2e-6
         outs.append(t)
     assert not np.array_equal(outs[0], outs[1])      # the orders really differ
     assert not np.array_equal(outs[2], outs[4])
+    assert not np.array_equal(outs[4], outs[6])      # the row placement changes the order
+    assert not np.array_equal(outs[4], outs[7])
 
 
 def _dyadic_rows(rng, n, lens):
@@ -448,8 +451,9 @@ def _dyadic_rows(rng, n, lens):
     return rp, col, coef
 
 
+@pytest.mark.parametrize("order", ["stream", "stream-runs", "stream-by-index"])
 @pytest.mark.parametrize("tile", [512, 4096])
-def test_stream_order_adds_every_entry_once(tile):
+def test_stream_order_adds_every_entry_once(tile, order):
     # uneven rows (1 .. 2048 entries), several row chunks and column tiles, and
     # segments that cross passes: the stream order must equal the exact sum
     rng = np.random.default_rng(11)
@@ -460,16 +464,34 @@ def test_stream_order_adds_every_entry_once(tile):
     refl = np.full((n, 3), 0.5, np.float32)
     area = np.ones(n, np.float32)
     t, en, k = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=1, stop_frac=0.0,
-                                 order="stream", tile_cols=tile, threads=4)
+                                 order=order, tile_cols=tile, threads=4)
     t64, _, _ = oracle.bounce_f64(rp, col, coef, e0, refl, area, max_bounces=1, stop_frac=0.0)
     assert np.array_equal(t.astype(np.float64), t64)
     # FP16 storage with row scale 1: coefficients 2^-m are exact in FP16 too
     raw = np.array([oracle.f32_to_f16(v) for v in np.unique(coef)], np.uint16)[
         np.searchsorted(np.unique(coef), coef)]
     t16, _, _ = oracle.bounce_f32(rp, col, raw, e0, refl, area, max_bounces=1, stop_frac=0.0,
-                                  order="stream", tile_cols=tile, kind=1,
+                                  order=order, tile_cols=tile, kind=1,
                                   row_scale=np.ones(n, np.float32), threads=4)
     assert np.array_equal(t16.astype(np.float64), t64)
+
+
+@pytest.mark.parametrize("order", ["stream", "stream-runs", "stream-by-index"])
+def test_stream_order_beyond_256_full_chunks(order):
+    # more rows than 256 chunks of 1024 hold: 512 chunks, still every entry once
+    rng = np.random.default_rng(12)
+    n = 270_000
+    lens = 2 ** rng.integers(0, 4, n)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    coef = np.concatenate([np.full(l, 1.0 / l) for l in lens]).astype(np.float32)
+    e0 = (rng.integers(0, 64, (n, 3)) / 64.0).astype(np.float32)
+    refl = np.full((n, 3), 0.5, np.float32)
+    area = np.ones(n, np.float32)
+    t, _, _ = oracle.bounce_f32(rp, col, coef, e0, refl, area, max_bounces=1, stop_frac=0.0,
+                                order=order, threads=8)
+    t64, _, _ = oracle.bounce_f64(rp, col, coef, e0, refl, area, max_bounces=1, stop_frac=0.0)
+    assert np.array_equal(t.astype(np.float64), t64)
 
 
 def test_stream_order_independent_of_threads():

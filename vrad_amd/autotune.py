@@ -9,7 +9,8 @@ keeps the fastest:
               wavefront ("trace_rays_per_lane", 64 x 1/2/4/8 rays per wave)
   bounce      gather tile width, kernel and prefetch depth ("bounce_kernel",
               "bounce_tile_cols", "bounce_group", "bounce_passes",
-              "bounce_prefetch")
+              "bounce_prefetch"), optionally the row placement
+              ("bounce_placement")
 
 Visibility results never depend on these knobs.  For BounceLight the kernel
 and tile width fix the FP32 summation order (each order is reproduced bit for
@@ -143,15 +144,23 @@ class Autotuner:
         return self.sweep("visibility", key, cands, measure, force)
 
     def tune_bounce(self, run, key, keep_order=True, shapes=BOUNCE_SHAPES, force=False,
-                    measure=None, current=None):
+                    measure=None, current=None, placements=None):
         """run() performs the caller's bounces (BounceLight or a batch).  With
         keep_order, only shapes with the current kernel and tile width (the
-        FP32 summation order) are tried."""
+        FP32 summation order) are tried.  placements (e.g. (0, 1, 2)) also
+        sweeps "bounce_placement", the dealing of rows to the gather's chunks
+        (part of the stream gather's order, so keep_order holds it at the
+        current value for that kernel); None leaves it as it is."""
         if keep_order:
             cur = current or (get_knob("bounce_kernel", self.lib),
                               get_knob("bounce_tile_cols", self.lib))
-            shapes = [s for s in shapes if (s[0], s[1]) == tuple(cur)]
+            shapes = [s for s in shapes if (s[0], s[1]) == tuple(cur[:2])]
         cands = [dict(zip(BOUNCE_KNOBS, s)) for s in shapes]
+        if placements is not None:
+            cur_p = current[2] if current and len(current) > 2 else \
+                get_knob("bounce_placement", self.lib)
+            cands = [dict(c, bounce_placement=p) for c in cands for p in placements
+                     if not (keep_order and c["bounce_kernel"] == 2 and p != cur_p)]
         measure = measure or (lambda knobs: timed(run, "bounce_gather", self.reps, self.lib))
         return self.sweep(f"bounce-{'order' if keep_order else 'any'}", key, cands, measure,
                           force)

@@ -729,14 +729,37 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
   __shared__ __attribute__((aligned(16))) float s_e[2][4 * C];  // (r, g, b, -) per patch
   __shared__ __attribute__((aligned(16))) float s_acc[16 * 64 * 4];  // slot sums per wave
   __shared__ int64_t s_bs[kMaxTilesPerJob];  // this chunk's block starts
+  __shared__ int s_tn[kMaxTilesPerJob];      // this chunk's tiles holding entries, in order
+  __shared__ int s_wn[16];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t c = (int64_t)blockIdx.x - jb.chunk_begin;
   const int nt = tl.nt;
-  for (int t = (int)threadIdx.x; t < nt; t += T) s_bs[t] = tl.bstart[c * nt + t];
+  static_assert(kMaxTilesPerJob <= T, "one thread per tile lists the tiles");
+  // Tiles where no row of the chunk has entries add nothing to any sum: the
+  // chunk walks only the others (block totals are boff[block][1024]), in
+  // tile order, so the sums do not change.
+  const int tt = (int)threadIdx.x;
+  const bool tile_live = tt < nt && tl.boff[(c * nt + tt) * (int64_t)RS + 1024] > 0;
+  const uint64_t live_bits = __ballot(tile_live);
+  if (tt < nt) s_bs[tt] = tl.bstart[c * nt + tt];
+  if (lane == 0) s_wn[wave] = __builtin_popcountll(live_bits);
   f32x3 *const acc = reinterpret_cast<f32x3 *>(s_acc) + 64 * wave;
   acc[lane] = f32x3{0.0f, 0.0f, 0.0f};
   __syncthreads();
+  int live_before = 0, n_live = 0;
+  for (int w2 = 0; w2 < 16; ++w2) {
+    live_before += w2 < wave ? s_wn[w2] : 0;
+    n_live += s_wn[w2];
+  }
+  if (tile_live)
+    s_tn[live_before + __builtin_popcountll(live_bits & ((1ull << lane) - 1ull))] = tt;
+  __syncthreads();
+  // k-th tile walked (a chunk with no entries at all walks tile 0, empty)
+  const int ntl = __builtin_amdgcn_readfirstlane(n_live > 0 ? n_live : 1);
+  auto tile_at = [&](int k) {
+    return n_live > 0 ? __builtin_amdgcn_readfirstlane(s_tn[k]) : 0;
+  };
   const int64_t oi = c + slot_rank(wave, lane) * tl.n_chunks;  // this lane's owner slot
   const int64_t row = oi < x.n ? (int64_t)x.order[oi] : -1;
   const char *bidx = reinterpret_cast<const char *>(tl.bidx);
@@ -851,11 +874,11 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
   };
 
   int sa0, sb0, sa1, sb1;
-  stream_bounds(0, sa0, sb0);
-  stream_bounds(min(1, nt - 1), sa1, sb1);
-  StreamMeta m0 = meta(0, sa0, sb0, true);
+  stream_bounds(tile_at(0), sa0, sb0);
+  stream_bounds(tile_at(min(1, ntl - 1)), sa1, sb1);
+  StreamMeta m0 = meta(tile_at(0), sa0, sb0, true);
   PassLoad<KIND> ring[2][D + 1];  // [tile parity][pass % (D + 1)]
-  load_emit(0);
+  load_emit(tile_at(0));
 #pragma unroll
   for (int k = 0; k < D; ++k) issue(k, m0, ring[0][k]);
   store_emit(0);
@@ -863,16 +886,18 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
   // Before a tile's barrier each wave needs its LDS writes of the tile's emit
   // done (and its LDS reads of the previous tile): lgkmcnt 0, vmcnt left open.
   constexpr int WAIT_LGKM0 = 15 | (3 << 14) | (7 << 4);
-  auto tile = [&](auto parc, int t) {
+  // the k-th walked tile
+  auto tile = [&](auto parc, int k) {
     constexpr int p = decltype(parc)::value;
     __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
     __asm__ volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();  // every wave's copy; tile t - 1 is done
+    __builtin_amdgcn_s_barrier();  // every wave's copy; the tile before is done
     __asm__ volatile("" ::: "memory");
-    load_emit(min(t + 1, nt - 1));  // on the last tile a harmless re-load
+    const int tn1 = tile_at(min(k + 1, ntl - 1));
+    load_emit(tn1);  // on the last tile a harmless re-load
     int sa2, sb2;
-    stream_bounds(min(t + 2, nt - 1), sa2, sb2);
-    const StreamMeta m1 = meta(min(t + 1, nt - 1), sa1, sb1, t + 1 < nt);
+    stream_bounds(tile_at(min(k + 2, ntl - 1)), sa2, sb2);
+    const StreamMeta m1 = meta(tn1, sa1, sb1, k + 1 < ntl);
 #pragma unroll
     for (int k = 0; k < D; ++k) issue(k, m1, ring[p ^ 1][k]);
     const char *te = reinterpret_cast<const char *>(s_e[p]);
@@ -896,9 +921,9 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
     sa1 = sa2;
     sb1 = sb2;
   };
-  for (int t = 0; t < nt; t += 2) {
-    tile(IntC<0>{}, t);
-    if (t + 1 < nt) tile(IntC<1>{}, t + 1);
+  for (int k = 0; k < ntl; k += 2) {
+    tile(IntC<0>{}, k);
+    if (k + 1 < ntl) tile(IntC<1>{}, k + 1);
   }
   __builtin_amdgcn_s_waitcnt(0);  // the last (unused) loads
   if (row >= 0) {

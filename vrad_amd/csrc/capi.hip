@@ -214,6 +214,7 @@ struct vr_xfers {
   std::vector<int64_t> rp_host;  // padded starts
   std::vector<int32_t> cnt_host; // logical lengths
   DBuf rp, col, coef32, coef16, scale, order, cnt;
+  int placement = 0;  // the row placement `order` was built for (bounce_placement)
   DBuf tp;            // tile boundaries for the LDS-tiled gather
   int tp_cols = 0;
   // tile-blocked copy of the coefficients for the LDS-tiled gather
@@ -243,21 +244,58 @@ struct vr_gen {
 namespace vr {
 namespace {
 
-// rows longest-first (ties by index), padded starts
-void layout_rows(const std::vector<int32_t> &cnt, std::vector<int64_t> &rp,
-                 std::vector<int32_t> &order) {
+// The gather kernels read chunk c's k-th row as order[c + k n_chunks] (and
+// deal a chunk's k-th row to wave slot_rank^-1(k)).
+// Placement 0: order = every row longest-first (ties by index), so chunk c
+// takes a stride through it.  Placement 1: chunk c's rows are the c-th run of
+// consecutive rows (runs of ceil((n - c) / n_chunks) rows, the chunk sizes of
+// placement 0), longest-first inside the run.  Placement 2: chunk c's rows
+// are placement 0's, ranked by row index, so every wave holds rows from all
+// over the index range (and so, for MakeTransfers rows, all over the scene).
+void row_order(const std::vector<int32_t> &cnt, int placement, std::vector<int32_t> &order) {
+  const int64_t n = (int64_t)cnt.size();
+  order.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+  auto longer = [&](int32_t a, int32_t b) { return cnt[a] > cnt[b]; };
+  if (placement != 1) {
+    std::stable_sort(order.begin(), order.end(), longer);
+    if (placement == 2) {
+      const int64_t nc = gather_chunks(n);
+      std::vector<int32_t> mine;
+      for (int64_t c = 0; c < nc; ++c) {
+        mine.clear();
+        for (int64_t k = c; k < n; k += nc) mine.push_back(order[(size_t)k]);
+        std::sort(mine.begin(), mine.end());
+        for (size_t k = 0; k < mine.size(); ++k) order[(size_t)(c + (int64_t)k * nc)] = mine[k];
+      }
+    }
+    return;
+  }
+  const int64_t nc = gather_chunks(n);
+  std::vector<int32_t> run;
+  int64_t first = 0;
+  for (int64_t c = 0; c < nc && first < n; ++c) {
+    const int64_t size = (n - c + nc - 1) / nc;
+    run.resize((size_t)size);
+    for (int64_t k = 0; k < size; ++k) run[(size_t)k] = (int32_t)(first + k);
+    std::stable_sort(run.begin(), run.end(), longer);
+    for (int64_t k = 0; k < size; ++k) order[(size_t)(c + k * nc)] = run[(size_t)k];
+    first += size;
+  }
+}
+
+// padded row starts (rows padded to a multiple of 8 entries)
+void layout_rows(const std::vector<int32_t> &cnt, std::vector<int64_t> &rp) {
   const int64_t n = (int64_t)cnt.size();
   rp.assign((size_t)n + 1, 0);
   for (int64_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + (((int64_t)cnt[i] + 7) & ~(int64_t)7);
-  order.resize((size_t)n);
-  for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t a, int32_t b) { return cnt[a] > cnt[b]; });
 }
 
 int finish_layout(vr_xfers *x, hipStream_t st) {
   std::vector<int32_t> order;
-  layout_rows(x->cnt_host, x->rp_host, order);
+  layout_rows(x->cnt_host, x->rp_host);
+  row_order(x->cnt_host, 0, order);
+  x->placement = 0;
   int rc = upload_vec(x->rp, x->rp_host, st);
   if (!rc) rc = upload_vec(x->order, order, st);
   if (!rc) rc = upload_vec(x->cnt, x->cnt_host, st);
@@ -412,6 +450,11 @@ int vr_set_tuning(const char *name, int value) {
     if (value < 0 || value > 7)
       return fail(VR_E_INVALID, "bounce_prefetch must be 0 (by kernel) to 7");
     tuning().bounce_prefetch = value;
+  } else if (k == "bounce_placement") {
+    if (value < 0 || value > 2)
+      return fail(VR_E_INVALID,
+                  "bounce_placement must be 0 (strided), 1 (row runs) or 2 (strided, by index)");
+    tuning().bounce_placement = value;
   } else if (k == "bounce_passes") {
     if (value < 0 || value > 2)
       return fail(VR_E_INVALID, "bounce_passes must be 0 (by precision), 1 or 2");
@@ -445,6 +488,7 @@ int vr_get_tuning(const char *name, int *value) {
   else if (k == "bounce_group") *value = tuning().bounce_group;
   else if (k == "bounce_passes") *value = tuning().bounce_passes;
   else if (k == "bounce_prefetch") *value = tuning().bounce_prefetch;
+  else if (k == "bounce_placement") *value = tuning().bounce_placement;
   else if (k == "kd_build") *value = tuning().kd_build;
   else if (k == "kd_build_device_min") *value = tuning().kd_build_device_min;
   else if (k == "patch_build") *value = tuning().patch_build;
@@ -1491,6 +1535,7 @@ int tiled_shape(const vr_xfers *x, const Tuning &tu, TiledShape &sh) {
     sh.passes = tu.bounce_passes ? tu.bounce_passes : 1;
   }
   sh.prefetch = tu.bounce_prefetch ? tu.bounce_prefetch : (sh.kernel == 2 ? 1 : 3);
+  sh.placement = tu.bounce_placement;
   if (!bounce_gather_variant(sh))
     return fail(VR_E_UNSUPPORTED, "no compiled LDS gather for this tuning shape");
   return VR_OK;
@@ -1503,7 +1548,7 @@ bool tiles_ready(const vr_xfers *x, const TiledShape &sh) {
   const int ostride = sh.kernel == 2 ? 16 : 12;
   return x->tp_cols == sh.cols && x->tp.p && x->blk_cols == sh.cols &&
          x->blk_prec == x->precision && x->blk_chunks == gather_chunks(x->n) &&
-         x->blk_stride == ostride;
+         x->blk_stride == ostride && x->placement == sh.placement;
 }
 
 // Tile pointers and the tile-blocked copy of x for the tile width, built once
@@ -1521,9 +1566,22 @@ int prepare_tiles(vr_xfers *x, const TiledShape &sh, DevTiles &tl, hipStream_t s
   tl.group = sh.group;
   tl.n_chunks = gather_chunks(n);
   if (n == 0) return VR_OK;
-  const DevXfers dx = x->dev();
   if (x->served && !tiles_ready(x, sh))
     return fail(VR_E_STATE, "transfers are held by a server with another gather layout");
+  if (x->placement != sh.placement) {  // deal the rows again; the blocked copy follows them
+    std::vector<int32_t> order;
+    row_order(x->cnt_host, sh.placement, order);
+    // into the same buffer (n rows either way), so the order pointer in every
+    // DevXfers already taken of x stays valid
+    if (x->order.bytes < sizeof(int32_t) * order.size())
+      return fail(VR_E_STATE, "row order buffer smaller than the rows");
+    VR_HIP(hipMemcpyAsync(x->order.p, order.data(), sizeof(int32_t) * order.size(),
+                          hipMemcpyHostToDevice, st));
+    VR_HIP(hipStreamSynchronize(st));
+    x->placement = sh.placement;
+    x->blk_cols = 0;
+  }
+  const DevXfers dx = x->dev();
   if (x->tp_cols != cols || !x->tp.p) {
     TRY(x->tp.alloc(4 * (size_t)n * (size_t)(nt + 1)));
     launch_tile_ptrs(n, x->rp.as<int64_t>(), x->cnt.as<int32_t>(), x->col.as<int32_t>(), cols, nt,
@@ -2285,7 +2343,7 @@ int vr_server_add_scene(vr_server *sv, vr_scene *scene, vr_xfers *x, const vr_sa
     if (!sv->scenes.empty() &&
         (sh.kernel != sv->shape.kernel || sh.cols != sv->shape.cols ||
          sh.group != sv->shape.group || sh.passes != sv->shape.passes ||
-         sh.prefetch != sv->shape.prefetch))
+         sh.prefetch != sv->shape.prefetch || sh.placement != sv->shape.placement))
       return fail(VR_E_INVALID, "the scenes of a server share one gather shape");
     TRY(vr::prepare_tiles(x, sh, sc->tl, g_stream));
     sv->shape = sh;

@@ -24,6 +24,14 @@ struct Tuning {
   int bounce_group = 0;        // 8 / 16 lanes per row segment, 0 = by precision
   int bounce_passes = 0;       // passes of a segment loaded ahead of use, 0 = by precision
   int bounce_prefetch = 0;     // steps/passes the loads run ahead, 0 = by kernel (3 / 1)
+  // rows dealt to the gather's 1024-row chunks: 0 = a stride through the
+  // longest-first order of all rows (every chunk the same mix), 1 = chunk c
+  // takes the c-th run of consecutive rows, longest-first inside the run (a
+  // chunk's rows share their column tiles when neighbouring patches see the
+  // same patches, as MakeTransfers rows do), 2 = placement 0's chunks with
+  // their rows dealt to the waves in row-index order (each wave's rows spread
+  // over the scene, so the waves' work per column tile evens out)
+  int bounce_placement = 0;
   // kd-tree construction in vr_scene_build (the same tree either way)
   int kd_build = 2;                 // 0 = host threads, 1 = GPU, 2 = GPU from a size on
   int kd_build_device_min = 2048;   // triangles at which kd_build = 2 uses the GPU
@@ -181,15 +189,20 @@ struct BounceJob {
   double stop_frac;
 };
 constexpr int kMaxTilesPerJob = 1024;  // block starts of a chunk cached in LDS
-// Chunks of a transfer set: at most 1024 rows each, and at least
-// kChunkTarget of them while a chunk keeps >= 256 rows.  A constant (not the
-// device's CU count) because the stream gather's summation order depends on
-// the chunking.
+// Chunks of a transfer set, at most 1024 rows each (one gather workgroup,
+// whose LDS takes a whole CU, so kChunkTarget = 256 of them run at once):
+// up to kChunkTarget full chunks, min(kChunkTarget, ceil(n / 256)) chunks
+// (at least 256 rows each while that stays under 1024); beyond, a whole
+// multiple of kChunkTarget, so the launch runs in full rounds.  A constant
+// (not the device's CU count) because the stream gather's summation order
+// depends on the chunking.
 constexpr int64_t kChunkTarget = 256;
 __host__ __device__ inline int64_t gather_chunks(int64_t n) {
   if (n <= 0) return 0;
-  const int64_t a = (n + 1023) / 1024, b = (n + 255) / 256;
-  return a > (b < kChunkTarget ? b : kChunkTarget) ? a : (b < kChunkTarget ? b : kChunkTarget);
+  constexpr int64_t round_rows = kChunkTarget * 1024;
+  if (n > round_rows) return kChunkTarget * ((n + round_rows - 1) / round_rows);
+  const int64_t b = (n + 255) / 256;
+  return b < kChunkTarget ? b : kChunkTarget;
 }
 // Rank (in the longest-first row order, within chunk c: row order[c + rank *
 // n_chunks]) of owner slot (wave w, lane j): serpentine over the 16 waves, so
@@ -200,7 +213,7 @@ __host__ __device__ inline int64_t slot_rank(int w, int j) {
 // Gather kernel and shape: kernel 0 = LDS-tiled (group, passes, prefetch),
 // 2 = stream (prefetch only); cols = tile width.
 struct GatherShape {
-  int kernel, cols, group, passes, prefetch;
+  int kernel, cols, group, passes, prefetch, placement;
 };
 void launch_bounce(const DevXfers &x, const float *emit_in, float *emit_out, float *total,
                    const float *refl, const float *area, double *e_row, uint32_t *row_counter,
