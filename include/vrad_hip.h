@@ -530,7 +530,7 @@ int vr_lighting_lump(const float *luxel_rgb, int64_t n_luxels, const vr_face_lig
                      int64_t n_faces, uint8_t *lump, int64_t lump_cap, int32_t *lightofs,
                      int64_t *lump_bytes);
 
-/* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- *//* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- */
+/* ---- BSP clusters and PVS: rad/clustertable, rad/lightmap/vis.go -------- */
 /* The BSP lumps PointInLeaf reads (cache.GetLumpCache().Planes / Nodes /
  * Leafs, rad/clustertable/point.go:14-36): a plane's normal and distance, a
  * node's plane and children (child < 0: leaf -1-child), a leaf's cluster.

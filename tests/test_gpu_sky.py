@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from tests import helpers as H
-from vrad_amd import _lib, lightmap, scenes
+from vrad_amd import _lib, autotune, lightmap, scenes
 from vrad_amd.raytracer import Environment
 
 pytestmark = pytest.mark.gpu
@@ -46,9 +46,10 @@ def test_precompute_general_goldens(golden_dir):
         assert (r["NCoordSelect0"], r["NCoordSelect1"]) == (t["c0"], t["c1"])
 
 
-@pytest.mark.parametrize("lds", [4, 8, 24])
+@pytest.mark.parametrize("lds", [4, 10, 24])
 def test_leaf_sky_closed_forms(lds):
     print(f"[gpu] CanLeafTraceToSky closed forms lds={lds}", flush=True)
+    old = autotune.get_knob("trace_lds_entries")
     _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", lds))
     try:
         ctr = np.array([[48, 48, 18]], np.int16)
@@ -60,7 +61,7 @@ def test_leaf_sky_closed_forms(lds):
             got = lightmap.CanLeafTraceToSky(env, ctr, ctr + 4)
             assert list(got) == [want]
     finally:
-        _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", 4))
+        _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", old))
 
 
 def test_leaf_sky_level_vs_oracle():

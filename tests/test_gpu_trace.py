@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from tests import helpers as H
-from vrad_amd import _lib, scenes
+from vrad_amd import _lib, autotune, scenes
 from vrad_amd.raytracer import Environment, FourRays, RayTracingResult
 from vrad_amd.trace import TestLineDoesHitSky
 
@@ -25,21 +25,23 @@ def _env(v9, ids):
     return env
 
 
-@pytest.fixture(params=[(_lib.VR_PATH_OPT, 8, 1), (_lib.VR_PATH_OPT, 4, 1), (_lib.VR_PATH_OPT, 24, 1),
-                        (_lib.VR_PATH_OPT, 2, 1), (_lib.VR_PATH_OPT, 2, 8),
-                        (_lib.VR_PATH_OPT, 8, 2), (_lib.VR_PATH_OPT, 4, 4), (_lib.VR_PATH_OPT, 8, 8),
-                        (_lib.VR_PATH_PLAIN, 8, 1)],
-                ids=["opt-lds8", "opt-lds4", "opt-lds24", "opt-lds2", "opt-lds2-r8", "opt-lds8-r2", "opt-lds4-r4", "opt-lds8-r8",
+@pytest.fixture(params=[(_lib.VR_PATH_OPT, 10, 1), (_lib.VR_PATH_OPT, 8, 1), (_lib.VR_PATH_OPT, 4, 1),
+                        (_lib.VR_PATH_OPT, 24, 1), (_lib.VR_PATH_OPT, 2, 1), (_lib.VR_PATH_OPT, 2, 8),
+                        (_lib.VR_PATH_OPT, 6, 2), (_lib.VR_PATH_OPT, 8, 2), (_lib.VR_PATH_OPT, 4, 4),
+                        (_lib.VR_PATH_OPT, 10, 4), (_lib.VR_PATH_OPT, 8, 8), (_lib.VR_PATH_PLAIN, 8, 1)],
+                ids=["opt-lds10", "opt-lds8", "opt-lds4", "opt-lds24", "opt-lds2", "opt-lds2-r8", "opt-lds6-r2",
+                     "opt-lds8-r2", "opt-lds4-r4", "opt-lds10-r4", "opt-lds8-r8",
                      "plain"])
 def path(request):
     p, lds, rays = request.param
     print(f"[gpu] path={p} trace_lds_entries={lds} rays_per_lane={rays}", flush=True)
+    old = autotune.get_knob("trace_lds_entries")
     _lib.check(_lib.lib.vr_set_path(p))
     _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", lds))
     _lib.check(_lib.lib.vr_set_tuning(b"trace_rays_per_lane", rays))
     yield p
     _lib.check(_lib.lib.vr_set_path(_lib.VR_PATH_OPT))
-    _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", 4))
+    _lib.check(_lib.lib.vr_set_tuning(b"trace_lds_entries", old))
     _lib.check(_lib.lib.vr_set_tuning(b"trace_rays_per_lane", 1))
 
 

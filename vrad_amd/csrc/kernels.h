@@ -13,7 +13,7 @@ constexpr int kStackDepth = 24;  // > max kd depth (21 + 1 leaf level)
 
 // Launch-shape knobs (vr_set_tuning); every value gives identical results.
 struct Tuning {
-  int trace_lds_entries = 4;   // traversal-stack entries per lane kept in LDS: 2/4/8/12/16/24
+  int trace_lds_entries = 10;  // traversal-stack entries per lane kept in LDS: 2/4/6/8/10/12/16/24
   int trace_rays_per_lane = 1; // rays per lane of the visibility kernels (1, 2, 4, 8): a wave
                                // covers 64 x this many rays
   int bounce_waves = 8192;     // persistent waves of the wave-per-row gather
