@@ -155,6 +155,20 @@ struct ScratchStack {
   }
 };
 
+// Interval bounds in the walk are compared as int32 bit patterns: one
+// v_min_i32 / v_max_i32, with no NaN quieting of loop-carried values.  For
+// two non-negative floats, or one of each sign, this is the float min / max;
+// for two negatives (only when a caller's tmin < 0) "min" returns the larger
+// and "max" the smaller, so an upper bound (min) only grows and a lower
+// bound (max) only shrinks: intervals widen, the walk visits more, and the
+// result is unchanged.
+__device__ __forceinline__ float tmin_wide(float a, float b) {
+  return __int_as_float(min(__float_as_int(a), __float_as_int(b)));
+}
+__device__ __forceinline__ float tmax_wide(float a, float b) {
+  return __int_as_float(max(__float_as_int(a), __float_as_int(b)));
+}
+
 // Walks one kd-tree (nodes/refs/bounds) for hits with tmin <= t < tmax that
 // come before (tbest, best) in (t, triangle index) order, updating them; the
 // result is the first of the hits in that order, bit for bit the brute-force
@@ -189,6 +203,14 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
     tb = fminf(tb, fmaxf(t1, t2));
   }
   if (!(ta <= tb)) return false;
+  // a split plane s on axis a bounds the child slabs at
+  //   t = ((s +- dl) - o_a) i_a = s i_a + (+-dl - o_a) i_a,
+  // the near child's exit with +dl toward the ray (copysign(dl, i_a)), the
+  // far child's entry with -dl; the terms after s i_a are fixed per ray, and
+  // their rounding (and the fma's) stays far inside the dl margin
+  const float ex = (copysignf(dl, ix) - ox) * ix, fx = (-copysignf(dl, ix) - ox) * ix;
+  const float ey = (copysignf(dl, iy) - oy) * iy, fy = (-copysignf(dl, iy) - oy) * iy;
+  const float ez = (copysignf(dl, iz) - oz) * iz, fz = (-copysignf(dl, iz) - oz) * iz;
   int sp = 0;
   uint32_t node = 0;
   // next stacked node whose interval can still hold a hit before tbest
@@ -219,17 +241,15 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
     while ((nd.x & 3u) != 3u) {
       const uint32_t type = nd.x & 3u;
       const float sv = __uint_as_float(nd.y);
-      const float o_a = sel3((int)type, ox, oy, oz);
       const float i_a = sel3((int)type, ix, iy, iz);
-      const float lo = ((sv - dl) - o_a) * i_a;
-      const float hi = ((sv + dl) - o_a) * i_a;
+      const float near_end = __builtin_fmaf(sv, i_a, sel3((int)type, ex, ey, ez));
+      const float far_start = __builtin_fmaf(sv, i_a, sel3((int)type, fx, fy, fz));
       const uint32_t left = nd.x >> 2;
       const bool fwd = i_a >= 0.0f;
       const uint32_t near_n = fwd ? left : left + 1, far_n = fwd ? left + 1 : left;
-      const float near_end = fwd ? hi : lo, far_start = fwd ? lo : hi;
-      const float tbe = fminf(tb, tbest);
-      const bool near_ok = ta <= fminf(near_end, tbe);
-      const float fs = fmaxf(far_start, ta);
+      const float tbe = tmin_wide(tb, tbest);
+      const bool near_ok = ta <= tmin_wide(near_end, tbe);
+      const float fs = tmax_wide(far_start, ta);
       const bool far_ok = fs <= tbe;
       // the stack holds at most one entry per level of the path to the
       // current node, and the host checks tree depth < D
@@ -238,7 +258,7 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
         ++sp;
       }
       node = near_ok ? near_n : far_n;
-      const float tb_near = fminf(tb, near_end);
+      const float tb_near = tmin_wide(tb, near_end);
       ta = near_ok ? ta : fs;
       tb = near_ok ? tb_near : tb;
       if (!near_ok && !far_ok) {
