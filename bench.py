@@ -449,10 +449,15 @@ def run_bounce(cx, args):
         for k, val in zip(knobs, shape):
             check(lib.vr_set_tuning(k, val))
 
-    def bounce_ms(nb, reps=1):
-        # one untimed call (tile layout for the shape, clocks), then the median
-        # over reps calls of the gather's mean launch time
+    def bounce_ms(nb, reps=1, warm_s=0.0):
+        # one untimed call (tile layout for the shape), then untimed calls for
+        # warm_s seconds (the GPU reaches steady clocks only after some
+        # tenths of a second of this work), then the median over reps calls
+        # of the gather's mean launch time
         lightmap.BounceLight(xf, e0, refl, area, max_bounces=1, stop_frac=0.0)
+        t_end = time.perf_counter() + warm_s
+        while time.perf_counter() < t_end:
+            lightmap.BounceLight(xf, e0, refl, area, max_bounces=nb, stop_frac=0.0)
         out = []
         for _ in range(reps):
             check(lib.vr_profile_reset())
@@ -494,7 +499,7 @@ def run_bounce(cx, args):
         sweep = {shape_name(tuple(r["knobs"][k] for k in autotune.BOUNCE_KNOBS)): round(r["ms"], 4)
                  for r in rec["table"]}
         best = shape_name(tuple(rec["best"][k] for k in autotune.BOUNCE_KNOBS))
-        per = bounce_ms(8, reps=5)
+        per = bounce_ms(16, reps=5, warm_s=0.3)
         gbps = B / (per * 1e-3) / 1e9
         out[name] = {"ms_per_bounce": round(per, 4), "GBps": round(gbps, 1),
                      "frac": round(gbps / HBM_PEAK_GBPS, 4), "bytes_per_bounce": B,
@@ -724,9 +729,9 @@ def run_level_bake(cx, args):
 
 def real_csr_bounce(cx, xf, s):
     """BounceLight GB/s on a transfer set MakeTransfers built (clustered
-    columns, rows of every length), FP32 and FP16, with both row placements:
+    columns, rows of every length), FP32 and FP16, with each row placement:
     8 bounces per call, the median of 3 calls after a warm-up call that
-    builds the tile layout."""
+    builds the tile layout and 0.2 s of untimed calls."""
     from vrad_amd import autotune, lightmap
     lib, check, L = cx.lib, cx.check, cx.L
     inf = xf.info()
@@ -742,6 +747,10 @@ def real_csr_bounce(cx, xf, s):
                                        ("fp16", L.VR_COEF_F16, 2, 4)):
                 xf.set_precision(prec)
                 lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=1, stop_frac=0.0)
+                t_end = time.perf_counter() + 0.2   # steady clocks before timing
+                while time.perf_counter() < t_end:
+                    lightmap.BounceLight(xf, e0, s.p_refl, s.p_area, max_bounces=8,
+                                         stop_frac=0.0)
                 ms = []
                 for _ in range(3):
                     check(lib.vr_profile_reset())
