@@ -89,8 +89,13 @@ __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p
                                                        uint32_t *__restrict__ bits,
                                                        int64_t words) {
   __shared__ uint2 s_stk[S * 64];
+  // The grid is sized for cap pairs; only the first ceil(n / 64) blocks have
+  // work.  Blocks are dealt to the XCDs round-robin, so those blocks spread
+  // over all of them, and the XCD-aware remap runs over just those.
   const uint32_t n = min(*count, cap);
-  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
+  const uint32_t nb = (n + 63u) / 64u;
+  if (blockIdx.x >= nb) return;
+  const uint32_t k = xcd_chunk(blockIdx.x, nb) * 64 + threadIdx.x;
   if (k >= n) return;
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
@@ -111,7 +116,9 @@ __global__ __launch_bounds__(256) void k_pair_trace_plain(DevScene sc, DevPatche
                                                           uint32_t *__restrict__ bits,
                                                           int64_t words) {
   const uint32_t n = min(*count, cap);
-  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const uint32_t nb = (n + 255u) / 256u;  // blocks with work (see k_pair_trace_lds)
+  if (blockIdx.x >= nb) return;
+  const uint32_t k = xcd_chunk(blockIdx.x, nb) * 256 + threadIdx.x;
   if (k >= n) return;
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
