@@ -48,6 +48,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
 LEVEL = dict(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
+# row placement of the gather in the level bake (rows clustered by room)
+LEVEL_BAKE_PLACEMENT = 2
 # the precision form's scene: ~1.5 M triangles, ~250k patches
 C3_LEVEL = dict(seed=3, rooms_x=16, rooms_y=16, chop=72.0, tess=40.0, n_lights=16, sun=True)
 
@@ -676,10 +678,19 @@ def run_level_bake(cx, args):
     direct = lightmap.GatherSampleLight(env, s.p_origin, s.p_normal, s.lights)
     times["direct_light_ms"] = (time.perf_counter() - t0) * 1e3
     emit0 = (direct * s.p_refl).astype(np.float32)
-    t0 = time.perf_counter()
-    tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct,
-                                      max_bounces=100, stop_frac=0.0)
-    times["bounces_ms"] = (time.perf_counter() - t0) * 1e3
+    # MakeTransfers rows cluster by room: the gather deals each chunk's rows
+    # to its waves by row index (bounce_placement 2, its own restated order);
+    # the tile layout for it is built inside the timed bounce call
+    from vrad_amd import autotune
+    old_place = autotune.get_knob("bounce_placement")
+    autotune.set_knob("bounce_placement", LEVEL_BAKE_PLACEMENT)
+    try:
+        t0 = time.perf_counter()
+        tot, en, k = lightmap.BounceLight(xf, emit0, s.p_refl, s.p_area, total=direct,
+                                          max_bounces=100, stop_frac=0.0)
+        times["bounces_ms"] = (time.perf_counter() - t0) * 1e3
+    finally:
+        autotune.set_knob("bounce_placement", old_place)
     times["total_ms"] = sum(times.values())
     times = {a: round(b, 1) for a, b in times.items()}
     inf = xf.info()
@@ -716,7 +727,8 @@ def run_level_bake(cx, args):
     out = {"triangles": int(len(s.ids)), "patches": int(np_),
            "lights": int(len(s.lights)), "sun": bool((s.lights["type"] == 3).any()),
            "transfers": int(inf.nnz), "segments_traced": int(inf.segments_traced),
-           "bounces": int(k), "energy_first_last": [float(en[0]), float(en[-1])],
+           "bounces": int(k), "bounce_placement": LEVEL_BAKE_PLACEMENT,
+           "energy_first_last": [float(en[0]), float(en[-1])],
            "times": times,
            "cpu_check": {"direct_light_4096_patches_bitexact": direct_ok,
                          "rows_checked": 64, "segments_checked": int(seg_checked),

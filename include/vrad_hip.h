@@ -229,9 +229,9 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t n_patches,
  * counterpart): scene k's patches[k] (n_patches[k] of them) give out[k],
  * equal bit for bit to vr_make_transfers(scenes[k], ...).  Every round, each
  * unfinished scene culls its next candidate rows into its own region of one
- * pair buffer (params->batch_pairs pairs in total, default 16M, at least a
- * row per scene), and one visibility launch walks all regions through a
- * table of scene descriptors.  The plain path runs the scenes one by one.
+ * pair buffer (params->batch_pairs pairs per round, default 32M, shared
+ * among the scenes still culling, at least a row each), and one visibility
+ * launch walks all regions through a table of scene descriptors.  The plain path runs the scenes one by one.
  * On failure no out[k] is set. */
 int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
                             const int64_t *n_patches, int32_t n_scenes,

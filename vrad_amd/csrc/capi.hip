@@ -1043,9 +1043,12 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
   vr_xfer_params P{0.1f, -1, 1, 0};
   if (prm) P = *prm;
   hipStream_t st = g_stream;
-  // every scene culls its next rows into its own region of one pair buffer;
-  // the regions' pairs are then traced in one launch
-  const int64_t budget = P.batch_pairs > 0 ? P.batch_pairs : (int64_t)16 << 20;
+  // every round, each unfinished scene culls its next rows into its own
+  // region of one pair buffer, and the regions' pairs are traced in one
+  // launch.  The round's budget is shared among the scenes still culling
+  // (at least one whole row each), so the rounds do not outlast the small
+  // scenes with small shares.
+  const int64_t budget = P.batch_pairs > 0 ? P.batch_pairs : (int64_t)32 << 20;
   struct One {
     std::unique_ptr<vr_xfers> x;
     Staging S;
@@ -1054,7 +1057,7 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
     int64_t np = 0, words = 0, cap = 0, off = 0, i0 = 0;
   };
   std::vector<One> sc((size_t)n_scenes);
-  int64_t total_cap = 0;
+  int64_t total_cap = budget;  // a round's shares plus one row per scene
   for (int32_t k = 0; k < n_scenes; ++k) {
     One &o = sc[(size_t)k];
     const vr_patches *p = &patches[k];
@@ -1071,10 +1074,9 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
     o.words = (o.np + 31) / 32;
     TRY(o.bits.alloc((size_t)std::max<int64_t>(o.np, 1) * (size_t)std::max<int64_t>(o.words, 1) * 4));
     VR_HIP(hipMemsetAsync(o.bits.p, 0, o.bits.bytes, st));
-    o.cap = std::max<int64_t>(o.np - 1, std::max<int64_t>(1, budget / n_scenes));
-    if (o.cap >= (1ll << 31)) return fail(VR_E_INVALID, "batch_pairs too large");
-    o.off = total_cap;
-    total_cap += o.cap;
+    if (budget >= (1ll << 31) || o.np - 1 >= (1ll << 31))
+      return fail(VR_E_INVALID, "batch_pairs too large");
+    total_cap += std::max<int64_t>(o.np - 1, 0);
   }
   DBuf pairs, counts, d_jobs;
   TRY(pairs.alloc((size_t)std::max<int64_t>(total_cap, 1) * sizeof(int2)));
@@ -1087,12 +1089,19 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
     Prof pr("make_transfers", st);
     for (;;) {
       bool any = false;
+      int64_t active = 0, off = 0;
+      for (int32_t k = 0; k < n_scenes; ++k) active += sc[(size_t)k].i0 + 1 < sc[(size_t)k].np;
+      const int64_t share = std::max<int64_t>(1, budget / std::max<int64_t>(active, 1));
       VR_HIP(hipMemsetAsync(counts.p, 0, 4 * (size_t)n_scenes, st));
       for (int32_t k = 0; k < n_scenes; ++k) {
         One &o = sc[(size_t)k];
         if (o.i0 + 1 >= o.np) continue;
         any = true;
         const int64_t rowlen = o.np - 1 - o.i0;
+        // share <= budget / active and rowlen <= np - 1, so the regions fit
+        o.cap = std::max<int64_t>(rowlen, share);
+        o.off = off;
+        off += o.cap;
         const int64_t rows = std::max<int64_t>(1, o.cap / rowlen);
         const int64_t i1 = std::min<int64_t>(o.np - 1, o.i0 + rows);
         o.x->pairs += (2 * o.np - o.i0 - i1 - 1) * (i1 - o.i0) / 2;
