@@ -55,6 +55,7 @@ class Environment:
         self.Flags = flags
         self.MinBound = np.zeros(3, np.float32)
         self.MaxBound = np.zeros(3, np.float32)
+        # ingested triangles as chunks (one per call), joined at build time
         self._v9, self._ids, self._flags = [], [], []
         self.TriangleColors, self.TriangleMaterials = [], []
         self._scene = None
@@ -67,9 +68,9 @@ class Environment:
         self.AddTriangleWithMaterial(id, v1, v2, v3, colour, 0, 0)
 
     def AddTriangleWithMaterial(self, id, v1, v2, v3, colour, flags, materialIndex):
-        self._v9.append(np.concatenate([f32(v1, 3), f32(v2, 3), f32(v3, 3)]))
-        self._ids.append(np.int32(id))
-        self._flags.append(np.uint8(flags & 0xFF))
+        self._v9.append(np.concatenate([f32(v1, 3), f32(v2, 3), f32(v3, 3)]).reshape(1, 9))
+        self._ids.append(np.array([id], np.int32))
+        self._flags.append(np.array([flags & 0xFF], np.uint8))
         if not (flags & RTE_FLAGS_DONT_STORE_TRIANGLE_COLORS):
             self.TriangleColors.append(f32(colour, 3))
         if not (flags & RTE_FLAGS_DONT_STORE_TRIANGLE_MATERIALS):
@@ -93,16 +94,23 @@ class Environment:
     def AddTriangles(self, verts9, ids, flags=None):
         """Bulk append (n x 9 float32 VertexCoordData, n ids)."""
         v = f32(verts9).reshape(-1, 9)
-        self._v9.extend(v)
-        self._ids.extend(np.asarray(ids, np.int32))
-        fl = np.zeros(len(v), np.uint8) if flags is None else np.asarray(flags, np.uint8)
-        self._flags.extend(fl)
+        i = np.asarray(ids, np.int32).reshape(-1)
+        if len(i) != len(v):
+            raise ValueError(f"{len(v)} triangles but {len(i)} ids")
+        fl = np.zeros(len(v), np.uint8) if flags is None else np.asarray(flags, np.uint8).reshape(-1)
+        # copies: later changes to the caller's arrays do not reach the scene
+        self._v9.append(v.copy())
+        self._ids.append(i.copy())
+        self._flags.append(np.array(fl, np.uint8))
 
     def triangle_arrays(self):
-        n = len(self._ids)
-        v9 = f32(np.array(self._v9, np.float32).reshape(n, 9)) if n else np.zeros((0, 9), np.float32)
-        ids = np.ascontiguousarray(np.array(self._ids, np.int32))
-        fl = np.ascontiguousarray(np.array(self._flags, np.uint8))
+        if not self._ids:
+            return np.zeros((0, 9), np.float32), np.zeros(0, np.int32), np.zeros(0, np.uint8)
+        def join(parts):
+            return parts[0] if len(parts) == 1 else np.concatenate(parts)
+        v9 = f32(join(self._v9).reshape(-1, 9))
+        ids = np.ascontiguousarray(join(self._ids), np.int32)
+        fl = np.ascontiguousarray(join(self._flags), np.uint8)
         return v9, ids, fl
 
     # ---- raytracer/environment.go:119-138 ----
