@@ -59,9 +59,9 @@ class Environment:
         self._v9, self._ids, self._flags = [], [], []
         self.TriangleColors, self.TriangleMaterials = [], []
         self._scene = None
-        self.OptimizedKDTree = None      # (children uint32, split float32)
-        self.TriangleIndexList = None
-        self.OptimizedTriangleList = None  # TriIntersectData records
+        # the built scene's tree and triangle records, copied out on first use
+        self._kd = None    # (children uint32, split float32, triangle index list int32)
+        self._tri = None   # TriIntersectData records
 
     # ---- ingest: raytracer/environment.go:41-117 ----
     def AddTriangle(self, id, v1, v2, v3, colour=(1.0, 1.0, 1.0)):
@@ -124,16 +124,38 @@ class Environment:
         info = self.info()
         self.MinBound = np.array(info.bmin[:], np.float32)
         self.MaxBound = np.array(info.bmax[:], np.float32)
-        ch = np.zeros(info.n_nodes, np.uint32)
-        sp = np.zeros(info.n_nodes, np.float32)
-        idx = np.zeros(info.n_leaf_refs, np.int32)
-        check(lib.vr_scene_get_kdtree(h, ptr(ch), ptr(sp), ptr(idx)))
-        self.OptimizedKDTree = (ch, sp)
-        self.TriangleIndexList = idx
-        tri = np.zeros(info.n_tri, TRI_INTERSECT_DTYPE)
-        if info.n_tri:
-            check(lib.vr_scene_get_triangles(h, ptr(tri)))
-        self.OptimizedTriangleList = tri
+
+    def _tree(self):
+        if self._kd is None and self._scene is not None:
+            info = self.info()
+            ch = np.zeros(info.n_nodes, np.uint32)
+            sp = np.zeros(info.n_nodes, np.float32)
+            idx = np.zeros(info.n_leaf_refs, np.int32)
+            check(lib.vr_scene_get_kdtree(self._scene, ptr(ch), ptr(sp), ptr(idx)))
+            self._kd = (ch, sp, idx)
+        return self._kd
+
+    @property
+    def OptimizedKDTree(self):
+        """(children, split) of the built tree (optimisedkdnode.go:15-54)."""
+        t = self._tree()
+        return None if t is None else (t[0], t[1])
+
+    @property
+    def TriangleIndexList(self):
+        t = self._tree()
+        return None if t is None else t[2]
+
+    @property
+    def OptimizedTriangleList(self):
+        """TriIntersectData records of the built scene, in ingest order."""
+        if self._tri is None and self._scene is not None:
+            n = self.info().n_tri
+            tri = np.zeros(n, TRI_INTERSECT_DTYPE)
+            if n:
+                check(lib.vr_scene_get_triangles(self._scene, ptr(tri)))
+            self._tri = tri
+        return self._tri
 
     def info(self):
         inf = _lib.SceneInfo()
@@ -199,6 +221,7 @@ class Environment:
         return self.TriangleColors[triangleId]
 
     def close(self):
+        self._kd = self._tri = None
         if self._scene is not None:
             lib.vr_scene_destroy(self._scene)
             self._scene = None
