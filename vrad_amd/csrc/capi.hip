@@ -628,6 +628,7 @@ int vr_scene_build(vr_scene *s, int32_t n_threads) {
   const int kb = tuning().kd_build;
   const bool device = kb == 1 || (kb == 2 && s->n_tri >= tuning().kd_build_device_min);
   TRY(build_scene_trees(s, n_threads, device));
+  const int nt = host_threads(n_threads);
   const KdTree &kd = s->kd;
   // device records of a tree: nodes, and 3 float4 per leaf reference whose
   // meta holds the triangle's index in OptimizedTriangleList (`map` turns a
@@ -635,23 +636,27 @@ int vr_scene_build(vr_scene *s, int32_t n_threads) {
   auto flatten = [&](const KdTree &t, const std::vector<int32_t> *map, std::vector<uint2> &nodes,
                      std::vector<float4> &refs) {
     nodes.resize(t.children.size());
-    for (size_t k = 0; k < nodes.size(); ++k) {
-      uint32_t sb;
-      std::memcpy(&sb, &t.split[k], 4);
-      nodes[k] = make_uint2(t.children[k], sb);
-    }
+    parallel_for((int64_t)nodes.size(), nt, [&](int64_t b, int64_t e) {
+      for (int64_t k = b; k < e; ++k) {
+        uint32_t sb;
+        std::memcpy(&sb, &t.split[(size_t)k], 4);
+        nodes[(size_t)k] = make_uint2(t.children[(size_t)k], sb);
+      }
+    });
     refs.resize(t.idx.size() * 3);
-    for (size_t k = 0; k < t.idx.size(); ++k) {
-      const int32_t ti = map ? (*map)[(size_t)t.idx[k]] : t.idx[k];
-      const TriIntersect &tr = s->tris[(size_t)ti];
-      uint32_t meta = (uint32_t)ti | ((uint32_t)tr.c0 << 28) | ((uint32_t)tr.c1 << 30);
-      float idf, metaf;
-      std::memcpy(&idf, &tr.id, 4);
-      std::memcpy(&metaf, &meta, 4);
-      refs[3 * k + 0] = make_float4(tr.nx, tr.ny, tr.nz, tr.d);
-      refs[3 * k + 1] = make_float4(tr.e[0], tr.e[1], tr.e[2], tr.e[3]);
-      refs[3 * k + 2] = make_float4(tr.e[4], tr.e[5], idf, metaf);
-    }
+    parallel_for((int64_t)t.idx.size(), nt, [&](int64_t b, int64_t e) {
+      for (int64_t k = b; k < e; ++k) {
+        const int32_t ti = map ? (*map)[(size_t)t.idx[(size_t)k]] : t.idx[(size_t)k];
+        const TriIntersect &tr = s->tris[(size_t)ti];
+        uint32_t meta = (uint32_t)ti | ((uint32_t)tr.c0 << 28) | ((uint32_t)tr.c1 << 30);
+        float idf, metaf;
+        std::memcpy(&idf, &tr.id, 4);
+        std::memcpy(&metaf, &meta, 4);
+        refs[(size_t)(3 * k + 0)] = make_float4(tr.nx, tr.ny, tr.nz, tr.d);
+        refs[(size_t)(3 * k + 1)] = make_float4(tr.e[0], tr.e[1], tr.e[2], tr.e[3]);
+        refs[(size_t)(3 * k + 2)] = make_float4(tr.e[4], tr.e[5], idf, metaf);
+      }
+    });
   };
   std::vector<uint2> nodes, sky_nodes;
   std::vector<float4> refs, sky_refs;
@@ -662,10 +667,12 @@ int vr_scene_build(vr_scene *s, int32_t n_threads) {
     TRY(upload_vec(s->sky_refs, sky_refs, g_stream));
   }
   std::vector<float4> planes((size_t)s->n_tri);
-  for (int64_t i = 0; i < s->n_tri; ++i) {
-    const TriIntersect &t = s->tris[(size_t)i];
-    planes[(size_t)i] = make_float4(t.nx, t.ny, t.nz, t.d);
-  }
+  parallel_for(s->n_tri, nt, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const TriIntersect &t = s->tris[(size_t)i];
+      planes[(size_t)i] = make_float4(t.nx, t.ny, t.nz, t.d);
+    }
+  });
   TRY(upload_vec(s->nodes, nodes, g_stream));
   TRY(upload_vec(s->refs, refs, g_stream));
   TRY(upload_vec(s->planes, planes, g_stream));

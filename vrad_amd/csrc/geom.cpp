@@ -36,6 +36,10 @@ static void parallel_ranges(int64_t n, int n_threads, F fn) {
   for (auto &x : th) x.join();
 }
 
+void parallel_for(int64_t n, int n_threads, const std::function<void(int64_t, int64_t)> &fn) {
+  parallel_ranges(n, n_threads, [&](int64_t b, int64_t e) { fn(b, e); });
+}
+
 // GetEdgeEquation, vmath/polygon/edge.go:5-25
 static void edge_equation(const float p1[3], const float p2[3], int c0, int c1,
                           const float in[3], float *out) {

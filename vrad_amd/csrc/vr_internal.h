@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/vrad_hip.h"
@@ -64,5 +65,8 @@ constexpr int kRefQuads = 3;
 constexpr uint32_t kTriIndexMask = (1u << 28) - 1u;
 
 int host_threads(int requested);
+// fn(begin, end) over [0, n) split into n_threads contiguous ranges on host
+// threads (in the calling thread when n is small or n_threads <= 1)
+void parallel_for(int64_t n, int n_threads, const std::function<void(int64_t, int64_t)> &fn);
 
 }  // namespace vr
