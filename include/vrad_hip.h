@@ -76,7 +76,7 @@ int vr_set_path(int path);
 /* Kernel-shape knobs (the autotuner's interface; no reference counterpart).
  * Results do not depend on:
  *   "trace_lds_entries"  traversal-stack entries per lane held in LDS
- *                        (2, 4, 8, 12, 16, 24, default 4; the rest spill
+ *                        (2, 4, 6, 8, 10, 12, 16, 24, default 10; the rest spill
  *                        to scratch)
  *   "trace_rays_per_lane" rays each lane of the visibility kernels takes
  *                        (1, 2, 4, 8): rays per wavefront = 64 x this; for
@@ -105,8 +105,13 @@ int vr_set_path(int path);
  *   "bounce_prefetch"    passes (stream: 1, 2) or steps (tiled: 1, 3, 7) the
  *                        gather's loads run ahead; 0 (default) = 1 for the
  *                        stream gather, 3 for the tiled one
- * The kernel, tile width and group fix the summation order; the other bounce
- * knobs change only the schedule.  vr_bounce_light fails with
+ *   "bounce_placement"   stream gather: how rows are dealt to its 1024-row
+ *                        chunks and their waves; 0 (default) = a stride
+ *                        through the longest-first order of all rows, 1 =
+ *                        runs of consecutive rows, 2 = placement 0's chunks
+ *                        with their rows dealt in row-index order
+ * The kernel, tile width, group and placement fix the summation order; the
+ * other bounce knobs change only the schedule.  vr_bounce_light fails with
  * VR_E_UNSUPPORTED for a combination that has no compiled kernel. */
 int vr_set_tuning(const char *name, int value);
 int vr_get_tuning(const char *name, int *value);
