@@ -563,6 +563,15 @@ __global__ __launch_bounds__(1024) void k_bounce_tiled(const BounceJob *__restri
 }
 
 // ---- stream gather ------------------------------------------------------------
+// Record slot of patch p (tile-relative) in the stream gather's LDS emit
+// tile: p with its low 4 bits xor'ed with bits 3-6.  A permutation inside
+// each aligned group of 16 records.  A pass's lane l reads its chunk entry m
+// at stream entry 8 l + m, so a run of consecutive columns (MakeTransfers
+// rows are mostly such runs) would put lanes l and l + 2 on the same bank
+// group; after the xor any 16 consecutive lanes of such a run read 16
+// different record slots (8 lanes: 8 different slots mod 8).
+__host__ __device__ inline int emit_slot(int p) { return p ^ ((p >> 3) & 15); }
+
 // Lane source through DPP for the segmented scan (0 where the source lane is
 // outside the row; the scan's conditions never use such a lane).
 template <int CTRL, int ROWS>
@@ -704,10 +713,10 @@ __host__ __device__ inline int stream_pass_slot(int c, int nc) {
 // order; each slot gets at most one addition per pass.  Loads run D passes
 // ahead; the first D passes of tile t + 1 are issued when tile t starts, into
 // the other half of a two-tile register ring.  Emit tiles sit in LDS as
-// 16-byte records (offsets 16 (col - t C)); the next emit tile is loaded into
-// registers when a tile starts and written to the other LDS buffer when it
-// ends (no LDS DMA: the compiler waits for every pending load before an LDS
-// read while a DMA to LDS is in flight).
+// 16-byte records (offsets 16 emit_slot(col - t C)); the next emit tile is
+// loaded into registers when a tile starts and written to the other LDS
+// buffer when it ends (no LDS DMA: the compiler waits for every pending load
+// before an LDS read while a DMA to LDS is in flight).
 template <int KIND, int C, int D>
 __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restrict__ jobs,
                                                         int n_jobs) {
@@ -782,7 +791,7 @@ __global__ __launch_bounds__(1024) void k_bounce_stream(const BounceJob *__restr
   auto store_emit = [&](int buf) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m)
-      *reinterpret_cast<f32x3 *>(&s_e[buf][4 * (T * m + (int)threadIdx.x)]) = epc[m];
+      *reinterpret_cast<f32x3 *>(&s_e[buf][4 * emit_slot(T * m + (int)threadIdx.x)]) = epc[m];
   };
   // the wave's stream [a, e) of block (c, tn): its first and past-the-last
   // entries, read through the scalar cache (wave-uniform, read-only here), so
@@ -972,8 +981,9 @@ __global__ __launch_bounds__(1024) void k_block_offsets(DevXfers x, DevTiles tl,
 }
 
 // Blocked layout, pass 2: copy each owner's segment with its columns turned
-// into LDS byte offsets ostride (col - t C), bit 0 set on the segment's first
-// entry; pad with (offset 0, coef 0).  One wave per (block, owner slot).
+// into LDS byte offsets ostride (col - t C) (stream layout: 16 emit_slot(col
+// - t C)), bit 0 set on the segment's first entry; pad with (offset 0,
+// coef 0).  One wave per (block, owner slot).
 __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int G,
                                                     int coef_bytes, uint16_t *__restrict__ bidx,
                                                     void *__restrict__ bcoef) {
@@ -1012,7 +1022,9 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
     } else {  // tiled layout: bit 0 marks the segment's first entry
       bits = q == 0 ? 1u : 0u;
     }
-    const uint16_t off = (uint16_t)((real ? tl.ostride * (x.col[src + q] - p0) : 0) | (int)bits);
+    const int cp = real ? x.col[src + q] - p0 : 0;
+    const uint16_t off =
+        (uint16_t)((tl.ostride == 16 ? 16 * emit_slot(cp) : tl.ostride * cp) | (int)bits);
     if (tl.ostride == 16 && coef_bytes == 2) {
       // stream layout, 16-bit: chunk record of 4 offsets then 4 coefficients
       // (chunks start at multiples of G = 4 entries)

@@ -271,16 +271,33 @@ void row_order(const std::vector<int32_t> &cnt, int placement, std::vector<int32
     }
     return;
   }
+  // Run r (consecutive rows, r-th in index order) has the size chunk r would
+  // have.  A chunk's rows alone fix each row's sums, so the runs of one size
+  // may go to any chunk of that size: the heaviest first, since workgroups
+  // start in chunk order and the heaviest then finish soonest.
   const int64_t nc = gather_chunks(n);
-  std::vector<int32_t> run;
+  std::vector<int64_t> run_first((size_t)nc, 0), run_size((size_t)nc, 0), weight((size_t)nc, 0);
   int64_t first = 0;
-  for (int64_t c = 0; c < nc && first < n; ++c) {
-    const int64_t size = (n - c + nc - 1) / nc;
+  for (int64_t r = 0; r < nc; ++r) {
+    run_first[(size_t)r] = first;
+    run_size[(size_t)r] = (n - r + nc - 1) / nc;
+    for (int64_t k = 0; k < run_size[(size_t)r]; ++k) weight[(size_t)r] += cnt[(size_t)(first + k)];
+    first += run_size[(size_t)r];
+  }
+  std::vector<int64_t> slot((size_t)nc);
+  for (int64_t r = 0; r < nc; ++r) slot[(size_t)r] = r;
+  // chunks 0 .. big-1 hold one row more than the rest
+  const int64_t big = n % nc ? n % nc : nc;
+  auto heavier = [&](int64_t a, int64_t b) { return weight[(size_t)a] > weight[(size_t)b]; };
+  std::stable_sort(slot.begin(), slot.begin() + big, heavier);
+  std::stable_sort(slot.begin() + big, slot.end(), heavier);
+  std::vector<int32_t> run;
+  for (int64_t c = 0; c < nc; ++c) {
+    const int64_t r = slot[(size_t)c], size = run_size[(size_t)r];
     run.resize((size_t)size);
-    for (int64_t k = 0; k < size; ++k) run[(size_t)k] = (int32_t)(first + k);
+    for (int64_t k = 0; k < size; ++k) run[(size_t)k] = (int32_t)(run_first[(size_t)r] + k);
     std::stable_sort(run.begin(), run.end(), longer);
     for (int64_t k = 0; k < size; ++k) order[(size_t)(c + k * nc)] = run[(size_t)k];
-    first += size;
   }
 }
 
