@@ -983,7 +983,8 @@ __global__ __launch_bounds__(1024) void k_block_offsets(DevXfers x, DevTiles tl,
 // Blocked layout, pass 2: copy each owner's segment with its columns turned
 // into LDS byte offsets ostride (col - t C) (stream layout: 16 emit_slot(col
 // - t C)), bit 0 set on the segment's first entry; pad with (offset 0,
-// coef 0).  One wave per (block, owner slot).
+// coef 0).  One wave per (block, gather wave w): it copies the segments of
+// w's 64 owner slots one after another, each with all 64 lanes.
 __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int G,
                                                     int coef_bytes, uint16_t *__restrict__ bidx,
                                                     void *__restrict__ bcoef) {
@@ -991,57 +992,60 @@ __global__ __launch_bounds__(256) void k_block_fill(DevXfers x, DevTiles tl, int
   const int R = 1024, RS = R + 4;
   const int64_t gw = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t nblk = tl.n_chunks * (int64_t)tl.nt;
-  if (gw >= nblk * R) return;
-  const int64_t blk = gw / R;
-  const int u = (int)(gw % R), w = u / 64, j = u % 64;
+  if (gw >= nblk * 16) return;
+  const int64_t blk = gw / 16;
+  const int w = (int)(gw % 16);
   const int64_t c = blk / tl.nt;
   const int t = (int)(blk % tl.nt);
-  const int64_t oi = c + slot_rank(w, j) * tl.n_chunks;
-  if (oi >= x.n) return;
-  const int64_t row = x.order[oi];
-  const int32_t *tp = tl.tp + row * (int64_t)(tl.nt + 1);
-  const int64_t src = x.rp[row] + tp[t];
-  const int len = tp[t + 1] - tp[t];
-  const int32_t *bo = tl.boff + blk * RS;
-  const int64_t dst = tl.bstart[blk] + bo[u];
-  const int plen = bo[u + 1] - bo[u];
-  const int p0 = t * tl.cols;
-  const int ss = bo[u] - bo[64 * w];  // the segment's start in the wave's stream
-  const int wlen = bo[64 * w + 64] - bo[64 * w];
-  const int nch = plen / G;
-  for (int q = lane; q < plen; q += 64) {
-    const bool real = q < len;
-    uint32_t bits;
-    int64_t at = dst + q;
-    if (tl.ostride == 16) {  // stream layout: chunk metadata, A/B chunk order per pass
-      const int pw = 128 * G, pos = ss + q, ps = pos / pw * pw, k = q / G;
-      const int sl = ss >= ps ? ((ss - ps) / G) >> 1 : 0;
-      bits = stream_chunk_meta(q % G, k == 0, k == nch - 1, sl, j);
-      const int nc = min(128, (wlen - ps) / G);
-      at = tl.bstart[blk] + bo[64 * w] + ps + G * stream_pass_slot((pos - ps) / G, nc) + q % G;
-    } else {  // tiled layout: bit 0 marks the segment's first entry
-      bits = q == 0 ? 1u : 0u;
+  for (int j = 0; j < 64; ++j) {
+    const int u = 64 * w + j;
+    const int64_t oi = c + slot_rank(w, j) * tl.n_chunks;
+    if (oi >= x.n) continue;
+    const int64_t row = x.order[oi];
+    const int32_t *tp = tl.tp + row * (int64_t)(tl.nt + 1);
+    const int64_t src = x.rp[row] + tp[t];
+    const int len = tp[t + 1] - tp[t];
+    const int32_t *bo = tl.boff + blk * RS;
+    const int64_t dst = tl.bstart[blk] + bo[u];
+    const int plen = bo[u + 1] - bo[u];
+    const int p0 = t * tl.cols;
+    const int ss = bo[u] - bo[64 * w];  // the segment's start in the wave's stream
+    const int wlen = bo[64 * w + 64] - bo[64 * w];
+    const int nch = plen / G;
+    for (int q = lane; q < plen; q += 64) {
+      const bool real = q < len;
+      uint32_t bits;
+      int64_t at = dst + q;
+      if (tl.ostride == 16) {  // stream layout: chunk metadata, A/B chunk order per pass
+        const int pw = 128 * G, pos = ss + q, ps = pos / pw * pw, k = q / G;
+        const int sl = ss >= ps ? ((ss - ps) / G) >> 1 : 0;
+        bits = stream_chunk_meta(q % G, k == 0, k == nch - 1, sl, j);
+        const int nc = min(128, (wlen - ps) / G);
+        at = tl.bstart[blk] + bo[64 * w] + ps + G * stream_pass_slot((pos - ps) / G, nc) + q % G;
+      } else {  // tiled layout: bit 0 marks the segment's first entry
+        bits = q == 0 ? 1u : 0u;
+      }
+      const int cp = real ? x.col[src + q] - p0 : 0;
+      const uint16_t off =
+          (uint16_t)((tl.ostride == 16 ? 16 * emit_slot(cp) : tl.ostride * cp) | (int)bits);
+      if (tl.ostride == 16 && coef_bytes == 2) {
+        // stream layout, 16-bit: chunk record of 4 offsets then 4 coefficients
+        // (chunks start at multiples of G = 4 entries)
+        const int64_t rec = (at >> 2) * 8, m4 = at & 3;
+        bidx[rec + m4] = off;
+        bidx[rec + 4 + m4] = real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
+        continue;
+      }
+      // stream layout, FP32: offsets in stream order (the A and B chunks of a
+      // lane adjacent), coefficients in the A-then-B slots
+      bidx[tl.ostride == 16 ? dst + q : at] = off;
+      if (coef_bytes == 4)
+        reinterpret_cast<float *>(bcoef)[at] =
+            real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;
+      else
+        reinterpret_cast<uint16_t *>(bcoef)[at] =
+            real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
     }
-    const int cp = real ? x.col[src + q] - p0 : 0;
-    const uint16_t off =
-        (uint16_t)((tl.ostride == 16 ? 16 * emit_slot(cp) : tl.ostride * cp) | (int)bits);
-    if (tl.ostride == 16 && coef_bytes == 2) {
-      // stream layout, 16-bit: chunk record of 4 offsets then 4 coefficients
-      // (chunks start at multiples of G = 4 entries)
-      const int64_t rec = (at >> 2) * 8, m4 = at & 3;
-      bidx[rec + m4] = off;
-      bidx[rec + 4 + m4] = real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
-      continue;
-    }
-    // stream layout, FP32: offsets in stream order (the A and B chunks of a
-    // lane adjacent), coefficients in the A-then-B slots
-    bidx[tl.ostride == 16 ? dst + q : at] = off;
-    if (coef_bytes == 4)
-      reinterpret_cast<float *>(bcoef)[at] =
-          real ? reinterpret_cast<const float *>(x.coef)[src + q] : 0.0f;
-    else
-      reinterpret_cast<uint16_t *>(bcoef)[at] =
-          real ? reinterpret_cast<const uint16_t *>(x.coef)[src + q] : (uint16_t)0;
   }
 }
 
@@ -1055,7 +1059,7 @@ void launch_block_offsets(const DevXfers &x, const DevTiles &tl, int G, int64_t 
 
 void launch_block_fill(const DevXfers &x, const DevTiles &tl, int G, int coef_bytes,
                        uint16_t *bidx, void *bcoef, hipStream_t st) {
-  const int64_t waves = tl.n_chunks * (int64_t)tl.nt * 1024;
+  const int64_t waves = tl.n_chunks * (int64_t)tl.nt * 16;
   if (waves <= 0) return;
   hipLaunchKernelGGL(k_block_fill, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, x, tl, G,
                      coef_bytes, bidx, bcoef);

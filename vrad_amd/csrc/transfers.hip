@@ -267,15 +267,19 @@ __device__ __forceinline__ float form_factor(const DevPatches &p, int64_t i, int
   return ((di * dj) * p.area[j]) / ((VR_PI_F * r2) * r2);
 }
 
-// one wave per row: ascending columns from the bit matrix
-__global__ __launch_bounds__(256) void k_row_fill(const uint32_t *__restrict__ bits, int64_t np,
-                                                  int64_t words, DevPatches p,
-                                                  const int64_t *__restrict__ rp,
-                                                  const int32_t *__restrict__ cnt,
-                                                  int32_t *__restrict__ col,
-                                                  float *__restrict__ coef) {
-  const int lane = threadIdx.x & 63;
-  const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+// One wave per row, over the bit matrix row in chunks of 64 words (2048
+// columns): each lane takes a word, the set bits' columns go to LDS at their
+// rank in the chunk (ascending), and consecutive lanes then write consecutive
+// entries (col and F_ij) to the CSR, so the stores are contiguous.
+__global__ __launch_bounds__(64) void k_row_fill(const uint32_t *__restrict__ bits, int64_t np,
+                                                 int64_t words, DevPatches p,
+                                                 const int64_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ cnt,
+                                                 int32_t *__restrict__ col,
+                                                 float *__restrict__ coef) {
+  __shared__ int32_t s_col[64 * 32];
+  const int lane = (int)threadIdx.x;
+  const int64_t row = blockIdx.x;
   if (row >= np) return;
   const uint32_t *rb = bits + row * words;
   const int64_t s = rp[row];
@@ -283,23 +287,28 @@ __global__ __launch_bounds__(256) void k_row_fill(const uint32_t *__restrict__ b
   for (int64_t w0 = 0; w0 < words; w0 += 64) {
     const int64_t w = w0 + lane;
     uint32_t m = w < words ? rb[w] : 0u;
-    int pc = __popc(m);
+    const int pc = __popc(m);
     // inclusive wave scan of pc
     int inc = pc;
     for (int off = 1; off < 64; off <<= 1) {
-      int v = __shfl_up(inc, off);
+      const int v = __shfl_up(inc, off);
       if (lane >= off) inc += v;
     }
     const int tot = __shfl(inc, 63);
-    int64_t pos = s + run + (inc - pc);
+    if (tot == 0) continue;
+    int at = inc - pc;
     while (m) {
       const int bit = __ffs(m) - 1;
       m &= m - 1u;
-      const int64_t j = w * 32 + bit;
-      col[pos] = (int32_t)j;
-      coef[pos] = form_factor(p, row, j);
-      ++pos;
+      s_col[at++] = (int32_t)(w * 32 + bit);
     }
+    __syncthreads();  // one wave: the chunk's columns are in LDS before they are read
+    for (int k = lane; k < tot; k += 64) {
+      const int32_t j = s_col[k];
+      col[s + run + k] = j;
+      coef[s + run + k] = form_factor(p, row, j);
+    }
+    __syncthreads();  // and read before the next chunk overwrites them
     run += tot;
   }
   const int64_t c = cnt[row], e = rp[row + 1];
@@ -313,8 +322,8 @@ void launch_row_fill(const uint32_t *bits, int64_t np, int64_t words, const DevP
                      const int64_t *rp, const int32_t *cnt, int32_t *col, float *coef,
                      hipStream_t st) {
   if (np <= 0) return;
-  hipLaunchKernelGGL(k_row_fill, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, bits, np, words,
-                     p, rp, cnt, col, coef);
+  hipLaunchKernelGGL(k_row_fill, dim3((unsigned)np), dim3(64), 0, st, bits, np, words, p, rp, cnt,
+                     col, coef);
 }
 
 // FP64 row sum in wave order (lane owns p with (p/4)%64 == lane), butterfly
