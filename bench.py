@@ -411,12 +411,20 @@ def run_cpu(v, args):
     threads = min(16, os.cpu_count() or 1)
     tri = oracle.precompute(s.verts9, s.ids)
     kd = oracle.KdTree(s.verts9)
-    passes, cpu_s = 0, 0.0
-    while passes < 512 and (passes == 0 or cpu_s < args.cpu_seconds):
+    passes, cpu_s, per = 0, 0.0, []
+    while passes < 512 and (passes < 5 or cpu_s < args.cpu_seconds):
         c0 = time.perf_counter()
         cv, _ = oracle.testline(kd, tri, st, en, threads=threads)
-        cpu_s += time.perf_counter() - c0
+        per.append(time.perf_counter() - c0)
+        cpu_s += per[-1]
         passes += 1
+    # one thread, on the batch's first 1 M segments (at least 5 passes)
+    m1 = min(n, 1 << 20)
+    one = []
+    while len(one) < 5 or sum(one) < 2.0:
+        c0 = time.perf_counter()
+        oracle.testline(kd, tri, st[:m1], en[:m1], threads=1)
+        one.append(time.perf_counter() - c0)
     # the traversal's work on this batch (closest-hit rule, untimed pass)
     wv, visits, tests = oracle.testline_stats(kd, tri, st, en, threads=threads)
     v["work"] = {"kd_node_visits_per_segment": round(visits / n, 2),
@@ -424,10 +432,20 @@ def run_cpu(v, args):
                  "gpu_giga_node_visits_per_s": round(visits / (v["kern_ms"] * 1e-3) / 1e9, 2),
                  "counted_by": "oracle kd traversal (closest-hit order)",
                  "visible_matches": bool(np.array_equal(wv, cv))}
-    return {"value": round(n * passes / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": f"the {n}-segment batch x {passes} passes, kd-tree TestLine "
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                         "")
+    except OSError:
+        pass
+    return {"value": round(n / float(np.median(per)) / 1e6, 3), "unit": "Mrays/s",
+            "cores": threads, "kind": "port",
+            "sample": f"the {n}-segment batch x {passes} passes (median pass), kd-tree TestLine "
                       f"(oracle/oracle.c), {cpu_s:.1f} s",
+            "value_1_thread": round(m1 / float(np.median(one)) / 1e6, 3),
+            "sample_1_thread": f"the batch's first {m1} segments x {len(one)} passes (median)",
+            "nproc": os.cpu_count(), "cpu_model": model,
             "bitexact_vs_gpu": bool(np.array_equal(cv, v["vis"]))}
 
 
