@@ -673,7 +673,7 @@ __device__ __forceinline__ void gather_stream(const CH &ch, const char *te, floa
   for (int m = 0; m < 4; ++m) {
     const uint32_t d = m < 2 ? ch.o.x : ch.o.y;
     const uint32_t off = (m & 1) ? (d >> 16) & 0xfff0u : d & 0xfff0u;
-    const f32x3 e = *reinterpret_cast<const f32x3 *>(te + off);
+    const f32x4 e = *reinterpret_cast<const f32x4 *>(te + off);
     const float w = ch.coef(m);
     r = __builtin_fmaf(w, e.x, r);
     g = __builtin_fmaf(w, e.y, g);
