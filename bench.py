@@ -50,8 +50,8 @@ SEG_BYTES = 26  # 24 B segment in + 1 B visible + 1 B sky out
 LEVEL = dict(seed=2, rooms_x=8, rooms_y=8, chop=32.0, tess=64.0, n_lights=16, sun=True)
 # row placement of the gather in the level bake (rows clustered by room)
 LEVEL_BAKE_PLACEMENT = 2
-# the precision form's scene: ~1.5 M triangles, ~250k patches
-C3_LEVEL = dict(seed=3, rooms_x=16, rooms_y=16, chop=72.0, tess=40.0, n_lights=16, sun=True)
+# the largest generated level: ~1.55 M triangles, ~264k patches
+LEVEL_16X16 = dict(seed=3, rooms_x=16, rooms_y=16, chop=72.0, tess=40.0, n_lights=16, sun=True)
 
 
 def progress(msg):
@@ -358,15 +358,15 @@ def run_visibility(cx, args):
                 info=info, build_s=build_s, extra=extra)
 
 
-def run_c3_scene(cx, args):
-    """Visibility on the precision form's scene at its stated size: the level
-    generator at ~1.55 M triangles / ~264k patches.  Scene build with the tree
+def run_level16x16(cx, args):
+    """Visibility on the largest generated level: 16 x 16 rooms,
+    ~1.55 M triangles / ~264k patches.  Scene build with the tree
     on the GPU and on the host (same tree), then a 4 M-segment MakeTransfers
     batch in row order and 4 M closest-hit rays, each at its autotuned shape."""
-    progress("C3 scene: build")
+    progress("16x16-room level: build")
     from vrad_amd import autotune, scenes
     from vrad_amd.raytracer import Environment
-    s = scenes.level(**C3_LEVEL)
+    s = scenes.level(**LEVEL_16X16)
     env = Environment()
     env.AddTriangles(s.verts9, s.ids)
     t0 = time.perf_counter()
@@ -374,14 +374,14 @@ def run_c3_scene(cx, args):
     build_ms = (time.perf_counter() - t0) * 1e3
     info = env.info()
     kd = kd_build_compare(s, env)
-    out = {"generator": dict(C3_LEVEL), "triangles": int(info.n_tri),
+    out = {"generator": dict(LEVEL_16X16), "triangles": int(info.n_tri),
            "patches": int(len(s.p_area)), "kd_nodes": int(info.n_nodes),
            "kd_leaf_refs": int(info.n_leaf_refs), "scene_build_ms": round(build_ms, 1),
            "kd_build": kd}
-    progress("C3 scene: TestLine batch")
+    progress("16x16-room level: TestLine batch")
     st, en = row_batch(s, args.segments, i0=len(s.p_area) // 2)
     b = Batch(cx, env, st, en)
-    rec = autotune.Autotuner(reps=3).tune_visibility(b.step, key=f"c3:{b.n}",
+    rec = autotune.Autotuner(reps=3).tune_visibility(b.step, key=f"level16x16:{b.n}",
                                                      stacks=(6, 8, 10, 12), rays=(1,))
     _, ms = b.run(args.steps, args.warmup)
     out["test_line"] = {"segments": b.n, "Mrays": round(b.n / (ms * 1e-3) / 1e6, 1),
@@ -389,10 +389,10 @@ def run_c3_scene(cx, args):
                         "visible_fraction": round(float(b.vis.float().mean().item()), 4),
                         "note": "MakeTransfers rows from the middle patch on, facing cull"}
     del b
-    progress("C3 scene: closest-hit rays")
+    progress("16x16-room level: closest-hit rays")
     ro, rd = hemisphere_rays(s, args.segments, seed=33)
     rb = RayBatch(cx, env, ro, rd)
-    rec = autotune.Autotuner(reps=3).tune_visibility(rb.step, key=f"c3-closest:{rb.n}",
+    rec = autotune.Autotuner(reps=3).tune_visibility(rb.step, key=f"level16x16-closest:{rb.n}",
                                                      stacks=(6, 8, 10, 12), rays=(1,),
                                                      family="trace_closest")
     _, ms = rb.run(args.steps, args.warmup)
@@ -944,7 +944,7 @@ def run_luxel_stage(cx, args):
 
 
 def batching_scenes(n_scenes, seed=64):
-    """The batching form's scenes: triangle and patch targets drawn
+    """Scenes for the many-scene bake: triangle and patch targets drawn
     log-uniformly from [2k, 200k] and [1k, 60k], 1-16 lights, sun on half.
     Rooms follow the patch target (about 3000 patches per room at chop 32),
     then the wall tessellation is solved for the triangle target and the chop
@@ -970,7 +970,7 @@ def batching_scenes(n_scenes, seed=64):
 
 
 def run_batch_bake(cx, args):
-    """The batching form: 64 procedural scenes of uneven size baked together.
+    """Many-scene bake: 64 procedural scenes of uneven size baked together.
     Visibility of every scene's MakeTransfers candidate pairs shares one
     launch per round (vr_make_transfers_batch), direct light of every scene
     is one launch (vr_gather_sample_light_batch), and every scene's FP32
@@ -1083,9 +1083,9 @@ def run_batch_bake(cx, args):
 
 
 def run_serving(cx, args):
-    """The serving form: resident scenes from the precision-form generator
-    (mixed sizes, synthetic lognormal transfers, FP16 with per-row scale), a
-    seeded Poisson stream of re-bake requests (1-4 lights moved by
+    """Serving: resident scenes from the level generator (mixed sizes,
+    synthetic lognormal transfers, FP16 with per-row scale), a seeded
+    Poisson stream of re-bake requests (1-4 lights moved by
     U[-64,64]^3, direct re-gather + 8 bounces); p50/p99 latency (completion -
     arrival) and throughput of the batched server and of the plain sequential
     server on the same arrival trace; a sample of requests re-run alone must
@@ -1266,7 +1266,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
     ap.add_argument("--no-pvs", action="store_true", help="skip the PVS-culled MakeTransfers")
     ap.add_argument("--no-luxels", action="store_true", help="skip the luxel stage")
-    ap.add_argument("--no-c3", action="store_true", help="skip the 1.5M-triangle scene")
+    ap.add_argument("--no-big-level", action="store_true",
+                    help="skip the 16x16-room level (1.55 M triangles)")
     ap.add_argument("--no-level", action="store_true",
                     help="skip the 8x8-room level bake end to end")
     ap.add_argument("--batch-scenes", type=int, default=64)
@@ -1293,7 +1294,7 @@ def main():
     cx = Ctx()
     v = None if args.no_trace else run_visibility(cx, args)
     cpu = None if (v is None or args.no_cpu) else run_cpu(v, args)
-    c3 = None if (args.no_trace or args.no_c3) else run_c3_scene(cx, args)
+    big = None if (args.no_trace or args.no_big_level) else run_level16x16(cx, args)
     bounce = None if args.no_bounce else run_bounce(cx, args)
     room = None if args.no_room else run_room_bake(cx, args)
     batch = None if args.no_batch else run_batch_bake(cx, args)
@@ -1319,7 +1320,7 @@ def main():
                                "4M MakeTransfers TestLine segments per batch (row order)"},
         "roofline": None, "cpu_baseline": cpu, "visibility": None, "bounce": bounce,
         "room_bake": room, "batch_bake": batch, "serving": serving, "pvs_culling": pvs_out,
-        "level_bake": level, "c3_scene": c3,
+        "level_bake": level, "level_16x16": big,
         "luxel_stage": lux,
     }
     if v is not None:

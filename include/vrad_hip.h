@@ -230,13 +230,14 @@ typedef struct {
  * get F_ij = cos_i cos_j A_j / (pi r^2) and the CSR stays resident. */
 int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t n_patches,
                       const vr_xfer_params *params, vr_xfers **out);
-/* MakeTransfers for many scenes at once (the batching form; no reference
- * counterpart): scene k's patches[k] (n_patches[k] of them) give out[k],
+/* MakeTransfers for many scenes at once (no reference counterpart): scene
+ * k's patches[k] (n_patches[k] of them) give out[k],
  * equal bit for bit to vr_make_transfers(scenes[k], ...).  Every round, each
  * unfinished scene culls its next candidate rows into its own region of one
  * pair buffer (params->batch_pairs pairs per round, default 32M, shared
  * among the scenes still culling, at least a row each), and one visibility
- * launch walks all regions through a table of scene descriptors.  The plain path runs the scenes one by one.
+ * launch walks all regions through a table of scene descriptors.  The plain
+ * path runs the scenes one by one.
  * On failure no out[k] is set. */
 int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
                             const int64_t *n_patches, int32_t n_scenes,
@@ -291,8 +292,8 @@ int vr_gather_sample_light(vr_scene *s, const vr_samples *smp, int64_t n,
                            const vr_light *lights, int32_t n_lights,
                            float normal_offset, int32_t skip_id, float *rgb);
 
-/* GatherSampleLight for many scenes in one launch (the batching form; no
- * reference counterpart): scene k's n[k] samples with its lights[k]
+/* GatherSampleLight for many scenes in one launch (no reference
+ * counterpart): scene k's n[k] samples with its lights[k]
  * (n_lights[k]) write rgb[k] (3 n[k]); per scene equal bit for bit to
  * vr_gather_sample_light.  The plain path runs the scenes one by one. */
 int vr_gather_sample_light_batch(vr_scene *const *scenes, const vr_samples *samples,
@@ -344,9 +345,9 @@ int vr_bounce_light_noalloc(vr_xfers *x, const float *emit0, const float *refl,
                             const vr_bounce_params *params, double *energy,
                             int32_t *n_bounces);
 
-/* Batched BounceLight (no reference counterpart; the batching form of the
- * bake): every job is an independent transfer set with its own host arrays;
- * one launch per bounce covers all jobs' rows (the LDS-tiled gather), each
+/* Batched BounceLight (no reference counterpart): every job is an
+ * independent transfer set with its own host arrays; one launch per bounce
+ * covers all jobs' rows (the stream or the LDS-tiled gather), each
  * job keeps its own FP64 energy and stop test on the device.  Jobs share the
  * coefficient precision.  use_graph != 0 captures 8 bounces as one hipGraph
  * and replays it until every job has stopped.  Per job the results equal
@@ -362,7 +363,7 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs,
                           const vr_bounce_params *params, int32_t use_graph);
 
 /* ---- serving: re-bake requests against resident scenes ----------------- */
-/* No reference counterpart (the serving form of the bake).  A server keeps
+/* No reference counterpart.  A server keeps
  * each scene's tree, patches, transfers and the transfers' tile-blocked copy
  * resident in HBM and runs re-bake requests: the scene's lights with 1..k of
  * them moved, direct light re-gathered, then `bounces` bounces from it.
@@ -372,7 +373,7 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs,
  * working memory (carved from an hbm_budget arena) are free; every iteration
  * runs one bounce of every admitted request in one batched launch, so
  * requests that arrive while others run join at the next bounce.  The
- * scenes share one coefficient precision (FP16 for the serving form).
+ * scenes share one coefficient precision (FP16 in the serving bench).
  * VR_SERVE_PLAIN: the comparison path, one request at a time in arrival
  * order with the plain kernels (scratch-stack gather, one thread per CSR row
  * on the FP32 coefficients, a host read of the energy after every bounce).

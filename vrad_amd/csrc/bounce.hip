@@ -1308,7 +1308,7 @@ void launch_emit_init(int64_t n, const float *direct, const float *refl, float *
                      refl, emit, total);
 }
 
-// ---- synthetic transfers (config of the precision form) -------------------
+// ---- synthetic transfers (lognormal row lengths, uniform columns) ---------
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;

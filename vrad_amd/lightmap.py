@@ -267,8 +267,9 @@ def BounceLightBatch(jobs, max_bounces=100, stop_frac=1e-3, use_graph=True):
         energy = np.zeros(max(max_bounces, 1), np.float64)
         keep.append((e0, rf, ar, tot, energy))
         h = xf.handle
-        arr[i] = _lib.BounceJob(h.value if isinstance(h, C.c_void_p) else h, _lib.fptr(e0), _lib.fptr(rf), _lib.fptr(ar),
-                                _lib.fptr(tot), energy.ctypes.data_as(C.POINTER(C.c_double)), 0)
+        arr[i] = _lib.BounceJob(h.value if isinstance(h, C.c_void_p) else h, _lib.fptr(e0),
+                                _lib.fptr(rf), _lib.fptr(ar), _lib.fptr(tot),
+                                energy.ctypes.data_as(C.POINTER(C.c_double)), 0)
     prm = _lib.BounceParams(int(max_bounces), float(stop_frac))
     check(lib.vr_bounce_light_batch(arr, len(jobs), C.byref(prm), 1 if use_graph else 0))
     return [(k[3], k[4][:arr[i].n_bounces], arr[i].n_bounces) for i, k in enumerate(keep)]
