@@ -1,9 +1,10 @@
 """Re-bake serving against resident scenes (no reference counterpart).  A
 Server keeps each scene's kd-tree, patches and transfers on the device; a
 request moves some of the scene's lights, re-gathers the direct light and
-runs a fixed number of bounces.  The batched mode admits requests by free slots and device memory and runs one bounce of
-every admitted request per launch (continuous batching); the plain mode runs
-one request at a time on the plain kernels."""
+runs a fixed number of bounces.  The batched mode admits requests by free
+slots and device memory and runs one bounce of every admitted request per
+launch (continuous batching); the plain mode runs one request at a time on
+the plain kernels."""
 import ctypes as C
 
 import numpy as np
