@@ -214,7 +214,7 @@ typedef struct {
   float normal_offset;  /* segment end offset along each normal (0.1) */
   int32_t skip_id;
   int32_t normalize;    /* scale rows with FP64 sum > 1 by 1/sum */
-  int64_t batch_pairs;  /* candidate pairs per visibility batch (<=0: 8M) */
+  int64_t batch_pairs;  /* candidate pairs per visibility batch (<=0: 32M) */
 } vr_xfer_params;
 typedef struct vr_xfers vr_xfers;
 typedef struct {

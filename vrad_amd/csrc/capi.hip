@@ -946,10 +946,11 @@ static int make_transfers_impl(vr_scene *s, const vr_patches *p, int64_t np,
   TRY(ensure_init());
   vr_xfer_params P{0.1f, -1, 1, 0};
   if (prm) P = *prm;
-  // a batch holds one row of the unculled enumeration, or one block of the
-  // PVS enumeration
+  // a batch holds at most every pair of the scene, and at least one row of
+  // the unculled enumeration or one block of the PVS one
+  const int64_t all_pairs = np * (np - 1) / 2;
   const int64_t cap = std::max<int64_t>(
-      P.batch_pairs > 0 ? P.batch_pairs : (int64_t)8 << 20,
+      std::min<int64_t>(P.batch_pairs > 0 ? P.batch_pairs : (int64_t)32 << 20, all_pairs),
       std::max<int64_t>(np - 1, nc > 0 ? (int64_t)vr::pvs_block_cols() : 1));
   if (cap >= (1ll << 31)) return fail(VR_E_INVALID, "batch_pairs too large");
   hipStream_t st = g_stream;
