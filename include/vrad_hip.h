@@ -236,8 +236,10 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t n_patches,
  * unfinished scene culls its next candidate rows into its own region of one
  * pair buffer (params->batch_pairs pairs per round, default 32M, shared
  * among the scenes still culling, at least a row each), and one visibility
- * launch walks all regions through a table of scene descriptors.  The plain
- * path runs the scenes one by one.
+ * launch walks all regions through a table of scene descriptors.  A scene
+ * with params->batch_pairs candidate pairs or more fills a round alone and
+ * runs as vr_make_transfers would.  The plain path runs the scenes one by
+ * one.
  * On failure no out[k] is set. */
 int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
                             const int64_t *n_patches, int32_t n_scenes,

@@ -39,8 +39,10 @@ def _same_csr(a, b):
             and np.array_equal(wa.view(np.uint32), wb.view(np.uint32)))
 
 
-@pytest.mark.parametrize("batch_pairs", [0, 40000], ids=["default", "many-rounds"])
+@pytest.mark.parametrize("batch_pairs", [0, 500000, 40000], ids=["default", "split", "many-rounds"])
 def test_make_transfers_batch_equals_single(mixed, batch_pairs):
+    """Scenes with batch_pairs candidate pairs or more run on their own, the
+    rest share launches: 0 (32M) shares all, 40000 runs most on their own."""
     print(f"[gpu] MakeTransfersBatch {len(mixed)} scenes batch_pairs={batch_pairs}", flush=True)
     items = [(env, s.p_origin, s.p_normal, s.p_area) for s, env in mixed]
     # an empty patch set and a single patch ride along
@@ -48,6 +50,11 @@ def test_make_transfers_batch_equals_single(mixed, batch_pairs):
     items.insert(2, (env0, s0.p_origin[:0], s0.p_normal[:0], s0.p_area[:0]))
     items.insert(4, (env0, s0.p_origin[:1], s0.p_normal[:1], s0.p_area[:1]))
     got = MakeTransfersBatch(items, batch_pairs=batch_pairs)
+    n = np.array([len(a) for _, _, _, a in items], np.int64)
+    solo = n * (n - 1) // 2 >= (batch_pairs or 32 << 20)
+    print(f"[gpu] {int(solo.sum())} of {len(items)} scenes on their own", flush=True)
+    if batch_pairs == 500000:
+        assert 0 < solo.sum() < len(items) - 2
     for k, (env, o, nrm, a) in enumerate(items):
         one = MakeTransfers(env, o, nrm, a, batch_pairs=batch_pairs)
         assert _same_csr(got[k], one), k

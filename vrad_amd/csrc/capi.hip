@@ -1031,6 +1031,10 @@ int vr_make_transfers(vr_scene *s, const vr_patches *p, int64_t np, const vr_xfe
   return make_transfers_impl(s, p, np, prm, nullptr, nullptr, 0, out);
 }
 
+static int make_transfers_shared(vr_scene *const *scenes, const vr_patches *patches,
+                                 const int64_t *n_patches, const std::vector<int32_t> &which,
+                                 const vr_xfer_params &P, int64_t budget, vr_xfers **out);
+
 int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
                             const int64_t *n_patches, int32_t n_scenes,
                             const vr_xfer_params *prm, vr_xfers **out) {
@@ -1067,13 +1071,41 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
   TRY(ensure_init());
   vr_xfer_params P{0.1f, -1, 1, 0};
   if (prm) P = *prm;
+  const int64_t budget = P.batch_pairs > 0 ? P.batch_pairs : (int64_t)32 << 20;
+  // A scene with a whole round's budget of candidate pairs fills the GPU on
+  // its own: it takes the single-scene path (the same result), one launch
+  // per round of its own, and the shared launches serve the smaller scenes.
+  std::vector<int32_t> shared;
+  int rc = VR_OK;
+  for (int32_t k = 0; k < n_scenes && rc == VR_OK; ++k) {
+    const int64_t np = n_patches[k];
+    if (np * (np - 1) / 2 >= budget)
+      rc = make_transfers_impl(scenes[k], &patches[k], np, &P, nullptr, nullptr, 0, &out[k]);
+    else
+      shared.push_back(k);
+  }
+  if (rc == VR_OK) rc = make_transfers_shared(scenes, patches, n_patches, shared, P, budget, out);
+  if (rc != VR_OK) {
+    for (int32_t k = 0; k < n_scenes; ++k) {
+      delete out[k];
+      out[k] = nullptr;
+    }
+  }
+  return rc;
+}
+
+// vr_make_transfers_batch's shared launches for the scenes in `which`.
+static int make_transfers_shared(vr_scene *const *scenes, const vr_patches *patches,
+                                 const int64_t *n_patches, const std::vector<int32_t> &which,
+                                 const vr_xfer_params &P, int64_t budget, vr_xfers **out) {
   hipStream_t st = g_stream;
+  const int32_t n_scenes = (int32_t)which.size();
+  if (n_scenes == 0) return VR_OK;
   // every round, each unfinished scene culls its next rows into its own
   // region of one pair buffer, and the regions' pairs are traced in one
   // launch.  The round's budget is shared among the scenes still culling
   // (at least one whole row each), so the rounds do not outlast the small
   // scenes with small shares.
-  const int64_t budget = P.batch_pairs > 0 ? P.batch_pairs : (int64_t)32 << 20;
   struct One {
     std::unique_ptr<vr_xfers> x;
     Staging S;
@@ -1085,8 +1117,8 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
   int64_t total_cap = budget;  // a round's shares plus one row per scene
   for (int32_t k = 0; k < n_scenes; ++k) {
     One &o = sc[(size_t)k];
-    const vr_patches *p = &patches[k];
-    o.np = n_patches[k];
+    const vr_patches *p = &patches[which[(size_t)k]];
+    o.np = n_patches[which[(size_t)k]];
     o.x.reset(new vr_xfers);
     o.x->n = o.np;
     const size_t fb = 4 * (size_t)std::max<int64_t>(o.np, 1);
@@ -1146,7 +1178,7 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
         if (c <= 0) continue;
         o.x->segments += c;
         TraceJob j;
-        j.sc = scenes[k]->dev;
+        j.sc = scenes[which[(size_t)k]]->dev;
         j.p = o.dp;
         j.pairs = pairs.as<int2>() + o.off;
         j.bits = o.bits.as<uint32_t>();
@@ -1172,7 +1204,7 @@ int vr_make_transfers_batch(vr_scene *const *scenes, const vr_patches *patches,
       o.bits.release();
     }
   }
-  for (int32_t k = 0; k < n_scenes; ++k) out[k] = sc[(size_t)k].x.release();
+  for (int32_t k = 0; k < n_scenes; ++k) out[which[(size_t)k]] = sc[(size_t)k].x.release();
   return VR_OK;
 }
 
