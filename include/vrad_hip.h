@@ -83,6 +83,11 @@ int vr_set_path(int path);
  *                        TestLine above 1 a lane's consecutive segments are
  *                        first tested against the triangle that blocked the
  *                        one before (results unchanged)
+ *   "trace_xcd_group"    TestLine, closest hit and the MakeTransfers pair
+ *                        trace deal their workgroups to the 8 XCDs in
+ *                        groups of this many consecutive 64-ray chunks,
+ *                        round-robin (default 256; 0 = one contiguous range
+ *                        per XCD)
  *   "bounce_waves"       persistent waves of the wave-per-row gather
  *   "kd_build"           kd-tree construction in vr_scene_build: 0 = host
  *                        threads, 1 = GPU, 2 (default) = GPU for a tree

@@ -189,3 +189,31 @@ def test_test_line_rows_with_sky_vs_oracle(path):
           flush=True)
     assert 0.0 < vo.mean() < 1.0 and so.any()
     assert np.array_equal(vg, vo) and np.array_equal(sg, so)
+
+
+@pytest.mark.parametrize("xg", [0, 16, 256, 1000])
+def test_xcd_group_dealing_keeps_results(xg):
+    """trace_xcd_group deals the visibility kernels' workgroups to the XCDs in
+    groups of xg (0: one contiguous range per XCD).  300k segments and rays
+    give whole rounds of 8 groups and a tail; every mapping equals the oracle."""
+    s = scenes.level(seed=2, rooms_x=4, rooms_y=4, chop=32.0, tess=64.0, n_lights=4)
+    env = _env(s.verts9, s.ids)
+    tri = oracle.precompute(s.verts9, s.ids)
+    kd = oracle.KdTree(s.verts9)
+    mn, mx = s.verts9.reshape(-1, 3).min(0), s.verts9.reshape(-1, 3).max(0)
+    rng = np.random.default_rng(77 + xg)
+    o, d = H.random_rays(rng, 300000, mn, mx)
+    a = o + d * 400.0
+    old = autotune.get_knob("trace_xcd_group")
+    print(f"[gpu] trace_xcd_group={xg}", flush=True)
+    try:
+        _lib.check(_lib.lib.vr_set_tuning(b"trace_xcd_group", xg))
+        vis, sky = env.TestLines(o, a)
+        hg, tg = env.TraceRays(o, d, 0.0, 3000.0)
+    finally:
+        _lib.check(_lib.lib.vr_set_tuning(b"trace_xcd_group", old))
+    vo, so = oracle.testline(kd, tri, o, a, threads=8)
+    assert np.array_equal(vis, vo) and np.array_equal(sky, so)
+    hk, tk = oracle.trace_kd(kd, tri, o, d, 0.0, 3000.0, threads=8)
+    assert np.array_equal(hg, hk) and np.array_equal(_bits(tg), _bits(tk))
+    assert 0.05 < vis.mean() < 0.95

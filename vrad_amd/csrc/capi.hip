@@ -449,6 +449,9 @@ int vr_set_tuning(const char *name, int value) {
     if (value != 1 && value != 2 && value != 4 && value != 8)
       return fail(VR_E_INVALID, "trace_rays_per_lane must be 1, 2, 4 or 8");
     tuning().trace_rays_per_lane = value;
+  } else if (k == "trace_xcd_group") {
+    if (value < 0 || value > 65536) return fail(VR_E_INVALID, "trace_xcd_group must be 0..65536");
+    tuning().trace_xcd_group = value;
   } else if (k == "bounce_waves") {
     if (value < 1 || value > (1 << 20)) return fail(VR_E_INVALID, "bounce_waves out of range");
     tuning().bounce_waves = value;
@@ -499,6 +502,7 @@ int vr_get_tuning(const char *name, int *value) {
   const std::string k(name);
   if (k == "trace_lds_entries") *value = tuning().trace_lds_entries;
   else if (k == "trace_rays_per_lane") *value = tuning().trace_rays_per_lane;
+  else if (k == "trace_xcd_group") *value = tuning().trace_xcd_group;
   else if (k == "bounce_waves") *value = tuning().bounce_waves;
   else if (k == "bounce_kernel") *value = tuning().bounce_kernel;
   else if (k == "bounce_tile_cols") *value = tuning().bounce_tile_cols;

@@ -16,6 +16,8 @@ struct Tuning {
   int trace_lds_entries = 10;  // traversal-stack entries per lane kept in LDS: 2/4/6/8/10/12/16/24
   int trace_rays_per_lane = 1; // rays per lane of the visibility kernels (1, 2, 4, 8): a wave
                                // covers 64 x this many rays
+  int trace_xcd_group = 256;   // TestLine, closest hit, MakeTransfers: workgroups dealt to the
+                               // XCDs in groups of this many (0 = one contiguous range each)
   int bounce_waves = 8192;     // persistent waves of the wave-per-row gather
   // Bounce gather kernel and its shape.  Unlike the knobs above these fix the
   // FP32 summation order (the CPU restatement follows the same settings).

@@ -105,7 +105,8 @@ __global__ __launch_bounds__(64) void k_gather_lds(DevScene sc, DevSamples s, in
                                                    float eps, int32_t skip,
                                                    float *__restrict__ rgb) {
   __shared__ uint2 s_stk[S * 64];
-  const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
+  const int64_t i =
+      (int64_t)xcd_group_chunk(blockIdx.x, gridDim.x, kXcdGroup) * 64 + threadIdx.x;
   if (i >= n) return;
   StackSpill<S, D> spill;
   HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(64) void k_gather_jobs(const GatherJob *__restrict_
                                                     const float *__restrict__ dirs, float eps,
                                                     int32_t skip) {
   __shared__ uint2 s_stk[S * 64];
-  const int64_t b = (int64_t)xcd_chunk(blockIdx.x, gridDim.x);
+  const int64_t b = (int64_t)xcd_group_chunk(blockIdx.x, gridDim.x, kXcdGroup);
   int lo = 0, hi = n_jobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(64) void k_leaf_sky(DevScene sc, const int16_t *__r
                                                  const float *__restrict__ dirs,
                                                  uint8_t *__restrict__ out) {
   __shared__ uint2 s_stk[S * 64];
-  const int64_t leaf = (int64_t)xcd_chunk(blockIdx.x, gridDim.x);
+  const int64_t leaf = (int64_t)xcd_group_chunk(blockIdx.x, gridDim.x, kXcdGroup);
   if (leaf >= n) return;
   StackSpill<S, D> spill;
   HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);

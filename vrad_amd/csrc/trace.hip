@@ -15,9 +15,10 @@ template <int S, int D, int R>
 __global__ __launch_bounds__(64) void k_trace_closest_lds(DevScene sc, DevRays r, int64_t n,
                                                           int32_t skip, int32_t *__restrict__ hit,
                                                           float *__restrict__ hit_t,
-                                                          float *__restrict__ hit_n) {
+                                                          float *__restrict__ hit_n, uint32_t xg) {
   __shared__ uint2 s_stk[S * 64];
-  const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R + threadIdx.x;
+  const int64_t base =
+      (int64_t)xcd_group_chunk(blockIdx.x, gridDim.x, xg) * 64 * R + threadIdx.x;
   StackSpill<S, D> spill;
   HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
   for (int q = 0; q < R; ++q) {
@@ -70,11 +71,11 @@ __global__ __launch_bounds__(256) void k_trace_closest_plain(DevScene sc, DevRay
 template <int S, int D, int R>
 __global__ __launch_bounds__(64) void k_test_line_lds(DevScene sc, DevSegs s, int64_t n,
                                                       int32_t skip, uint8_t *__restrict__ vis,
-                                                      uint8_t *__restrict__ sky) {
+                                                      uint8_t *__restrict__ sky, uint32_t xg) {
   __shared__ uint2 s_stk[S * 64];
   __shared__ int32_t s_q[R > 1 ? 64 * (R - 1) : 1];  // queued segments (offsets from base)
   const int lane = (int)threadIdx.x;
-  const int64_t base = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * 64 * R;
+  const int64_t base = (int64_t)xcd_group_chunk(blockIdx.x, gridDim.x, xg) * 64 * R;
   StackSpill<S, D> spill;
   HybridStack<S, D> st(s_stk, lane, spill);
   auto walk = [&](int64_t i, uint32_t &occ) {
@@ -158,7 +159,7 @@ void launch_trace_closest(const DevScene &sc, const DevRays &r, int64_t n, int32
 #define X(S, RR)                                                                              \
   case S * 8 + RR:                                                                            \
     hipLaunchKernelGGL((k_trace_closest_lds<S, kStackDepth, RR>), g, dim3(64), 0, st, sc, r, n, \
-                       skip, hit, hit_t, hit_n);                                              \
+                       skip, hit, hit_t, hit_n, (uint32_t)tuning().trace_xcd_group);          \
     break;
     VR_LDS_VARIANTS(X)
 #undef X
@@ -179,7 +180,7 @@ void launch_test_line(const DevScene &sc, const DevSegs &s, int64_t n, int32_t s
 #define X(S, RR)                                                                               \
   case S * 8 + RR:                                                                             \
     hipLaunchKernelGGL((k_test_line_lds<S, kStackDepth, RR>), g, dim3(64), 0, st, sc, s, n, skip, \
-                       vis, sky);                                                              \
+                       vis, sky, (uint32_t)tuning().trace_xcd_group);                          \
     break;
     VR_LDS_VARIANTS(X)
 #undef X

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p
                                                        const uint32_t *__restrict__ count,
                                                        uint32_t cap, float eps, int32_t skip,
                                                        uint32_t *__restrict__ bits,
-                                                       int64_t words) {
+                                                       int64_t words, uint32_t xg) {
   __shared__ uint2 s_stk[S * 64];
   // The grid is sized for cap pairs; only the first ceil(n / 64) blocks have
   // work.  Blocks are dealt to the XCDs round-robin, so those blocks spread
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p
   const uint32_t n = min(*count, cap);
   const uint32_t nb = (n + 63u) / 64u;
   if (blockIdx.x >= nb) return;
-  const uint32_t k = xcd_chunk(blockIdx.x, nb) * 64 + threadIdx.x;
+  const uint32_t k = xcd_group_chunk(blockIdx.x, nb, xg) * 64 + threadIdx.x;
   if (k >= n) return;
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
@@ -141,7 +141,8 @@ void launch_pair_trace(const DevScene &sc, const DevPatches &p, const int2 *pair
     // the traversal-stack entries kept in LDS follow the autotuned knob
 #define VR_PAIR_TRACE(S)                                                                    \
   hipLaunchKernelGGL((k_pair_trace_lds<S, kStackDepth>), dim3((cap + 63) / 64), dim3(64), 0, st, \
-                     sc, p, pairs, count, cap, eps, skip, bits, words)
+                     sc, p, pairs, count, cap, eps, skip, bits, words,                          \
+                     (uint32_t)tuning().trace_xcd_group)
     switch (tuning().trace_lds_entries) {
       case 2: VR_PAIR_TRACE(2); break;
       case 4: VR_PAIR_TRACE(4); break;
@@ -165,7 +166,7 @@ template <int S, int D>
 __global__ __launch_bounds__(64) void k_pair_trace_jobs(const TraceJob *__restrict__ jobs,
                                                         int n_jobs, float eps, int32_t skip) {
   __shared__ uint2 s_stk[S * 64];
-  const int64_t b = (int64_t)xcd_chunk(blockIdx.x, gridDim.x);
+  const int64_t b = (int64_t)xcd_group_chunk(blockIdx.x, gridDim.x, kXcdGroup);
   int lo = 0, hi = n_jobs - 1;
   while (lo < hi) {  // last job with blk_begin <= b
     const int mid = (lo + hi + 1) >> 1;
