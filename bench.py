@@ -311,7 +311,7 @@ def run_visibility(cx, args):
     best_s = rec["best"]
     elapsed, kern_ms = batch.run(args.steps, args.warmup)
     vis = batch.vis.cpu().numpy()
-    # the plain path (scratch stacks, no XCD remap) and an incoherent batch
+    # the plain path (one thread per segment, scratch-memory stacks) and an incoherent batch
     check(lib.vr_set_path(L.VR_PATH_PLAIN))
     _, plain_ms = batch.run(3, 1)
     check(lib.vr_set_path(L.VR_PATH_OPT))
