@@ -40,6 +40,9 @@ std::recursive_mutex g_mu;
 bool g_init = false;
 int g_device = 0;
 hipStream_t g_stream = nullptr;
+// a second stream, on which MakeTransfers runs every other round of its
+// visibility batches (default priority, like g_stream)
+hipStream_t g_stream2 = nullptr;
 int g_path = VR_PATH_OPT;
 bool g_prof = false;
 
@@ -129,6 +132,7 @@ int ensure_init() {
     return fail(VR_E_INVALID, "no such device");
   VR_HIP(hipSetDevice(g_device));
   VR_HIP(hipStreamCreate(&g_stream));
+  VR_HIP(hipStreamCreate(&g_stream2));
   // this device's copy of the allocation-free bounce's working set
   void *sp = nullptr;
   VR_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -857,7 +861,7 @@ int vr_test_line(vr_scene *s, const vr_segs *g, int64_t n, int32_t skip_id, uint
 
 // ---- transfers ------------------------------------------------------------
 __global__ void k_add_count(const uint32_t *c, uint32_t cap, unsigned long long *tot) {
-  *tot += (unsigned long long)min(*c, cap);
+  atomicAdd(tot, (unsigned long long)min(*c, cap));  // rounds on two streams add here
 }
 
 // The candidate-pair plan of a PVS: patches grouped by cluster, the group
@@ -969,20 +973,60 @@ static int make_transfers_impl(vr_scene *s, const vr_patches *p, int64_t np,
   DevPatches dp{S.ptr<float>(0), S.ptr<float>(1), S.ptr<float>(2), S.ptr<float>(3),
                 S.ptr<float>(4), S.ptr<float>(5), S.ptr<float>(6)};
   const int64_t words = (np + 31) / 32;
-  DBuf bits, pairs, count, segtot;
+  // The optimised path alternates rounds between two streams, each with its
+  // own pair buffer and count: a round's cull and the start of its trace
+  // run beside the end of the previous round's trace (the launch's tail).
+  // Both set bits of one matrix (atomicOr) and add to one segment total.
+  const int n_sides = g_path == VR_PATH_PLAIN ? 1 : 2;
+  DBuf bits, pairs[2], count[2], segtot;
   TRY(bits.alloc((size_t)np * (size_t)words * 4));
   VR_HIP(hipMemsetAsync(bits.p, 0, bits.bytes, st));
-  TRY(pairs.alloc((size_t)cap * sizeof(int2)));
-  TRY(count.alloc(4));
+  for (int q = 0; q < n_sides; ++q) {
+    TRY(pairs[q].alloc((size_t)cap * sizeof(int2)));
+    TRY(count[q].alloc(4));
+  }
   TRY(segtot.alloc(8));
   VR_HIP(hipMemsetAsync(segtot.p, 0, 8, st));
+  hipStream_t side_st[2] = {st, g_stream2};
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;
+  if (n_sides == 2) {
+    // the second stream starts after the clears on the first, and the first
+    // goes on (CSR build, segment total) after everything on the second
+    VR_HIP(hipEventCreateWithFlags(&ev_start, hipEventDisableTiming));
+    if (hipEventCreateWithFlags(&ev_end, hipEventDisableTiming) != hipSuccess) {
+      (void)hipEventDestroy(ev_start);
+      return fail(VR_E_HIP, "hipEventCreate failed");
+    }
+  }
+  struct Events {
+    hipEvent_t a, b;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } events{ev_start, ev_end};
+  // the second stream waits for everything queued on the first so far (the
+  // clears, the patch and PVS uploads); called before the first round
+  auto start_second = [&]() -> int {
+    if (n_sides == 2) {
+      VR_HIP(hipEventRecord(ev_start, st));
+      VR_HIP(hipStreamWaitEvent(g_stream2, ev_start, 0));
+    }
+    return VR_OK;
+  };
+  int round = 0;
   {
     Prof pr(nc > 0 ? "make_transfers_pvs" : "make_transfers", st);
+    // the buffers and stream of the current round
+    auto side = [&]() { return round % n_sides; };
     auto trace_batch = [&]() {
-      launch_pair_trace(s->dev, dp, pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap,
-                        P.normal_offset, P.skip_id, bits.as<uint32_t>(), words, g_path, st);
-      hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, st, count.as<uint32_t>(),
+      const int q = side();
+      launch_pair_trace(s->dev, dp, pairs[q].as<int2>(), count[q].as<uint32_t>(),
+                        (uint32_t)cap, P.normal_offset, P.skip_id, bits.as<uint32_t>(), words,
+                        g_path, side_st[q]);
+      hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, side_st[q], count[q].as<uint32_t>(),
                          (uint32_t)cap, segtot.as<unsigned long long>());
+      ++round;
       return check_launch();
     };
     if (nc > 0) {
@@ -997,27 +1041,38 @@ static int make_transfers_impl(vr_scene *s, const vr_patches *p, int64_t np,
         TRY(pre.upload(plan.blk_pre.data(), 8 * plan.blk_pre.size(), st));
         // a block keeps at most pvs_block_cols() pairs
         const int64_t per = std::max<int64_t>(1, cap / vr::pvs_block_cols());
+        TRY(start_second());
         for (int64_t b0 = 0; b0 < n_blk; b0 += per) {
-          VR_HIP(hipMemsetAsync(count.p, 0, 4, st));
+          const int q = side();
+          VR_HIP(hipMemsetAsync(count[q].p, 0, 4, side_st[q]));
           launch_pair_cull_pvs(dp, mem.as<int32_t>(), tab.as<vr::PvsEntry>(), pre.as<int64_t>(),
                                (int32_t)plan.tab.size(), b0, std::min(per, n_blk - b0),
-                               pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap, st);
+                               pairs[q].as<int2>(), count[q].as<uint32_t>(), (uint32_t)cap,
+                               side_st[q]);
           TRY(trace_batch());
         }
-        VR_HIP(hipStreamSynchronize(st));  // mem / tab / pre are freed on return
+        // mem / tab / pre are freed on return
+        for (int q = 0; q < n_sides; ++q) VR_HIP(hipStreamSynchronize(side_st[q]));
       }
     } else {
+      TRY(start_second());
       int64_t i0 = 0;
       while (i0 + 1 < np) {
         const int64_t rowlen = np - 1 - i0;
         int64_t rows = std::max<int64_t>(1, cap / rowlen);
         const int64_t i1 = std::min<int64_t>(np - 1, i0 + rows);
         x->pairs += (2 * np - i0 - i1 - 1) * (i1 - i0) / 2;
-        VR_HIP(hipMemsetAsync(count.p, 0, 4, st));
-        launch_pair_cull(dp, np, i0, i1, pairs.as<int2>(), count.as<uint32_t>(), (uint32_t)cap, st);
+        const int q = side();
+        VR_HIP(hipMemsetAsync(count[q].p, 0, 4, side_st[q]));
+        launch_pair_cull(dp, np, i0, i1, pairs[q].as<int2>(), count[q].as<uint32_t>(),
+                         (uint32_t)cap, side_st[q]);
         TRY(trace_batch());
         i0 = i1;
       }
+    }
+    if (n_sides == 2) {
+      VR_HIP(hipEventRecord(ev_end, g_stream2));
+      VR_HIP(hipStreamWaitEvent(st, ev_end, 0));
     }
     unsigned long long segs = 0;
     VR_HIP(hipMemcpyAsync(&segs, segtot.p, 8, hipMemcpyDeviceToHost, st));
