@@ -2294,6 +2294,9 @@ int serve_plain(vr_server *sv, ServeReq *r) {
   return VR_OK;
 }
 
+// bounce launches of the batched server between two reads of the states
+constexpr int kServeBurst = 4;
+
 void serve_loop(vr_server *sv) {
   (void)hipSetDevice(g_device);
   std::vector<BounceState> hs((size_t)sv->prm.max_active);
@@ -2337,7 +2340,19 @@ void serve_loop(vr_server *sv) {
       }
       const int na = (int)sv->active.size();
       if (na > 0) {
-        // one bounce of every active request in one batched launch
+        // one bounce of every active request per batched launch, up to
+        // kServeBurst launches before the host reads the requests' states:
+        // each launch skips a request that has stopped, so a burst only
+        // saves host round trips; requests that arrive meanwhile join at
+        // the next burst
+        int most = 0;
+        for (ServeReq *r : sv->active) {
+          const bool fresh =
+              std::find(admitted.begin(), admitted.end(), r) != admitted.end();
+          const int done = fresh ? 0 : hs[(size_t)r->slot].k_done;
+          most = std::max(most, sv->prm.bounces - done);
+        }
+        const int burst = std::max(1, std::min(kServeBurst, most));
         int64_t chunks = 0, eblocks = 0;
         for (int j = 0; j < na; ++j) {
           BounceJob &jb = sv->jobs_host[j];
@@ -2350,7 +2365,7 @@ void serve_loop(vr_server *sv) {
         hipStream_t st = sv->stream;
         hipError_t e = hipMemcpyAsync(sv->jobs_dev.p, sv->jobs_host, sizeof(BounceJob) * (size_t)na,
                                       hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) {
+        for (int b = 0; b < burst && e == hipSuccess; ++b) {
           launch_bounce_jobs(sv->scenes[0]->xfers->precision, sv->shape,
                              sv->jobs_dev.as<BounceJob>(), na, chunks, eblocks, st);
           e = hipGetLastError();
@@ -2376,7 +2391,7 @@ void serve_loop(vr_server *sv) {
         }
         sv->active.swap(still);
         (void)hipStreamSynchronize(st);
-        sv->st.iterations += 1;
+        sv->st.iterations += burst;  // batched bounce launches
         sv->st.max_active_seen = std::max<int64_t>(sv->st.max_active_seen, na);
       }
     }
