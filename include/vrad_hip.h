@@ -378,8 +378,9 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs,
  * VR_SERVE_BATCHED: continuous batching.  A scheduler thread admits queued
  * requests in arrival order while a slot (max_active) and the request's
  * working memory (carved from an hbm_budget arena) are free; every iteration
- * runs one bounce of every admitted request in one batched launch, so
- * requests that arrive while others run join at the next bounce.  The
+ * runs up to four bounces of every admitted request, one batched launch per
+ * bounce, so requests that arrive while others run join at the next
+ * iteration.  The
  * scenes share one coefficient precision (FP16 in the serving bench).
  * VR_SERVE_PLAIN: the comparison path, one request at a time in arrival
  * order with the plain kernels (scratch-stack gather, one thread per CSR row
