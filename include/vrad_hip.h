@@ -378,7 +378,7 @@ int vr_bounce_light_batch(vr_bounce_job *jobs, int32_t n_jobs,
  * VR_SERVE_BATCHED: continuous batching.  A scheduler thread admits queued
  * requests in arrival order while a slot (max_active) and the request's
  * working memory (carved from an hbm_budget arena) are free; every iteration
- * runs up to four bounces of every admitted request, one batched launch per
+ * runs up to eight bounces of every admitted request, one batched launch per
  * bounce, so requests that arrive while others run join at the next
  * iteration.  The
  * scenes share one coefficient precision (FP16 in the serving bench).

@@ -2295,7 +2295,7 @@ int serve_plain(vr_server *sv, ServeReq *r) {
 }
 
 // bounce launches of the batched server between two reads of the states
-constexpr int kServeBurst = 4;
+constexpr int kServeBurst = 8;
 
 void serve_loop(vr_server *sv) {
   (void)hipSetDevice(g_device);
@@ -2341,7 +2341,8 @@ void serve_loop(vr_server *sv) {
       const int na = (int)sv->active.size();
       if (na > 0) {
         // one bounce of every active request per batched launch, up to
-        // kServeBurst launches before the host reads the requests' states:
+        // kServeBurst launches (no more than the most bounces any active
+        // request still needs) before the host reads the requests' states:
         // each launch skips a request that has stopped, so a burst only
         // saves host round trips; requests that arrive meanwhile join at
         // the next burst
