@@ -220,6 +220,12 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
     tb = fminf(tb, fmaxf(t1, t2));
   }
   if (!(ta <= tb)) return false;
+  // An any-hit walk returns at its first accepted hit, so tbest is fixed
+  // while it runs: clamping tb to it here leaves every min(tb, tbest) of
+  // the walk unchanged (an integer-order min, associative), and a stacked
+  // entry, pushed only when its start is at most that bound, never needs
+  // the pop's prune.
+  if constexpr (ANY) tb = tmin_wide(tb, tbest);
   // a split plane s on axis a bounds the child slabs at
   //   t = ((s +- dl) - o_a) i_a = s i_a + (+-dl - o_a) i_a,
   // the near child's exit with +dl toward the ray (copysign(dl, i_a)), the
@@ -237,7 +243,7 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
       float a, b;
       uint32_t n;
       st.get(sp, n, a, b);
-      if (a > tbest) continue;
+      if (!ANY && a > tbest) continue;
       node = n;
       ta = a;
       tb = b;
@@ -283,7 +289,7 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
           const uint32_t left = nd.x >> 2;
           const bool fwd = i_a >= 0.0f;
           const uint32_t near_n = fwd ? left : left + 1, far_n = fwd ? left + 1 : left;
-          const float tbe = tmin_wide(tb, tbest);
+          const float tbe = ANY ? tb : tmin_wide(tb, tbest);
           const bool near_ok = ta <= tmin_wide(near_end, tbe);
           const float fs = tmax_wide(far_start, ta);
           const bool far_ok = fs <= tbe;
@@ -332,7 +338,7 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
         const uint32_t left = nd.x >> 2;
         const bool fwd = i_a >= 0.0f;
         const uint32_t near_n = fwd ? left : left + 1, far_n = fwd ? left + 1 : left;
-        const float tbe = tmin_wide(tb, tbest);
+        const float tbe = ANY ? tb : tmin_wide(tb, tbest);
         const bool near_ok = ta <= tmin_wide(near_end, tbe);
         const float fs = tmax_wide(far_start, ta);
         const bool far_ok = fs <= tbe;
