@@ -193,18 +193,42 @@ __device__ __forceinline__ float tmax_wide(float a, float b) {
 // until every lane of the wave holds one (below).  Returns whether a hit was accepted (best_ref =
 // its leaf-reference index).  D = stack capacity; the host guarantees tree
 // depth < D.
+// What a walk needs of its ray besides origin and direction, the same for
+// every tree of a scene (the margin dl depends on the scene's extent M):
+// the reciprocal direction, and the slab terms of a split plane s on axis a,
+//   t = ((s +- dl) - o_a) i_a = s i_a + (+-dl - o_a) i_a,
+// the near child's exit with +dl toward the ray (copysign(dl, i_a)), the far
+// child's entry with -dl; the terms after s i_a are fixed per ray, and their
+// rounding (and the fma's) stays far inside the dl margin.  A segment that
+// walks the sky tree and then the full tree computes this once.
+struct RayPre {
+  float ix, iy, iz, dl, ex, ey, ez, fx, fy, fz;
+};
+__device__ __forceinline__ RayPre ray_pre(float M, float ox, float oy, float oz, float dx,
+                                          float dy, float dz) {
+  RayPre r;
+  const float om = fmaxf(fabsf(ox), fmaxf(fabsf(oy), fabsf(oz)));
+  r.dl = (M + om) * 0x1p-15f;
+  r.ix = 1.0f / (fabsf(dx) < kTiny ? copysignf(kTiny, dx) : dx);
+  r.iy = 1.0f / (fabsf(dy) < kTiny ? copysignf(kTiny, dy) : dy);
+  r.iz = 1.0f / (fabsf(dz) < kTiny ? copysignf(kTiny, dz) : dz);
+  r.ex = (copysignf(r.dl, r.ix) - ox) * r.ix;
+  r.fx = (-copysignf(r.dl, r.ix) - ox) * r.ix;
+  r.ey = (copysignf(r.dl, r.iy) - oy) * r.iy;
+  r.fy = (-copysignf(r.dl, r.iy) - oy) * r.iy;
+  r.ez = (copysignf(r.dl, r.iz) - oz) * r.iz;
+  r.fz = (-copysignf(r.dl, r.iz) - oz) * r.iz;
+  return r;
+}
+
 template <int D, bool ANY, bool SKIPSKY, bool POSTPONE, class Stack>
 __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restrict__ refs,
-                          const float *bmin, const float *bmax, float M, Stack &st, float ox,
-                          float oy, float oz, float dx, float dy, float dz, float tmin,
-                          float tmax, int32_t skip, float &tbest, int32_t &best,
+                          const float *bmin, const float *bmax, const RayPre &rp, Stack &st,
+                          float ox, float oy, float oz, float dx, float dy, float dz,
+                          float tmin, float tmax, int32_t skip, float &tbest, int32_t &best,
                           int32_t &best_id, uint32_t &best_ref) {
   const int32_t best0 = best;
-  const float om = fmaxf(fabsf(ox), fmaxf(fabsf(oy), fabsf(oz)));
-  const float dl = (M + om) * 0x1p-15f;
-  const float ix = 1.0f / (fabsf(dx) < kTiny ? copysignf(kTiny, dx) : dx);
-  const float iy = 1.0f / (fabsf(dy) < kTiny ? copysignf(kTiny, dy) : dy);
-  const float iz = 1.0f / (fabsf(dz) < kTiny ? copysignf(kTiny, dz) : dz);
+  const float dl = rp.dl, ix = rp.ix, iy = rp.iy, iz = rp.iz;
   float ta = tmin, tb = tmax;
   {
     float t1 = ((bmin[0] - dl) - ox) * ix, t2 = ((bmax[0] + dl) - ox) * ix;
@@ -226,14 +250,7 @@ __device__ bool walk_tree(const uint2 *__restrict__ nodes, const float4 *__restr
   // entry, pushed only when its start is at most that bound, never needs
   // the pop's prune.
   if constexpr (ANY) tb = tmin_wide(tb, tbest);
-  // a split plane s on axis a bounds the child slabs at
-  //   t = ((s +- dl) - o_a) i_a = s i_a + (+-dl - o_a) i_a,
-  // the near child's exit with +dl toward the ray (copysign(dl, i_a)), the
-  // far child's entry with -dl; the terms after s i_a are fixed per ray, and
-  // their rounding (and the fma's) stays far inside the dl margin
-  const float ex = (copysignf(dl, ix) - ox) * ix, fx = (-copysignf(dl, ix) - ox) * ix;
-  const float ey = (copysignf(dl, iy) - oy) * iy, fy = (-copysignf(dl, iy) - oy) * iy;
-  const float ez = (copysignf(dl, iz) - oz) * iz, fz = (-copysignf(dl, iz) - oz) * iz;
+  const float ex = rp.ex, ey = rp.ey, ez = rp.ez, fx = rp.fx, fy = rp.fy, fz = rp.fz;
   int sp = 0;
   uint32_t node = 0;
   // next stacked node whose interval can still hold a hit before tbest
@@ -382,8 +399,9 @@ __device__ void trace_ray(const DevScene &sc, Stack &st, float ox, float oy, flo
   id_out = 0;
   if (sc.empty) return;
   uint32_t ref = kNoRef;
-  if (walk_tree<D, ANY, false, !ANY>(sc.nodes, sc.refs, sc.bmin, sc.bmax, sc.M, st, ox, oy, oz,
-                                     dx, dy, dz, tmin, tmax, skip, tbest, best, best_id, ref)) {
+  const RayPre rp = ray_pre(sc.M, ox, oy, oz, dx, dy, dz);
+  if (walk_tree<D, ANY, false, !ANY>(sc.nodes, sc.refs, sc.bmin, sc.bmax, rp, st, ox, oy, oz, dx,
+                                     dy, dz, tmin, tmax, skip, tbest, best, best_id, ref)) {
     t_out = tbest;
     best_out = best;
     id_out = best_id;
@@ -418,10 +436,10 @@ __device__ __forceinline__ bool test_line_cached(const DevScene &sc, Stack &st, 
   float t = len;
   int32_t best = -1, id = 0;
   uint32_t ref = kNoRef;
+  const RayPre rp = ray_pre(sc.M, sx, sy, sz, dx, dy, dz);
   if (sc.has_sky)
-    walk_tree<D, false, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, sc.M,
-                                      st, sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id,
-                                      ref);
+    walk_tree<D, false, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, rp, st,
+                                      sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
   const int32_t sky_best = best;
   bool occluded = false;
   if (occluder != kNoRef) {
@@ -430,7 +448,7 @@ __device__ __forceinline__ bool test_line_cached(const DevScene &sc, Stack &st, 
   }
   if (!occluded) {
     ref = kNoRef;
-    occluded = walk_tree<D, true, true, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, sc.M, st, sx,
+    occluded = walk_tree<D, true, true, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, rp, st, sx,
                                                sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id,
                                                ref);
     if (occluded) occluder = ref;
@@ -464,9 +482,9 @@ __device__ __forceinline__ bool occluded_by(const DevScene &sc, Stack &st, float
   int32_t best = -1, id = 0;
   uint32_t ref = kNoRef;
   if (sc.has_sky)
-    walk_tree<D, false, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, sc.M,
-                                      st, sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id,
-                                      ref);
+    walk_tree<D, false, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax,
+                                      ray_pre(sc.M, sx, sy, sz, dx, dy, dz), st, sx, sy, sz, dx,
+                                      dy, dz, 0.0f, len, skip, t, best, id, ref);
   const int32_t sky_best = best;
   tri_test<true>(sc.refs, occluder, sx, sy, sz, dx, dy, dz, 0.0f, skip, t, best, id, ref);
   return best != sky_best;
