@@ -79,3 +79,35 @@ def test_transfers_occluder_blocks():
     env = _env(v9, ids)
     rp, col, coef = MakeTransfers(env, o, nrm, a, normal_offset=0.01).download()
     assert len(col) == 0
+
+
+@pytest.mark.parametrize("layers,visible", [
+    ("sky", True),             # the closest hit is sky: visible
+    ("sky-then-slab", True),   # a sky quad before an opaque slab: still visible
+    ("slab-then-sky", False),  # an opaque slab before the sky quad: occluded
+], ids=["sky", "sky-then-slab", "slab-then-sky"])
+def test_transfers_sky_layers_vs_oracle(layers, visible):
+    # two facing squares; between them a sky quad and/or an opaque slab.  A
+    # segment is occluded iff its closest hit is not sky, so a slab behind
+    # the sky quad does not occlude (MakeTransfers walks the full tree for
+    # any opaque hit first, then the sky tree, then the full tree again)
+    n = 4
+    pa = H.square_patches((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), n)
+    pb = H.square_patches((0, 0, 1), (1, 0, 0), (0, 1, 0), (0, 0, -1), n)
+    o, nrm, a = [np.concatenate(x) for x in zip(pa, pb)]
+    sky_z, slab = {"sky": (0.5, None), "sky-then-slab": (0.4, (0.6, 0.7)),
+                   "slab-then-sky": (0.6, (0.3, 0.4))}[layers]
+    tris = H.quad(H.SKY, (-5, -5, sky_z), (5, -5, sky_z), (5, 5, sky_z), (-5, 5, sky_z))
+    if slab is not None:
+        tris += H.box((-5, -5, slab[0]), (5, 5, slab[1]))
+    v9, ids = H.to_arrays(tris)
+    print(f"[gpu] transfers sky layers={layers}", flush=True)
+    rp, col, coef = MakeTransfers(_env(v9, ids), o, nrm, a).download()
+    tri = oracle.precompute(v9, ids)
+    kd = oracle.KdTree(v9)
+    orp, ocol, ocoef, _ = oracle.make_transfers(kd, tri, o, nrm, a, threads=4)
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    assert np.array_equal(coef.view(np.uint32), ocoef.view(np.uint32))
+    # every bottom patch sees every top patch, or none does
+    real = col[coef != 0]
+    assert (len(real) == 2 * (n * n) ** 2) if visible else (len(real) == 0)

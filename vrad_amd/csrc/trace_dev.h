@@ -464,6 +464,40 @@ __device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float s
   return test_line_cached<D>(sc, st, sx, sy, sz, ex, ey, ez, skip, sky, none);
 }
 
+// test_line's visibility without its sky output, with the walks in the
+// order that suits a visible segment: first any non-sky hit on [0, len)
+// (none: visible, and the sky tree is never walked); then the closest sky
+// hit before that one in (t, index) order (none: the closest hit is not
+// sky, occluded); then, as test_line does, any non-sky hit before that sky
+// hit.  The three steps decide what test_line decides: a sky hit found in
+// the second is the closest of all, since every sky hit it passed over lies
+// after a non-sky hit.
+template <int D, class Stack>
+__device__ __forceinline__ bool segment_visible(const DevScene &sc, Stack &st, float sx, float sy,
+                                                float sz, float ex, float ey, float ez,
+                                                int32_t skip) {
+  const float vx = ex - sx, vy = ey - sy, vz = ez - sz;
+  const float len2 = (vx * vx + vy * vy) + vz * vz;
+  const float len = __builtin_sqrtf(len2);
+  const float inv = 1.0f / len;
+  const float dx = vx * inv, dy = vy * inv, dz = vz * inv;
+  if (sc.empty) return true;
+  const RayPre rp = ray_pre(sc.M, sx, sy, sz, dx, dy, dz);
+  float t = len;
+  int32_t best = -1, id = 0;
+  uint32_t ref = kNoRef;
+  if (!walk_tree<D, true, true, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, rp, st, sx, sy, sz,
+                                       dx, dy, dz, 0.0f, len, skip, t, best, id, ref))
+    return true;
+  if (!sc.has_sky) return false;
+  const int32_t first = best;
+  walk_tree<D, false, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, rp, st,
+                                    sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
+  if (best == first) return false;
+  return !walk_tree<D, true, true, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, rp, st, sx, sy, sz,
+                                          dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
+}
+
 // The first half of test_line_cached alone: whether the segment is occluded
 // by `occluder` (a full-tree leaf reference) as test_line would decide it
 // (the closest sky hit first, then the same hit test and bound).  False

@@ -101,10 +101,10 @@ __global__ __launch_bounds__(64) void k_pair_trace_lds(DevScene sc, DevPatches p
   const int i = ij.x, j = ij.y;
   StackSpill<S, D> spill;
   HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
-  bool sky;
-  const bool vis = test_line<D>(sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
-                                p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
-                                p.oy[j] + eps * p.ny[j], p.oz[j] + eps * p.nz[j], skip, sky);
+  const bool vis = segment_visible<D>(sc, st, p.ox[i] + eps * p.nx[i],
+                                      p.oy[i] + eps * p.ny[i], p.oz[i] + eps * p.nz[i],
+                                      p.ox[j] + eps * p.nx[j], p.oy[j] + eps * p.ny[j],
+                                      p.oz[j] + eps * p.nz[j], skip);
   if (vis) set_pair_bits(bits, words, i, j);
 }
 
@@ -123,10 +123,10 @@ __global__ __launch_bounds__(256) void k_pair_trace_plain(DevScene sc, DevPatche
   const int2 ij = pairs[k];
   const int i = ij.x, j = ij.y;
   ScratchStack<D> st;
-  bool sky;
-  const bool vis = test_line<D>(sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
-                                p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
-                                p.oy[j] + eps * p.ny[j], p.oz[j] + eps * p.nz[j], skip, sky);
+  const bool vis = segment_visible<D>(sc, st, p.ox[i] + eps * p.nx[i],
+                                      p.oy[i] + eps * p.ny[i], p.oz[i] + eps * p.nz[i],
+                                      p.ox[j] + eps * p.nx[j], p.oy[j] + eps * p.ny[j],
+                                      p.oz[j] + eps * p.nz[j], skip);
   if (vis) set_pair_bits(bits, words, i, j);
 }
 
@@ -182,10 +182,10 @@ __global__ __launch_bounds__(64) void k_pair_trace_jobs(const TraceJob *__restri
   const DevPatches &p = J.p;
   StackSpill<S, D> spill;
   HybridStack<S, D> st(s_stk, (int)threadIdx.x, spill);
-  bool sky;
-  const bool vis = test_line<D>(J.sc, st, p.ox[i] + eps * p.nx[i], p.oy[i] + eps * p.ny[i],
-                                p.oz[i] + eps * p.nz[i], p.ox[j] + eps * p.nx[j],
-                                p.oy[j] + eps * p.ny[j], p.oz[j] + eps * p.nz[j], skip, sky);
+  const bool vis = segment_visible<D>(J.sc, st, p.ox[i] + eps * p.nx[i],
+                                      p.oy[i] + eps * p.ny[i], p.oz[i] + eps * p.nz[i],
+                                      p.ox[j] + eps * p.nx[j], p.oy[j] + eps * p.ny[j],
+                                      p.oz[j] + eps * p.nz[j], skip);
   if (vis) set_pair_bits(J.bits, J.words, i, j);
 }
 
