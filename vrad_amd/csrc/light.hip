@@ -16,7 +16,6 @@ __device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, fl
   const float sx = px + eps * nx, sy = py + eps * ny, sz = pz + eps * nz;
   const float MTL = VR_MAX_TRACE_LENGTH_F;
   float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-  bool sky;
   for (int32_t li = 0; li < n_lights; ++li) {
     const vr_light &L = lights[li];
     float scale = 0.0f;
@@ -31,7 +30,7 @@ __device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, fl
       if (!(dot > 0.0f)) continue;
       const float dot2 = -((L.normal[0] * ux + L.normal[1] * uy) + L.normal[2] * uz);
       if (!(dot2 > 0.0f)) continue;
-      if (!test_line<D>(sc, st, sx, sy, sz, L.origin[0], L.origin[1], L.origin[2], skip, sky))
+      if (!segment_visible<D>(sc, st, sx, sy, sz, L.origin[0], L.origin[1], L.origin[2], skip))
         continue;
       scale = (dot * dot2) / dist2;
     } else if (L.type == VR_EMIT_POINT || L.type == VR_EMIT_SPOTLIGHT) {
@@ -64,7 +63,7 @@ __device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, fl
           fall = fall * (1.0f - sm);
         }
       }
-      if (!test_line<D>(sc, st, sx, sy, sz, L.origin[0], L.origin[1], L.origin[2], skip, sky))
+      if (!segment_visible<D>(sc, st, sx, sy, sz, L.origin[0], L.origin[1], L.origin[2], skip))
         continue;
       scale = dot * fall;
     } else if (L.type == VR_EMIT_SKYLIGHT) {
@@ -72,7 +71,7 @@ __device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, fl
       if (!(dot > 0.0f)) continue;
       const float ex = sx - MTL * L.normal[0], ey = sy - MTL * L.normal[1],
                   ez = sz - MTL * L.normal[2];
-      if (!test_line<D>(sc, st, sx, sy, sz, ex, ey, ez, skip, sky)) continue;
+      if (!segment_visible<D>(sc, st, sx, sy, sz, ex, ey, ez, skip)) continue;
       scale = dot;
     } else if (L.type == VR_EMIT_SKYAMBIENT) {
       float a = 0.0f, w = 0.0f;
@@ -82,7 +81,7 @@ __device__ void gather_one(const DevScene &sc, Stack &st, float px, float py, fl
         if (!(ck > 0.0f)) continue;
         w = w + ck;
         const float ex = sx + MTL * dx, ey = sy + MTL * dy, ez = sz + MTL * dz;
-        if (test_line<D>(sc, st, sx, sy, sz, ex, ey, ez, skip, sky)) a = a + ck;
+        if (segment_visible<D>(sc, st, sx, sy, sz, ex, ey, ez, skip)) a = a + ck;
       }
       if (!(w > 0.0f)) continue;
       scale = a / w;
@@ -212,9 +211,8 @@ __global__ __launch_bounds__(64) void k_leaf_sky(DevScene sc, const int16_t *__r
     const int k = k0 + (int)threadIdx.x;
     bool vis = false;
     if (k < 162) {
-      bool sky;
-      vis = test_line<D>(sc, st, cx, cy, cz, dirs[3 * k] * nm + cx, dirs[3 * k + 1] * nm + cy,
-                         dirs[3 * k + 2] * nm + cz, -1, sky);
+      vis = segment_visible<D>(sc, st, cx, cy, cz, dirs[3 * k] * nm + cx,
+                               dirs[3 * k + 1] * nm + cy, dirs[3 * k + 2] * nm + cz, -1);
     }
     if (__any(vis)) {
       any = 1;
