@@ -87,8 +87,13 @@ __global__ __launch_bounds__(64) void k_test_line_lds(DevScene sc, DevSegs s, in
   };
   if constexpr (R == 1) {
     const int64_t i = base + lane;
-    uint32_t none = kNoRef;
-    if (i < n) walk(i, none);
+    if (i < n) {
+      bool hs;
+      const bool v = test_line<D>(sc, st, s.sx[i], s.sy[i], s.sz[i], s.ex[i], s.ey[i], s.ez[i],
+                                  skip, hs);
+      vis[i] = v ? 1 : 0;
+      if (sky) sky[i] = hs ? 1 : 0;
+    }
   } else {
     const int64_t first = base + (int64_t)lane * R;
     uint32_t occ = kNoRef;

@@ -456,12 +456,45 @@ __device__ __forceinline__ bool test_line_cached(const DevScene &sc, Stack &st, 
   sky = !occluded && sky_best >= 0;
   return !occluded;
 }
+// TestLineDoesHitSky without an occluder hint, with the walks in the order
+// that suits a visible segment: any opaque hit on [0, len) first.  Without
+// one the segment is visible, and its closest hit is sky iff the sky tree
+// has any hit on [0, len).  With one, the closest sky hit before it in
+// (t, index) order (none: occluded), then any opaque hit before that sky
+// hit (test_line_cached's second walk) decide; a sky hit found that way is
+// the closest of all, since every sky hit it passed over lies after an
+// opaque one.  Same results as test_line_cached.
 template <int D, class Stack>
 __device__ __forceinline__ bool test_line(const DevScene &sc, Stack &st, float sx, float sy,
                                           float sz, float ex, float ey, float ez, int32_t skip,
                                           bool &sky) {
-  uint32_t none = kNoRef;
-  return test_line_cached<D>(sc, st, sx, sy, sz, ex, ey, ez, skip, sky, none);
+  const float vx = ex - sx, vy = ey - sy, vz = ez - sz;
+  const float len2 = (vx * vx + vy * vy) + vz * vz;
+  const float len = __builtin_sqrtf(len2);
+  const float inv = 1.0f / len;
+  const float dx = vx * inv, dy = vy * inv, dz = vz * inv;
+  sky = false;
+  if (sc.empty) return true;
+  const RayPre rp = ray_pre(sc.M, sx, sy, sz, dx, dy, dz);
+  float t = len;
+  int32_t best = -1, id = 0;
+  uint32_t ref = kNoRef;
+  if (!walk_tree<D, true, true, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, rp, st, sx, sy, sz,
+                                       dx, dy, dz, 0.0f, len, skip, t, best, id, ref)) {
+    if (sc.has_sky)
+      sky = walk_tree<D, true, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax,
+                                             rp, st, sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t,
+                                             best, id, ref);
+    return true;
+  }
+  if (!sc.has_sky) return false;
+  const int32_t first = best;
+  walk_tree<D, false, false, false>(sc.sky_nodes, sc.sky_refs, sc.sky_bmin, sc.sky_bmax, rp, st,
+                                    sx, sy, sz, dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
+  if (best == first) return false;
+  sky = !walk_tree<D, true, true, false>(sc.nodes, sc.refs, sc.bmin, sc.bmax, rp, st, sx, sy, sz,
+                                         dx, dy, dz, 0.0f, len, skip, t, best, id, ref);
+  return sky;
 }
 
 // test_line's visibility without its sky output, with the walks in the
