@@ -305,7 +305,8 @@ def run_visibility(cx, args):
     batch = Batch(cx, env, st, en)
     # autotune traversal-stack entries in LDS x rays per wavefront on this batch
     from vrad_amd import autotune
-    rec = autotune.Autotuner(reps=3).tune_visibility(batch.step, key=f"level:{n}")
+    rec = autotune.Autotuner(reps=3, final_rounds=2).tune_visibility(batch.step,
+                                                                     key=f"level:{n}")
     sweep = {"S%d-R%d" % (r["knobs"]["trace_lds_entries"], r["knobs"]["trace_rays_per_lane"]):
              round(n / (r["ms"] * 1e-3) / 1e6, 1) for r in rec["table"]}
     best_s = rec["best"]
@@ -321,7 +322,7 @@ def run_visibility(cx, args):
     progress("visibility: PVS-culled (visible-heavy) batch")
     pst, pen = pvs_pair_batch(s, args.segments, first_room=27)
     pb = Batch(cx, env, pst, pen)
-    prec = autotune.Autotuner(reps=3).tune_visibility(
+    prec = autotune.Autotuner(reps=3, final_rounds=2).tune_visibility(
         pb.step, key=f"level-pvs:{pb.n}", stacks=(4, 6, 8, 10, 12), rays=(1,))
     _, pvs_ms = pb.run(args.steps, args.warmup)
     extra["pvs_batch"] = {
@@ -337,7 +338,7 @@ def run_visibility(cx, args):
     progress("visibility: closest-hit rays")
     ro, rd = hemisphere_rays(s, args.segments, seed=21)
     rb = RayBatch(cx, env, ro, rd)
-    crec = autotune.Autotuner(reps=3).tune_visibility(
+    crec = autotune.Autotuner(reps=3, final_rounds=2).tune_visibility(
         rb.step, key=f"level-closest:{rb.n}", stacks=(4, 6, 8, 10, 12), rays=(1,),
         family="trace_closest")
     _, cl_ms = rb.run(args.steps, args.warmup)
@@ -381,8 +382,8 @@ def run_level16x16(cx, args):
     progress("16x16-room level: TestLine batch")
     st, en = row_batch(s, args.segments, i0=len(s.p_area) // 2)
     b = Batch(cx, env, st, en)
-    rec = autotune.Autotuner(reps=3).tune_visibility(b.step, key=f"level16x16:{b.n}",
-                                                     stacks=(6, 8, 10, 12), rays=(1,))
+    rec = autotune.Autotuner(reps=3, final_rounds=2).tune_visibility(
+        b.step, key=f"level16x16:{b.n}", stacks=(6, 8, 10, 12), rays=(1,))
     _, ms = b.run(args.steps, args.warmup)
     out["test_line"] = {"segments": b.n, "Mrays": round(b.n / (ms * 1e-3) / 1e6, 1),
                         "kernel_ms": round(ms, 4), "tuned": rec["best"],
@@ -392,9 +393,9 @@ def run_level16x16(cx, args):
     progress("16x16-room level: closest-hit rays")
     ro, rd = hemisphere_rays(s, args.segments, seed=33)
     rb = RayBatch(cx, env, ro, rd)
-    rec = autotune.Autotuner(reps=3).tune_visibility(rb.step, key=f"level16x16-closest:{rb.n}",
-                                                     stacks=(6, 8, 10, 12), rays=(1,),
-                                                     family="trace_closest")
+    rec = autotune.Autotuner(reps=3, final_rounds=2).tune_visibility(
+        rb.step, key=f"level16x16-closest:{rb.n}", stacks=(6, 8, 10, 12), rays=(1,),
+        family="trace_closest")
     _, ms = rb.run(args.steps, args.warmup)
     out["closest_hit"] = {"rays": rb.n, "Mrays": round(rb.n / (ms * 1e-3) / 1e6, 1),
                           "kernel_ms": round(ms, 4), "tuned": rec["best"],

@@ -84,3 +84,35 @@ def test_bounce_sweep_of_row_placements():
     rec = tu.tune_bounce(run=None, key="q", keep_order=False, measure=measure,
                          placements=(0, 1, 2))
     assert len(seen) == 3 * len(autotune.BOUNCE_SHAPES)
+
+
+def test_final_rounds_retime_the_fastest_candidates(tmp_path):
+    # the first pass catches S10 in a slow sample and sees S6 fastest; the
+    # finalists' later samples show S10 fastest, and the choice follows each
+    # one's best time.  Without final rounds the first pass decides.
+    def scripted():
+        first = {(6, 1): 0.611, (8, 1): 0.612, (10, 1): 0.640}
+        later = {(6, 1): 0.618, (8, 1): 0.612, (10, 1): 0.605}
+        seen = {}
+
+        def measure(knobs):
+            k = (knobs["trace_lds_entries"], knobs["trace_rays_per_lane"])
+            seen[k] = seen.get(k, 0) + 1
+            return (first if seen[k] == 1 else later).get(k, 1.0)
+        return measure, seen
+
+    kn = Knobs()
+    measure, seen = scripted()
+    rec = autotune.Autotuner(setter=kn.set, final_rounds=2).tune_visibility(
+        lambda: None, key="k", measure=measure)
+    assert rec["best"] == {"trace_lds_entries": 10, "trace_rays_per_lane": 1}
+    assert kn.v["trace_lds_entries"] == 10
+    assert [f["knobs"]["trace_lds_entries"] for f in rec["final"]] == [6, 8, 10]
+    assert seen[(8, 1)] == seen[(10, 1)] == seen[(6, 1)] == 3
+    assert all(n == 1 for k, n in seen.items() if k not in ((8, 1), (10, 1), (6, 1)))
+    kn2 = Knobs()
+    measure2, _ = scripted()
+    rec2 = autotune.Autotuner(setter=kn2.set).tune_visibility(lambda: None, key="k",
+                                                               measure=measure2)
+    assert rec2["best"] == {"trace_lds_entries": 6, "trace_rays_per_lane": 1}
+    assert "final" not in rec2

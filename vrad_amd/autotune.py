@@ -85,10 +85,17 @@ class Autotuner:
     the knobs with vr_set_tuning and times run() with the kernel timers.  A
     caller may pass its own measure (the tests do, without a GPU)."""
 
-    def __init__(self, device="gpu0", cache_path=None, reps=3, lib=None, setter=None):
+    def __init__(self, device="gpu0", cache_path=None, reps=3, lib=None, setter=None,
+                 final_rounds=0, finalists=3):
         self.device = device
         self.cache_path = cache_path
         self.reps = reps
+        # final_rounds > 0: after the sweep, time the `finalists` fastest
+        # candidates again, in turn, that many rounds, and keep each one's
+        # best time, so a choice between near-equal settings does not rest
+        # on one sample taken while the clocks drifted
+        self.final_rounds = final_rounds
+        self.finalists = finalists
         self.lib = lib
         self.setter = setter or (lambda k, v: set_knob(k, v, self.lib))
         self.cache = {}
@@ -127,9 +134,19 @@ class Autotuner:
             self.apply(knobs)
             table.append((measure(knobs), knobs))
         ms, best = min(table, key=lambda x: x[0])
+        final = None
+        if self.final_rounds > 0 and len(table) > 1:
+            final = [[m, k] for m, k in sorted(table, key=lambda x: x[0])[:self.finalists]]
+            for _ in range(self.final_rounds):
+                for f in final:
+                    self.apply(f[1])
+                    f[0] = min(f[0], measure(f[1]))
+            ms, best = min(final, key=lambda x: x[0])
         self.apply(best)
         rec = {"best": best, "best_ms": ms, "table": [{"ms": m, "knobs": k} for m, k in table],
                "tuned_at": time.strftime("%Y-%m-%dT%H:%M:%S")}
+        if final is not None:
+            rec["final"] = [{"ms": m, "knobs": k} for m, k in final]
         self.cache[ck] = rec
         self._save()
         return dict(rec, cached=False)
